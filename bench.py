@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Throughput of the 2D LLG solver hot path on one MI355X.
+
+Workload: 1536x512-cell square-hole film (centred 444x444 hole, 25.07 %),
+two-domain +-x initial state with seeded 0.05 rad noise, FFT demag of
+strength 4 and thickness 1 on the zero-padded 3072x1024 grid, FP64,
+BDF + Newton + SPGMR(5) + analytic Jv + 3x3 block-Jacobi, rtol = atol = 1e-4.
+A step advances the simulation by 1.0 time unit (ten CVode outputs at the
+T1 = 0.1 cadence); the default 100 steps cover t in [0, 100].
+
+Prints one JSON line: sim-time/wall-s, SPGMR iterations per step, the
+demag column kernel's achieved bandwidth against MI355X HBM peak, and the
+CPU oracle timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for p in (str(ROOT / "gpu-micromagnetics-irregular_amd"), str(ROOT)):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+KERNELS = {"rhs": 1, "jtv": 2, "build_p": 3, "apply_p": 4, "demag_row_fwd": 5,
+           "demag_col": 6, "demag_row_inv": 7, "demag": 8}
+
+
+def algorithmic_bytes(kernel, ng, ny, ncell, padded=True):
+    """Unique bytes read + written by one launch (FP64, byte mask)."""
+    px, py = (2 * ng, 2 * ny) if padded else (ng, ny)
+    hx = px // 2 + 1
+    spec = 3 * ny * hx * 16                     # intermediate spectrum (rows stored)
+    tensor = (4 if padded else 5) * hx * py * 8  # real tensor planes
+    state = 3 * ncell * 8
+    return {
+        "demag_row_fwd": state + spec,
+        "demag_col": 2 * spec + tensor,
+        "demag_row_inv": spec + state,
+        "demag": 2 * state + 4 * spec + tensor,
+        "rhs": 3 * state + ncell,                # y, hd, yd + mask byte
+        "jtv": 3 * state + ncell,                # y, v, Jv + mask byte
+        "apply_p": 3 * ncell * 8 * 3 + 2 * state + ncell,
+        "build_p": 2 * state + 9 * ncell * 8,
+    }[kernel]
+
+
+def cpu_baseline(la, cfg, geom, y0, budget_s, threads):
+    """The CPU oracle (OpenMP C restatement of the callbacks + serial
+    N_Vector + the same BDF integrator) on the same workload from t = 0,
+    stopped after ~budget_s seconds of integration."""
+    from oracle import oracle as orc
+    L = orc.load()
+    if threads:
+        L.orc_set_num_threads(threads)
+    nthreads = L.orc_num_threads()
+    t_init = time.perf_counter()
+    sim = orc.OracleSim(cfg, geom)
+    init_s = time.perf_counter() - t_init
+    n = sim.neq
+    y = orc.SerialVector(n, y0)
+    P = la.lib()
+    mem = P.LLGCVodeCreate(2, None)
+    P.LLGCVodeSetMaxOrd(mem, 5)
+    P.LLGCVodeInit(mem, orc.addr("orc_cv_rhs"), 0.0, y.v)
+    P.LLGCVodeSetUserData(mem, sim.ctx)
+    P.LLGCVodeSStolerances(mem, 1e-4, 1e-4)
+    P.LLGCVodeSetLinearSolverSPGMR(mem, 1, 5, 2 if n < 500000 else 1)
+    P.LLGCVodeSetJacTimes(mem, None, orc.addr("orc_cv_jtv"))
+    P.LLGCVodeSetPreconditioner(mem, orc.addr("orc_cv_psetup"), orc.addr("orc_cv_psolve"))
+    tret = C.c_double(0.0)
+    t0 = time.perf_counter()
+    tout, dt = 0.01, 0.01
+    while True:
+        rc = P.LLGCVode(mem, tout, y.v, C.byref(tret), 1)
+        if rc < 0 or time.perf_counter() - t0 > budget_s:
+            break
+        tout += dt
+    wall = time.perf_counter() - t0
+    st = la.native.llgcv_stats()
+    P.LLGCVodeGetStats(mem, C.byref(st))
+    m = C.c_void_p(mem)
+    P.LLGCVodeFree(C.byref(m))
+    sim.close()
+    return {"value": tret.value / wall, "unit": "sim-time/wall-s", "cores": nthreads,
+            "kind": "port",
+            "sample": (f"same workload from t=0 to t={tret.value:.3f} ({st.nst} BDF steps, "
+                       f"{st.nfe} f evals) in {wall:.1f} s on {nthreads} OpenMP threads; "
+                       f"oracle setup (tensor + plans) {init_s:.1f} s excluded")}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100, help="timed steps (1.0 sim time each)")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed warmup steps (separate run)")
+    ap.add_argument("--ng", type=int, default=1536)
+    ap.add_argument("--ny", type=int, default=512)
+    ap.add_argument("--periodic", action="store_true", help="reference periodic demag instead of padded")
+    ap.add_argument("--profile-kernel", default="demag_col", choices=sorted(KERNELS))
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--gstype", type=int, default=0, help="0 auto, 1 modified, 2 classical GS")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1 "
+                                   "(the multi-GPU decomposition is design-only)"}))
+        sys.exit(2)
+
+    import llg_amd as la
+    la.lib()   # raises when the native library is missing
+
+    ng, ny = args.ng, args.ny
+    padded = not args.periodic
+    geom = la.Geometry.square_hole(ng, ny, halfside_frac=222.0 / 512.0)
+    cfg = la.preset("square_hole", ng=ng, ny=ny, padded=padded)
+    y0 = geom.ic_split_x(max_angle=0.05, seed=1)
+    print(f"[bench] workload {ng}x{ny} cells, hole {int((geom.active == 0).sum())} cells, "
+          f"{'padded' if padded else 'periodic'} demag; building context", file=sys.stderr, flush=True)
+    t_setup = time.perf_counter()
+    sim = la.Simulation(cfg, geom)
+    setup_s = time.perf_counter() - t_setup
+    step = 1.0
+    ropts = dict(rtol=1e-4, atol=1e-4, maxl=5, gstype=args.gstype, max_order=5, t_out=0.1,
+                 mxstep=100000)
+
+    if args.warmup > 0:
+        w = sim.integrator(y0, **ropts)
+        w.advance(args.warmup * step)
+        w.close()
+        print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+
+    run = sim.integrator(y0, **ropts)
+    L = la.lib()
+    L.LLG_ProfileStart(KERNELS[args.profile_kernel], 200000)
+    out = run.advance(args.steps * step)
+    tot_ms, nl, dropped = C.c_double(), C.c_long(), C.c_long()
+    L.LLG_ProfileStop(C.byref(tot_ms), C.byref(nl), C.byref(dropped))
+    wall = out["wall_s"]
+    value = args.steps * step / wall
+    print(f"[bench] timed {args.steps} steps in {wall:.3f} s: nst={out['nst']} nfe={out['nfe']} "
+          f"nli={out['nli']} nni={out['nni']} netf={out['netf']} ncfn={out['ncfn']}",
+          file=sys.stderr, flush=True)
+
+    ncell = ng * ny
+    nbytes = algorithmic_bytes(args.profile_kernel, ng, ny, ncell, padded)
+    avg_s = (tot_ms.value / 1e3) / max(nl.value, 1)
+    achieved = nbytes / avg_s / 1e9 if nl.value else 0.0
+    roof = {"bound": "hbm", "kernel": args.profile_kernel, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "bytes_per_launch": nbytes, "avg_launch_us": round(avg_s * 1e6, 2),
+            "launches": nl.value}
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(la, cfg, geom, y0, args.cpu_budget,
+                           args.cpu_threads or min(16, os.cpu_count() or 1))
+
+    rec = {
+        "metric": "sim-time/wall-s at rtol=1e-4 (1536x512+demag)",
+        "value": round(value, 4),
+        "unit": "sim-time/wall-s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded two-domain state on the square-hole geometry)",
+        "config": {"workload": f"square-hole {ng}x{ny} cells, "
+                               f"{'zero-padded %dx%d' % (2 * ng, 2 * ny) if padded else 'periodic'} "
+                               "FFT demag 4.0, rtol=atol=1e-4, Krylov 5, block-Jacobi",
+                   "t_end": args.steps * step,
+                   "spgmr_iters_per_step": round(out["nli"] / max(out["nst"], 1), 3),
+                   "nli_per_nni": round(out["nli"] / max(out["nni"], 1), 3),
+                   "nst": out["nst"], "nfe": out["nfe"], "nni": out["nni"], "nli": out["nli"],
+                   "nsetups": out["nsetups"], "netf": out["netf"], "ncfn": out["ncfn"],
+                   "host_syncs": out["nsyncs"], "setup_s": round(setup_s, 2)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,767 @@
+// demag_fft.hip -- FFT demagnetising field on gfx950 with a hand-written
+// LDS Stockham FFT.  Replaces the reference's cuFFT pipeline
+// gather -> D2Z(batch 3) -> multiply(9 complex planes) -> Z2D -> shift/scale
+// (src/irregular/2d_fft/2d_i4/demag_fft.cu:94-161,184-363) with three
+// fused passes per Demag_Apply:
+//
+//   1. row pass   : per (row, component) workgroup: gather y (x w for the
+//                   windowed form) straight from the SoA state, real FFT of
+//                   length Px as a complex FFT of length Px/2 in LDS plus
+//                   the split post-process; writes Px/2+1 bins.
+//   2. column pass: per spectral column kx: the three components' column
+//                   FFTs (length Py) in LDS, the 3x3 tensor product, the
+//                   inverse column FFTs; only the rows the output needs
+//                   are written back.
+//   3. row pass   : inverse real FFT per (row, component), writes the
+//                   field directly into the SoA output (overwrite).
+//
+// The tensor is stored as real planes because the centred thin-film
+// kernels are even (xx, yy, zz) or odd-odd (xy); only the periodic xy
+// plane keeps an imaginary part (its -n/2 line breaks the odd symmetry).
+// xz/yz/zx/zy vanish for the z=0 source (their spectra are rounding
+// noise); the periodic shift and the strength/(Px*Py) scale are folded
+// into the stored spectrum.  The Newell tensor itself (`ctt`/`calt`,
+// 2d_i4/demag_fft.cu:27-91) is evaluated on the GPU over one quadrant of
+// displacements and mirrored.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "llg/demag_fft.h"
+#include "llg/nvector_hip.h"
+#include "kprof.h"
+
+namespace {
+
+thread_local std::string g_derr;
+void derr(const std::string& e) { g_derr = e; }
+
+constexpr int kMaxPasses = 16;
+constexpr int kXcd = 8;
+
+struct Plan1D {
+  int L;
+  int npass;
+  int radix[kMaxPasses];
+  int bpt[kMaxPasses];      // butterflies per thread per pass (<= 8)
+  const double2* tw;        // exp(-2 pi i q / L), q in [0, L)
+};
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// multiply by sign*i
+__device__ inline double2 cmuli(double2 a, double sign) { return make_double2(-sign * a.y, sign * a.x); }
+
+// in-register DFT of length R, kernel exp(sign * 2 pi i j k / R)
+template <int R>
+__device__ inline void dft(double2 (&v)[R], double sign);
+
+template <>
+__device__ inline void dft<2>(double2 (&v)[2], double) {
+  const double2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+template <>
+__device__ inline void dft<4>(double2 (&v)[4], double sign) {
+  const double2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+  const double2 c = cadd(v[1], v[3]), d = cmuli(csub(v[1], v[3]), sign);
+  v[0] = cadd(a, c);
+  v[2] = csub(a, c);
+  v[1] = cadd(b, d);
+  v[3] = csub(b, d);
+}
+template <>
+__device__ inline void dft<8>(double2 (&v)[8], double sign) {
+  double2 e[4] = {v[0], v[2], v[4], v[6]};
+  double2 o[4] = {v[1], v[3], v[5], v[7]};
+  dft<4>(e, sign);
+  dft<4>(o, sign);
+  const double r = 0.70710678118654752440;
+  // W^k = exp(sign 2 pi i k / 8)
+  const double2 w1 = make_double2(r, sign * r);
+  const double2 w3 = make_double2(-r, sign * r);
+  const double2 t0 = o[0];
+  const double2 t1 = cmul(o[1], w1);
+  const double2 t2 = cmuli(o[2], sign);
+  const double2 t3 = cmul(o[3], w3);
+  v[0] = cadd(e[0], t0); v[4] = csub(e[0], t0);
+  v[1] = cadd(e[1], t1); v[5] = csub(e[1], t1);
+  v[2] = cadd(e[2], t2); v[6] = csub(e[2], t2);
+  v[3] = cadd(e[3], t3); v[7] = csub(e[3], t3);
+}
+template <>
+__device__ inline void dft<3>(double2 (&v)[3], double sign) {
+  const double c = -0.5, s = sign * 0.86602540378443864676;
+  const double2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+  const double2 a = make_double2(v[0].x + c * t1.x, v[0].y + c * t1.y);
+  const double2 b = make_double2(-s * t2.y, s * t2.x);   // i*s*t2
+  v[0] = cadd(v[0], t1);
+  v[1] = cadd(a, b);
+  v[2] = csub(a, b);
+}
+template <>
+__device__ inline void dft<5>(double2 (&v)[5], double sign) {
+  const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+  const double s1 = sign * 0.95105651629515357212, s2 = sign * 0.58778525229247312917;
+  const double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  const double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  const double2 a1 = make_double2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+  const double2 a2 = make_double2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+  const double2 b1r = make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+  const double2 b2r = make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+  const double2 b1 = make_double2(-b1r.y, b1r.x);   // i * b1r
+  const double2 b2 = make_double2(-b2r.y, b2r.x);
+  v[0] = cadd(v[0], cadd(t1, t2));
+  v[1] = cadd(a1, b1);
+  v[4] = csub(a1, b1);
+  v[2] = cadd(a2, b2);
+  v[3] = csub(a2, b2);
+}
+
+// One Stockham pass of radix R over ncomp transforms of length L stored at
+// buf + comp*cstride (in place: all reads, barrier, all writes, barrier).
+template <int R, int B>
+__device__ void pass(double2* buf, int L, int Ns, double sign, const double2* __restrict__ tw,
+                     int ncomp, int cstride) {
+  const int nb = L / R;
+  const int total = nb * ncomp;
+  const int tstep = L / (Ns * R);
+  double2 v[B][R];
+#pragma unroll
+  for (int r = 0; r < B; r++) {
+    const int b = threadIdx.x + r * blockDim.x;
+    if (b < total) {
+      const int comp = b / nb, j = b - comp * nb;
+      const double2* src = buf + comp * cstride;
+      const int k = j % Ns;
+#pragma unroll
+      for (int t = 0; t < R; t++) {
+        double2 x = src[j + t * nb];
+        if (t > 0 && k > 0) {
+          double2 w = tw[t * k * tstep];
+          if (sign > 0) w.y = -w.y;
+          x = cmul(x, w);
+        }
+        v[r][t] = x;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < B; r++) {
+    const int b = threadIdx.x + r * blockDim.x;
+    if (b < total) {
+      const int comp = b / nb, j = b - comp * nb;
+      double2* dst = buf + comp * cstride;
+      const int k = j % Ns;
+      dft<R>(v[r], sign);
+      const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int t = 0; t < R; t++) dst[base + t * Ns] = v[r][t];
+    }
+  }
+  __syncthreads();
+}
+
+// register budget: R*B complex values per thread
+template <int R>
+struct MaxB { static constexpr int value = R == 8 ? 2 : (R == 5 ? 3 : (R == 2 ? 8 : 4)); };
+
+template <int R, int B>
+__device__ inline void pass_if(double2* buf, int L, int Ns, double sign, const double2* tw,
+                               int ncomp, int cstride) {
+  if constexpr (B <= MaxB<R>::value) pass<R, B>(buf, L, Ns, sign, tw, ncomp, cstride);
+}
+
+template <int R>
+__device__ void pass_b(double2* buf, int L, int Ns, double sign, const double2* tw, int ncomp,
+                       int cstride, int B) {
+  switch (B) {
+    case 1: pass_if<R, 1>(buf, L, Ns, sign, tw, ncomp, cstride); break;
+    case 2: pass_if<R, 2>(buf, L, Ns, sign, tw, ncomp, cstride); break;
+    case 3: pass_if<R, 3>(buf, L, Ns, sign, tw, ncomp, cstride); break;
+    case 4: pass_if<R, 4>(buf, L, Ns, sign, tw, ncomp, cstride); break;
+    default: pass_if<R, 8>(buf, L, Ns, sign, tw, ncomp, cstride); break;
+  }
+}
+
+__device__ void fft_lds(double2* buf, const Plan1D& p, double sign, int ncomp, int cstride) {
+  int Ns = 1;
+  for (int i = 0; i < p.npass; i++) {
+    const int R = p.radix[i], B = p.bpt[i];
+    switch (R) {
+      case 8: pass_b<8>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
+      case 4: pass_b<4>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
+      case 2: pass_b<2>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
+      case 3: pass_b<3>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
+      default: pass_b<5>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
+    }
+    Ns *= R;
+  }
+}
+
+// ----------------------------------------------------------- row forward
+// grid (rows, ncomp); in = base + comp*comp_stride + row*nx_in
+__global__ __launch_bounds__(1024) void k_row_fwd(const double* __restrict__ in,
+                                                 const double* __restrict__ w, long comp_stride,
+                                                 int nx_in, int rows, Plan1D plan,
+                                                 const double2* __restrict__ post,
+                                                 double2* __restrict__ spec) {
+  extern __shared__ double2 lds[];
+  const int row = blockIdx.x, comp = blockIdx.y;
+  const int L = plan.L, Hx = L + 1;
+  const double* src = in + comp * comp_stride + (long)row * nx_in;
+  const double* wr = w ? w + (long)row * nx_in : nullptr;
+  for (int m = threadIdx.x; m < L; m += blockDim.x) {
+    const int n0 = 2 * m, n1 = 2 * m + 1;
+    double a = 0.0, b = 0.0;
+    if (n0 < nx_in) a = wr ? wr[n0] * src[n0] : src[n0];
+    if (n1 < nx_in) b = wr ? wr[n1] * src[n1] : src[n1];
+    lds[m] = make_double2(a, b);
+  }
+  __syncthreads();
+  fft_lds(lds, plan, -1.0, 1, L);
+  double2* out = spec + ((long)comp * rows + row) * Hx;
+  for (int k = threadIdx.x; k <= L; k += blockDim.x) {
+    const double2 zk = lds[k == L ? 0 : k];
+    const double2 zc = cconj(lds[k == 0 ? 0 : L - k]);
+    const double2 e = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
+    const double2 d = csub(zk, zc);
+    const double2 o = make_double2(0.5 * d.y, -0.5 * d.x);
+    out[k] = cadd(e, cmul(post[k], o));
+  }
+}
+
+// ---------------------------------------------------------- row inverse
+__global__ __launch_bounds__(1024) void k_row_inv(const double2* __restrict__ spec, int rows,
+                                                 Plan1D plan, const double2* __restrict__ post,
+                                                 double* __restrict__ out, long comp_stride,
+                                                 int nx_out) {
+  extern __shared__ double2 lds[];
+  const int row = blockIdx.x, comp = blockIdx.y;
+  const int L = plan.L, Hx = L + 1;
+  const double2* in = spec + ((long)comp * rows + row) * Hx;
+  for (int k = threadIdx.x; k < L; k += blockDim.x) {
+    const double2 xk = in[k];
+    const double2 xc = cconj(in[L - k]);
+    const double2 a = cadd(xk, xc);
+    const double2 b = cmul(csub(xk, xc), cconj(post[k]));
+    lds[k] = make_double2(a.x - b.y, a.y + b.x);   // a + i b
+  }
+  __syncthreads();
+  fft_lds(lds, plan, +1.0, 1, L);
+  double* dst = out + comp * comp_stride + (long)row * nx_out;
+  for (int m = threadIdx.x; m < L; m += blockDim.x) {
+    const int n0 = 2 * m;
+    const double2 z = lds[m];
+    if (n0 < nx_out) dst[n0] = z.x;
+    if (n0 + 1 < nx_out) dst[n0 + 1] = z.y;
+  }
+}
+
+__device__ inline int xcd_remap(int bid, int G) {
+  const int per = G / kXcd, rem = G % kXcd;
+  const int x = bid % kXcd, idx = bid / kXcd;
+  return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
+}
+
+// --------------------------------------------------------------- column
+// mode 0: fwd FFT, 3x3 tensor product, inverse FFT (3 components)
+// mode 1: fwd FFT only (1 component; tensor spectra at init)
+__global__ __launch_bounds__(1024) void k_col(double2* __restrict__ spec, int rows_in,
+                                             int rows_out, int Hx, Plan1D plan,
+                                             const double* __restrict__ K, int has_xy_im,
+                                             int mode) {
+  extern __shared__ double2 lds[];
+  const int kx = xcd_remap(blockIdx.x, gridDim.x);
+  const int Py = plan.L;
+  const int ncomp = mode == 0 ? 3 : 1;
+  for (int c = 0; c < ncomp; c++)
+    for (int y = threadIdx.x; y < Py; y += blockDim.x)
+      lds[c * Py + y] = y < rows_in ? spec[((long)c * rows_in + y) * Hx + kx] : make_double2(0.0, 0.0);
+  __syncthreads();
+  fft_lds(lds, plan, -1.0, ncomp, Py);
+  if (mode == 0) {
+    const long P = (long)Hx * Py;
+    const double* kxx = K + (long)kx * Py;
+    const double* kyy = kxx + P;
+    const double* kzz = kyy + P;
+    const double* kxyr = kzz + P;
+    const double* kxyi = kxyr + P;
+    for (int ky = threadIdx.x; ky < Py; ky += blockDim.x) {
+      const double2 mx = lds[ky], my = lds[Py + ky], mz = lds[2 * Py + ky];
+      const double a = kxx[ky], b = kyy[ky], cz = kzz[ky];
+      const double2 xy = make_double2(kxyr[ky], has_xy_im ? kxyi[ky] : 0.0);
+      const double2 hx = cadd(make_double2(a * mx.x, a * mx.y), cmul(xy, my));
+      const double2 hy = cadd(cmul(xy, mx), make_double2(b * my.x, b * my.y));
+      lds[ky] = hx;
+      lds[Py + ky] = hy;
+      lds[2 * Py + ky] = make_double2(cz * mz.x, cz * mz.y);
+    }
+    __syncthreads();
+    fft_lds(lds, plan, +1.0, 3, Py);
+  }
+  for (int c = 0; c < ncomp; c++)
+    for (int y = threadIdx.x; y < rows_out; y += blockDim.x)
+      spec[((long)c * rows_out + y) * Hx + kx] = lds[c * Py + y];
+}
+
+// ------------------------------------------------------------ tensor
+__device__ void ctt_dev(double b, double a, double sx, double sy, double dm[10]) {
+  const double sz = 0.0;
+  const double xn = sx - a, xp = sx + a, yn = sy - a, yp = sy + a, zn = sz - b, zp = sz + b;
+  const double xn2 = xn * xn, xp2 = xp * xp, yn2 = yn * yn, yp2 = yp * yp, zn2 = zn * zn,
+               zp2 = zp * zp;
+  const double dnnn = sqrt(xn2 + yn2 + zn2), dpnn = sqrt(xp2 + yn2 + zn2);
+  const double dnpn = sqrt(xn2 + yp2 + zn2), dnnp = sqrt(xn2 + yn2 + zp2);
+  const double dppn = sqrt(xp2 + yp2 + zn2), dnpp = sqrt(xn2 + yp2 + zp2);
+  const double dpnp = sqrt(xp2 + yn2 + zp2), dppp = sqrt(xp2 + yp2 + zp2);
+  dm[1] = atan(zn * yn / (xn * dnnn)) - atan(zp * yn / (xn * dnnp)) -
+          atan(zn * yp / (xn * dnpn)) + atan(zp * yp / (xn * dnpp)) -
+          atan(zn * yn / (xp * dpnn)) + atan(zp * yn / (xp * dpnp)) +
+          atan(zn * yp / (xp * dppn)) - atan(zp * yp / (xp * dppp));
+  dm[2] = log((dnnn - zn) / (dnnp - zp)) - log((dpnn - zn) / (dpnp - zp)) -
+          log((dnpn - zn) / (dnpp - zp)) + log((dppn - zn) / (dppp - zp));
+  dm[5] = atan(zn * xn / (yn * dnnn)) - atan(zp * xn / (yn * dnnp)) -
+          atan(zn * xp / (yn * dpnn)) + atan(zp * xp / (yn * dpnp)) -
+          atan(zn * xn / (yp * dnpn)) + atan(zp * xn / (yp * dnpp)) +
+          atan(zn * xp / (yp * dppn)) - atan(zp * xp / (yp * dppp));
+  dm[9] = atan(xn * yn / (zn * dnnn)) - atan(xp * yn / (zn * dpnn)) -
+          atan(xn * yp / (zn * dnpn)) + atan(xp * yp / (zn * dppn)) -
+          atan(xn * yn / (zp * dnnp)) + atan(xp * yn / (zp * dpnp)) +
+          atan(xn * yp / (zp * dnpp)) - atan(xp * yp / (zp * dppp));
+}
+
+// quadrant Q[c][dy][dx], dx in [0,qx), dy in [0,qy); c = xx, yy, zz, xy
+__global__ void k_tensor_quadrant(double thick, int qx, int qy, double* __restrict__ Q) {
+  const int dx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int dy = blockIdx.y;
+  if (dx >= qx || dy >= qy) return;
+  const double a = 0.49999, b = 0.5 * thick;
+  double sxx = 0, syy = 0, szz = 0, sxy = 0;
+  double dm[10];
+  for (int jy = -4; jy <= 4; jy++) {
+    const double sy = (double)dy + 0.1 * (double)jy;
+    for (int ix = -4; ix <= 4; ix++) {
+      const double sx = (double)dx + 0.1 * (double)ix;
+      ctt_dev(b, a, sx, sy, dm);
+      sxx += dm[1]; sxy += dm[2]; syy += dm[5]; szz += dm[9];
+    }
+  }
+  const long P = (long)qx * qy, k = (long)dy * qx + dx;
+  Q[k] = sxx / 81.;
+  Q[P + k] = syy / 81.;
+  Q[2 * P + k] = szz / 81.;
+  Q[3 * P + k] = sxy / 81.;
+}
+
+// fill the four real kernel planes K[c][r][s] (Py x Px) from the quadrant.
+// periodic: index s <-> displacement wrapped into [-Px/2, Px - Px/2)
+// padded  : s < Px/2 -> d = s, s > Px/2 -> d = s - Px, s == Px/2 -> 0
+__global__ void k_tensor_fill(const double* __restrict__ Q, int qx, int qy, int Px, int Py,
+                              int padded, double* __restrict__ K) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (s >= Px) return;
+  const long P = (long)Px * Py, k = (long)r * Px + s;
+  int dx, dy;
+  bool zero = false;
+  if (padded) {
+    const int hx = Px / 2, hy = Py / 2;
+    if (s == hx || r == hy) zero = true;
+    dx = s < hx ? s : s - Px;
+    dy = r < hy ? r : r - Py;
+  } else {
+    const int hx = Px / 2, hy = Py / 2;
+    dx = ((s + hx) % Px) - hx;
+    dy = ((r + hy) % Py) - hy;
+  }
+  if (zero) {
+    K[k] = K[P + k] = K[2 * P + k] = K[3 * P + k] = 0.0;
+    return;
+  }
+  const int ax = dx < 0 ? -dx : dx, ay = dy < 0 ? -dy : dy;
+  const long PQ = (long)qx * qy, q = (long)ay * qx + ax;
+  const double sgn = ((dx < 0) != (dy < 0)) ? -1.0 : 1.0;
+  K[k] = Q[q];
+  K[P + k] = Q[PQ + q];
+  K[2 * P + k] = Q[2 * PQ + q];
+  K[3 * P + k] = (dx == 0 || dy == 0) ? sgn * Q[3 * PQ + q] : sgn * Q[3 * PQ + q];
+}
+
+// spectra [ky][kx] (one plane) -> stored plane [kx][ky] (transposed), scaled
+__global__ void k_extract(const double2* __restrict__ S, int Py, int Hx, double scale,
+                          int take_imag, double* __restrict__ outp) {
+  const int kx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ky = blockIdx.y;
+  if (kx >= Hx) return;
+  const double2 v = S[(long)ky * Hx + kx];
+  outp[(long)kx * Py + ky] = scale * (take_imag ? v.y : v.x);
+}
+
+// ---------------------------------------------------------- host helpers
+bool factorize(int L, std::vector<int>& rad) {
+  rad.clear();
+  int n = L;
+  while (n % 8 == 0 && n > 1) { rad.push_back(8); n /= 8; }
+  if (n % 2 == 0 && n % 4 != 0 && !rad.empty() && rad.back() == 8) {
+    rad.back() = 4;   // 8*2 -> 4*4
+    n *= 2;
+  }
+  while (n % 4 == 0 && n > 1) { rad.push_back(4); n /= 4; }
+  while (n % 2 == 0 && n > 1) { rad.push_back(2); n /= 2; }
+  while (n % 3 == 0 && n > 1) { rad.push_back(3); n /= 3; }
+  while (n % 5 == 0 && n > 1) { rad.push_back(5); n /= 5; }
+  return n == 1 && (int)rad.size() <= kMaxPasses;
+}
+
+struct HostPlan {
+  Plan1D p{};
+  int block = 64;
+  double2* d_tw = nullptr;
+};
+
+int max_b(int r) { return r == 8 ? 2 : (r == 5 ? 3 : (r == 2 ? 8 : 4)); }
+
+bool make_plan(int L, int ncomp, HostPlan& hp) {
+  std::vector<int> rad;
+  if (L < 1 || !factorize(L, rad)) {
+    derr("unsupported FFT length " + std::to_string(L) + " (need 2^a 3^b 5^c)");
+    return false;
+  }
+  hp.p.L = L;
+  hp.p.npass = (int)rad.size();
+  // smallest multiple-of-64 workgroup (>= 64) for which every pass fits its
+  // per-thread register budget
+  int block = 64;
+  for (int r : rad) {
+    const int tot = ncomp * L / r;
+    const int need = (tot + max_b(r) - 1) / max_b(r);
+    block = std::max(block, ((need + 63) / 64) * 64);
+  }
+  int maxb = 1;
+  for (int r : rad) maxb = std::max(maxb, ncomp * L / r);
+  block = std::max(block, std::min(((maxb + 63) / 64) * 64, 256));
+  if (block > 1024) { derr("FFT length too large for one workgroup: " + std::to_string(L)); return false; }
+  hp.block = block;
+  for (int i = 0; i < hp.p.npass; i++) {
+    hp.p.radix[i] = rad[i];
+    const int tot = ncomp * L / rad[i];
+    int b = (tot + block - 1) / block;
+    if (b > 4) b = 8;
+    hp.p.bpt[i] = b;
+  }
+  std::vector<double2> tw(L);
+  for (int q = 0; q < L; q++) {
+    const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)q / (long double)L;
+    tw[q] = make_double2((double)cosl(ang), (double)sinl(ang));
+  }
+  if (hipMalloc(&hp.d_tw, sizeof(double2) * L) != hipSuccess) { derr("hipMalloc twiddles"); return false; }
+  hipMemcpy(hp.d_tw, tw.data(), sizeof(double2) * L, hipMemcpyHostToDevice);
+  hp.p.tw = hp.d_tw;
+  return true;
+}
+
+void free_plan(HostPlan& hp) {
+  if (hp.d_tw) hipFree(hp.d_tw);
+  hp.d_tw = nullptr;
+}
+
+// host restatement of ctt (used only for the self-coupling N(0))
+void ctt_host(double b, double a, double sx, double sy, double dm[10]) {
+  const double sz = 0.0;
+  const double xn = sx - a, xp = sx + a, yn = sy - a, yp = sy + a, zn = sz - b, zp = sz + b;
+  const double xn2 = xn * xn, xp2 = xp * xp, yn2 = yn * yn, yp2 = yp * yp, zn2 = zn * zn,
+               zp2 = zp * zp;
+  const double dnnn = std::sqrt(xn2 + yn2 + zn2), dpnn = std::sqrt(xp2 + yn2 + zn2);
+  const double dnpn = std::sqrt(xn2 + yp2 + zn2), dnnp = std::sqrt(xn2 + yn2 + zp2);
+  const double dppn = std::sqrt(xp2 + yp2 + zn2), dnpp = std::sqrt(xn2 + yp2 + zp2);
+  const double dpnp = std::sqrt(xp2 + yn2 + zp2), dppp = std::sqrt(xp2 + yp2 + zp2);
+  using std::atan;
+  using std::log;
+  dm[1] = atan(zn * yn / (xn * dnnn)) - atan(zp * yn / (xn * dnnp)) -
+          atan(zn * yp / (xn * dnpn)) + atan(zp * yp / (xn * dnpp)) -
+          atan(zn * yn / (xp * dpnn)) + atan(zp * yn / (xp * dpnp)) +
+          atan(zn * yp / (xp * dppn)) - atan(zp * yp / (xp * dppp));
+  dm[2] = log((dnnn - zn) / (dnnp - zp)) - log((dpnn - zn) / (dpnp - zp)) -
+          log((dnpn - zn) / (dnpp - zp)) + log((dppn - zn) / (dppp - zp));
+  dm[3] = log((dnnn - yn) / (dnpn - yp)) - log((dpnn - yn) / (dppn - yp)) -
+          log((dnnp - yn) / (dnpp - yp)) + log((dpnp - yn) / (dppp - yp));
+  dm[4] = log((dnnn - zn) / (dnnp - zp)) - log((dnpn - zn) / (dnpp - zp)) -
+          log((dpnn - zn) / (dpnp - zp)) + log((dppn - zn) / (dppp - zp));
+  dm[5] = atan(zn * xn / (yn * dnnn)) - atan(zp * xn / (yn * dnnp)) -
+          atan(zn * xp / (yn * dpnn)) + atan(zp * xp / (yn * dpnp)) -
+          atan(zn * xn / (yp * dnpn)) + atan(zp * xn / (yp * dnpp)) +
+          atan(zn * xp / (yp * dppn)) - atan(zp * xp / (yp * dppp));
+  dm[6] = log((dnnn - xn) / (dpnn - xp)) - log((dnpn - xn) / (dppn - xp)) -
+          log((dnnp - xn) / (dpnp - xp)) + log((dnpp - xn) / (dppp - xp));
+  dm[7] = log((dnnn - yn) / (dnpn - yp)) - log((dnnp - yn) / (dnpp - yp)) -
+          log((dpnn - yn) / (dppn - yp)) + log((dpnp - yn) / (dppp - yp));
+  dm[8] = log((dnnn - xn) / (dpnn - xp)) - log((dnnp - xn) / (dpnp - xp)) -
+          log((dnpn - xn) / (dppn - xp)) + log((dnpp - xn) / (dppp - xp));
+  dm[9] = atan(xn * yn / (zn * dnnn)) - atan(xp * yn / (zn * dpnn)) -
+          atan(xn * yp / (zn * dnpn)) + atan(xp * yp / (zn * dppn)) -
+          atan(xn * yn / (zp * dnnp)) + atan(xp * yn / (zp * dpnp)) +
+          atan(xn * yp / (zp * dnpp)) - atan(xp * yp / (zp * dppp));
+}
+
+void tensor_at_host(double thick, int dx, int dy, double out9[9]) {
+  const double a = 0.49999, b = 0.5 * thick;
+  double dm[10];
+  for (int c = 0; c < 9; c++) out9[c] = 0.0;
+  for (int jy = -4; jy <= 4; jy++) {
+    const double sy = (double)dy + 0.1 * (double)jy;
+    for (int ix = -4; ix <= 4; ix++) {
+      const double sx = (double)dx + 0.1 * (double)ix;
+      ctt_host(b, a, sx, sy, dm);
+      for (int c = 0; c < 9; c++) out9[c] += dm[c + 1];
+    }
+  }
+  for (int c = 0; c < 9; c++) out9[c] /= 81.;
+}
+
+}  // namespace
+
+// ================================================================= state
+struct DemagData {
+  int nx, ny, px, py, hx, padded;
+  double strength;
+  double n0[3];
+  hipStream_t stream;
+  HostPlan row, col, col1;
+  double2* d_post;    // exp(-2 pi i k / Px), k in [0, Px/2]
+  double2* d_spec;    // 3 * ny * hx complex
+  double* d_K;        // 5 planes [hx][py]
+  int has_xy_im;
+  int fp32;
+};
+
+namespace {
+
+bool launch_ok(const char* what) {
+  const hipError_t e = hipPeekAtLastError();
+  if (e != hipSuccess) {
+    derr(std::string(what) + ": " + hipGetErrorString(e));
+    return false;
+  }
+  return true;
+}
+
+void destroy(DemagData* d) {
+  if (!d) return;
+  free_plan(d->row);
+  free_plan(d->col);
+  free_plan(d->col1);
+  hipFree(d->d_post);
+  hipFree(d->d_spec);
+  hipFree(d->d_K);
+  delete d;
+}
+
+DemagData* init_common(int nx, int ny, double thick, double strength, int padded) {
+  g_derr.clear();
+  if (nx <= 0 || ny <= 0) { derr("Demag: bad grid"); return nullptr; }
+  DemagData* d = new DemagData();
+  d->nx = nx; d->ny = ny; d->padded = padded; d->strength = strength;
+  d->px = padded ? 2 * nx : nx;
+  d->py = padded ? 2 * ny : ny;
+  if (d->px % 2) { derr("Demag: FFT row length must be even"); delete d; return nullptr; }
+  d->hx = d->px / 2 + 1;
+  d->stream = (hipStream_t)NVHip_DefaultStream();
+  d->has_xy_im = padded ? 0 : 1;
+  // self coupling N(0) on the host (2d_i4/demag_fft.cu:230-239)
+  double t9[9];
+  tensor_at_host(thick, 0, 0, t9);
+  d->n0[0] = t9[0] * strength;
+  d->n0[1] = t9[4] * strength;
+  d->n0[2] = t9[8] * strength;
+  if (!make_plan(d->px / 2, 1, d->row) || !make_plan(d->py, 3, d->col) ||
+      !make_plan(d->py, 1, d->col1)) {
+    destroy(d);
+    return nullptr;
+  }
+  const int Px = d->px, Py = d->py, Hx = d->hx;
+  std::vector<double2> post(Hx);
+  for (int k = 0; k < Hx; k++) {
+    const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)Px;
+    post[k] = make_double2((double)cosl(ang), (double)sinl(ang));
+  }
+  const size_t spec_elems = std::max((size_t)3 * ny * Hx, (size_t)Py * Hx);
+  if (hipMalloc(&d->d_post, sizeof(double2) * Hx) != hipSuccess ||
+      hipMalloc(&d->d_spec, sizeof(double2) * spec_elems) != hipSuccess ||
+      hipMalloc(&d->d_K, sizeof(double) * 5 * (size_t)Hx * Py) != hipSuccess) {
+    derr("Demag: device allocation failed");
+    destroy(d);
+    return nullptr;
+  }
+  hipMemcpy(d->d_post, post.data(), sizeof(double2) * Hx, hipMemcpyHostToDevice);
+  hipMemset(d->d_K, 0, sizeof(double) * 5 * (size_t)Hx * Py);
+
+  // tensor quadrant on the GPU
+  const int qx = padded ? nx : Px / 2 + 1;
+  const int qy = padded ? ny : Py / 2 + 1;
+  double *dQ = nullptr, *dKr = nullptr;
+  if (hipMalloc(&dQ, sizeof(double) * 4 * (size_t)qx * qy) != hipSuccess ||
+      hipMalloc(&dKr, sizeof(double) * 4 * (size_t)Px * Py) != hipSuccess) {
+    derr("Demag: tensor scratch allocation failed");
+    hipFree(dQ);
+    destroy(d);
+    return nullptr;
+  }
+  hipLaunchKernelGGL(k_tensor_quadrant, dim3((qx + 63) / 64, qy), dim3(64), 0, d->stream, thick,
+                     qx, qy, dQ);
+  hipLaunchKernelGGL(k_tensor_fill, dim3((Px + 255) / 256, Py), dim3(256), 0, d->stream, dQ, qx,
+                     qy, Px, Py, padded, dKr);
+  // spectra of the 4 real planes: row R2C + column forward FFT
+  const double scale = strength / ((double)Px * (double)Py);
+  const int plane_of[4] = {0, 1, 2, 3};   // xx, yy, zz, xy(re/im)
+  for (int c = 0; c < 4; c++) {
+    hipLaunchKernelGGL(k_row_fwd, dim3(Py, 1), dim3(d->row.block),
+                       sizeof(double2) * d->row.p.L, d->stream,
+                       (const double*)(dKr + (size_t)c * Px * Py), (const double*)nullptr,
+                       (long)0, Px, Py, d->row.p, (const double2*)d->d_post, d->d_spec);
+    hipLaunchKernelGGL(k_col, dim3(Hx), dim3(d->col1.block), sizeof(double2) * Py, d->stream,
+                       d->d_spec, Py, Py, Hx, d->col1.p, (const double*)nullptr, 0, 1);
+    const long P = (long)Hx * Py;
+    hipLaunchKernelGGL(k_extract, dim3((Hx + 255) / 256, Py), dim3(256), 0, d->stream,
+                       (const double2*)d->d_spec, Py, Hx, scale, 0, d->d_K + plane_of[c] * P);
+    if (c == 3 && d->has_xy_im)
+      hipLaunchKernelGGL(k_extract, dim3((Hx + 255) / 256, Py), dim3(256), 0, d->stream,
+                         (const double2*)d->d_spec, Py, Hx, scale, 1, d->d_K + 4 * P);
+  }
+  hipStreamSynchronize(d->stream);
+  hipFree(dQ);
+  hipFree(dKr);
+  if (!launch_ok("Demag_Init")) { destroy(d); return nullptr; }
+  return d;
+}
+
+void apply(DemagData* d, const double* y, const double* w, double* h) {
+  const int nx = d->nx, ny = d->ny, Hx = d->hx, Py = d->py;
+  const long N = (long)nx * ny;
+  hipStream_t s = d->stream;
+  kprof::begin(kprof::DEMAG_ALL, s);
+  kprof::begin(kprof::DEMAG_ROW_FWD, s);
+  hipLaunchKernelGGL(k_row_fwd, dim3(ny, 3), dim3(d->row.block), sizeof(double2) * d->row.p.L, s,
+                     y, w, N, nx, ny, d->row.p, (const double2*)d->d_post, d->d_spec);
+  kprof::end(kprof::DEMAG_ROW_FWD, s);
+  kprof::begin(kprof::DEMAG_COL, s);
+  hipLaunchKernelGGL(k_col, dim3(Hx), dim3(d->col.block), sizeof(double2) * 3 * Py, s,
+                     d->d_spec, ny, ny, Hx, d->col.p, (const double*)d->d_K, d->has_xy_im, 0);
+  kprof::end(kprof::DEMAG_COL, s);
+  kprof::begin(kprof::DEMAG_ROW_INV, s);
+  hipLaunchKernelGGL(k_row_inv, dim3(ny, 3), dim3(d->row.block), sizeof(double2) * d->row.p.L, s,
+                     (const double2*)d->d_spec, ny, d->row.p, (const double2*)d->d_post, h, N, nx);
+  kprof::end(kprof::DEMAG_ROW_INV, s);
+  kprof::end(kprof::DEMAG_ALL, s);
+  launch_ok("Demag_Apply");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* Demag_LastError(void) { return g_derr.c_str(); }
+
+DemagData* Demag_Init(int nx, int ny, double thick, double demag_strength) {
+  return init_common(nx, ny, thick, demag_strength, 0);
+}
+
+DemagData* Demag_InitPadded(int nx, int ny, double thick, double demag_strength) {
+  return init_common(nx, ny, thick, demag_strength, 1);
+}
+
+void Demag_Apply(DemagData* d, const double* y_dev, double* h_out_dev) {
+  if (!d) return;
+  g_derr.clear();
+  apply(d, y_dev, nullptr, h_out_dev);
+}
+
+void Demag_ApplyWindowed(DemagData* d, const double* y_dev, const double* w_dev, double* h_out_dev) {
+  if (!d) return;
+  g_derr.clear();
+  apply(d, y_dev, w_dev, h_out_dev);
+}
+
+void Demag_SetStream(DemagData* d, void* stream) {
+  if (!d) return;
+  d->stream = stream ? (hipStream_t)stream : (hipStream_t)NVHip_DefaultStream();
+}
+
+void Demag_GetSelfCoupling(DemagData* d, double* nxx0, double* nyy0, double* nzz0) {
+  if (!d) {
+    if (nxx0) *nxx0 = 0.0;
+    if (nyy0) *nyy0 = 0.0;
+    if (nzz0) *nzz0 = 0.0;
+    return;
+  }
+  if (nxx0) *nxx0 = d->n0[0];
+  if (nyy0) *nyy0 = d->n0[1];
+  if (nzz0) *nzz0 = d->n0[2];
+}
+
+void Demag_Destroy(DemagData* d) { destroy(d); }
+
+int Demag_SetPrecision(DemagData* d, int fp32) {
+  if (!d) return -1;
+  if (fp32) { derr("FP32 spectral demag not available in this build"); return -1; }
+  d->fp32 = 0;
+  return 0;
+}
+
+int Demag_GetPlanShape(DemagData* d, int* px, int* py, int* padded) {
+  if (!d) return -1;
+  if (px) *px = d->px;
+  if (py) *py = d->py;
+  if (padded) *padded = d->padded;
+  return 0;
+}
+
+int Demag_GetSpectrum(DemagData* d, double* h_planes) {
+  if (!d || !h_planes) return -1;
+  const int Hx = d->hx, Py = d->py;
+  const size_t P = (size_t)Hx * Py;
+  std::vector<double> tmp(5 * P);
+  hipStreamSynchronize(d->stream);
+  if (hipMemcpy(tmp.data(), d->d_K, sizeof(double) * 5 * P, hipMemcpyDeviceToHost) != hipSuccess)
+    return -1;
+  for (int c = 0; c < 5; c++)
+    for (int ky = 0; ky < Py; ky++)
+      for (int kx = 0; kx < Hx; kx++)
+        h_planes[c * P + (size_t)ky * Hx + kx] = tmp[c * P + (size_t)kx * Py + ky];
+  return 0;
+}
+
+int Demag_NewellTensor(double thick, int mdx, int mdy, double* planes9, int nthreads) {
+  if (mdx <= 0 || mdy <= 0 || !planes9) return -1;
+  const int mdx2 = mdx / 2, mdy2 = mdy / 2;
+  const size_t plane = (size_t)mdx * mdy;
+  int nt = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
+  if (nt < 1) nt = 1;
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; t++) {
+    pool.emplace_back([=]() {
+      double t9[9];
+      for (int j = t; j < mdy; j += nt)
+        for (int i = 0; i < mdx; i++) {
+          tensor_at_host(thick, i - mdx2, j - mdy2, t9);
+          for (int c = 0; c < 9; c++) planes9[c * plane + (size_t)j * mdx + i] = t9[c];
+        }
+    });
+  }
+  for (auto& th : pool) th.join();
+  return 0;
+}
+
+}  // extern "C"

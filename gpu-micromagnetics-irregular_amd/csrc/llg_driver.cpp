@@ -1,0 +1,143 @@
+// llg_driver.cpp -- the host driver of the reference's main(): wires the
+// LLG plug-ins into the BDF integrator and runs the output loop
+// (src/irregular/2d_fft/2d_i4/2d_fft.cu:545-700):
+//   y/abstol vectors, FusedNVec_Init, CVodeInit(f), SVtolerances,
+//   SPGMR(PREC_LEFT, KRYLOV_DIM) with classical GS iff neq < 500000,
+//   SetJacTimes(JtvProduct), SetPreconditioner(PrecondSetup, PrecondSolve),
+//   SetMaxOrd, then CVode(tout) every T1 with a timed region around the loop.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "llg/llg_cvode.h"
+#include "llg/llg_driver.h"
+#include "llg/llg_plugin.h"
+#include "llg/nvector_hip.h"
+
+namespace {
+
+struct Run {
+  llg_ctx* ctx = nullptr;
+  N_Vector y = nullptr, abstol = nullptr;
+  void* cv = nullptr;
+  struct _SUNContext sctx{};
+  llg_run_opts opts{};
+  double t = 0.0;
+};
+
+}  // namespace
+
+extern "C" {
+
+void LLG_DefaultRunOpts(llg_run_opts* o) {
+  std::memset(o, 0, sizeof(*o));
+  o->t0 = 0.0;
+  o->t_out = 0.1;
+  o->rtol = 1e-4;
+  o->atol = 1e-4;
+  o->maxl = 5;
+  o->gstype = 0;        // auto: classical iff neq < 500000 (2d_i4/2d_fft.cu:645-650)
+  o->max_order = 5;
+  o->mxstep = 500;
+  o->fused = 1;
+  o->use_jtv = 1;
+  o->use_prec = 1;
+  o->pretype = SUN_PREC_LEFT;
+}
+
+void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_host) {
+  if (!ctx || !opts || !y0_host) return nullptr;
+  Run* r = new Run();
+  r->ctx = ctx;
+  r->opts = *opts;
+  int ncell = 0;
+  LLG_GetCounts(ctx, &ncell, nullptr, nullptr);
+  const sunindextype neq = 3 * (sunindextype)ncell;
+  void* stream = LLG_GetStream(ctx);
+  r->y = N_VNewWithStream_Hip(neq, stream, &r->sctx);
+  r->abstol = N_VNewWithStream_Hip(neq, stream, &r->sctx);
+  if (!r->y || !r->abstol) { LLG_RunDestroy(r); return nullptr; }
+  if (opts->fused) FusedNVec_Init(r->y);
+  N_VSetFromHost_Hip(r->y, y0_host);
+  N_VConst(opts->atol, r->abstol);
+  r->cv = LLGCVodeCreate(LLGCV_BDF, &r->sctx);
+  int gs = opts->gstype;
+  if (gs == 0) gs = (neq < 500000) ? SUN_CLASSICAL_GS : SUN_MODIFIED_GS;
+  if (!r->cv || LLGCVodeSetMaxOrd(r->cv, opts->max_order) != 0 ||
+      LLGCVodeInit(r->cv, LLG_Rhs, opts->t0, r->y) != 0 ||
+      LLGCVodeSetUserData(r->cv, ctx) != 0 ||
+      LLGCVodeSVtolerances(r->cv, opts->rtol, r->abstol) != 0 ||
+      LLGCVodeSetLinearSolverSPGMR(r->cv, opts->use_prec ? opts->pretype : SUN_PREC_NONE,
+                                   opts->maxl, gs) != 0 ||
+      LLGCVodeSetJacTimes(r->cv, nullptr, opts->use_jtv ? JtvProduct : nullptr) != 0 ||
+      LLGCVodeSetMaxNumSteps(r->cv, opts->mxstep) != 0) {
+    LLG_RunDestroy(r);
+    return nullptr;
+  }
+  if (opts->use_prec) LLGCVodeSetPreconditioner(r->cv, PrecondSetup, PrecondSolve);
+  r->t = opts->t0;
+  return r;
+}
+
+int LLG_RunAdvance(void* run, double t_end, llg_run_result* res) {
+  Run* r = static_cast<Run*>(run);
+  if (!r) return -1;
+  hipStream_t s = (hipStream_t)LLG_GetStream(r->ctx);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipStreamSynchronize(s);
+  const auto c0 = std::chrono::steady_clock::now();
+  hipEventRecord(e0, s);
+  const double t_start = r->t;
+  double tout = r->t + r->opts.t_out;
+  long nout = 0;
+  int retval = 0;
+  double tret = r->t;
+  while (tout <= t_end + 1e-9 * std::fabs(t_end)) {
+    retval = LLGCVode(r->cv, tout, r->y, &tret, LLGCV_NORMAL);
+    if (retval < 0) break;
+    nout++;
+    r->t = tret;
+    tout += r->opts.t_out;
+  }
+  hipEventRecord(e1, s);
+  hipEventSynchronize(e1);
+  const auto c1 = std::chrono::steady_clock::now();
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (res) {
+    std::memset(res, 0, sizeof(*res));
+    LLGCVodeGetStats(r->cv, &res->st);
+    res->st.nsyncs = NVHip_NumSyncs();
+    res->wall_s = std::chrono::duration<double>(c1 - c0).count();
+    res->gpu_ms = ms;
+    res->t_reached = r->t;
+    res->sim_per_wall = res->wall_s > 0 ? (r->t - t_start) / res->wall_s : 0.0;
+    res->retval = retval;
+    res->nout = nout;
+  }
+  return retval < 0 ? retval : 0;
+}
+
+int LLG_RunGetState(void* run, double* y_host) {
+  Run* r = static_cast<Run*>(run);
+  if (!r || !y_host) return -1;
+  return N_VGetToHost_Hip(r->y, y_host);
+}
+
+void LLG_RunDestroy(void* run) {
+  Run* r = static_cast<Run*>(run);
+  if (!r) return;
+  if (r->cv) LLGCVodeFree(&r->cv);
+  N_VDestroy(r->y);
+  N_VDestroy(r->abstol);
+  delete r;
+}
+
+}  // extern "C"

@@ -1,0 +1,768 @@
+// llg_kernels.hip -- the LLG CVODE plug-ins on gfx950: RHS, analytic Jv,
+// 3x3 block-Jacobi setup/solve, unit-sphere normalisation, and the
+// context that replaces the reference's byte-mirrored UserData structs.
+//
+// Reference behaviour restated here (paths under src/irregular/2d_fft/):
+//   RHS    2d_i4/2d_fft.cu:262-319 (ymsk), 2d_i6/2d_i6.cu:250-343 (compact),
+//          2d_i1/2d_fft.cu:296-360 (linear anisotropy + DMI),
+//          2d_i3/2d_fft.cu:451-627 (LDS-tiled windowed polycrystal);
+//          wrapper order 2d_i4/2d_fft.cu:328-366
+//   Jv     2d_i4/jtv.cu:47-147, 2d_i6/jtv.cu:47-183, 2d_i1/jtv.cu:43-119
+//   Psetup 2d_i4/precond.cu:120-252,324-358; 2d_i6/precond.cu:118-249;
+//          two-phase jok protocol src/2d/2d_heter/precond.cu:154-192,265-314
+//   Psolve 2d_i4/precond.cu:255-286,361-385; 2d_i6/precond.cu:252-285,361-396
+//   normalize 2d_i1/2d_fft.cu:253-275 (exact), 2d_i2/2d_fft.cu:262-283 (+1e-3)
+//
+// MI355X layout: SoA FP64 state [mx|my|mz]; geometry as ONE byte per cell
+// (the reference replicates an FP64 0/1 mask in three planes: 24 B/cell
+// of extra traffic); compact mode launches one thread per active cell and
+// zeroes the hole cells in the same launch.  Stencil kernels use 64x4
+// workgroups so one wavefront covers 64 consecutive cells of a row
+// (512 B coalesced per plane); the neighbour rows come through L1/L2.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "llg/llg_plugin.h"
+#include "llg/nvector_hip.h"
+#include "kprof.h"
+
+namespace {
+
+thread_local std::string g_err;
+void set_err(const std::string& e) { g_err = e; }
+
+constexpr int kBX = 64, kBY = 4;   // 2D stencil workgroup (one wave per row)
+constexpr int k1D = 256;
+
+struct TermsD {
+  double che, chk, cha, chb;
+  double msk[3], nsk[3], happ[3];
+  int aniso, cell_axes, window, gate;
+};
+
+struct GeoD {
+  int ng, ny, ncell;
+  int boundary, exch_self;
+  int compact;
+  const unsigned char* active;  // ncell
+  const double* w;              // ncell or null
+  const double* cmsk;           // 3*ncell or null
+  const double* cnsk;           // 3*ncell or null
+  const int* ids;               // active ids (compact)
+  const int* hole_ids;          // inactive ids (compact)
+  int n_active, n_hole;
+  double alpha, chg;
+};
+
+TermsD to_dev(const llg_terms& t) {
+  TermsD d;
+  d.che = t.che; d.chk = t.chk; d.cha = t.cha; d.chb = t.chb;
+  for (int a = 0; a < 3; a++) { d.msk[a] = t.msk[a]; d.nsk[a] = t.nsk[a]; d.happ[a] = t.happ[a]; }
+  d.aniso = t.aniso; d.cell_axes = t.cell_axes; d.window = t.window; d.gate = t.gate;
+  return d;
+}
+
+__device__ inline void nbrs(const GeoD& g, int gx, int gy, int& lc, int& rc, int& uc, int& dc) {
+  const int self = gy * g.ng + gx;
+  if (g.boundary == LLG_BC_NEUMANN) {
+    lc = gx > 0 ? self - 1 : self;
+    rc = gx < g.ng - 1 ? self + 1 : self;
+    uc = gy > 0 ? self - g.ng : self;
+    dc = gy < g.ny - 1 ? self + g.ng : self;
+  } else {
+    const int xl = gx == 0 ? g.ng - 1 : gx - 1;
+    const int xr = gx == g.ng - 1 ? 0 : gx + 1;
+    const int yu = gy == 0 ? g.ny - 1 : gy - 1;
+    const int yd = gy == g.ny - 1 ? 0 : gy + 1;
+    lc = gy * g.ng + xl; rc = gy * g.ng + xr; uc = yu * g.ng + gx; dc = yd * g.ng + gx;
+  }
+}
+
+__device__ inline void axes(const GeoD& g, const TermsD& T, int cell, double msk[3], double nsk[3]) {
+  const int N = g.ncell;
+  if (T.cell_axes && g.cmsk) {
+    msk[0] = g.cmsk[cell]; msk[1] = g.cmsk[N + cell]; msk[2] = g.cmsk[2 * N + cell];
+  } else {
+    msk[0] = T.msk[0]; msk[1] = T.msk[1]; msk[2] = T.msk[2];
+  }
+  if (T.cell_axes && g.cnsk) {
+    nsk[0] = g.cnsk[cell]; nsk[1] = g.cnsk[N + cell]; nsk[2] = g.cnsk[2 * N + cell];
+  } else {
+    nsk[0] = T.nsk[0]; nsk[1] = T.nsk[1]; nsk[2] = T.nsk[2];
+  }
+}
+
+// h (without demag) from terms T; optional linearised partner dh for v
+template <bool LIN>
+__device__ inline void field(const GeoD& g, const TermsD& T, const double* __restrict__ y,
+                             const double* __restrict__ v, int cell, int gx, int gy,
+                             double h[3], double dh[3]) {
+  const int N = g.ncell;
+  int lc, rc, uc, dc;
+  nbrs(g, gx, gy, lc, rc, uc, dc);
+  double msk[3], nsk[3];
+  axes(g, T, cell, msk, nsk);
+  const double m[3] = {y[cell], y[N + cell], y[2 * N + cell]};
+  const bool win = T.window && g.w;
+  const double wl = win ? g.w[lc] : 1.0, wr = win ? g.w[rc] : 1.0;
+  const double wu = win ? g.w[uc] : 1.0, wd = win ? g.w[dc] : 1.0;
+  const double mdotmsk = m[0] * msk[0] + m[1] * msk[1] + m[2] * msk[2];
+  double vv[3] = {0, 0, 0}, vdotmsk = 0.0;
+  if (LIN) {
+    vv[0] = v[cell]; vv[1] = v[N + cell]; vv[2] = v[2 * N + cell];
+    vdotmsk = vv[0] * msk[0] + vv[1] * msk[1] + vv[2] * msk[2];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const double L = wl * y[a * N + lc], R = wr * y[a * N + rc];
+    const double U = wu * y[a * N + uc], D = wd * y[a * N + dc];
+    double ex = L + R + U + D;
+    if (g.exch_self) ex = ex - 4.0 * m[a];
+    double an;
+    if (T.aniso == LLG_ANISO_CUBIC) an = msk[a] * T.chk * m[a] * (m[a] * m[a] - 1.0);
+    else an = msk[a] * (T.chk * mdotmsk + T.cha);
+    h[a] = T.happ[a] + T.che * ex + an + T.chb * nsk[a] * (L + R);
+    if (LIN) {
+      const double vL = wl * v[a * N + lc], vR = wr * v[a * N + rc];
+      const double vU = wu * v[a * N + uc], vD = wd * v[a * N + dc];
+      double vex = vL + vR + vU + vD;
+      if (g.exch_self) vex = vex - 4.0 * vv[a];
+      double dan;
+      if (T.aniso == LLG_ANISO_CUBIC) dan = (msk[a] * T.chk * (3.0 * m[a] * m[a] - 1.0)) * vv[a];
+      else dan = msk[a] * T.chk * vdotmsk;
+      dh[a] = T.che * vex + dan + T.chb * nsk[a] * (vL + vR);
+    }
+  }
+}
+
+__device__ inline double gate_of(const GeoD& g, int gate, int cell) {
+  if (gate == LLG_GATE_MASK) return g.active[cell] ? 1.0 : 0.0;
+  if (gate == LLG_GATE_WINDOW) return g.w ? g.w[cell] : 1.0;
+  return 1.0;
+}
+
+// thread -> cell mapping; returns false when the thread has no cell.
+// compact: threads [0, n_active) take active ids, [n_active, n_active+n_hole)
+// zero a hole cell of `out` (3 planes) and return false.
+__device__ inline bool map_cell(const GeoD& g, double* out, int& cell, int& gx, int& gy) {
+  if (g.compact) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < g.n_active) {
+      cell = g.ids[t];
+    } else {
+      const int hidx = t - g.n_active;
+      if (hidx < g.n_hole) {
+        const int hc = g.hole_ids[hidx];
+        out[hc] = 0.0; out[g.ncell + hc] = 0.0; out[2 * g.ncell + hc] = 0.0;
+      }
+      return false;
+    }
+    gx = cell % g.ng;
+    gy = cell / g.ng;
+    return true;
+  }
+  gx = blockIdx.x * blockDim.x + threadIdx.x;
+  gy = blockIdx.y * blockDim.y + threadIdx.y;
+  if (gx >= g.ng || gy >= g.ny) return false;
+  cell = gy * g.ng + gx;
+  return true;
+}
+
+// ------------------------------------------------------------------ RHS
+__global__ __launch_bounds__(256) void k_rhs(GeoD g, TermsD T, const double* __restrict__ y,
+                                             const double* __restrict__ hd, double* __restrict__ yd) {
+  int cell, gx, gy;
+  if (!map_cell(g, yd, cell, gx, gy)) return;
+  const int N = g.ncell;
+  double h[3], dummy[3];
+  field<false>(g, T, y, nullptr, cell, gx, gy, h, dummy);
+  if (hd) { h[0] += hd[cell]; h[1] += hd[N + cell]; h[2] += hd[2 * N + cell]; }
+  const double m1 = y[cell], m2 = y[N + cell], m3 = y[2 * N + cell];
+  const double mh = m1 * h[0] + m2 * h[1] + m3 * h[2];
+  const double f1 = g.chg * (m3 * h[1] - m2 * h[2]) + g.alpha * (h[0] - mh * m1);
+  const double f2 = g.chg * (m1 * h[2] - m3 * h[0]) + g.alpha * (h[1] - mh * m2);
+  const double f3 = g.chg * (m2 * h[0] - m1 * h[1]) + g.alpha * (h[2] - mh * m3);
+  if (g.compact) {
+    yd[cell] = f1; yd[N + cell] = f2; yd[2 * N + cell] = f3;
+  } else {
+    const double gt = gate_of(g, T.gate, cell);
+    yd[cell] = gt * f1; yd[N + cell] = gt * f2; yd[2 * N + cell] = gt * f3;
+  }
+}
+
+// LDS-tiled windowed RHS (the polycrystal form, 2d_i3/2d_fft.cu:451-627):
+// a (BX+2)x(BY+2) halo of w*m per component is staged in LDS once and the
+// four neighbours are read from it.  Periodic boundary only.
+template <int BX, int BY>
+__global__ __launch_bounds__(BX* BY) void k_rhs_tiled(GeoD g, TermsD T,
+                                                       const double* __restrict__ y,
+                                                       const double* __restrict__ hd,
+                                                       double* __restrict__ yd) {
+  constexpr int SX = BX + 2, SY = BY + 2;
+  __shared__ double tile[3][SY][SX];
+  const int N = g.ncell;
+  const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY;
+  const bool win = T.window && g.w;
+  for (int s = threadIdx.y * BX + threadIdx.x; s < SX * SY; s += BX * BY) {
+    const int sx = s % SX, sy = s / SX;
+    int gx = x0 + sx - 1, gy = y0 + sy - 1;
+    gx = gx < 0 ? gx + g.ng : (gx >= g.ng ? gx - g.ng : gx);
+    gy = gy < 0 ? gy + g.ny : (gy >= g.ny ? gy - g.ny : gy);
+    const int c = gy * g.ng + gx;
+    const double ww = win ? g.w[c] : 1.0;
+    tile[0][sy][sx] = ww * y[c];
+    tile[1][sy][sx] = ww * y[N + c];
+    tile[2][sy][sx] = ww * y[2 * N + c];
+  }
+  __syncthreads();
+  const int gx = x0 + threadIdx.x, gy = y0 + threadIdx.y;
+  if (gx >= g.ng || gy >= g.ny) return;
+  const int cell = gy * g.ng + gx;
+  const int sx = threadIdx.x + 1, sy = threadIdx.y + 1;
+  double msk[3], nsk[3];
+  axes(g, T, cell, msk, nsk);
+  const double m[3] = {y[cell], y[N + cell], y[2 * N + cell]};
+  const double mdotmsk = m[0] * msk[0] + m[1] * msk[1] + m[2] * msk[2];
+  double h[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const double L = tile[a][sy][sx - 1], R = tile[a][sy][sx + 1];
+    const double U = tile[a][sy - 1][sx], D = tile[a][sy + 1][sx];
+    double ex = L + R + U + D;
+    if (g.exch_self) ex = ex - 4.0 * m[a];
+    double an;
+    if (T.aniso == LLG_ANISO_CUBIC) an = msk[a] * T.chk * m[a] * (m[a] * m[a] - 1.0);
+    else an = msk[a] * (T.chk * mdotmsk + T.cha);
+    h[a] = T.happ[a] + T.che * ex + an + T.chb * nsk[a] * (L + R);
+  }
+  if (hd) { h[0] += hd[cell]; h[1] += hd[N + cell]; h[2] += hd[2 * N + cell]; }
+  const double m1 = m[0], m2 = m[1], m3 = m[2];
+  const double mh = m1 * h[0] + m2 * h[1] + m3 * h[2];
+  const double gt = gate_of(g, T.gate, cell);
+  yd[cell] = gt * (g.chg * (m3 * h[1] - m2 * h[2]) + g.alpha * (h[0] - mh * m1));
+  yd[N + cell] = gt * (g.chg * (m1 * h[2] - m3 * h[0]) + g.alpha * (h[1] - mh * m2));
+  yd[2 * N + cell] = gt * (g.chg * (m2 * h[0] - m1 * h[1]) + g.alpha * (h[2] - mh * m3));
+}
+
+// ------------------------------------------------------------------- Jv
+__global__ __launch_bounds__(256) void k_jtv(GeoD g, TermsD T, const double* __restrict__ y,
+                                             const double* __restrict__ v, double* __restrict__ Jv) {
+  int cell, gx, gy;
+  if (!map_cell(g, Jv, cell, gx, gy)) return;
+  const int N = g.ncell;
+  double h[3], dh[3];
+  field<true>(g, T, y, v, cell, gx, gy, h, dh);
+  const double m1 = y[cell], m2 = y[N + cell], m3 = y[2 * N + cell];
+  const double v1 = v[cell], v2 = v[N + cell], v3 = v[2 * N + cell];
+  const double mh = m1 * h[0] + m2 * h[1] + m3 * h[2];
+  const double dmh = (v1 * h[0] + v2 * h[1] + v3 * h[2]) + (m1 * dh[0] + m2 * dh[1] + m3 * dh[2]);
+  const double j1 = g.chg * (v3 * h[1] + m3 * dh[1] - v2 * h[2] - m2 * dh[2]) +
+                    g.alpha * (dh[0] - dmh * m1 - mh * v1);
+  const double j2 = g.chg * (v1 * h[2] + m1 * dh[2] - v3 * h[0] - m3 * dh[0]) +
+                    g.alpha * (dh[1] - dmh * m2 - mh * v2);
+  const double j3 = g.chg * (v2 * h[0] + m2 * dh[0] - v1 * h[1] - m1 * dh[1]) +
+                    g.alpha * (dh[2] - dmh * m3 - mh * v3);
+  if (g.compact) {
+    Jv[cell] = j1; Jv[N + cell] = j2; Jv[2 * N + cell] = j3;
+  } else {
+    const double gt = gate_of(g, T.gate, cell);
+    Jv[cell] = gt * j1; Jv[N + cell] = gt * j2; Jv[2 * N + cell] = gt * j3;
+  }
+}
+
+// ---------------------------------------------------------- block-Jacobi
+__device__ inline void invert_block(const double J[9], double gamma, double P[9]) {
+  const double A00 = 1.0 - gamma * J[0], A01 = -gamma * J[1], A02 = -gamma * J[2];
+  const double A10 = -gamma * J[3], A11 = 1.0 - gamma * J[4], A12 = -gamma * J[5];
+  const double A20 = -gamma * J[6], A21 = -gamma * J[7], A22 = 1.0 - gamma * J[8];
+  const double c00 = A11 * A22 - A12 * A21;
+  const double c10 = A12 * A20 - A10 * A22;
+  const double c20 = A10 * A21 - A11 * A20;
+  const double det = A00 * c00 + A01 * c10 + A02 * c20;
+  if (!(fabs(det) > 1e-30)) {
+#pragma unroll
+    for (int i = 0; i < 9; i++) P[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    return;
+  }
+  const double inv = 1.0 / det;
+  P[0] = (A11 * A22 - A12 * A21) * inv;
+  P[1] = (A02 * A21 - A01 * A22) * inv;
+  P[2] = (A01 * A12 - A02 * A11) * inv;
+  P[3] = (A12 * A20 - A10 * A22) * inv;
+  P[4] = (A00 * A22 - A02 * A20) * inv;
+  P[5] = (A02 * A10 - A00 * A12) * inv;
+  P[6] = (A10 * A21 - A11 * A20) * inv;
+  P[7] = (A01 * A20 - A00 * A21) * inv;
+  P[8] = (A00 * A11 - A01 * A10) * inv;
+}
+
+__global__ __launch_bounds__(256) void k_build_P(GeoD g, TermsD T, const double* __restrict__ y,
+                                                 const double* __restrict__ hd, double n0x,
+                                                 double n0y, double n0z, double gamma,
+                                                 int rebuild_J, double* __restrict__ Jst,
+                                                 double* __restrict__ P) {
+  int cell, gx, gy;
+  GeoD gg = g;
+  gg.n_hole = 0;   // hole blocks of P are never read: skip them
+  if (!map_cell(gg, nullptr, cell, gx, gy)) return;
+  const int N = g.ncell;
+  double J[9];
+  if (rebuild_J) {
+    double h[3], dummy[3];
+    field<false>(g, T, y, nullptr, cell, gx, gy, h, dummy);
+    if (hd) { h[0] += hd[cell]; h[1] += hd[N + cell]; h[2] += hd[2 * N + cell]; }
+    double msk[3], nsk[3];
+    axes(g, T, cell, msk, nsk);
+    const double m1 = y[cell], m2 = y[N + cell], m3 = y[2 * N + cell];
+    const double m[3] = {m1, m2, m3};
+    double self_ex = 0.0;
+    if (g.boundary == LLG_BC_NEUMANN || g.exch_self) {
+      int ghosts = 0;
+      if (g.boundary == LLG_BC_NEUMANN)
+        ghosts = (gx == 0) + (gx == g.ng - 1) + (gy == 0) + (gy == g.ny - 1);
+      self_ex = T.che * ((double)ghosts - (g.exch_self ? 4.0 : 0.0));
+    }
+    const double n0[3] = {n0x, n0y, n0z};
+    double k[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      double dan;
+      if (T.aniso == LLG_ANISO_CUBIC) dan = T.chk * (3.0 * m[a] * m[a] - 1.0) * msk[a];
+      else dan = msk[a] * msk[a] * T.chk;
+      k[a] = dan + n0[a] + self_ex;
+    }
+    const double e1 = h[0] + m1 * k[0], e2 = h[1] + m2 * k[1], e3 = h[2] + m3 * k[2];
+    const double mh = m1 * h[0] + m2 * h[1] + m3 * h[2];
+    const double al = g.alpha, cg = g.chg;
+    J[0] = al * (k[0] - e1 * m1 - mh);
+    J[1] = cg * (m3 * k[1] - h[2]) - al * e2 * m1;
+    J[2] = cg * (h[1] - m2 * k[2]) - al * e3 * m1;
+    J[3] = cg * (h[2] - m3 * k[0]) - al * e1 * m2;
+    J[4] = al * (k[1] - e2 * m2 - mh);
+    J[5] = cg * (m1 * k[2] - h[0]) - al * e3 * m2;
+    J[6] = cg * (m2 * k[0] - h[1]) - al * e1 * m3;
+    J[7] = cg * (h[0] - m1 * k[1]) - al * e2 * m3;
+    J[8] = al * (k[2] - e3 * m3 - mh);
+    if (Jst) {
+#pragma unroll
+      for (int s = 0; s < 9; s++) Jst[s * N + cell] = J[s];
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < 9; s++) J[s] = Jst[s * N + cell];
+  }
+  double Pi[9];
+  invert_block(J, gamma, Pi);
+#pragma unroll
+  for (int s = 0; s < 9; s++) P[s * N + cell] = Pi[s];
+}
+
+__global__ __launch_bounds__(256) void k_apply_P(GeoD g, int gate, const double* __restrict__ P,
+                                                 const double* __restrict__ r,
+                                                 double* __restrict__ z) {
+  const int N = g.ncell;
+  int cell;
+  if (g.compact) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.n_active) {
+      const int hidx = t - g.n_active;
+      if (hidx < g.n_hole) {
+        const int hc = g.hole_ids[hidx];
+        z[hc] = 0.0; z[N + hc] = 0.0; z[2 * N + hc] = 0.0;
+      }
+      return;
+    }
+    cell = g.ids[t];
+  } else {
+    cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= N) return;
+  }
+  const double r0 = r[cell], r1 = r[N + cell], r2 = r[2 * N + cell];
+  const double z0 = P[0 * N + cell] * r0 + P[1 * N + cell] * r1 + P[2 * N + cell] * r2;
+  const double z1 = P[3 * N + cell] * r0 + P[4 * N + cell] * r1 + P[5 * N + cell] * r2;
+  const double z2 = P[6 * N + cell] * r0 + P[7 * N + cell] * r1 + P[8 * N + cell] * r2;
+  if (g.compact) {
+    z[cell] = z0; z[N + cell] = z1; z[2 * N + cell] = z2;
+  } else {
+    const double gt = gate_of(g, gate, cell);
+    z[cell] = gt * z0; z[N + cell] = gt * z1; z[2 * N + cell] = gt * z2;
+  }
+}
+
+// --------------------------------------------------------------- normalise
+__global__ void k_normalize(double* y, int ncell, int mode) {
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= ncell) return;
+  const double m1 = y[cell], m2 = y[ncell + cell], m3 = y[2 * ncell + cell];
+  const double ymp = sqrt(m1 * m1 + m2 * m2 + m3 * m3);
+  if (mode == LLG_NORM_EXACT) {
+    if (ymp > 1.0e-30) {
+      const double inv = 1.0 / ymp;
+      y[cell] = m1 * inv; y[ncell + cell] = m2 * inv; y[2 * ncell + cell] = m3 * inv;
+    }
+  } else {
+    const double inv = 1.0 / (ymp + 1.0e-3);
+    y[cell] = m1 * inv; y[ncell + cell] = m2 * inv; y[2 * ncell + cell] = m3 * inv;
+  }
+}
+
+}  // namespace
+
+// ============================================================ context
+struct PrecondData {
+  int ng, ny, ncell;
+  double* d_P;   // 9*ncell SoA: P[s*ncell + cell]
+  double* d_J;   // 9*ncell, two-phase protocol only
+};
+
+struct llg_ctx {
+  llg_config cfg;
+  int ncell, neq;
+  int n_active, n_hole;
+  hipStream_t stream;
+  unsigned char* d_active;
+  double* d_w;
+  double* d_cmsk;
+  double* d_cnsk;
+  int* d_ids;
+  int* d_hole_ids;
+  double* d_hd;        // field of the most recent f() (read by PrecondSetup)
+  bool hd_valid;
+  PrecondData* pd;
+  DemagData* demag;
+  double n0[3];
+  std::vector<unsigned char> h_active;
+  std::vector<int> h_ids, h_hole_ids;
+  long nf, njv, npset, npsol;
+};
+
+namespace {
+
+GeoD geo_of(const llg_ctx* c) {
+  GeoD g;
+  g.ng = c->cfg.ng; g.ny = c->cfg.ny; g.ncell = c->ncell;
+  g.boundary = c->cfg.boundary; g.exch_self = c->cfg.exch_self;
+  g.compact = c->cfg.exec == LLG_EXEC_COMPACT;
+  g.active = c->d_active; g.w = c->d_w; g.cmsk = c->d_cmsk; g.cnsk = c->d_cnsk;
+  g.ids = c->d_ids; g.hole_ids = c->d_hole_ids;
+  g.n_active = c->n_active; g.n_hole = c->n_hole;
+  g.alpha = c->cfg.alpha; g.chg = c->cfg.chg;
+  return g;
+}
+
+void grid_for(const llg_ctx* c, dim3& grid, dim3& block) {
+  if (c->cfg.exec == LLG_EXEC_COMPACT) {
+    const int n = c->n_active + c->n_hole;
+    block = dim3(k1D);
+    grid = dim3((n + k1D - 1) / k1D);
+  } else {
+    block = dim3(kBX, kBY);
+    grid = dim3((c->cfg.ng + kBX - 1) / kBX, (c->cfg.ny + kBY - 1) / kBY);
+  }
+}
+
+hipStream_t stream_of(N_Vector v, const llg_ctx* c) {
+  if (v && v->ops && v->ops->nvgetvectorid && v->ops->nvgetvectorid(v) == SUNDIALS_NVEC_HIP)
+    return (hipStream_t)N_VGetStream_Hip(v);
+  return c->stream;
+}
+
+int launch_status(const char* what) {
+  hipError_t e = hipPeekAtLastError();
+  if (e != hipSuccess) {
+    fprintf(stderr, ">>> ERROR in %s: kernel launch failed: %s\n", what, hipGetErrorString(e));
+    set_err(std::string(what) + ": " + hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+template <class T>
+bool upload(T** dst, const T* src, size_t n, hipStream_t s) {
+  *dst = nullptr;
+  if (n == 0) return true;
+  if (hipMalloc(dst, sizeof(T) * n) != hipSuccess) return false;
+  if (src) {
+    if (hipMemcpyAsync(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice, s) != hipSuccess) return false;
+  } else {
+    if (hipMemsetAsync(*dst, 0, sizeof(T) * n, s) != hipSuccess) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+namespace kprof {
+State& state() {
+  static State st;
+  return st;
+}
+}  // namespace kprof
+
+extern "C" {
+
+const char* LLG_LastError(void) { return g_err.c_str(); }
+
+int LLG_ProfileStart(int kernel_id, int capacity) {
+  kprof::State& st = kprof::state();
+  for (hipEvent_t e : st.ev) hipEventDestroy(e);
+  st.ev.clear();
+  st.used = 0;
+  st.dropped = 0;
+  st.target = kprof::NONE;
+  if (kernel_id == kprof::NONE) return 0;
+  if (capacity < 1) capacity = 1;
+  st.ev.resize(2 * (size_t)capacity);
+  for (auto& e : st.ev)
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+  st.target = kernel_id;
+  return 0;
+}
+
+int LLG_ProfileStop(double* total_ms, long* launches, long* dropped) {
+  kprof::State& st = kprof::state();
+  st.target = kprof::NONE;
+  double tot = 0.0;
+  if (st.used) hipEventSynchronize(st.ev[st.used - 1]);
+  for (size_t i = 0; i + 1 < st.used; i += 2) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, st.ev[i], st.ev[i + 1]);
+    tot += ms;
+  }
+  if (total_ms) *total_ms = tot;
+  if (launches) *launches = (long)(st.used / 2);
+  if (dropped) *dropped = st.dropped;
+  return 0;
+}
+
+PrecondData* Precond_Create(int ng, int ny, int ncell) {
+  if (ng <= 0 || ny <= 0 || ncell != ng * ny) { set_err("Precond_Create: bad shape"); return nullptr; }
+  PrecondData* pd = (PrecondData*)calloc(1, sizeof(PrecondData));
+  if (!pd) return nullptr;
+  pd->ng = ng; pd->ny = ny; pd->ncell = ncell;
+  const size_t bytes = sizeof(double) * 9 * (size_t)ncell;
+  if (hipMalloc(&pd->d_P, bytes) != hipSuccess) { free(pd); set_err("Precond_Create: hipMalloc"); return nullptr; }
+  hipMemset(pd->d_P, 0, bytes);
+  return pd;
+}
+
+void Precond_Destroy(PrecondData* pd) {
+  if (!pd) return;
+  if (pd->d_P) hipFree(pd->d_P);
+  if (pd->d_J) hipFree(pd->d_J);
+  free(pd);
+}
+
+llg_ctx* LLG_Create(const llg_config* cfg, const unsigned char* active, const double* w,
+                    const double* cell_msk, const double* cell_nsk, void* stream) {
+  if (!cfg || cfg->ng <= 0 || cfg->ny <= 0) { set_err("LLG_Create: bad grid"); return nullptr; }
+  if (cfg->exec != LLG_EXEC_YMSK && cfg->exec != LLG_EXEC_COMPACT) { set_err("LLG_Create: bad exec"); return nullptr; }
+  llg_ctx* c = new llg_ctx();
+  c->cfg = *cfg;
+  c->ncell = cfg->ng * cfg->ny;
+  c->neq = 3 * c->ncell;
+  c->stream = stream ? (hipStream_t)stream : (hipStream_t)NVHip_DefaultStream();
+  const int N = c->ncell;
+  c->h_active.resize(N);
+  for (int i = 0; i < N; i++) c->h_active[i] = active ? (active[i] != 0) : 1;
+  // ascending id lists (2d_i6/2d_i6.cu:514-518)
+  for (int i = 0; i < N; i++) (c->h_active[i] ? c->h_ids : c->h_hole_ids).push_back(i);
+  c->n_active = (int)c->h_ids.size();
+  c->n_hole = (int)c->h_hole_ids.size();
+  bool ok = upload(&c->d_active, c->h_active.data(), N, c->stream) &&
+            upload(&c->d_w, w, w ? (size_t)N : 0, c->stream) &&
+            upload(&c->d_cmsk, cell_msk, cell_msk ? (size_t)3 * N : 0, c->stream) &&
+            upload(&c->d_cnsk, cell_nsk, cell_nsk ? (size_t)3 * N : 0, c->stream) &&
+            upload(&c->d_ids, c->h_ids.data(), c->h_ids.size(), c->stream) &&
+            upload(&c->d_hole_ids, c->h_hole_ids.data(), c->h_hole_ids.size(), c->stream) &&
+            upload<double>(&c->d_hd, nullptr, (size_t)3 * N, c->stream);
+  if (!ok) { set_err("LLG_Create: device allocation failed"); LLG_Destroy(c); return nullptr; }
+  hipStreamSynchronize(c->stream);
+  c->pd = Precond_Create(cfg->ng, cfg->ny, N);
+  if (!c->pd) { LLG_Destroy(c); return nullptr; }
+  if (cfg->pc_mode == LLG_PC_JOK) {
+    if (hipMalloc(&c->pd->d_J, sizeof(double) * 9 * (size_t)N) != hipSuccess) {
+      set_err("LLG_Create: J storage"); LLG_Destroy(c); return nullptr;
+    }
+    hipMemset(c->pd->d_J, 0, sizeof(double) * 9 * (size_t)N);
+  }
+  if (cfg->demag != LLG_DEMAG_OFF && cfg->demag_strength > 0.0) {
+    c->demag = (cfg->demag == LLG_DEMAG_PADDED)
+                   ? Demag_InitPadded(cfg->ng, cfg->ny, cfg->demag_thick, cfg->demag_strength)
+                   : Demag_Init(cfg->ng, cfg->ny, cfg->demag_thick, cfg->demag_strength);
+    if (!c->demag) { set_err(std::string("LLG_Create: demag: ") + Demag_LastError()); LLG_Destroy(c); return nullptr; }
+    Demag_SetStream(c->demag, c->stream);
+    if (cfg->demag_fp32) Demag_SetPrecision(c->demag, 1);
+    Demag_GetSelfCoupling(c->demag, &c->n0[0], &c->n0[1], &c->n0[2]);
+  }
+  hipDeviceSynchronize();
+  return c;
+}
+
+void LLG_Destroy(llg_ctx* c) {
+  if (!c) return;
+  hipFree(c->d_active); hipFree(c->d_w); hipFree(c->d_cmsk); hipFree(c->d_cnsk);
+  hipFree(c->d_ids); hipFree(c->d_hole_ids); hipFree(c->d_hd);
+  Precond_Destroy(c->pd);
+  Demag_Destroy(c->demag);
+  delete c;
+}
+
+int LLG_GetCounts(llg_ctx* c, int* ncell, int* n_active, int* n_inactive) {
+  if (!c) return -1;
+  if (ncell) *ncell = c->ncell;
+  if (n_active) *n_active = c->n_active;
+  if (n_inactive) *n_inactive = c->n_hole;
+  return 0;
+}
+
+int LLG_GetMask(llg_ctx* c, double* h) {
+  if (!c || !h) return -1;
+  const int N = c->ncell;
+  for (int i = 0; i < N; i++) {
+    const double m = c->h_active[i] ? 1.0 : 0.0;
+    h[i] = h[N + i] = h[2 * N + i] = m;
+  }
+  return 0;
+}
+
+int LLG_GetActiveIds(llg_ctx* c, int* ids, int* hole_ids) {
+  if (!c) return -1;
+  if (ids && c->n_active)
+    hipMemcpy(ids, c->d_ids, sizeof(int) * c->n_active, hipMemcpyDeviceToHost);
+  if (hole_ids && c->n_hole)
+    hipMemcpy(hole_ids, c->d_hole_ids, sizeof(int) * c->n_hole, hipMemcpyDeviceToHost);
+  return 0;
+}
+
+int LLG_GetHdemag(llg_ctx* c, double* h) {
+  if (!c || !h) return -1;
+  hipStreamSynchronize(c->stream);
+  return hipMemcpy(h, c->d_hd, sizeof(double) * 3 * (size_t)c->ncell, hipMemcpyDeviceToHost) ==
+                 hipSuccess ? 0 : -1;
+}
+
+int LLG_GetPinv(llg_ctx* c, double* h) {
+  if (!c || !h) return -1;
+  hipDeviceSynchronize();
+  return hipMemcpy(h, c->pd->d_P, sizeof(double) * 9 * (size_t)c->ncell, hipMemcpyDeviceToHost) ==
+                 hipSuccess ? 0 : -1;
+}
+
+DemagData* LLG_GetDemag(llg_ctx* c) { return c ? c->demag : nullptr; }
+void* LLG_GetStream(llg_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int LLG_GetCallCounts(llg_ctx* c, long* nf, long* njv, long* nps, long* npv) {
+  if (!c) return -1;
+  if (nf) *nf = c->nf;
+  if (njv) *njv = c->njv;
+  if (nps) *nps = c->npset;
+  if (npv) *npv = c->npsol;
+  return 0;
+}
+
+// f(): [normalize] -> demag (overwrite hd) -> RHS (compact zeroes holes in-launch)
+int LLG_Rhs(sunrealtype t, N_Vector y, N_Vector ydot, void* user_data) {
+  (void)t;
+  llg_ctx* c = (llg_ctx*)user_data;
+  if (!c) return -1;
+  c->nf++;
+  double* yd = N_VGetDeviceArrayPointer(y);
+  double* fd = N_VGetDeviceArrayPointer(ydot);
+  hipStream_t s = stream_of(ydot, c);
+  if (c->cfg.normalize != LLG_NORM_NONE) {
+    hipLaunchKernelGGL(k_normalize, dim3((c->ncell + k1D - 1) / k1D), dim3(k1D), 0, s, yd,
+                       c->ncell, c->cfg.normalize);
+  }
+  const double* hd = nullptr;
+  if (c->demag) {
+    Demag_SetStream(c->demag, s);
+    if (c->cfg.rhs.window && c->d_w) Demag_ApplyWindowed(c->demag, yd, c->d_w, c->d_hd);
+    else Demag_Apply(c->demag, yd, c->d_hd);
+    if (Demag_LastError()[0]) { set_err(Demag_LastError()); return -1; }
+    hd = c->d_hd;
+  }
+  const GeoD g = geo_of(c);
+  const TermsD T = to_dev(c->cfg.rhs);
+  if (c->cfg.exec == LLG_EXEC_YMSK && c->cfg.rhs.window && c->cfg.boundary == LLG_BC_PERIODIC) {
+    dim3 block(32, 8), grid((c->cfg.ng + 31) / 32, (c->cfg.ny + 7) / 8);
+    kprof::begin(kprof::RHS, s);
+    hipLaunchKernelGGL((k_rhs_tiled<32, 8>), grid, block, 0, s, g, T, yd, hd, fd);
+    kprof::end(kprof::RHS, s);
+  } else {
+    dim3 grid, block;
+    grid_for(c, grid, block);
+    kprof::begin(kprof::RHS, s);
+    hipLaunchKernelGGL(k_rhs, grid, block, 0, s, g, T, yd, hd, fd);
+    kprof::end(kprof::RHS, s);
+  }
+  return launch_status("f");
+}
+
+int JtvProduct(N_Vector v, N_Vector Jv, sunrealtype t, N_Vector y, N_Vector fy, void* user_data,
+               N_Vector tmp) {
+  (void)t; (void)fy; (void)tmp;
+  llg_ctx* c = (llg_ctx*)user_data;
+  if (!c) return -1;
+  c->njv++;
+  hipStream_t s = stream_of(Jv, c);
+  dim3 grid, block;
+  grid_for(c, grid, block);
+  kprof::begin(kprof::JTV, s);
+  hipLaunchKernelGGL(k_jtv, grid, block, 0, s, geo_of(c), to_dev(c->cfg.jv),
+                     (const double*)N_VGetDeviceArrayPointer(y),
+                     (const double*)N_VGetDeviceArrayPointer(v), N_VGetDeviceArrayPointer(Jv));
+  kprof::end(kprof::JTV, s);
+  return launch_status("jtv");
+}
+
+int PrecondSetup(sunrealtype t, N_Vector y, N_Vector fy, sunbooleantype jok,
+                 sunbooleantype* jcurPtr, sunrealtype gamma, void* user_data) {
+  (void)t; (void)fy;
+  llg_ctx* c = (llg_ctx*)user_data;
+  if (!c) return -1;
+  c->npset++;
+  const bool two_phase = c->cfg.pc_mode == LLG_PC_JOK;
+  const int rebuild = (!two_phase || !jok) ? 1 : 0;
+  *jcurPtr = rebuild ? SUNTRUE : SUNFALSE;
+  hipStream_t s = stream_of(y, c);
+  dim3 grid, block;
+  GeoD g = geo_of(c);
+  if (c->cfg.exec == LLG_EXEC_COMPACT) {
+    block = dim3(k1D);
+    grid = dim3((c->n_active + k1D - 1) / k1D > 0 ? (c->n_active + k1D - 1) / k1D : 1);
+  } else {
+    grid_for(c, grid, block);
+  }
+  kprof::begin(kprof::BUILD_P, s);
+  hipLaunchKernelGGL(k_build_P, grid, block, 0, s, g, to_dev(c->cfg.pc),
+                     (const double*)N_VGetDeviceArrayPointer(y),
+                     (const double*)(c->demag ? c->d_hd : nullptr), c->n0[0], c->n0[1],
+                     c->n0[2], gamma, rebuild, c->pd->d_J, c->pd->d_P);
+  kprof::end(kprof::BUILD_P, s);
+  return launch_status("PrecondSetup");
+}
+
+int PrecondSolve(sunrealtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z,
+                 sunrealtype gamma, sunrealtype delta, int lr, void* user_data) {
+  (void)t; (void)y; (void)fy; (void)gamma; (void)delta; (void)lr;
+  llg_ctx* c = (llg_ctx*)user_data;
+  if (!c) return -1;
+  c->npsol++;
+  hipStream_t s = stream_of(z, c);
+  const int n = (c->cfg.exec == LLG_EXEC_COMPACT) ? c->n_active + c->n_hole : c->ncell;
+  kprof::begin(kprof::APPLY_P, s);
+  hipLaunchKernelGGL(k_apply_P, dim3((n + k1D - 1) / k1D), dim3(k1D), 0, s, geo_of(c),
+                     c->cfg.pc.gate, (const double*)c->pd->d_P,
+                     (const double*)N_VGetDeviceArrayPointer(r), N_VGetDeviceArrayPointer(z));
+  kprof::end(kprof::APPLY_P, s);
+  return launch_status("PrecondSolve");
+}
+
+}  // extern "C"

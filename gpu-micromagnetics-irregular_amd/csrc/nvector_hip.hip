@@ -1,0 +1,813 @@
+// nvector_hip.hip -- HIP N_Vector for gfx950 with fused ops and
+// deterministic single-launch reductions.
+//
+// Replaces NVECTOR_CUDA and the reference's fused layer
+// (src/irregular/2d_fft/2d_i4/deferred_nvector.cu:95-235,
+//  src/2d/2d_heter/deferred_nvector.cu:135-151,209-236).
+//
+// Design (MI355X): every vector carries the stream its ops launch on.
+// Element-wise ops are grid-stride kernels over 64-wide wavefronts.
+// A reduction is ONE launch: each workgroup reduces its slice (wave
+// shuffles, then LDS across the 4 waves), writes a partial in a fixed
+// slot, and the last workgroup to finish (atomic ticket) sums the
+// partials in slot order and stores the K results straight into pinned
+// host memory.  The host then waits on the stream once.  The result does
+// not depend on workgroup scheduling, unlike the reference's atomicAdd
+// accumulation (deferred_nvector.cu:127-129).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+
+#include "llg/nvector_hip.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxRedBlocks = 1024;
+constexpr int kMaxK = NVHIP_FUSED_MAX;
+
+struct NvContent {
+  sunindextype n = 0;
+  double* d = nullptr;
+  double* h = nullptr;
+  bool own_d = false;
+  hipStream_t stream = nullptr;
+};
+
+inline NvContent* C(N_Vector v) { return static_cast<NvContent*>(v->content); }
+inline double* D(N_Vector v) { return C(v)->d; }
+inline sunindextype LEN(N_Vector v) { return C(v)->n; }
+inline hipStream_t S(N_Vector v) { return C(v)->stream; }
+
+// ----------------------------------------------------- reduction scratch
+struct RedWs {
+  double* partials = nullptr;   // kMaxRedBlocks * kMaxK
+  unsigned* ticket = nullptr;   // zero between launches
+  double* host = nullptr;       // pinned, kMaxK
+  double* host_dev = nullptr;   // device alias of host
+};
+
+std::mutex g_mu;
+std::unordered_map<hipStream_t, RedWs*> g_ws;
+hipStream_t g_default_stream = nullptr;
+bool g_default_stream_made = false;
+long g_syncs = 0;
+
+RedWs* workspace(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_ws.find(s);
+  if (it != g_ws.end()) return it->second;
+  RedWs* w = new RedWs();
+  if (hipMalloc(&w->partials, sizeof(double) * kMaxRedBlocks * kMaxK) != hipSuccess ||
+      hipMalloc(&w->ticket, sizeof(unsigned)) != hipSuccess ||
+      hipHostMalloc(&w->host, sizeof(double) * kMaxK, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&w->host_dev, w->host, 0) != hipSuccess) {
+    fprintf(stderr, "[nvector_hip] reduction workspace allocation failed\n");
+    abort();
+  }
+  hipMemset(w->ticket, 0, sizeof(unsigned));
+  hipDeviceSynchronize();
+  g_ws[s] = w;
+  return w;
+}
+
+inline int ew_grid(sunindextype n) {
+  sunindextype g = (n + kBlock - 1) / kBlock;
+  if (g > 65536) g = 65536;
+  return (int)(g < 1 ? 1 : g);
+}
+inline int red_grid(sunindextype n) {
+  sunindextype g = (n + 4 * kBlock - 1) / (4 * kBlock);
+  if (g > kMaxRedBlocks) g = kMaxRedBlocks;
+  return (int)(g < 1 ? 1 : g);
+}
+
+void check_launch(const char* what) {
+  hipError_t e = hipPeekAtLastError();
+  if (e != hipSuccess) {
+    fprintf(stderr, "[nvector_hip] %s launch failed: %s\n", what, hipGetErrorString(e));
+  }
+}
+
+// ------------------------------------------------------- element-wise ops
+#define GRID_STRIDE(i, n) \
+  for (sunindextype i = (sunindextype)blockIdx.x * blockDim.x + threadIdx.x; i < (n); \
+       i += (sunindextype)gridDim.x * blockDim.x)
+
+__global__ void k_linearsum(double a, const double* __restrict__ x, double b,
+                            const double* __restrict__ y, double* z, sunindextype n) {
+  GRID_STRIDE(i, n) z[i] = a * x[i] + b * y[i];
+}
+__global__ void k_const(double c, double* z, sunindextype n) { GRID_STRIDE(i, n) z[i] = c; }
+__global__ void k_prod(const double* x, const double* y, double* z, sunindextype n) {
+  GRID_STRIDE(i, n) z[i] = x[i] * y[i];
+}
+__global__ void k_div(const double* x, const double* y, double* z, sunindextype n) {
+  GRID_STRIDE(i, n) z[i] = x[i] / y[i];
+}
+__global__ void k_scale(double c, const double* x, double* z, sunindextype n) {
+  GRID_STRIDE(i, n) z[i] = c * x[i];
+}
+__global__ void k_abs(const double* x, double* z, sunindextype n) {
+  GRID_STRIDE(i, n) z[i] = fabs(x[i]);
+}
+__global__ void k_inv(const double* x, double* z, sunindextype n) {
+  GRID_STRIDE(i, n) z[i] = 1.0 / x[i];
+}
+__global__ void k_addconst(const double* x, double b, double* z, sunindextype n) {
+  GRID_STRIDE(i, n) z[i] = x[i] + b;
+}
+__global__ void k_compare(double c, const double* x, double* z, sunindextype n) {
+  GRID_STRIDE(i, n) z[i] = (fabs(x[i]) >= c) ? 1.0 : 0.0;
+}
+
+struct PtrPack { const double* p[kMaxK]; };
+struct OutPack { double* p[kMaxK]; };
+struct CoefPack { double c[kMaxK]; };
+
+// z = sum_j c_j X_j   (z may alias any X_j: each element read before write)
+__global__ void k_lincomb(int nv, CoefPack c, PtrPack X, double* z, sunindextype n) {
+  GRID_STRIDE(i, n) {
+    double s = c.c[0] * X.p[0][i];
+    for (int j = 1; j < nv; j++) s += c.c[j] * X.p[j][i];
+    z[i] = s;
+  }
+}
+// Z_j = a_j x + Y_j
+__global__ void k_scaleaddmulti(int nv, CoefPack a, const double* x, PtrPack Y, OutPack Z,
+                                sunindextype n) {
+  GRID_STRIDE(i, n) {
+    const double xi = x[i];
+    for (int j = 0; j < nv; j++) Z.p[j][i] = a.c[j] * xi + Y.p[j][i];
+  }
+}
+// Z_j = a X_j + b Y_j
+__global__ void k_lsumarray(int nv, double a, PtrPack X, double b, PtrPack Y, OutPack Z,
+                            sunindextype n) {
+  GRID_STRIDE(i, n) {
+    for (int j = 0; j < nv; j++) Z.p[j][i] = a * X.p[j][i] + b * Y.p[j][i];
+  }
+}
+// Z_j = c_j X_j
+__global__ void k_scalearray(int nv, CoefPack c, PtrPack X, OutPack Z, sunindextype n) {
+  GRID_STRIDE(i, n) {
+    for (int j = 0; j < nv; j++) Z.p[j][i] = c.c[j] * X.p[j][i];
+  }
+}
+__global__ void k_constarray(int nv, double c, OutPack Z, sunindextype n) {
+  GRID_STRIDE(i, n) {
+    for (int j = 0; j < nv; j++) Z.p[j][i] = c;
+  }
+}
+
+// ------------------------------------------------------------ reductions
+enum RedKind { RED_SUM = 0, RED_MAX = 1, RED_MIN = 2 };
+
+__device__ inline double combine(int kind, double a, double b) {
+  return kind == RED_SUM ? a + b : (kind == RED_MAX ? fmax(a, b) : fmin(a, b));
+}
+__device__ inline double identity(int kind) {
+  return kind == RED_SUM ? 0.0 : (kind == RED_MAX ? -INFINITY : INFINITY);
+}
+
+// element contribution functors
+struct FDot { const double *x, *y; __device__ double operator()(sunindextype i) const { return x[i] * y[i]; } };
+struct FWsq { const double *x, *w; __device__ double operator()(sunindextype i) const { const double p = x[i] * w[i]; return p * p; } };
+struct FWsqMask {
+  const double *x, *w, *id;
+  __device__ double operator()(sunindextype i) const {
+    const double p = x[i] * w[i];
+    return id[i] > 0.0 ? p * p : 0.0;
+  }
+};
+struct FAbs { const double* x; __device__ double operator()(sunindextype i) const { return fabs(x[i]); } };
+struct FVal { const double* x; __device__ double operator()(sunindextype i) const { return x[i]; } };
+struct FQuot {
+  const double *a, *b;
+  __device__ double operator()(sunindextype i) const { return b[i] != 0.0 ? a[i] / b[i] : INFINITY; }
+};
+// invtest: z = 1/x where x != 0; contribution 1 when x == 0
+struct FInvTest {
+  const double* x; double* z;
+  __device__ double operator()(sunindextype i) const {
+    const double xi = x[i];
+    if (xi == 0.0) return 1.0;
+    z[i] = 1.0 / xi;
+    return 0.0;
+  }
+};
+struct FConstr {
+  const double *c, *x; double* m;
+  __device__ double operator()(sunindextype i) const {
+    const double ci = c[i], xi = x[i];
+    double bad = 0.0;
+    if (ci != 0.0) {
+      const bool b = (fabs(ci) > 1.5 && xi * ci <= 0.0) || (fabs(ci) > 0.5 && xi * ci < 0.0);
+      bad = b ? 1.0 : 0.0;
+    }
+    m[i] = bad;
+    return bad;
+  }
+};
+
+// Block-wide reduction of K values per thread; result valid in thread 0.
+template <int K>
+__device__ void block_reduce(int kind, double (&v)[K], double (*sm)[kWaves]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    double a = v[k];
+    for (int off = 32; off > 0; off >>= 1) a = combine(kind, a, __shfl_xor(a, off, 64));
+    if (lane == 0) sm[k][wave] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      double a = sm[k][0];
+      for (int w = 1; w < kWaves; w++) a = combine(kind, a, sm[k][w]);
+      v[k] = a;
+    }
+  }
+}
+
+// Final pass run by the last workgroup: partials[b*K + k] for b < nblk.
+template <int K>
+__device__ void finish(int kind, int nblk, const double* partials, double* out,
+                       double (*sm)[kWaves]) {
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = identity(kind);
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      // agent-scope atomic load: never served from a stale per-CU L1 line
+      const double p = __hip_atomic_load(partials + b * K + k, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+      v[k] = combine(kind, v[k], p);
+    }
+  }
+  __syncthreads();
+  block_reduce<K>(kind, v, sm);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) out[k] = v[k];
+  }
+}
+
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_reduce1(int kind, F f, sunindextype n,
+                                                    double* partials, unsigned* ticket,
+                                                    double* out) {
+  __shared__ double sm[1][kWaves];
+  __shared__ bool last;
+  double v[1] = {identity(kind)};
+  GRID_STRIDE(i, n) v[0] = combine(kind, v[0], f(i));
+  block_reduce<1>(kind, v, sm);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = v[0];
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last) {
+    __threadfence();
+    finish<1>(kind, gridDim.x, partials, out, sm);
+    if (threadIdx.x == 0) *ticket = 0u;
+  }
+}
+
+// K dot products x . Y_k (and K weighted squares when W given)
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_dotmulti(const double* __restrict__ x, PtrPack Y,
+                                                     sunindextype n, double* partials,
+                                                     unsigned* ticket, double* out) {
+  __shared__ double sm[K][kWaves];
+  __shared__ bool last;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = 0.0;
+  GRID_STRIDE(i, n) {
+    const double xi = x[i];
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] += xi * Y.p[k][i];
+  }
+  block_reduce<K>(RED_SUM, v, sm);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) partials[blockIdx.x * K + k] = v[k];
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last) {
+    __threadfence();
+    finish<K>(RED_SUM, gridDim.x, partials, out, sm);
+    if (threadIdx.x == 0) *ticket = 0u;
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_wsqarray(PtrPack X, PtrPack W, const double* id,
+                                                     sunindextype n, double* partials,
+                                                     unsigned* ticket, double* out) {
+  __shared__ double sm[K][kWaves];
+  __shared__ bool last;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = 0.0;
+  GRID_STRIDE(i, n) {
+    const bool on = id ? (id[i] > 0.0) : true;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const double p = X.p[k][i] * W.p[k][i];
+      v[k] += on ? p * p : 0.0;
+    }
+  }
+  block_reduce<K>(RED_SUM, v, sm);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) partials[blockIdx.x * K + k] = v[k];
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last) {
+    __threadfence();
+    finish<K>(RED_SUM, gridDim.x, partials, out, sm);
+    if (threadIdx.x == 0) *ticket = 0u;
+  }
+}
+
+template <class F>
+double reduce1(N_Vector v, int kind, F f) {
+  const sunindextype n = LEN(v);
+  hipStream_t s = S(v);
+  RedWs* w = workspace(s);
+  const int g = red_grid(n);
+  hipLaunchKernelGGL(k_reduce1<F>, dim3(g), dim3(kBlock), 0, s, kind, f, n, w->partials,
+                     w->ticket, w->host_dev);
+  check_launch("reduce");
+  hipStreamSynchronize(s);
+  g_syncs++;
+  return w->host[0];
+}
+
+template <int K>
+void launch_dotmulti(hipStream_t s, const double* x, const PtrPack& Y, sunindextype n,
+                     RedWs* w) {
+  hipLaunchKernelGGL(k_dotmulti<K>, dim3(red_grid(n)), dim3(kBlock), 0, s, x, Y, n,
+                     w->partials, w->ticket, w->host_dev);
+}
+
+void dotmulti_dispatch(int K, hipStream_t s, const double* x, const PtrPack& Y, sunindextype n,
+                       RedWs* w) {
+  switch (K) {
+    case 1: launch_dotmulti<1>(s, x, Y, n, w); break;
+    case 2: launch_dotmulti<2>(s, x, Y, n, w); break;
+    case 3: launch_dotmulti<3>(s, x, Y, n, w); break;
+    case 4: launch_dotmulti<4>(s, x, Y, n, w); break;
+    case 5: launch_dotmulti<5>(s, x, Y, n, w); break;
+    case 6: launch_dotmulti<6>(s, x, Y, n, w); break;
+    case 7: launch_dotmulti<7>(s, x, Y, n, w); break;
+    case 8: launch_dotmulti<8>(s, x, Y, n, w); break;
+    case 9: launch_dotmulti<9>(s, x, Y, n, w); break;
+    case 10: launch_dotmulti<10>(s, x, Y, n, w); break;
+    case 11: launch_dotmulti<11>(s, x, Y, n, w); break;
+    case 12: launch_dotmulti<12>(s, x, Y, n, w); break;
+    case 13: launch_dotmulti<13>(s, x, Y, n, w); break;
+    case 14: launch_dotmulti<14>(s, x, Y, n, w); break;
+    case 15: launch_dotmulti<15>(s, x, Y, n, w); break;
+    default: launch_dotmulti<16>(s, x, Y, n, w); break;
+  }
+}
+
+template <int K>
+void launch_wsq(hipStream_t s, const PtrPack& X, const PtrPack& W, const double* id,
+                sunindextype n, RedWs* w) {
+  hipLaunchKernelGGL(k_wsqarray<K>, dim3(red_grid(n)), dim3(kBlock), 0, s, X, W, id, n,
+                     w->partials, w->ticket, w->host_dev);
+}
+void wsq_dispatch(int K, hipStream_t s, const PtrPack& X, const PtrPack& W, const double* id,
+                  sunindextype n, RedWs* w) {
+  switch (K) {
+    case 1: launch_wsq<1>(s, X, W, id, n, w); break;
+    case 2: launch_wsq<2>(s, X, W, id, n, w); break;
+    case 3: launch_wsq<3>(s, X, W, id, n, w); break;
+    case 4: launch_wsq<4>(s, X, W, id, n, w); break;
+    case 5: launch_wsq<5>(s, X, W, id, n, w); break;
+    case 6: launch_wsq<6>(s, X, W, id, n, w); break;
+    case 7: launch_wsq<7>(s, X, W, id, n, w); break;
+    case 8: launch_wsq<8>(s, X, W, id, n, w); break;
+    default: launch_wsq<16>(s, X, W, id, n, w); break;
+  }
+}
+
+void install_ops(N_Vector v, bool fused);
+
+N_Vector make_vector(sunindextype n, hipStream_t s, bool alloc, SUNContext sunctx) {
+  N_Vector v = (N_Vector)calloc(1, sizeof(*v));
+  if (!v) return nullptr;
+  v->ops = (N_Vector_Ops)calloc(1, sizeof(struct _generic_N_Vector_Ops));
+  NvContent* c = new NvContent();
+  c->n = n;
+  c->stream = s;
+  if (alloc && n > 0) {
+    if (hipMalloc(&c->d, sizeof(double) * (size_t)n) != hipSuccess) {
+      delete c;
+      free(v->ops);
+      free(v);
+      return nullptr;
+    }
+    c->own_d = true;
+  }
+  v->content = c;
+  v->sunctx = sunctx;
+  install_ops(v, false);
+  return v;
+}
+
+}  // namespace
+
+// =================================================================== C API
+extern "C" {
+
+void* NVHip_DefaultStream(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_default_stream_made) {
+    if (hipStreamCreateWithFlags(&g_default_stream, hipStreamNonBlocking) != hipSuccess)
+      g_default_stream = nullptr;
+    g_default_stream_made = true;
+  }
+  return (void*)g_default_stream;
+}
+
+long NVHip_NumSyncs(void) { return g_syncs; }
+
+N_Vector N_VNewWithStream_Hip(sunindextype length, void* stream, SUNContext sunctx) {
+  hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)NVHip_DefaultStream();
+  return make_vector(length, s, true, sunctx);
+}
+N_Vector N_VNew_Hip(sunindextype length, SUNContext sunctx) {
+  return N_VNewWithStream_Hip(length, nullptr, sunctx);
+}
+N_Vector N_VNewEmpty_Hip(SUNContext sunctx) {
+  return make_vector(0, (hipStream_t)NVHip_DefaultStream(), false, sunctx);
+}
+N_Vector N_VMake_Hip(sunindextype length, sunrealtype* d_data, void* stream, SUNContext sunctx) {
+  hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)NVHip_DefaultStream();
+  N_Vector v = make_vector(length, s, false, sunctx);
+  if (v) C(v)->d = d_data;
+  return v;
+}
+
+N_Vector_ID N_VGetVectorID_Hip(N_Vector v) { (void)v; return SUNDIALS_NVEC_HIP; }
+
+N_Vector N_VCloneEmpty_Hip(N_Vector w) {
+  N_Vector v = make_vector(LEN(w), S(w), false, w->sunctx);
+  if (!v) return nullptr;
+  std::memcpy(v->ops, w->ops, sizeof(struct _generic_N_Vector_Ops));
+  return v;
+}
+N_Vector N_VClone_Hip(N_Vector w) {
+  N_Vector v = make_vector(LEN(w), S(w), true, w->sunctx);
+  if (!v) return nullptr;
+  std::memcpy(v->ops, w->ops, sizeof(struct _generic_N_Vector_Ops));
+  return v;
+}
+void N_VDestroy_Hip(N_Vector v) {
+  if (!v) return;
+  NvContent* c = C(v);
+  if (c) {
+    if (c->own_d && c->d) hipFree(c->d);
+    if (c->h) hipHostFree(c->h);
+    delete c;
+  }
+  free(v->ops);
+  free(v);
+}
+void N_VSpace_Hip(N_Vector v, sunindextype* lrw, sunindextype* liw) {
+  *lrw = LEN(v);
+  *liw = 2;
+}
+sunrealtype* N_VGetDeviceArrayPointer_Hip(N_Vector v) { return D(v); }
+sunrealtype* N_VGetHostArrayPointer_Hip(N_Vector v) {
+  NvContent* c = C(v);
+  if (!c->h && c->n > 0) {
+    if (hipHostMalloc(&c->h, sizeof(double) * (size_t)c->n, hipHostMallocDefault) != hipSuccess)
+      c->h = nullptr;
+  }
+  return c->h;
+}
+static void nv_setarray(sunrealtype* d, N_Vector v) { C(v)->d = d; }
+sunindextype N_VGetLength_Hip(N_Vector v) { return LEN(v); }
+void* N_VGetStream_Hip(N_Vector v) { return (void*)S(v); }
+
+void N_VCopyToDevice_Hip(N_Vector v) {
+  double* h = N_VGetHostArrayPointer_Hip(v);
+  if (!h) return;
+  hipMemcpyAsync(D(v), h, sizeof(double) * (size_t)LEN(v), hipMemcpyHostToDevice, S(v));
+  hipStreamSynchronize(S(v));
+}
+void N_VCopyFromDevice_Hip(N_Vector v) {
+  double* h = N_VGetHostArrayPointer_Hip(v);
+  if (!h) return;
+  hipMemcpyAsync(h, D(v), sizeof(double) * (size_t)LEN(v), hipMemcpyDeviceToHost, S(v));
+  hipStreamSynchronize(S(v));
+}
+int N_VSetFromHost_Hip(N_Vector v, const sunrealtype* src) {
+  if (hipMemcpyAsync(D(v), src, sizeof(double) * (size_t)LEN(v), hipMemcpyHostToDevice, S(v)) !=
+      hipSuccess)
+    return -1;
+  return hipStreamSynchronize(S(v)) == hipSuccess ? 0 : -1;
+}
+int N_VGetToHost_Hip(N_Vector v, sunrealtype* dst) {
+  if (hipMemcpyAsync(dst, D(v), sizeof(double) * (size_t)LEN(v), hipMemcpyDeviceToHost, S(v)) !=
+      hipSuccess)
+    return -1;
+  return hipStreamSynchronize(S(v)) == hipSuccess ? 0 : -1;
+}
+
+#define EW_LAUNCH(kern, z, ...)                                                        \
+  do {                                                                                 \
+    const sunindextype n_ = LEN(z);                                                    \
+    if (n_ > 0) hipLaunchKernelGGL(kern, dim3(ew_grid(n_)), dim3(kBlock), 0, S(z), __VA_ARGS__); \
+    check_launch(#kern);                                                               \
+  } while (0)
+
+void N_VLinearSum_Hip(sunrealtype a, N_Vector x, sunrealtype b, N_Vector y, N_Vector z) {
+  EW_LAUNCH(k_linearsum, z, a, D(x), b, D(y), D(z), LEN(z));
+}
+void N_VConst_Hip(sunrealtype c, N_Vector z) { EW_LAUNCH(k_const, z, c, D(z), LEN(z)); }
+void N_VProd_Hip(N_Vector x, N_Vector y, N_Vector z) { EW_LAUNCH(k_prod, z, D(x), D(y), D(z), LEN(z)); }
+void N_VDiv_Hip(N_Vector x, N_Vector y, N_Vector z) { EW_LAUNCH(k_div, z, D(x), D(y), D(z), LEN(z)); }
+void N_VScale_Hip(sunrealtype c, N_Vector x, N_Vector z) {
+  if (x == z && c == 1.0) return;
+  EW_LAUNCH(k_scale, z, c, D(x), D(z), LEN(z));
+}
+void N_VAbs_Hip(N_Vector x, N_Vector z) { EW_LAUNCH(k_abs, z, D(x), D(z), LEN(z)); }
+void N_VInv_Hip(N_Vector x, N_Vector z) { EW_LAUNCH(k_inv, z, D(x), D(z), LEN(z)); }
+void N_VAddConst_Hip(N_Vector x, sunrealtype b, N_Vector z) {
+  EW_LAUNCH(k_addconst, z, D(x), b, D(z), LEN(z));
+}
+void N_VCompare_Hip(sunrealtype c, N_Vector x, N_Vector z) {
+  EW_LAUNCH(k_compare, z, c, D(x), D(z), LEN(z));
+}
+
+sunrealtype N_VDotProd_Hip(N_Vector x, N_Vector y) {
+  return reduce1(x, RED_SUM, FDot{D(x), D(y)});
+}
+sunrealtype N_VMaxNorm_Hip(N_Vector x) { return reduce1(x, RED_MAX, FAbs{D(x)}); }
+sunrealtype N_VWSqrSumLocal_Hip(N_Vector x, N_Vector w) {
+  return reduce1(x, RED_SUM, FWsq{D(x), D(w)});
+}
+sunrealtype N_VWSqrSumMaskLocal_Hip(N_Vector x, N_Vector w, N_Vector id) {
+  return reduce1(x, RED_SUM, FWsqMask{D(x), D(w), D(id)});
+}
+sunrealtype N_VWrmsNorm_Hip(N_Vector x, N_Vector w) {
+  return std::sqrt(N_VWSqrSumLocal_Hip(x, w) / (double)LEN(x));
+}
+sunrealtype N_VWrmsNormMask_Hip(N_Vector x, N_Vector w, N_Vector id) {
+  return std::sqrt(N_VWSqrSumMaskLocal_Hip(x, w, id) / (double)LEN(x));
+}
+sunrealtype N_VMin_Hip(N_Vector x) { return reduce1(x, RED_MIN, FVal{D(x)}); }
+sunrealtype N_VWL2Norm_Hip(N_Vector x, N_Vector w) {
+  return std::sqrt(N_VWSqrSumLocal_Hip(x, w));
+}
+sunrealtype N_VL1Norm_Hip(N_Vector x) { return reduce1(x, RED_SUM, FAbs{D(x)}); }
+sunbooleantype N_VInvTest_Hip(N_Vector x, N_Vector z) {
+  const double nz = reduce1(z, RED_SUM, FInvTest{D(x), D(z)});
+  return nz > 0.0 ? SUNFALSE : SUNTRUE;
+}
+sunbooleantype N_VConstrMask_Hip(N_Vector c, N_Vector x, N_Vector m) {
+  const double nb = reduce1(m, RED_SUM, FConstr{D(c), D(x), D(m)});
+  return nb > 0.0 ? SUNFALSE : SUNTRUE;
+}
+sunrealtype N_VMinQuotient_Hip(N_Vector num, N_Vector denom) {
+  const double q = reduce1(num, RED_MIN, FQuot{D(num), D(denom)});
+  return std::isinf(q) ? 1.7976931348623157e308 : q;
+}
+
+SUNErrCode N_VLinearCombination_Hip(int nvec, sunrealtype* c, N_Vector* X, N_Vector z) {
+  if (nvec < 1) return SUN_SUCCESS;
+  if (nvec == 1) { N_VScale_Hip(c[0], X[0], z); return SUN_SUCCESS; }
+  // groups of kMaxK: first group writes z, later groups accumulate into z
+  int done = 0;
+  while (done < nvec) {
+    CoefPack cp{};
+    PtrPack xp{};
+    int k = 0;
+    if (done > 0) { cp.c[0] = 1.0; xp.p[0] = D(z); k = 1; }
+    while (k < kMaxK && done < nvec) { cp.c[k] = c[done]; xp.p[k] = D(X[done]); k++; done++; }
+    EW_LAUNCH(k_lincomb, z, k, cp, xp, D(z), LEN(z));
+  }
+  return SUN_SUCCESS;
+}
+
+SUNErrCode N_VScaleAddMulti_Hip(int nvec, sunrealtype* a, N_Vector x, N_Vector* Y, N_Vector* Z) {
+  for (int base = 0; base < nvec; base += kMaxK) {
+    const int k = (nvec - base < kMaxK) ? nvec - base : kMaxK;
+    CoefPack cp{};
+    PtrPack yp{};
+    OutPack zp{};
+    for (int j = 0; j < k; j++) {
+      cp.c[j] = a[base + j];
+      yp.p[j] = D(Y[base + j]);
+      zp.p[j] = D(Z[base + j]);
+    }
+    EW_LAUNCH(k_scaleaddmulti, x, k, cp, D(x), yp, zp, LEN(x));
+  }
+  return SUN_SUCCESS;
+}
+
+SUNErrCode N_VDotProdMulti_Hip(int nvec, N_Vector x, N_Vector* Y, sunrealtype* dotprods) {
+  if (nvec < 1) return SUN_SUCCESS;
+  hipStream_t s = S(x);
+  RedWs* w = workspace(s);
+  for (int base = 0; base < nvec; base += kMaxK) {
+    const int k = (nvec - base < kMaxK) ? nvec - base : kMaxK;
+    PtrPack yp{};
+    for (int j = 0; j < k; j++) yp.p[j] = D(Y[base + j]);
+    dotmulti_dispatch(k, s, D(x), yp, LEN(x), w);
+    check_launch("dotmulti");
+    hipStreamSynchronize(s);
+    g_syncs++;
+    for (int j = 0; j < k; j++) dotprods[base + j] = w->host[j];
+  }
+  return SUN_SUCCESS;
+}
+
+SUNErrCode N_VLinearSumVectorArray_Hip(int nvec, sunrealtype a, N_Vector* X, sunrealtype b,
+                                       N_Vector* Y, N_Vector* Z) {
+  for (int base = 0; base < nvec; base += kMaxK) {
+    const int k = (nvec - base < kMaxK) ? nvec - base : kMaxK;
+    PtrPack xp{}, yp{};
+    OutPack zp{};
+    for (int j = 0; j < k; j++) {
+      xp.p[j] = D(X[base + j]);
+      yp.p[j] = D(Y[base + j]);
+      zp.p[j] = D(Z[base + j]);
+    }
+    EW_LAUNCH(k_lsumarray, Z[0], k, a, xp, b, yp, zp, LEN(Z[0]));
+  }
+  return SUN_SUCCESS;
+}
+
+SUNErrCode N_VScaleVectorArray_Hip(int nvec, sunrealtype* c, N_Vector* X, N_Vector* Z) {
+  for (int base = 0; base < nvec; base += kMaxK) {
+    const int k = (nvec - base < kMaxK) ? nvec - base : kMaxK;
+    CoefPack cp{};
+    PtrPack xp{};
+    OutPack zp{};
+    for (int j = 0; j < k; j++) {
+      cp.c[j] = c[base + j];
+      xp.p[j] = D(X[base + j]);
+      zp.p[j] = D(Z[base + j]);
+    }
+    EW_LAUNCH(k_scalearray, Z[0], k, cp, xp, zp, LEN(Z[0]));
+  }
+  return SUN_SUCCESS;
+}
+
+SUNErrCode N_VConstVectorArray_Hip(int nvec, sunrealtype c, N_Vector* Z) {
+  for (int base = 0; base < nvec; base += kMaxK) {
+    const int k = (nvec - base < kMaxK) ? nvec - base : kMaxK;
+    OutPack zp{};
+    for (int j = 0; j < k; j++) zp.p[j] = D(Z[base + j]);
+    EW_LAUNCH(k_constarray, Z[0], k, c, zp, LEN(Z[0]));
+  }
+  return SUN_SUCCESS;
+}
+
+static SUNErrCode wsq_array(int nvec, N_Vector* X, N_Vector* W, N_Vector id, sunrealtype* nrm) {
+  if (nvec < 1) return SUN_SUCCESS;
+  hipStream_t s = S(X[0]);
+  RedWs* w = workspace(s);
+  const sunindextype n = LEN(X[0]);
+  for (int base = 0; base < nvec; base += 8) {
+    const int k = (nvec - base < 8) ? nvec - base : 8;
+    PtrPack xp{}, wp{};
+    for (int j = 0; j < k; j++) {
+      xp.p[j] = D(X[base + j]);
+      wp.p[j] = D(W[base + j]);
+    }
+    wsq_dispatch(k, s, xp, wp, id ? D(id) : nullptr, n, w);
+    check_launch("wsqarray");
+    hipStreamSynchronize(s);
+    g_syncs++;
+    for (int j = 0; j < k; j++) nrm[base + j] = std::sqrt(w->host[j] / (double)n);
+  }
+  return SUN_SUCCESS;
+}
+
+SUNErrCode N_VWrmsNormVectorArray_Hip(int nvec, N_Vector* X, N_Vector* W, sunrealtype* nrm) {
+  return wsq_array(nvec, X, W, nullptr, nrm);
+}
+SUNErrCode N_VWrmsNormMaskVectorArray_Hip(int nvec, N_Vector* X, N_Vector* W, N_Vector id,
+                                          sunrealtype* nrm) {
+  return wsq_array(nvec, X, W, id, nrm);
+}
+
+static sunrealtype nv_dot_local(N_Vector x, N_Vector y) { return N_VDotProd_Hip(x, y); }
+static SUNComm nv_comm(N_Vector v) { (void)v; return SUN_COMM_NULL; }
+static void nv_print(N_Vector v) {
+  double* h = N_VGetHostArrayPointer_Hip(v);
+  N_VCopyFromDevice_Hip(v);
+  for (sunindextype i = 0; i < LEN(v) && h; i++) printf("%.16e\n", h[i]);
+}
+static void nv_printfile(N_Vector v, FILE* f) {
+  double* h = N_VGetHostArrayPointer_Hip(v);
+  N_VCopyFromDevice_Hip(v);
+  for (sunindextype i = 0; i < LEN(v) && h; i++) fprintf(f, "%.16e\n", h[i]);
+}
+
+int N_VEnableFusedOps_Hip(N_Vector v, sunbooleantype tf) {
+  if (!v || !v->ops) return -1;
+  N_Vector_Ops o = v->ops;
+  o->nvlinearcombination = tf ? N_VLinearCombination_Hip : nullptr;
+  o->nvscaleaddmulti = tf ? N_VScaleAddMulti_Hip : nullptr;
+  o->nvdotprodmulti = tf ? N_VDotProdMulti_Hip : nullptr;
+  o->nvlinearsumvectorarray = tf ? N_VLinearSumVectorArray_Hip : nullptr;
+  o->nvscalevectorarray = tf ? N_VScaleVectorArray_Hip : nullptr;
+  o->nvconstvectorarray = tf ? N_VConstVectorArray_Hip : nullptr;
+  o->nvwrmsnormvectorarray = tf ? N_VWrmsNormVectorArray_Hip : nullptr;
+  o->nvwrmsnormmaskvectorarray = tf ? N_VWrmsNormMaskVectorArray_Hip : nullptr;
+  return 0;
+}
+
+void FusedNVec_Init(N_Vector v) {
+  if (!v) return;
+  N_VEnableFusedOps_Hip(v, SUNTRUE);
+  workspace(S(v));
+}
+
+void FusedNVec_FreePool(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto& kv : g_ws) {
+    hipFree(kv.second->partials);
+    hipFree(kv.second->ticket);
+    hipHostFree(kv.second->host);
+    delete kv.second;
+  }
+  g_ws.clear();
+}
+
+}  // extern "C"
+
+namespace {
+void install_ops(N_Vector v, bool fused) {
+  N_Vector_Ops o = v->ops;
+  std::memset(o, 0, sizeof(*o));
+  o->nvgetvectorid = N_VGetVectorID_Hip;
+  o->nvclone = N_VClone_Hip;
+  o->nvcloneempty = N_VCloneEmpty_Hip;
+  o->nvdestroy = N_VDestroy_Hip;
+  o->nvspace = N_VSpace_Hip;
+  o->nvgetarraypointer = N_VGetHostArrayPointer_Hip;
+  o->nvgetdevicearraypointer = N_VGetDeviceArrayPointer_Hip;
+  o->nvsetarraypointer = nv_setarray;
+  o->nvgetcommunicator = nv_comm;
+  o->nvgetlength = N_VGetLength_Hip;
+  o->nvgetlocallength = N_VGetLength_Hip;
+  o->nvlinearsum = N_VLinearSum_Hip;
+  o->nvconst = N_VConst_Hip;
+  o->nvprod = N_VProd_Hip;
+  o->nvdiv = N_VDiv_Hip;
+  o->nvscale = N_VScale_Hip;
+  o->nvabs = N_VAbs_Hip;
+  o->nvinv = N_VInv_Hip;
+  o->nvaddconst = N_VAddConst_Hip;
+  o->nvdotprod = N_VDotProd_Hip;
+  o->nvmaxnorm = N_VMaxNorm_Hip;
+  o->nvwrmsnorm = N_VWrmsNorm_Hip;
+  o->nvwrmsnormmask = N_VWrmsNormMask_Hip;
+  o->nvmin = N_VMin_Hip;
+  o->nvwl2norm = N_VWL2Norm_Hip;
+  o->nvl1norm = N_VL1Norm_Hip;
+  o->nvcompare = N_VCompare_Hip;
+  o->nvinvtest = N_VInvTest_Hip;
+  o->nvconstrmask = N_VConstrMask_Hip;
+  o->nvminquotient = N_VMinQuotient_Hip;
+  o->nvdotprodlocal = nv_dot_local;
+  o->nvmaxnormlocal = N_VMaxNorm_Hip;
+  o->nvminlocal = N_VMin_Hip;
+  o->nvl1normlocal = N_VL1Norm_Hip;
+  o->nvinvtestlocal = N_VInvTest_Hip;
+  o->nvconstrmasklocal = N_VConstrMask_Hip;
+  o->nvminquotientlocal = N_VMinQuotient_Hip;
+  o->nvwsqrsumlocal = N_VWSqrSumLocal_Hip;
+  o->nvwsqrsummasklocal = N_VWSqrSumMaskLocal_Hip;
+  o->nvprint = nv_print;
+  o->nvprintfile = nv_printfile;
+  if (fused) N_VEnableFusedOps_Hip(v, SUNTRUE);
+}
+}  // namespace

@@ -1,0 +1,430 @@
+"""llg_amd -- MI355X-native 2D LLG hot path (CVODE plug-ins on gfx950).
+
+Host-side mirror of the reference's plug-in surface: configurations that
+reproduce each reference variant's physics (configs/sim_config.h knobs,
+per-variant constants of src/irregular/2d_fft/2d_i{1..6}), geometry/IC
+builders, single-callback entry points (f, Jv, PrecondSetup/Solve,
+Demag_Apply) and the timed integration loop.  All compute runs in
+libllg_amd.so; this module only marshals arguments.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import native
+from .native import (llg_config, llg_terms, llg_run_opts, llg_run_result, llg_grain,
+                     EXEC_YMSK, EXEC_COMPACT, ANISO_LINEAR, ANISO_CUBIC, NORM_NONE,
+                     NORM_EXACT, NORM_REGULARIZED, BC_PERIODIC, BC_NEUMANN, DEMAG_OFF,
+                     DEMAG_PERIODIC, DEMAG_PADDED, GATE_NONE, GATE_MASK, GATE_WINDOW,
+                     PC_ALWAYS, PC_JOK, PREC_LEFT, PREC_RIGHT, MODIFIED_GS, CLASSICAL_GS)
+
+__all__ = [
+    "native", "lib", "terms", "make_config", "preset", "PRESETS", "Geometry", "Simulation",
+    "HipVector", "llg_config", "llg_terms", "llg_run_opts",
+]
+
+
+def lib():
+    return native.load()
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _ucp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_ubyte)) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int)) if a is not None else None
+
+
+# --------------------------------------------------------------- configs
+def terms(che=0.0, chk=0.0, cha=0.0, chb=0.0, msk=(0, 0, 0), nsk=(0, 0, 0), happ=(0, 0, 0),
+          aniso=ANISO_CUBIC, cell_axes=0, window=0, gate=GATE_NONE) -> llg_terms:
+    t = llg_terms()
+    t.che, t.chk, t.cha, t.chb = che, chk, cha, chb
+    for a in range(3):
+        t.msk[a], t.nsk[a], t.happ[a] = msk[a], nsk[a], happ[a]
+    t.aniso, t.cell_axes, t.window, t.gate = aniso, cell_axes, window, gate
+    return t
+
+
+def make_config(ng, ny, rhs, jv=None, pc=None, exec_=EXEC_YMSK, boundary=BC_PERIODIC,
+                exch_self=0, normalize=NORM_NONE, demag=DEMAG_OFF, demag_strength=0.0,
+                demag_thick=1.0, demag_fp32=0, alpha=0.2, chg=1.0, pc_mode=PC_ALWAYS):
+    c = llg_config()
+    c.ng, c.ny, c.exec_, c.boundary = ng, ny, exec_, boundary
+    c.exch_self, c.normalize, c.demag, c.demag_fp32 = exch_self, normalize, demag, demag_fp32
+    c.pc_mode = pc_mode
+    c.demag_strength, c.demag_thick, c.alpha, c.chg = demag_strength, demag_thick, alpha, chg
+    c.rhs = rhs
+    c.jv = jv if jv is not None else rhs
+    c.pc = pc if pc is not None else rhs
+    return c
+
+
+def _preset_i4(ng=512, ny=512, demag=4.0, demag_mode=DEMAG_PERIODIC, **kw):
+    """2d_i4: square hole, ymsk, Zeeman (-0.2,0,0); Jv without Zeeman/demag;
+    preconditioner field with pc_happ1 = -1.0 (2d_fft.cu:169-180, jtv.cu:36-39,
+    precond.cu:65-71)."""
+    cub = dict(che=20.0, chk=1.0, msk=(1, 1, 1), nsk=(1, 0, 0), aniso=ANISO_CUBIC, gate=GATE_MASK)
+    return make_config(ng, ny, terms(happ=(-0.2, 0, 0), **cub), terms(**cub),
+                       terms(happ=(-1.0, 0, 0), **cub), demag=demag_mode if demag > 0 else DEMAG_OFF,
+                       demag_strength=demag, **kw)
+
+
+def _preset_i2(ng=512, ny=512, demag=4.0, **kw):
+    cub = dict(che=50.0, chk=1.0, msk=(1, 1, 1), nsk=(1, 0, 0), aniso=ANISO_CUBIC, gate=GATE_MASK)
+    return make_config(ng, ny, terms(**cub), normalize=NORM_REGULARIZED,
+                       demag=DEMAG_PERIODIC if demag > 0 else DEMAG_OFF, demag_strength=demag, **kw)
+
+
+def _preset_i5(ng=512, ny=512, demag=4.0, **kw):
+    cub = dict(che=10.0, chk=1.0, msk=(1, 1, 1), nsk=(1, 0, 0), aniso=ANISO_CUBIC, gate=GATE_MASK)
+    return make_config(ng, ny, terms(**cub), demag=DEMAG_PERIODIC if demag > 0 else DEMAG_OFF,
+                       demag_strength=demag, **kw)
+
+
+def _preset_i6(ng=512, ny=512, demag=4.0, **kw):
+    cub = dict(che=10.0, chk=1.0, msk=(1, 1, 1), nsk=(1, 0, 0), aniso=ANISO_CUBIC, gate=GATE_NONE)
+    return make_config(ng, ny, terms(**cub), exec_=EXEC_COMPACT,
+                       demag=DEMAG_PERIODIC if demag > 0 else DEMAG_OFF, demag_strength=demag, **kw)
+
+
+def _preset_i1(ng=512, ny=512, demag=2.0, **kw):
+    lin = dict(che=4.0, chk=4.0, cha=0.0, chb=0.3, msk=(1, 0, 0), nsk=(1, 0, 0),
+               aniso=ANISO_LINEAR, gate=GATE_NONE)
+    return make_config(ng, ny, terms(**lin), normalize=NORM_EXACT,
+                       demag=DEMAG_PERIODIC if demag > 0 else DEMAG_OFF, demag_strength=demag, **kw)
+
+
+def _preset_i3(ng=768, ny=256, demag=1.0, **kw):
+    rhs = terms(che=4.0, chk=1.0, cha=0.0, chb=0.6, aniso=ANISO_LINEAR, cell_axes=1, window=1,
+                gate=GATE_WINDOW)
+    fixed = terms(che=4.0, chk=4.0, cha=0.0, chb=0.3, msk=(1, 0, 0), nsk=(1, 0, 0),
+                  aniso=ANISO_LINEAR, gate=GATE_NONE)
+    return make_config(ng, ny, rhs, fixed, fixed, normalize=NORM_REGULARIZED,
+                       demag=DEMAG_PERIODIC if demag > 0 else DEMAG_OFF, demag_strength=demag, **kw)
+
+
+def _preset_plain128(ng=128, ny=128, a=0.02, c_ex=1.0, k_u=0.1, h_ext=(0.0, 0.0, -0.05), **kw):
+    """Plain CPU-runnable workload: LL form -(1/(1+a^2))[m x h + a m x (m x h)]
+    mapped onto the reference form with chg = 1/(1+a^2), alpha = a/(1+a^2);
+    h = c_ex * Laplacian(m) (Neumann) + k_u (m.z) z + h_ext; no demag."""
+    t = terms(che=c_ex, chk=k_u, cha=0.0, chb=0.0, msk=(0, 0, 1), happ=h_ext,
+              aniso=ANISO_LINEAR, gate=GATE_NONE)
+    return make_config(ng, ny, t, boundary=BC_NEUMANN, exch_self=1, alpha=a / (1 + a * a),
+                       chg=1.0 / (1 + a * a), **kw)
+
+
+def _preset_square_hole(ng=1536, ny=512, demag=4.0, padded=True, **kw):
+    """1536x512-cell square hole (444^2 cells, 25.07 %), i4 physics constants,
+    zero-padded 3072x1024 FFT demag of strength 4, thickness 1."""
+    return _preset_i4(ng, ny, demag=demag, demag_mode=DEMAG_PADDED if padded else DEMAG_PERIODIC,
+                      **kw)
+
+
+PRESETS = {
+    "i1": _preset_i1, "i2": _preset_i2, "i3": _preset_i3, "i4": _preset_i4,
+    "i5": _preset_i5, "i6": _preset_i6, "plain128": _preset_plain128,
+    "square_hole": _preset_square_hole,
+}
+
+
+def preset(name, **kw) -> llg_config:
+    return PRESETS[name](**kw)
+
+
+# -------------------------------------------------------------- geometry
+class Geometry:
+    """Active mask (+ optional window and per-cell axes) of an ng x ny grid."""
+
+    def __init__(self, ng, ny, active=None, w=None, msk=None, nsk=None, grain_id=None,
+                 grains=None):
+        self.ng, self.ny = ng, ny
+        self.active = active
+        self.w, self.msk, self.nsk = w, msk, nsk
+        self.grain_id, self.grains = grain_id, grains
+
+    @property
+    def ncell(self):
+        return self.ng * self.ny
+
+    @classmethod
+    def full(cls, ng, ny):
+        return cls(ng, ny, np.ones(ng * ny, np.uint8))
+
+    @classmethod
+    def square_hole(cls, ng, ny, cx_frac=0.5, cy_frac=0.5, halfside_frac=0.22):
+        a = np.zeros(ng * ny, np.uint8)
+        lib().LLG_GeomSquareHole(ng, ny, cx_frac, cy_frac, halfside_frac, _ucp(a))
+        return cls(ng, ny, a)
+
+    @classmethod
+    def rectangle(cls, ng, ny, w_frac=0.5, h_frac=0.5):
+        a = np.zeros(ng * ny, np.uint8)
+        lib().LLG_GeomRectangle(ng, ny, w_frac, h_frac, _ucp(a))
+        return cls(ng, ny, a)
+
+    @classmethod
+    def ring(cls, ng, ny, w_frac=0.5, h_frac=0.5, inner_w=0.5, inner_h=0.5):
+        a = np.zeros(ng * ny, np.uint8)
+        lib().LLG_GeomRing(ng, ny, w_frac, h_frac, inner_w, inner_h, _ucp(a))
+        return cls(ng, ny, a)
+
+    @classmethod
+    def ellipse(cls, ng, ny, rx_frac=0.25, ry_frac=0.25):
+        a = np.zeros(ng * ny, np.uint8)
+        lib().LLG_GeomEllipse(ng, ny, rx_frac, ry_frac, _ucp(a))
+        return cls(ng, ny, a)
+
+    @classmethod
+    def polycrystal(cls, ng, ny, num_grains=72, dead_frac=0.16, seed=20251104, z_bias=1.6,
+                    eps_cells=2.2):
+        N = ng * ny
+        grains = (llg_grain * num_grains)()
+        gid = np.zeros(N, np.int32)
+        a = np.zeros(N, np.uint8)
+        w = np.zeros(N)
+        msk = np.zeros(3 * N)
+        nsk = np.zeros(3 * N)
+        lib().LLG_GeomPolycrystal(ng, ny, num_grains, dead_frac, seed, z_bias, eps_cells, grains,
+                                  _ip(gid), _ucp(a), _dp(w), _dp(msk), _dp(nsk))
+        return cls(ng, ny, a, w, msk, nsk, gid, grains)
+
+    # ---- initial states
+    def ic_uniform(self, m=(1.0, -0.0175, 0.0)):
+        y = np.zeros(3 * self.ncell)
+        lib().LLG_ICUniform(self.ng, self.ny, _ucp(self.active), m[0], m[1], m[2], _dp(y))
+        return y
+
+    def ic_two_domain(self, j_split=None, tail_my=-0.0175):
+        y = np.zeros(3 * self.ncell)
+        js = (7 * self.ny) // 8 if j_split is None else j_split
+        lib().LLG_ICTwoDomain(self.ng, self.ny, _ucp(self.active), js, tail_my, _dp(y))
+        return y
+
+    def ic_stripes(self, left=0.25, right=0.75, eps=0.01, seed=12345):
+        y = np.zeros(3 * self.ncell)
+        lib().LLG_ICStripes(self.ng, self.ny, left, right, eps, seed, _dp(y))
+        return y
+
+    def ic_grain_bumps(self, core_mz=0.95, seed=12345):
+        y = np.zeros(3 * self.ncell)
+        lib().LLG_ICGrainBumps(self.ng, self.ny, self.grains, _ip(self.grain_id), _dp(self.w),
+                               core_mz, seed, _dp(y))
+        return y
+
+    def ic_tilted_z(self, max_angle=0.1, seed=1):
+        y = np.zeros(3 * self.ncell)
+        lib().LLG_ICTiltedZ(self.ng, self.ny, _ucp(self.active), max_angle, seed, _dp(y))
+        return y
+
+    def ic_split_x(self, max_angle=0.05, seed=1):
+        y = np.zeros(3 * self.ncell)
+        lib().LLG_ICSplitX(self.ng, self.ny, _ucp(self.active), max_angle, seed, _dp(y))
+        return y
+
+    def ic_random_sphere(self, seed=1):
+        y = np.zeros(3 * self.ncell)
+        lib().LLG_ICRandomSphere(self.ng, self.ny, _ucp(self.active), seed, _dp(y))
+        return y
+
+
+# ---------------------------------------------------------------- vectors
+class _SunCtx(C.Structure):
+    _fields_ = [("in_use", C.c_int)]
+
+
+_SUNCTX = _SunCtx()
+
+
+class HipVector:
+    """Owning handle of an N_Vector_Hip with numpy transfer helpers."""
+
+    def __init__(self, n, stream=None, values=None):
+        L = lib()
+        self.n = n
+        self.v = L.N_VNewWithStream_Hip(n, stream, C.byref(_SUNCTX))
+        if not self.v:
+            raise RuntimeError("N_VNewWithStream_Hip failed")
+        if values is not None:
+            self.set(values)
+
+    def set(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == self.n
+        if lib().N_VSetFromHost_Hip(self.v, _dp(a)) != 0:
+            raise RuntimeError("host->device copy failed")
+
+    def get(self):
+        a = np.empty(self.n)
+        if lib().N_VGetToHost_Hip(self.v, _dp(a)) != 0:
+            raise RuntimeError("device->host copy failed")
+        return a
+
+    def fused(self, on=True):
+        lib().N_VEnableFusedOps_Hip(self.v, 1 if on else 0)
+        return self
+
+    def __del__(self):
+        try:
+            if self.v:
+                lib().N_VDestroy_Hip(self.v)
+                self.v = None
+        except Exception:
+            pass
+
+
+def _vp_array(vs):
+    arr = (C.c_void_p * len(vs))(*[v.v for v in vs])
+    return arr
+
+
+# ------------------------------------------------------------- simulation
+class Simulation:
+    """A context (the reference's UserData) plus single-callback access and
+    the timed integration loop."""
+
+    def __init__(self, cfg: llg_config, geom: Geometry | None = None):
+        L = lib()
+        self.cfg = cfg
+        self.geom = geom or Geometry.full(cfg.ng, cfg.ny)
+        g = self.geom
+        self.ctx = L.LLG_Create(C.byref(cfg), _ucp(g.active), _dp(g.w), _dp(g.msk), _dp(g.nsk),
+                                None)
+        if not self.ctx:
+            raise RuntimeError("LLG_Create failed: " + L.LLG_LastError().decode())
+        self.ncell = cfg.ng * cfg.ny
+        self.neq = 3 * self.ncell
+        self.stream = L.LLG_GetStream(self.ctx)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().LLG_Destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def vec(self, values=None):
+        return HipVector(self.neq, self.stream, values)
+
+    # --- single callbacks through the C ABI
+    def rhs(self, y, t=0.0):
+        vy, vf = self.vec(y), self.vec()
+        rc = lib().LLG_Rhs(t, vy.v, vf.v, self.ctx)
+        if rc != 0:
+            raise RuntimeError("LLG_Rhs failed: " + lib().LLG_LastError().decode())
+        return vf.get(), vy.get()
+
+    def jtv(self, y, v, t=0.0):
+        vy, vv, vj = self.vec(y), self.vec(v), self.vec()
+        rc = lib().JtvProduct(vv.v, vj.v, t, vy.v, None, self.ctx, None)
+        if rc != 0:
+            raise RuntimeError("JtvProduct failed")
+        return vj.get()
+
+    def psetup(self, y, gamma, jok=0, t=0.0):
+        vy = self.vec(y)
+        jcur = C.c_int(0)
+        rc = lib().PrecondSetup(t, vy.v, None, jok, C.byref(jcur), gamma, self.ctx)
+        if rc != 0:
+            raise RuntimeError("PrecondSetup failed")
+        return jcur.value
+
+    def psolve(self, r, gamma=0.0, t=0.0):
+        vr, vz = self.vec(r), self.vec()
+        rc = lib().PrecondSolve(t, None, None, vr.v, vz.v, gamma, 0.0, PREC_LEFT, self.ctx)
+        if rc != 0:
+            raise RuntimeError("PrecondSolve failed")
+        return vz.get()
+
+    def hdemag(self):
+        h = np.empty(self.neq)
+        lib().LLG_GetHdemag(self.ctx, _dp(h))
+        return h
+
+    def pinv(self):
+        p = np.empty(9 * self.ncell)
+        lib().LLG_GetPinv(self.ctx, _dp(p))
+        return p
+
+    def mask(self):
+        m = np.empty(self.neq)
+        lib().LLG_GetMask(self.ctx, _dp(m))
+        return m
+
+    def ids(self):
+        nc, na, ni = C.c_int(), C.c_int(), C.c_int()
+        lib().LLG_GetCounts(self.ctx, C.byref(nc), C.byref(na), C.byref(ni))
+        a = np.empty(na.value, np.int32)
+        h = np.empty(ni.value, np.int32)
+        lib().LLG_GetActiveIds(self.ctx, _ip(a), _ip(h))
+        return a, h
+
+    def self_coupling(self):
+        d = lib().LLG_GetDemag(self.ctx)
+        v = (C.c_double * 3)()
+        lib().Demag_GetSelfCoupling(d, C.byref(v, 0), C.byref(v, 8), C.byref(v, 16))
+        return tuple(v)
+
+    # --- integration
+    @staticmethod
+    def run_opts(**kw) -> llg_run_opts:
+        o = llg_run_opts()
+        lib().LLG_DefaultRunOpts(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+    def integrator(self, y0, **kw):
+        return Run(self, y0, self.run_opts(**kw))
+
+
+class Run:
+    """The reference's CVode output loop (T1 cadence) on this context."""
+
+    def __init__(self, sim: Simulation, y0, opts: llg_run_opts):
+        self.sim = sim
+        y0 = np.ascontiguousarray(y0, dtype=np.float64)
+        self.h = lib().LLG_RunCreate(sim.ctx, C.byref(opts), _dp(y0))
+        if not self.h:
+            raise RuntimeError("LLG_RunCreate failed")
+        self.opts = opts
+
+    def advance(self, t_end):
+        res = llg_run_result()
+        rc = lib().LLG_RunAdvance(self.h, t_end, C.byref(res))
+        out = {
+            "retval": res.retval, "wall_s": res.wall_s, "gpu_ms": res.gpu_ms,
+            "sim_per_wall": res.sim_per_wall, "t": res.t_reached, "nout": res.nout,
+            **res.st.as_dict(),
+        }
+        if rc != 0:
+            raise RuntimeError(f"integration failed (retval {res.retval}) at t={res.t_reached}")
+        return out
+
+    def state(self):
+        y = np.empty(self.sim.neq)
+        lib().LLG_RunGetState(self.h, _dp(y))
+        return y
+
+    def close(self):
+        if self.h:
+            lib().LLG_RunDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,65 @@
+/*
+ * demag_fft.h -- FFT demagnetising field on gfx950, hand-written LDS radix
+ * FFT (no vendor FFT library on the hot path).
+ *
+ * Replaces the reference module API src/irregular/2d_fft/2d_i4/demag_fft.h:29-42
+ * (Demag_Init / Demag_Apply / Demag_GetSelfCoupling / Demag_Destroy) and the
+ * polycrystal additions src/irregular/2d_fft/2d_i3/demag_fft.h
+ * (Demag_ApplyWindowed, Demag_SetStream).
+ *
+ * Operator (reference semantics, 2d_i4/demag_fft.cu:64-91,143-161,325-363):
+ *     h(r) = strength * sum_r' N(r - r') m(r')
+ * with N the 81-point face-averaged Newell cell tensor (`calt`/`ctt`).
+ *   Demag_Init       : periodic circular convolution on the nx x ny cell
+ *                      grid; displacement wrapped into [-n/2, n - n/2).
+ *   Demag_InitPadded : open-boundary (non-wrapped) convolution, evaluated
+ *                      as a circular one on the zero-padded 2nx x 2ny grid.
+ *                      No reference counterpart (the padded R2C of the
+ *                      1536x512 square-hole workload).
+ * y_dev and h_out_dev are SoA [mx|my|mz] device arrays of 3*nx*ny doubles;
+ * h_out_dev is OVERWRITTEN (same contract as the reference).
+ */
+#ifndef LLG_DEMAG_FFT_H
+#define LLG_DEMAG_FFT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct DemagData DemagData;
+
+DemagData* Demag_Init(int nx, int ny, double thick, double demag_strength);
+DemagData* Demag_InitPadded(int nx, int ny, double thick, double demag_strength);
+
+void Demag_Apply(DemagData* d, const double* y_dev, double* h_out_dev);
+/* gather multiplies m by the per-cell weight w_dev[nx*ny] (2d_i3/demag_fft.cu:125-139) */
+void Demag_ApplyWindowed(DemagData* d, const double* y_dev, const double* w_dev,
+                         double* h_out_dev);
+/* stream is a hipStream_t; NULL selects the library default stream */
+void Demag_SetStream(DemagData* d, void* stream);
+/* N_aa(0) * strength; zeros when d == NULL (2d_i4/demag_fft.cu:365-377) */
+void Demag_GetSelfCoupling(DemagData* d, double* nxx0, double* nyy0, double* nzz0);
+void Demag_Destroy(DemagData* d);
+
+/* FP32 spectral pipeline (FFT, tensor multiply and intermediates in FP32;
+ * input gather and output field stay FP64).  fp32 = 0 restores FP64. */
+int  Demag_SetPrecision(DemagData* d, int fp32);
+
+/* introspection used by tests and the bench roofline */
+int  Demag_GetPlanShape(DemagData* d, int* px, int* py, int* padded);
+/* copy the (strength-scaled, 1/(px*py)-normalised) tensor spectrum to host:
+ * planes xx, yy, zz, xy_re, xy_im, each py*(px/2+1) doubles, [ky][kx] */
+int  Demag_GetSpectrum(DemagData* d, double* h_planes);
+/* last error string of this module ("" when none) */
+const char* Demag_LastError(void);
+
+/* Newell tensor on the host: the nine components of `calt` for an
+ * mdx x mdy displacement grid centred at (mdx/2, mdy/2), each plane
+ * mdx*mdy doubles in the reference order aa,ab,ac,ba,bb,bc,ca,cb,cc
+ * (2d_i4/demag_fft.cu:64-91).  nthreads<=0 uses all host cores. */
+int  Demag_NewellTensor(double thick, int mdx, int mdy, double* planes9, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LLG_DEMAG_FFT_H */

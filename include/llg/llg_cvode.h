@@ -1,0 +1,84 @@
+/*
+ * llg_cvode.h -- CVODE-compatible variable-order BDF integrator (Nordsieck
+ * history, Newton iteration, SPGMR with left/right preconditioning) that
+ * drives the LLG plug-ins when SUNDIALS is not installed.
+ *
+ * The reference calls SUNDIALS CVODE 7.x (third-party, not vendored;
+ * src/irregular/2d_fft/2d_i4/Makefile:2-13) through
+ *   CVodeCreate(CV_BDF) / CVodeInit / CVodeSetUserData / CVodeSVtolerances /
+ *   SUNNonlinSol_Newton / SUNLinSol_SPGMR(PREC_LEFT, KRYLOV_DIM) /
+ *   CVodeSetJacTimes / CVodeSetPreconditioner / SetGSType / SetMaxOrd / CVode
+ * (2d_i4/2d_fft.cu:619-695).  This header offers the same sequence under an
+ * LLGCV prefix, so it can coexist with a real SUNDIALS.  The algorithm
+ * restates SUNDIALS' published CVODE/CVLS/SPGMR/Newton method and default
+ * constants (no reference source exists for it); it works on any N_Vector
+ * whose ops table is filled (HIP or serial).
+ */
+#ifndef LLG_CVODE_H
+#define LLG_CVODE_H
+
+#include "sundials_compat.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LLGCV_BDF     2
+#define LLGCV_NORMAL  1
+#define LLGCV_ONE_STEP 2
+
+/* return codes (CVODE values) */
+#define LLGCV_SUCCESS          0
+#define LLGCV_TSTOP_RETURN     1
+#define LLGCV_WARNING          99
+#define LLGCV_TOO_MUCH_WORK   -1
+#define LLGCV_TOO_MUCH_ACC    -2
+#define LLGCV_ERR_FAILURE     -3
+#define LLGCV_CONV_FAILURE    -4
+#define LLGCV_LINIT_FAIL      -5
+#define LLGCV_LSETUP_FAIL     -6
+#define LLGCV_LSOLVE_FAIL     -7
+#define LLGCV_RHSFUNC_FAIL    -8
+#define LLGCV_FIRST_RHSFUNC_ERR -9
+#define LLGCV_REPTD_RHSFUNC_ERR -10
+#define LLGCV_UNREC_RHSFUNC_ERR -11
+#define LLGCV_MEM_FAIL        -20
+#define LLGCV_MEM_NULL        -21
+#define LLGCV_ILL_INPUT       -22
+#define LLGCV_TOO_CLOSE       -27
+
+typedef struct llgcv_stats {
+  long nst, nfe, nsetups, nni, ncfn, netf, nli, nlcf, njvevals, npe, nps, nge;
+  long nfeLS;
+  int  qlast, qcur;
+  double hlast, hcur, tcur, h0u;
+  long nsyncs;            /* host synchronisations made by vector reductions */
+} llgcv_stats;
+
+void* LLGCVodeCreate(int lmm, SUNContext sunctx);
+int   LLGCVodeInit(void* mem, CVRhsFn f, sunrealtype t0, N_Vector y0);
+int   LLGCVodeReInit(void* mem, sunrealtype t0, N_Vector y0);
+int   LLGCVodeSetUserData(void* mem, void* user_data);
+int   LLGCVodeSStolerances(void* mem, sunrealtype rtol, sunrealtype atol);
+int   LLGCVodeSVtolerances(void* mem, sunrealtype rtol, N_Vector abstol);
+int   LLGCVodeSetMaxOrd(void* mem, int maxord);
+int   LLGCVodeSetMaxNumSteps(void* mem, long mxsteps);
+int   LLGCVodeSetInitStep(void* mem, sunrealtype hin);
+int   LLGCVodeSetMaxStep(void* mem, sunrealtype hmax);
+int   LLGCVodeSetMinStep(void* mem, sunrealtype hmin);
+int   LLGCVodeSetStopTime(void* mem, sunrealtype tstop);
+/* SPGMR linear solver: pretype SUN_PREC_*, maxl Krylov dim (<=0 -> 5),
+ * gstype SUN_MODIFIED_GS / SUN_CLASSICAL_GS */
+int   LLGCVodeSetLinearSolverSPGMR(void* mem, int pretype, int maxl, int gstype);
+int   LLGCVodeSetEpsLin(void* mem, sunrealtype eplifac);
+int   LLGCVodeSetJacTimes(void* mem, CVLsJacTimesSetupFn jtsetup, CVLsJacTimesVecFn jtimes);
+int   LLGCVodeSetPreconditioner(void* mem, CVLsPrecSetupFn pset, CVLsPrecSolveFn psolve);
+int   LLGCVode(void* mem, sunrealtype tout, N_Vector yout, sunrealtype* tret, int itask);
+int   LLGCVodeGetDky(void* mem, sunrealtype t, int k, N_Vector dky);
+int   LLGCVodeGetStats(void* mem, llgcv_stats* st);
+void  LLGCVodeFree(void** mem);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LLG_CVODE_H */

@@ -1,0 +1,54 @@
+/*
+ * llg_driver.h -- the reference's main() time loop as a library call.
+ *
+ * Replaces the setup and output loop of src/irregular/2d_fft/2d_i4/2d_fft.cu
+ * :545-700 (N_VNew x2, FusedNVec_Init, CVodeInit/SVtolerances/SPGMR/
+ * SetJacTimes/SetPreconditioner/SetMaxOrd, then CVode every T1 = 0.1 inside
+ * a timed region) and PrintFinalStats (:369-390, via llgcv_stats).
+ */
+#ifndef LLG_DRIVER_H
+#define LLG_DRIVER_H
+
+#include "llg_cvode.h"
+#include "llg_plugin.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct llg_run_opts {
+  double t0;          /* initial time */
+  double t_out;       /* output cadence T1 */
+  double rtol, atol;  /* RTOL_VAL / ATOL_VAL */
+  int    maxl;        /* KRYLOV_DIM */
+  int    gstype;      /* 0 = auto (classical iff neq < 500000), else SUN_*_GS */
+  int    max_order;   /* MAX_BDF_ORDER */
+  int    fused;       /* FusedNVec_Init on y */
+  long   mxstep;      /* max steps per CVode call (CVODE default 500) */
+  int    use_jtv;     /* analytic Jv (else difference quotient) */
+  int    use_prec;    /* block-Jacobi preconditioner */
+  int    pretype;     /* SUN_PREC_LEFT (reference) or SUN_PREC_RIGHT */
+  int    pad;
+} llg_run_opts;
+
+typedef struct llg_run_result {
+  llgcv_stats st;
+  double wall_s;        /* host wall time of the output loop */
+  double gpu_ms;        /* hipEvent time of the same loop */
+  double sim_per_wall;  /* simulated time advanced / wall_s */
+  double t_reached;
+  long   nout;
+  int    retval;
+  int    pad;
+} llg_run_result;
+
+void  LLG_DefaultRunOpts(llg_run_opts* o);
+void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_host);
+int   LLG_RunAdvance(void* run, double t_end, llg_run_result* res);
+int   LLG_RunGetState(void* run, double* y_host);
+void  LLG_RunDestroy(void* run);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LLG_DRIVER_H */

@@ -1,0 +1,141 @@
+/*
+ * llg_plugin.h -- the 2D LLG solver's CVODE plug-in surface on gfx950.
+ *
+ * Entry points and the reference interfaces they replace:
+ *   LLG_Rhs        CVRhsFn f()            src/irregular/2d_fft/2d_i4/2d_fft.cu:328-366
+ *                                         (compact: 2d_i6/2d_i6.cu:352-396,
+ *                                          polycrystal: 2d_i3/2d_fft.cu:656-726)
+ *   JtvProduct     CVLsJacTimesVecFn      2d_i4/jtv.h:26-27, jtv.cu:129-147
+ *   PrecondSetup   CVLsPrecSetupFn        2d_i4/precond.h:31-33, precond.cu:324-358
+ *                                         (two-phase jok: src/2d/2d_heter/precond.cu:265-314)
+ *   PrecondSolve   CVLsPrecSolveFn        2d_i4/precond.h:35-38, precond.cu:361-385
+ *   Precond_Create / Precond_Destroy      2d_i4/precond.h:28-29
+ *   LLG_Create / LLG_Destroy              the UserData struct + its setup in main()
+ *                                         (2d_i4/2d_fft.cu:219-233,472-543; 2d_i6/2d_i6.cu:214-231)
+ *
+ * `user_data` for every callback is an opaque llg_ctx* (instead of three
+ * byte-mirrored structs).  The knobs of configs/sim_config.h become the
+ * runtime struct llg_config.  Callbacks return 0 on success and -1 on a
+ * launch failure (the reference's convention, 2d_i4/2d_fft.cu:359-364).
+ * All kernels of a callback are launched on the HIP stream of the N_Vector
+ * they write.
+ */
+#ifndef LLG_PLUGIN_H
+#define LLG_PLUGIN_H
+
+#include "sundials_compat.h"
+#include "demag_fft.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* execution model (configs/sim_config.h:107-108) */
+#define LLG_EXEC_YMSK     0
+#define LLG_EXEC_COMPACT  1
+/* anisotropy form (configs/sim_config.h:279-280) */
+#define LLG_ANISO_LINEAR  0   /* h_a += msk_a (chk (m.msk) + cha)      (i1, i3) */
+#define LLG_ANISO_CUBIC   1   /* h_a += msk_a chk m_a (m_a^2 - 1)      (i2, i4, i5, i6) */
+/* normalisation of y at the top of f (A4) */
+#define LLG_NORM_NONE         0   /* i4, i5, i6 */
+#define LLG_NORM_EXACT        1   /* i1: m/|m| when |m| > 1e-30 */
+#define LLG_NORM_REGULARIZED  2   /* i2, i3: m/(|m| + 1e-3) */
+/* lattice boundary */
+#define LLG_BC_PERIODIC   0   /* all reference variants */
+#define LLG_BC_NEUMANN    1   /* mirror ghost cell (the 128x128 plain workload) */
+/* demag */
+#define LLG_DEMAG_OFF       0
+#define LLG_DEMAG_PERIODIC  1   /* reference operator */
+#define LLG_DEMAG_PADDED    2   /* open boundary, zero-padded 2nx x 2ny */
+/* output gate of a callback */
+#define LLG_GATE_NONE    0
+#define LLG_GATE_MASK    1   /* x ymsk (0/1 per cell) */
+#define LLG_GATE_WINDOW  2   /* x w (soft window, i3) */
+/* preconditioner setup protocol */
+#define LLG_PC_ALWAYS   0   /* rebuild every setup, *jcur = TRUE (i4/i6) */
+#define LLG_PC_JOK      1   /* build J only when !jok, P^-1 every setup (2d_heter) */
+
+/* One set of field terms.  RHS, Jv and the preconditioner each carry
+ * their own set because the reference's three mirrors differ
+ * (e.g. i4 pc_happ1 = -1.0 vs happ1 = -0.2; i4 Jv has no Zeeman). */
+typedef struct llg_terms {
+  double che;        /* exchange coupling */
+  double chk;        /* anisotropy */
+  double cha;        /* linear-anisotropy offset */
+  double chb;        /* DMI-like coupling to the x neighbours */
+  double msk[3];     /* uniform anisotropy axis */
+  double nsk[3];     /* uniform DMI direction */
+  double happ[3];    /* Zeeman field */
+  int    aniso;      /* LLG_ANISO_* */
+  int    cell_axes;  /* 1: per-cell axes (LLG_Create cell_msk/cell_nsk) override msk/nsk */
+  int    window;     /* 1: neighbour m enters as w*m (i3 RHS) */
+  int    gate;       /* LLG_GATE_* */
+} llg_terms;
+
+typedef struct llg_config {
+  int    ng, ny;           /* cell grid (reference ng = NX_VAL/3, ny) */
+  int    exec;             /* LLG_EXEC_* */
+  int    boundary;         /* LLG_BC_* */
+  int    exch_self;        /* 1: exchange = che * (sum_nb m_nb - n_nb m) (Laplacian form) */
+  int    normalize;        /* LLG_NORM_* */
+  int    demag;            /* LLG_DEMAG_* */
+  int    demag_fp32;       /* 1: FP32 spectral demag */
+  int    pc_mode;          /* LLG_PC_* */
+  int    pad0;
+  double demag_strength;
+  double demag_thick;
+  double alpha;            /* damping */
+  double chg;              /* gyromagnetic prefactor */
+  llg_terms rhs, jv, pc;
+} llg_config;
+
+typedef struct llg_ctx llg_ctx;
+
+/* Build a context (= the reference's UserData + its device buffers).
+ * active   : ng*ny bytes, nonzero = magnetic cell; NULL = all active
+ * w        : ng*ny soft window (NULL = none)
+ * cell_msk : 3*ng*ny SoA per-cell anisotropy axes (NULL = uniform)
+ * cell_nsk : 3*ng*ny SoA per-cell DMI directions  (NULL = uniform)
+ * stream   : hipStream_t for the context's own work (NULL = default)     */
+llg_ctx* LLG_Create(const llg_config* cfg, const unsigned char* active,
+                    const double* w, const double* cell_msk,
+                    const double* cell_nsk, void* stream);
+void     LLG_Destroy(llg_ctx* ctx);
+const char* LLG_LastError(void);
+
+/* accessors */
+int      LLG_GetCounts(llg_ctx* ctx, int* ncell, int* n_active, int* n_inactive);
+int      LLG_GetMask(llg_ctx* ctx, double* h_ymsk3);            /* 3*ncell, 1.0/0.0 */
+int      LLG_GetActiveIds(llg_ctx* ctx, int* h_ids, int* h_inactive_ids);
+int      LLG_GetHdemag(llg_ctx* ctx, double* h_hd3);            /* last f()'s field */
+int      LLG_GetPinv(llg_ctx* ctx, double* h_p9);               /* 9*ncell SoA */
+DemagData* LLG_GetDemag(llg_ctx* ctx);
+void*    LLG_GetStream(llg_ctx* ctx);
+/* per-callback launch counters (diagnostics) */
+int      LLG_GetCallCounts(llg_ctx* ctx, long* nf, long* njv, long* npsetup, long* npsolve);
+
+/* Per-kernel timing with HIP events on the launch stream (diagnostics; no
+ * reference counterpart).  kernel_id: 1 RHS, 2 Jv, 3 block-Jacobi setup,
+ * 4 block-Jacobi solve, 5 demag row forward, 6 demag column, 7 demag row
+ * inverse, 8 whole demag.  capacity = launches to record. */
+int LLG_ProfileStart(int kernel_id, int capacity);
+int LLG_ProfileStop(double* total_ms, long* launches, long* dropped);
+
+/* CVODE plug-ins (user_data = llg_ctx*) */
+int LLG_Rhs(sunrealtype t, N_Vector y, N_Vector ydot, void* user_data);
+int JtvProduct(N_Vector v, N_Vector Jv, sunrealtype t, N_Vector y, N_Vector fy,
+               void* user_data, N_Vector tmp);
+int PrecondSetup(sunrealtype t, N_Vector y, N_Vector fy, sunbooleantype jok,
+                 sunbooleantype* jcurPtr, sunrealtype gamma, void* user_data);
+int PrecondSolve(sunrealtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z,
+                 sunrealtype gamma, sunrealtype delta, int lr, void* user_data);
+
+/* Block-Jacobi storage (reference-compatible handle) */
+typedef struct PrecondData PrecondData;
+PrecondData* Precond_Create(int ng, int ny, int ncell);
+void         Precond_Destroy(PrecondData* pd);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LLG_PLUGIN_H */

@@ -1,0 +1,111 @@
+/*
+ * nvector_hip.h -- HIP N_Vector for gfx950 (MI355X): the device vector the
+ * LLG plug-ins and the integrator operate on, plus the fused-ops layer.
+ *
+ * Replaces, on the reference's hot path:
+ *   - SUNDIALS NVECTOR_CUDA as used by every variant (N_VNew_Cuda at
+ *     src/irregular/2d_fft/2d_i4/2d_fft.cu:545-546, device/host pointer
+ *     accessors at :332-333 and :554-555, copies at :612-613, :703);
+ *   - the fused N_Vector layer FusedNVec_Init / FusedNVec_FreePool
+ *     (2d_i4/deferred_nvector.h:58,66; multi-dot override
+ *     2d_i4/deferred_nvector.cu:95-180; linear-combination override
+ *     src/2d/2d_heter/deferred_nvector.cu:135-151,209-236; clone
+ *     propagation 2d_i4/deferred_nvector.cu:182-202).
+ *
+ * Layout: one contiguous FP64 device array per vector (SoA [mx|my|mz] for
+ * the LLG state), an optional host mirror, and the HIP stream every op of
+ * the vector is launched on.  Reductions are deterministic: per-block
+ * partials in fixed order, finished by the last block of the same launch,
+ * result written to pinned host memory (one launch + one stream sync).
+ */
+#ifndef LLG_NVECTOR_HIP_H
+#define LLG_NVECTOR_HIP_H
+
+#include "sundials_compat.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Largest number of vectors a fused op handles in one launch; beyond it
+ * the op loops over groups (the reference falls back to per-vector ops
+ * above 16, 2d_i4/deferred_nvector.cu:151-155). */
+#define NVHIP_FUSED_MAX 16
+
+/* constructors -- stream==NULL selects the library's default stream */
+N_Vector N_VNew_Hip(sunindextype length, SUNContext sunctx);
+N_Vector N_VNewWithStream_Hip(sunindextype length, void* stream, SUNContext sunctx);
+N_Vector N_VNewEmpty_Hip(SUNContext sunctx);
+N_Vector N_VMake_Hip(sunindextype length, sunrealtype* d_data, void* stream,
+                     SUNContext sunctx);
+
+/* accessors (NVECTOR_CUDA equivalents) */
+sunrealtype* N_VGetDeviceArrayPointer_Hip(N_Vector v);
+sunrealtype* N_VGetHostArrayPointer_Hip(N_Vector v);
+void         N_VCopyToDevice_Hip(N_Vector v);
+void         N_VCopyFromDevice_Hip(N_Vector v);
+void*        N_VGetStream_Hip(N_Vector v);
+sunindextype N_VGetLength_Hip(N_Vector v);
+int          N_VEnableFusedOps_Hip(N_Vector v, sunbooleantype tf);
+/* host <-> device copies of raw arrays of the vector's length */
+int          N_VSetFromHost_Hip(N_Vector v, const sunrealtype* h_src);
+int          N_VGetToHost_Hip(N_Vector v, sunrealtype* h_dst);
+
+/* the default stream used when a constructor gets stream==NULL */
+void* NVHip_DefaultStream(void);
+/* counters of host synchronisations made by reductions (diagnostics) */
+long  NVHip_NumSyncs(void);
+
+/* standard ops */
+N_Vector_ID  N_VGetVectorID_Hip(N_Vector v);
+N_Vector     N_VClone_Hip(N_Vector w);
+N_Vector     N_VCloneEmpty_Hip(N_Vector w);
+void         N_VDestroy_Hip(N_Vector v);
+void         N_VSpace_Hip(N_Vector v, sunindextype* lrw, sunindextype* liw);
+void         N_VLinearSum_Hip(sunrealtype a, N_Vector x, sunrealtype b, N_Vector y, N_Vector z);
+void         N_VConst_Hip(sunrealtype c, N_Vector z);
+void         N_VProd_Hip(N_Vector x, N_Vector y, N_Vector z);
+void         N_VDiv_Hip(N_Vector x, N_Vector y, N_Vector z);
+void         N_VScale_Hip(sunrealtype c, N_Vector x, N_Vector z);
+void         N_VAbs_Hip(N_Vector x, N_Vector z);
+void         N_VInv_Hip(N_Vector x, N_Vector z);
+void         N_VAddConst_Hip(N_Vector x, sunrealtype b, N_Vector z);
+sunrealtype  N_VDotProd_Hip(N_Vector x, N_Vector y);
+sunrealtype  N_VMaxNorm_Hip(N_Vector x);
+sunrealtype  N_VWrmsNorm_Hip(N_Vector x, N_Vector w);
+sunrealtype  N_VWrmsNormMask_Hip(N_Vector x, N_Vector w, N_Vector id);
+sunrealtype  N_VMin_Hip(N_Vector x);
+sunrealtype  N_VWL2Norm_Hip(N_Vector x, N_Vector w);
+sunrealtype  N_VL1Norm_Hip(N_Vector x);
+void         N_VCompare_Hip(sunrealtype c, N_Vector x, N_Vector z);
+sunbooleantype N_VInvTest_Hip(N_Vector x, N_Vector z);
+sunbooleantype N_VConstrMask_Hip(N_Vector c, N_Vector x, N_Vector m);
+sunrealtype  N_VMinQuotient_Hip(N_Vector num, N_Vector denom);
+sunrealtype  N_VWSqrSumLocal_Hip(N_Vector x, N_Vector w);
+sunrealtype  N_VWSqrSumMaskLocal_Hip(N_Vector x, N_Vector w, N_Vector id);
+
+/* fused ops */
+SUNErrCode N_VLinearCombination_Hip(int nvec, sunrealtype* c, N_Vector* X, N_Vector z);
+SUNErrCode N_VScaleAddMulti_Hip(int nvec, sunrealtype* a, N_Vector x, N_Vector* Y, N_Vector* Z);
+SUNErrCode N_VDotProdMulti_Hip(int nvec, N_Vector x, N_Vector* Y, sunrealtype* dotprods);
+
+/* vector-array ops */
+SUNErrCode N_VLinearSumVectorArray_Hip(int nvec, sunrealtype a, N_Vector* X, sunrealtype b,
+                                       N_Vector* Y, N_Vector* Z);
+SUNErrCode N_VScaleVectorArray_Hip(int nvec, sunrealtype* c, N_Vector* X, N_Vector* Z);
+SUNErrCode N_VConstVectorArray_Hip(int nvec, sunrealtype c, N_Vector* Z);
+SUNErrCode N_VWrmsNormVectorArray_Hip(int nvec, N_Vector* X, N_Vector* W, sunrealtype* nrm);
+SUNErrCode N_VWrmsNormMaskVectorArray_Hip(int nvec, N_Vector* X, N_Vector* W, N_Vector id,
+                                          sunrealtype* nrm);
+
+/* Fused-ops layer with the reference's entry-point names.
+ * FusedNVec_Init installs the one-launch multi-dot and linear-combination
+ * overrides on v and makes every clone of v inherit them;
+ * FusedNVec_FreePool releases the reduction scratch pools. */
+void FusedNVec_Init(N_Vector v);
+void FusedNVec_FreePool(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LLG_NVECTOR_HIP_H */

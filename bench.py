@@ -108,6 +108,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gstype", type=int, default=0, help="0 auto, 1 modified, 2 classical GS")
+    ap.add_argument("--sync-mode", type=int, default=0,
+                    help="reduction wait: 0 stream sync, 1 poll the pinned result word")
+    ap.add_argument("--fp32-demag", action="store_true", help="FP32 spectral demag form")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -117,11 +120,13 @@ def main():
 
     import llg_amd as la
     la.lib()   # raises when the native library is missing
+    la.lib().NVHip_SetSyncMode(args.sync_mode)
 
     ng, ny = args.ng, args.ny
     padded = not args.periodic
     geom = la.Geometry.square_hole(ng, ny, halfside_frac=222.0 / 512.0)
     cfg = la.preset("square_hole", ng=ng, ny=ny, padded=padded)
+    cfg.demag_fp32 = 1 if args.fp32_demag else 0
     y0 = geom.ic_split_x(max_angle=0.05, seed=1)
     print(f"[bench] workload {ng}x{ny} cells, hole {int((geom.active == 0).sum())} cells, "
           f"{'padded' if padded else 'periodic'} demag; building context", file=sys.stderr, flush=True)
@@ -176,7 +181,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if not args.fp32_demag else "f64 (demag spectra f32)",
         "data": "synthetic (seeded two-domain state on the square-hole geometry)",
         "config": {"workload": f"square-hole {ng}x{ny} cells, "
                                f"{'zero-padded %dx%d' % (2 * ng, 2 * ny) if padded else 'periodic'} "
@@ -186,7 +191,9 @@ def main():
                    "nli_per_nni": round(out["nli"] / max(out["nni"], 1), 3),
                    "nst": out["nst"], "nfe": out["nfe"], "nni": out["nni"], "nli": out["nli"],
                    "nsetups": out["nsetups"], "netf": out["netf"], "ncfn": out["ncfn"],
-                   "host_syncs": out["nsyncs"], "setup_s": round(setup_s, 2)},
+                   "host_syncs": out["nsyncs"], "setup_s": round(setup_s, 2),
+                   "gs": {0: "auto", 1: "modified", 2: "classical"}[args.gstype],
+                   "sync_mode": args.sync_mode},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -142,12 +142,15 @@ struct Integrator {
   bool nls_jcur = false;
   bool initialized = false;
   long lrw = 0;
+  llgcv_fused_ops fused{};
 };
 
 inline double rpow(double b, double e) { return std::pow(b, e); }
 
 // ---------------------------------------------------------------- helpers
 int ewt_set(Integrator* m, N_Vector ycur, N_Vector w) {
+  if (m->fused.ewt && !m->atolmin0)
+    return m->fused.ewt(ycur, m->reltol, m->sabstol, m->itol == 2 ? m->vabstol : nullptr, w);
   N_Vector t = m->tempv;
   N_VAbs(ycur, t);
   if (m->itol == 2) {
@@ -252,12 +255,14 @@ void set_coeffs(Integrator* m) {
 void predict(Integrator* m) {
   m->tn += m->h;
   if (m->tstopset && (m->tn - m->tstop) * m->h > 0.0) m->tn = m->tstop;
+  if (m->fused.pascal && m->fused.pascal(m->q, m->zn, 1.0) == 0) return;
   for (int k = 1; k <= m->q; k++)
     for (int j = m->q; j >= k; j--) N_VLinearSum(1.0, m->zn[j - 1], 1.0, m->zn[j], m->zn[j - 1]);
 }
 
 void restore(Integrator* m, double saved_t) {
   m->tn = saved_t;
+  if (m->fused.pascal && m->fused.pascal(m->q, m->zn, -1.0) == 0) return;
   for (int k = 1; k <= m->q; k++)
     for (int j = m->q; j >= k; j--) N_VLinearSum(1.0, m->zn[j - 1], -1.0, m->zn[j], m->zn[j - 1]);
 }
@@ -654,11 +659,23 @@ int nls_lsolve(Integrator* m, N_Vector delta) {
 
 int nls_convtest(Integrator* m, N_Vector ycor, N_Vector del, double tol) {
   const int mit = m->nls_curiter;
-  const double dnorm = N_VWrmsNorm(del, m->ewt);
+  double dnorm, ynorm = -1.0;
+  if (mit > 0 && del->ops->nvwrmsnormvectorarray) {
+    // both norms in one reduction (the second is only used on convergence);
+    // each value equals the separate N_VWrmsNorm result
+    N_Vector X[2] = {del, ycor};
+    N_Vector W[2] = {m->ewt, m->ewt};
+    double nr[2];
+    N_VWrmsNormVectorArray(2, X, W, nr);
+    dnorm = nr[0];
+    ynorm = nr[1];
+  } else {
+    dnorm = N_VWrmsNorm(del, m->ewt);
+  }
   if (mit > 0) m->crate = std::max(kCrDown * m->crate, dnorm / m->delp);
   const double dcon = dnorm * std::min(1.0, m->crate) / tol;
   if (dcon <= 1.0) {
-    m->acnrm = (mit == 0) ? dnorm : N_VWrmsNorm(ycor, m->ewt);
+    m->acnrm = (mit == 0) ? dnorm : (ynorm >= 0.0 ? ynorm : N_VWrmsNorm(ycor, m->ewt));
     return 0;
   }
   if (mit >= 1 && dnorm > kRDiv * m->delp) return NLS_CONV_RECVR;
@@ -998,6 +1015,14 @@ void* LLGCVodeCreate(int lmm, SUNContext sunctx) {
   if (!m) return nullptr;
   m->sunctx = sunctx;
   return m;
+}
+
+int LLGCVodeSetFusedOps(void* mem, const llgcv_fused_ops* ops) {
+  if (!mem) return LLGCV_MEM_NULL;
+  Integrator* m = static_cast<Integrator*>(mem);
+  if (ops) m->fused = *ops;
+  else m->fused = llgcv_fused_ops{};
+  return LLGCV_SUCCESS;
 }
 
 int LLGCVodeInit(void* mem, CVRhsFn f, sunrealtype t0, N_Vector y0) {

@@ -45,111 +45,143 @@ void derr(const std::string& e) { g_derr = e; }
 constexpr int kMaxPasses = 16;
 constexpr int kXcd = 8;
 
+// complex types of the two pipeline precisions
+template <class T> struct V2T;
+template <> struct V2T<double> { using type = double2; };
+template <> struct V2T<float> { using type = float2; };
+template <class T> using V2 = typename V2T<T>::type;
+
+template <class T>
+__device__ __host__ inline V2<T> mk(T x, T y) {
+  V2<T> r;
+  r.x = x;
+  r.y = y;
+  return r;
+}
+
+template <class T>
 struct Plan1D {
   int L;
   int npass;
   int radix[kMaxPasses];
   int bpt[kMaxPasses];      // butterflies per thread per pass (<= 8)
-  const double2* tw;        // exp(-2 pi i q / L), q in [0, L)
+  const V2<T>* tw;          // exp(-2 pi i q / L), q in [0, L)
 };
 
-__device__ inline double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+template <class C2> __device__ inline C2 cmul(C2 a, C2 b) {
+  C2 r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
 }
-__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+template <class C2> __device__ inline C2 cadd(C2 a, C2 b) { C2 r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <class C2> __device__ inline C2 csub(C2 a, C2 b) { C2 r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+template <class C2> __device__ inline C2 cconj(C2 a) { C2 r; r.x = a.x; r.y = -a.y; return r; }
 // multiply by sign*i
-__device__ inline double2 cmuli(double2 a, double sign) { return make_double2(-sign * a.y, sign * a.x); }
+template <class C2, class T> __device__ inline C2 cmuli(C2 a, T sign) {
+  C2 r;
+  r.x = -sign * a.y;
+  r.y = sign * a.x;
+  return r;
+}
 
 // in-register DFT of length R, kernel exp(sign * 2 pi i j k / R)
-template <int R>
-__device__ inline void dft(double2 (&v)[R], double sign);
+template <int R, class T>
+struct Dft;
 
-template <>
-__device__ inline void dft<2>(double2 (&v)[2], double) {
-  const double2 a = v[0], b = v[1];
-  v[0] = cadd(a, b);
-  v[1] = csub(a, b);
-}
-template <>
-__device__ inline void dft<4>(double2 (&v)[4], double sign) {
-  const double2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
-  const double2 c = cadd(v[1], v[3]), d = cmuli(csub(v[1], v[3]), sign);
-  v[0] = cadd(a, c);
-  v[2] = csub(a, c);
-  v[1] = cadd(b, d);
-  v[3] = csub(b, d);
-}
-template <>
-__device__ inline void dft<8>(double2 (&v)[8], double sign) {
-  double2 e[4] = {v[0], v[2], v[4], v[6]};
-  double2 o[4] = {v[1], v[3], v[5], v[7]};
-  dft<4>(e, sign);
-  dft<4>(o, sign);
-  const double r = 0.70710678118654752440;
-  // W^k = exp(sign 2 pi i k / 8)
-  const double2 w1 = make_double2(r, sign * r);
-  const double2 w3 = make_double2(-r, sign * r);
-  const double2 t0 = o[0];
-  const double2 t1 = cmul(o[1], w1);
-  const double2 t2 = cmuli(o[2], sign);
-  const double2 t3 = cmul(o[3], w3);
-  v[0] = cadd(e[0], t0); v[4] = csub(e[0], t0);
-  v[1] = cadd(e[1], t1); v[5] = csub(e[1], t1);
-  v[2] = cadd(e[2], t2); v[6] = csub(e[2], t2);
-  v[3] = cadd(e[3], t3); v[7] = csub(e[3], t3);
-}
-template <>
-__device__ inline void dft<3>(double2 (&v)[3], double sign) {
-  const double c = -0.5, s = sign * 0.86602540378443864676;
-  const double2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-  const double2 a = make_double2(v[0].x + c * t1.x, v[0].y + c * t1.y);
-  const double2 b = make_double2(-s * t2.y, s * t2.x);   // i*s*t2
-  v[0] = cadd(v[0], t1);
-  v[1] = cadd(a, b);
-  v[2] = csub(a, b);
-}
-template <>
-__device__ inline void dft<5>(double2 (&v)[5], double sign) {
-  const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
-  const double s1 = sign * 0.95105651629515357212, s2 = sign * 0.58778525229247312917;
-  const double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-  const double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-  const double2 a1 = make_double2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
-  const double2 a2 = make_double2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
-  const double2 b1r = make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
-  const double2 b2r = make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
-  const double2 b1 = make_double2(-b1r.y, b1r.x);   // i * b1r
-  const double2 b2 = make_double2(-b2r.y, b2r.x);
-  v[0] = cadd(v[0], cadd(t1, t2));
-  v[1] = cadd(a1, b1);
-  v[4] = csub(a1, b1);
-  v[2] = cadd(a2, b2);
-  v[3] = csub(a2, b2);
-}
+template <class T>
+struct Dft<2, T> {
+  __device__ static inline void run(V2<T> (&v)[2], T) {
+    const V2<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+template <class T>
+struct Dft<4, T> {
+  __device__ static inline void run(V2<T> (&v)[4], T sign) {
+    const V2<T> a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+    const V2<T> c = cadd(v[1], v[3]), d = cmuli(csub(v[1], v[3]), sign);
+    v[0] = cadd(a, c);
+    v[2] = csub(a, c);
+    v[1] = cadd(b, d);
+    v[3] = csub(b, d);
+  }
+};
+template <class T>
+struct Dft<8, T> {
+  __device__ static inline void run(V2<T> (&v)[8], T sign) {
+    V2<T> e[4] = {v[0], v[2], v[4], v[6]};
+    V2<T> o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4, T>::run(e, sign);
+    Dft<4, T>::run(o, sign);
+    const T r = T(0.70710678118654752440);
+    const V2<T> w1 = mk<T>(r, sign * r);   // exp(sign 2 pi i / 8)
+    const V2<T> w3 = mk<T>(-r, sign * r);
+    const V2<T> t0 = o[0];
+    const V2<T> t1 = cmul(o[1], w1);
+    const V2<T> t2 = cmuli(o[2], sign);
+    const V2<T> t3 = cmul(o[3], w3);
+    v[0] = cadd(e[0], t0); v[4] = csub(e[0], t0);
+    v[1] = cadd(e[1], t1); v[5] = csub(e[1], t1);
+    v[2] = cadd(e[2], t2); v[6] = csub(e[2], t2);
+    v[3] = cadd(e[3], t3); v[7] = csub(e[3], t3);
+  }
+};
+template <class T>
+struct Dft<3, T> {
+  __device__ static inline void run(V2<T> (&v)[3], T sign) {
+    const T c = T(-0.5), s = sign * T(0.86602540378443864676);
+    const V2<T> t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+    const V2<T> a = mk<T>(v[0].x + c * t1.x, v[0].y + c * t1.y);
+    const V2<T> b = mk<T>(-s * t2.y, s * t2.x);   // i*s*t2
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(a, b);
+    v[2] = csub(a, b);
+  }
+};
+template <class T>
+struct Dft<5, T> {
+  __device__ static inline void run(V2<T> (&v)[5], T sign) {
+    const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+    const T s1 = sign * T(0.95105651629515357212), s2 = sign * T(0.58778525229247312917);
+    const V2<T> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    const V2<T> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    const V2<T> a1 = mk<T>(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+    const V2<T> a2 = mk<T>(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+    const V2<T> b1r = mk<T>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+    const V2<T> b2r = mk<T>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    const V2<T> b1 = mk<T>(-b1r.y, b1r.x);   // i * b1r
+    const V2<T> b2 = mk<T>(-b2r.y, b2r.x);
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[3] = csub(a2, b2);
+  }
+};
 
 // One Stockham pass of radix R over ncomp transforms of length L stored at
 // buf + comp*cstride (in place: all reads, barrier, all writes, barrier).
-template <int R, int B>
-__device__ void pass(double2* buf, int L, int Ns, double sign, const double2* __restrict__ tw,
-                     int ncomp, int cstride) {
+template <int R, int B, class T>
+__device__ void pass(V2<T>* buf, int L, int Ns, T sign, const V2<T>* __restrict__ tw, int ncomp,
+                     int cstride) {
   const int nb = L / R;
   const int total = nb * ncomp;
   const int tstep = L / (Ns * R);
-  double2 v[B][R];
+  V2<T> v[B][R];
 #pragma unroll
   for (int r = 0; r < B; r++) {
     const int b = threadIdx.x + r * blockDim.x;
     if (b < total) {
       const int comp = b / nb, j = b - comp * nb;
-      const double2* src = buf + comp * cstride;
+      const V2<T>* src = buf + comp * cstride;
       const int k = j % Ns;
 #pragma unroll
       for (int t = 0; t < R; t++) {
-        double2 x = src[j + t * nb];
+        V2<T> x = src[j + t * nb];
         if (t > 0 && k > 0) {
-          double2 w = tw[t * k * tstep];
+          V2<T> w = tw[t * k * tstep];
           if (sign > 0) w.y = -w.y;
           x = cmul(x, w);
         }
@@ -163,9 +195,9 @@ __device__ void pass(double2* buf, int L, int Ns, double sign, const double2* __
     const int b = threadIdx.x + r * blockDim.x;
     if (b < total) {
       const int comp = b / nb, j = b - comp * nb;
-      double2* dst = buf + comp * cstride;
+      V2<T>* dst = buf + comp * cstride;
       const int k = j % Ns;
-      dft<R>(v[r], sign);
+      Dft<R, T>::run(v[r], sign);
       const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
       for (int t = 0; t < R; t++) dst[base + t * Ns] = v[r][t];
@@ -178,47 +210,50 @@ __device__ void pass(double2* buf, int L, int Ns, double sign, const double2* __
 template <int R>
 struct MaxB { static constexpr int value = R == 8 ? 2 : (R == 5 ? 3 : (R == 2 ? 8 : 4)); };
 
-template <int R, int B>
-__device__ inline void pass_if(double2* buf, int L, int Ns, double sign, const double2* tw,
-                               int ncomp, int cstride) {
-  if constexpr (B <= MaxB<R>::value) pass<R, B>(buf, L, Ns, sign, tw, ncomp, cstride);
+template <int R, int B, class T>
+__device__ inline void pass_if(V2<T>* buf, int L, int Ns, T sign, const V2<T>* tw, int ncomp,
+                               int cstride) {
+  if constexpr (B <= MaxB<R>::value) pass<R, B, T>(buf, L, Ns, sign, tw, ncomp, cstride);
 }
 
-template <int R>
-__device__ void pass_b(double2* buf, int L, int Ns, double sign, const double2* tw, int ncomp,
+template <int R, class T>
+__device__ void pass_b(V2<T>* buf, int L, int Ns, T sign, const V2<T>* tw, int ncomp,
                        int cstride, int B) {
   switch (B) {
-    case 1: pass_if<R, 1>(buf, L, Ns, sign, tw, ncomp, cstride); break;
-    case 2: pass_if<R, 2>(buf, L, Ns, sign, tw, ncomp, cstride); break;
-    case 3: pass_if<R, 3>(buf, L, Ns, sign, tw, ncomp, cstride); break;
-    case 4: pass_if<R, 4>(buf, L, Ns, sign, tw, ncomp, cstride); break;
-    default: pass_if<R, 8>(buf, L, Ns, sign, tw, ncomp, cstride); break;
+    case 1: pass_if<R, 1, T>(buf, L, Ns, sign, tw, ncomp, cstride); break;
+    case 2: pass_if<R, 2, T>(buf, L, Ns, sign, tw, ncomp, cstride); break;
+    case 3: pass_if<R, 3, T>(buf, L, Ns, sign, tw, ncomp, cstride); break;
+    case 4: pass_if<R, 4, T>(buf, L, Ns, sign, tw, ncomp, cstride); break;
+    default: pass_if<R, 8, T>(buf, L, Ns, sign, tw, ncomp, cstride); break;
   }
 }
 
-__device__ void fft_lds(double2* buf, const Plan1D& p, double sign, int ncomp, int cstride) {
+template <class T>
+__device__ void fft_lds(V2<T>* buf, const Plan1D<T>& p, T sign, int ncomp, int cstride) {
   int Ns = 1;
   for (int i = 0; i < p.npass; i++) {
     const int R = p.radix[i], B = p.bpt[i];
     switch (R) {
-      case 8: pass_b<8>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
-      case 4: pass_b<4>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
-      case 2: pass_b<2>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
-      case 3: pass_b<3>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
-      default: pass_b<5>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
+      case 8: pass_b<8, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
+      case 4: pass_b<4, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
+      case 2: pass_b<2, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
+      case 3: pass_b<3, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
+      default: pass_b<5, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
     }
     Ns *= R;
   }
 }
 
 // ----------------------------------------------------------- row forward
-// grid (rows, ncomp); in = base + comp*comp_stride + row*nx_in
+// grid (rows, ncomp); in = base + comp*comp_stride + row*nx_in (FP64 input)
+template <class T>
 __global__ __launch_bounds__(1024) void k_row_fwd(const double* __restrict__ in,
-                                                 const double* __restrict__ w, long comp_stride,
-                                                 int nx_in, int rows, Plan1D plan,
-                                                 const double2* __restrict__ post,
-                                                 double2* __restrict__ spec) {
-  extern __shared__ double2 lds[];
+                                                  const double* __restrict__ w, long comp_stride,
+                                                  int nx_in, int rows, Plan1D<T> plan,
+                                                  const V2<T>* __restrict__ post,
+                                                  V2<T>* __restrict__ spec) {
+  extern __shared__ char lds_raw[];
+  V2<T>* lds = reinterpret_cast<V2<T>*>(lds_raw);
   const int row = blockIdx.x, comp = blockIdx.y;
   const int L = plan.L, Hx = L + 1;
   const double* src = in + comp * comp_stride + (long)row * nx_in;
@@ -228,45 +263,47 @@ __global__ __launch_bounds__(1024) void k_row_fwd(const double* __restrict__ in,
     double a = 0.0, b = 0.0;
     if (n0 < nx_in) a = wr ? wr[n0] * src[n0] : src[n0];
     if (n1 < nx_in) b = wr ? wr[n1] * src[n1] : src[n1];
-    lds[m] = make_double2(a, b);
+    lds[m] = mk<T>((T)a, (T)b);
   }
   __syncthreads();
-  fft_lds(lds, plan, -1.0, 1, L);
-  double2* out = spec + ((long)comp * rows + row) * Hx;
+  fft_lds<T>(lds, plan, T(-1), 1, L);
+  V2<T>* out = spec + ((long)comp * rows + row) * Hx;
   for (int k = threadIdx.x; k <= L; k += blockDim.x) {
-    const double2 zk = lds[k == L ? 0 : k];
-    const double2 zc = cconj(lds[k == 0 ? 0 : L - k]);
-    const double2 e = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
-    const double2 d = csub(zk, zc);
-    const double2 o = make_double2(0.5 * d.y, -0.5 * d.x);
+    const V2<T> zk = lds[k == L ? 0 : k];
+    const V2<T> zc = cconj(lds[k == 0 ? 0 : L - k]);
+    const V2<T> e = mk<T>(T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y + zc.y));
+    const V2<T> d = csub(zk, zc);
+    const V2<T> o = mk<T>(T(0.5) * d.y, T(-0.5) * d.x);
     out[k] = cadd(e, cmul(post[k], o));
   }
 }
 
 // ---------------------------------------------------------- row inverse
-__global__ __launch_bounds__(1024) void k_row_inv(const double2* __restrict__ spec, int rows,
-                                                 Plan1D plan, const double2* __restrict__ post,
-                                                 double* __restrict__ out, long comp_stride,
-                                                 int nx_out) {
-  extern __shared__ double2 lds[];
+template <class T>
+__global__ __launch_bounds__(1024) void k_row_inv(const V2<T>* __restrict__ spec, int rows,
+                                                  Plan1D<T> plan, const V2<T>* __restrict__ post,
+                                                  double* __restrict__ out, long comp_stride,
+                                                  int nx_out) {
+  extern __shared__ char lds_raw[];
+  V2<T>* lds = reinterpret_cast<V2<T>*>(lds_raw);
   const int row = blockIdx.x, comp = blockIdx.y;
   const int L = plan.L, Hx = L + 1;
-  const double2* in = spec + ((long)comp * rows + row) * Hx;
+  const V2<T>* in = spec + ((long)comp * rows + row) * Hx;
   for (int k = threadIdx.x; k < L; k += blockDim.x) {
-    const double2 xk = in[k];
-    const double2 xc = cconj(in[L - k]);
-    const double2 a = cadd(xk, xc);
-    const double2 b = cmul(csub(xk, xc), cconj(post[k]));
-    lds[k] = make_double2(a.x - b.y, a.y + b.x);   // a + i b
+    const V2<T> xk = in[k];
+    const V2<T> xc = cconj(in[L - k]);
+    const V2<T> a = cadd(xk, xc);
+    const V2<T> b = cmul(csub(xk, xc), cconj(post[k]));
+    lds[k] = mk<T>(a.x - b.y, a.y + b.x);   // a + i b
   }
   __syncthreads();
-  fft_lds(lds, plan, +1.0, 1, L);
+  fft_lds<T>(lds, plan, T(1), 1, L);
   double* dst = out + comp * comp_stride + (long)row * nx_out;
   for (int m = threadIdx.x; m < L; m += blockDim.x) {
     const int n0 = 2 * m;
-    const double2 z = lds[m];
-    if (n0 < nx_out) dst[n0] = z.x;
-    if (n0 + 1 < nx_out) dst[n0 + 1] = z.y;
+    const V2<T> z = lds[m];
+    if (n0 < nx_out) dst[n0] = (double)z.x;
+    if (n0 + 1 < nx_out) dst[n0 + 1] = (double)z.y;
   }
 }
 
@@ -279,42 +316,48 @@ __device__ inline int xcd_remap(int bid, int G) {
 // --------------------------------------------------------------- column
 // mode 0: fwd FFT, 3x3 tensor product, inverse FFT (3 components)
 // mode 1: fwd FFT only (1 component; tensor spectra at init)
-__global__ __launch_bounds__(1024) void k_col(double2* __restrict__ spec, int rows_in,
-                                             int rows_out, int Hx, Plan1D plan,
-                                             const double* __restrict__ K, int has_xy_im,
-                                             int mode) {
-  extern __shared__ double2 lds[];
+template <class T>
+__global__ __launch_bounds__(1024) void k_col(V2<T>* __restrict__ spec, int rows_in,
+                                              int rows_out, int Hx, Plan1D<T> plan,
+                                              const T* __restrict__ K, int has_xy_im, int mode) {
+  extern __shared__ char lds_raw[];
+  V2<T>* lds = reinterpret_cast<V2<T>*>(lds_raw);
   const int kx = xcd_remap(blockIdx.x, gridDim.x);
   const int Py = plan.L;
   const int ncomp = mode == 0 ? 3 : 1;
   for (int c = 0; c < ncomp; c++)
     for (int y = threadIdx.x; y < Py; y += blockDim.x)
-      lds[c * Py + y] = y < rows_in ? spec[((long)c * rows_in + y) * Hx + kx] : make_double2(0.0, 0.0);
+      lds[c * Py + y] = y < rows_in ? spec[((long)c * rows_in + y) * Hx + kx] : mk<T>(T(0), T(0));
   __syncthreads();
-  fft_lds(lds, plan, -1.0, ncomp, Py);
+  fft_lds<T>(lds, plan, T(-1), ncomp, Py);
   if (mode == 0) {
     const long P = (long)Hx * Py;
-    const double* kxx = K + (long)kx * Py;
-    const double* kyy = kxx + P;
-    const double* kzz = kyy + P;
-    const double* kxyr = kzz + P;
-    const double* kxyi = kxyr + P;
+    const T* kxx = K + (long)kx * Py;
+    const T* kyy = kxx + P;
+    const T* kzz = kyy + P;
+    const T* kxyr = kzz + P;
+    const T* kxyi = kxyr + P;
     for (int ky = threadIdx.x; ky < Py; ky += blockDim.x) {
-      const double2 mx = lds[ky], my = lds[Py + ky], mz = lds[2 * Py + ky];
-      const double a = kxx[ky], b = kyy[ky], cz = kzz[ky];
-      const double2 xy = make_double2(kxyr[ky], has_xy_im ? kxyi[ky] : 0.0);
-      const double2 hx = cadd(make_double2(a * mx.x, a * mx.y), cmul(xy, my));
-      const double2 hy = cadd(cmul(xy, mx), make_double2(b * my.x, b * my.y));
+      const V2<T> mx = lds[ky], my = lds[Py + ky], mz = lds[2 * Py + ky];
+      const T a = kxx[ky], b = kyy[ky], cz = kzz[ky];
+      const V2<T> xy = mk<T>(kxyr[ky], has_xy_im ? kxyi[ky] : T(0));
+      const V2<T> hx = cadd(mk<T>(a * mx.x, a * mx.y), cmul(xy, my));
+      const V2<T> hy = cadd(cmul(xy, mx), mk<T>(b * my.x, b * my.y));
       lds[ky] = hx;
       lds[Py + ky] = hy;
-      lds[2 * Py + ky] = make_double2(cz * mz.x, cz * mz.y);
+      lds[2 * Py + ky] = mk<T>(cz * mz.x, cz * mz.y);
     }
     __syncthreads();
-    fft_lds(lds, plan, +1.0, 3, Py);
+    fft_lds<T>(lds, plan, T(1), 3, Py);
   }
   for (int c = 0; c < ncomp; c++)
     for (int y = threadIdx.x; y < rows_out; y += blockDim.x)
       spec[((long)c * rows_out + y) * Hx + kx] = lds[c * Py + y];
+}
+
+__global__ void k_to_float(const double* __restrict__ a, float* __restrict__ b, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
 }
 
 // ------------------------------------------------------------ tensor
@@ -426,15 +469,17 @@ bool factorize(int L, std::vector<int>& rad) {
   return n == 1 && (int)rad.size() <= kMaxPasses;
 }
 
+template <class T>
 struct HostPlan {
-  Plan1D p{};
+  Plan1D<T> p{};
   int block = 64;
-  double2* d_tw = nullptr;
+  V2<T>* d_tw = nullptr;
 };
 
 int max_b(int r) { return r == 8 ? 2 : (r == 5 ? 3 : (r == 2 ? 8 : 4)); }
 
-bool make_plan(int L, int ncomp, HostPlan& hp) {
+template <class T>
+bool make_plan(int L, int ncomp, HostPlan<T>& hp) {
   std::vector<int> rad;
   if (L < 1 || !factorize(L, rad)) {
     derr("unsupported FFT length " + std::to_string(L) + " (need 2^a 3^b 5^c)");
@@ -462,18 +507,19 @@ bool make_plan(int L, int ncomp, HostPlan& hp) {
     if (b > 4) b = 8;
     hp.p.bpt[i] = b;
   }
-  std::vector<double2> tw(L);
+  std::vector<V2<T>> tw(L);
   for (int q = 0; q < L; q++) {
     const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)q / (long double)L;
-    tw[q] = make_double2((double)cosl(ang), (double)sinl(ang));
+    tw[q] = mk<T>((T)cosl(ang), (T)sinl(ang));
   }
-  if (hipMalloc(&hp.d_tw, sizeof(double2) * L) != hipSuccess) { derr("hipMalloc twiddles"); return false; }
-  hipMemcpy(hp.d_tw, tw.data(), sizeof(double2) * L, hipMemcpyHostToDevice);
+  if (hipMalloc(&hp.d_tw, sizeof(V2<T>) * L) != hipSuccess) { derr("hipMalloc twiddles"); return false; }
+  hipMemcpy(hp.d_tw, tw.data(), sizeof(V2<T>) * L, hipMemcpyHostToDevice);
   hp.p.tw = hp.d_tw;
   return true;
 }
 
-void free_plan(HostPlan& hp) {
+template <class T>
+void free_plan(HostPlan<T>& hp) {
   if (hp.d_tw) hipFree(hp.d_tw);
   hp.d_tw = nullptr;
 }
@@ -539,10 +585,13 @@ struct DemagData {
   double strength;
   double n0[3];
   hipStream_t stream;
-  HostPlan row, col, col1;
+  HostPlan<double> row, col, col1;
+  HostPlan<float> row32, col32;
   double2* d_post;    // exp(-2 pi i k / Px), k in [0, Px/2]
-  double2* d_spec;    // 3 * ny * hx complex
+  float2* d_post32;
+  double2* d_spec;    // 3 * ny * hx complex (FP32 pipeline reuses it as float2)
   double* d_K;        // 5 planes [hx][py]
+  float* d_K32;
   int has_xy_im;
   int fp32;
 };
@@ -563,9 +612,13 @@ void destroy(DemagData* d) {
   free_plan(d->row);
   free_plan(d->col);
   free_plan(d->col1);
+  free_plan(d->row32);
+  free_plan(d->col32);
   hipFree(d->d_post);
+  hipFree(d->d_post32);
   hipFree(d->d_spec);
   hipFree(d->d_K);
+  hipFree(d->d_K32);
   delete d;
 }
 
@@ -625,17 +678,16 @@ DemagData* init_common(int nx, int ny, double thick, double strength, int padded
                      qy, Px, Py, padded, dKr);
   // spectra of the 4 real planes: row R2C + column forward FFT
   const double scale = strength / ((double)Px * (double)Py);
-  const int plane_of[4] = {0, 1, 2, 3};   // xx, yy, zz, xy(re/im)
   for (int c = 0; c < 4; c++) {
-    hipLaunchKernelGGL(k_row_fwd, dim3(Py, 1), dim3(d->row.block),
+    hipLaunchKernelGGL(k_row_fwd<double>, dim3(Py, 1), dim3(d->row.block),
                        sizeof(double2) * d->row.p.L, d->stream,
                        (const double*)(dKr + (size_t)c * Px * Py), (const double*)nullptr,
                        (long)0, Px, Py, d->row.p, (const double2*)d->d_post, d->d_spec);
-    hipLaunchKernelGGL(k_col, dim3(Hx), dim3(d->col1.block), sizeof(double2) * Py, d->stream,
-                       d->d_spec, Py, Py, Hx, d->col1.p, (const double*)nullptr, 0, 1);
+    hipLaunchKernelGGL(k_col<double>, dim3(Hx), dim3(d->col1.block), sizeof(double2) * Py,
+                       d->stream, d->d_spec, Py, Py, Hx, d->col1.p, (const double*)nullptr, 0, 1);
     const long P = (long)Hx * Py;
     hipLaunchKernelGGL(k_extract, dim3((Hx + 255) / 256, Py), dim3(256), 0, d->stream,
-                       (const double2*)d->d_spec, Py, Hx, scale, 0, d->d_K + plane_of[c] * P);
+                       (const double2*)d->d_spec, Py, Hx, scale, 0, d->d_K + c * P);
     if (c == 3 && d->has_xy_im)
       hipLaunchKernelGGL(k_extract, dim3((Hx + 255) / 256, Py), dim3(256), 0, d->stream,
                          (const double2*)d->d_spec, Py, Hx, scale, 1, d->d_K + 4 * P);
@@ -647,25 +699,35 @@ DemagData* init_common(int nx, int ny, double thick, double strength, int padded
   return d;
 }
 
-void apply(DemagData* d, const double* y, const double* w, double* h) {
+template <class T>
+void apply_t(DemagData* d, const HostPlan<T>& row, const HostPlan<T>& col, const V2<T>* post,
+             const T* K, const double* y, const double* w, double* h) {
   const int nx = d->nx, ny = d->ny, Hx = d->hx, Py = d->py;
   const long N = (long)nx * ny;
   hipStream_t s = d->stream;
+  V2<T>* spec = reinterpret_cast<V2<T>*>(d->d_spec);
   kprof::begin(kprof::DEMAG_ALL, s);
   kprof::begin(kprof::DEMAG_ROW_FWD, s);
-  hipLaunchKernelGGL(k_row_fwd, dim3(ny, 3), dim3(d->row.block), sizeof(double2) * d->row.p.L, s,
-                     y, w, N, nx, ny, d->row.p, (const double2*)d->d_post, d->d_spec);
+  hipLaunchKernelGGL(k_row_fwd<T>, dim3(ny, 3), dim3(row.block), sizeof(V2<T>) * row.p.L, s, y,
+                     w, N, nx, ny, row.p, post, spec);
   kprof::end(kprof::DEMAG_ROW_FWD, s);
   kprof::begin(kprof::DEMAG_COL, s);
-  hipLaunchKernelGGL(k_col, dim3(Hx), dim3(d->col.block), sizeof(double2) * 3 * Py, s,
-                     d->d_spec, ny, ny, Hx, d->col.p, (const double*)d->d_K, d->has_xy_im, 0);
+  hipLaunchKernelGGL(k_col<T>, dim3(Hx), dim3(col.block), sizeof(V2<T>) * 3 * Py, s, spec, ny, ny,
+                     Hx, col.p, K, d->has_xy_im, 0);
   kprof::end(kprof::DEMAG_COL, s);
   kprof::begin(kprof::DEMAG_ROW_INV, s);
-  hipLaunchKernelGGL(k_row_inv, dim3(ny, 3), dim3(d->row.block), sizeof(double2) * d->row.p.L, s,
-                     (const double2*)d->d_spec, ny, d->row.p, (const double2*)d->d_post, h, N, nx);
+  hipLaunchKernelGGL(k_row_inv<T>, dim3(ny, 3), dim3(row.block), sizeof(V2<T>) * row.p.L, s,
+                     (const V2<T>*)spec, ny, row.p, post, h, N, nx);
   kprof::end(kprof::DEMAG_ROW_INV, s);
   kprof::end(kprof::DEMAG_ALL, s);
   launch_ok("Demag_Apply");
+}
+
+void apply(DemagData* d, const double* y, const double* w, double* h) {
+  if (d->fp32)
+    apply_t<float>(d, d->row32, d->col32, d->d_post32, d->d_K32, y, w, h);
+  else
+    apply_t<double>(d, d->row, d->col, d->d_post, d->d_K, y, w, h);
 }
 
 }  // namespace
@@ -715,8 +777,29 @@ void Demag_Destroy(DemagData* d) { destroy(d); }
 
 int Demag_SetPrecision(DemagData* d, int fp32) {
   if (!d) return -1;
-  if (fp32) { derr("FP32 spectral demag not available in this build"); return -1; }
-  d->fp32 = 0;
+  g_derr.clear();
+  if (!fp32) { d->fp32 = 0; return 0; }
+  if (!d->d_K32) {
+    const int Hx = d->hx, Py = d->py;
+    const size_t P5 = 5 * (size_t)Hx * Py;
+    if (!make_plan(d->px / 2, 1, d->row32) || !make_plan(d->py, 3, d->col32)) return -1;
+    std::vector<float2> post(Hx);
+    for (int k = 0; k < Hx; k++) {
+      const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)d->px;
+      post[k] = make_float2((float)cosl(ang), (float)sinl(ang));
+    }
+    if (hipMalloc(&d->d_post32, sizeof(float2) * Hx) != hipSuccess ||
+        hipMalloc(&d->d_K32, sizeof(float) * P5) != hipSuccess) {
+      derr("Demag_SetPrecision: allocation failed");
+      return -1;
+    }
+    hipMemcpy(d->d_post32, post.data(), sizeof(float2) * Hx, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_to_float, dim3(1024), dim3(256), 0, d->stream, (const double*)d->d_K,
+                       d->d_K32, (long)P5);
+    hipStreamSynchronize(d->stream);
+    if (!launch_ok("Demag_SetPrecision")) return -1;
+  }
+  d->fp32 = 1;
   return 0;
 }
 

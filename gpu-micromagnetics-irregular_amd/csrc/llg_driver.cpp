@@ -78,6 +78,12 @@ void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_hos
     return nullptr;
   }
   if (opts->use_prec) LLGCVodeSetPreconditioner(r->cv, PrecondSetup, PrecondSolve);
+  if (opts->fused) {
+    llgcv_fused_ops fo{};
+    fo.pascal = NVHip_Pascal;
+    fo.ewt = NVHip_EwtSet;
+    LLGCVodeSetFusedOps(r->cv, &fo);
+  }
   r->t = opts->t0;
   return r;
 }

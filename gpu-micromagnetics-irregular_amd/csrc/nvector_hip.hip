@@ -16,6 +16,7 @@
 // accumulation (deferred_nvector.cu:127-129).
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -49,8 +50,9 @@ inline hipStream_t S(N_Vector v) { return C(v)->stream; }
 struct RedWs {
   double* partials = nullptr;   // kMaxRedBlocks * kMaxK
   unsigned* ticket = nullptr;   // zero between launches
-  double* host = nullptr;       // pinned, kMaxK
+  double* host = nullptr;       // pinned, kMaxK results + 1 sequence slot
   double* host_dev = nullptr;   // device alias of host
+  unsigned seq = 0;             // last sequence number issued on this stream
 };
 
 std::mutex g_mu;
@@ -58,6 +60,7 @@ std::unordered_map<hipStream_t, RedWs*> g_ws;
 hipStream_t g_default_stream = nullptr;
 bool g_default_stream_made = false;
 long g_syncs = 0;
+int g_sync_mode = 0;   // 0: hipStreamSynchronize, 1: spin on the pinned sequence word
 
 RedWs* workspace(hipStream_t s) {
   std::lock_guard<std::mutex> lk(g_mu);
@@ -66,12 +69,13 @@ RedWs* workspace(hipStream_t s) {
   RedWs* w = new RedWs();
   if (hipMalloc(&w->partials, sizeof(double) * kMaxRedBlocks * kMaxK) != hipSuccess ||
       hipMalloc(&w->ticket, sizeof(unsigned)) != hipSuccess ||
-      hipHostMalloc(&w->host, sizeof(double) * kMaxK, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostMalloc(&w->host, sizeof(double) * (kMaxK + 1), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
       hipHostGetDevicePointer((void**)&w->host_dev, w->host, 0) != hipSuccess) {
     fprintf(stderr, "[nvector_hip] reduction workspace allocation failed\n");
     abort();
   }
   hipMemset(w->ticket, 0, sizeof(unsigned));
+  std::memset(w->host, 0, sizeof(double) * (kMaxK + 1));
   hipDeviceSynchronize();
   g_ws[s] = w;
   return w;
@@ -166,6 +170,33 @@ __global__ void k_constarray(int nv, double c, OutPack Z, sunindextype n) {
   }
 }
 
+// Nordsieck predict (sign +1) / restore (-1): for k=1..q, j=q..k:
+// z[j-1] += sign*z[j]; z[q] unchanged.  Same per-element operations as the
+// q(q+1)/2 N_VLinearSum calls, one pass over memory.
+template <int Q>
+__global__ void k_pascal(OutPack Z, double sign, sunindextype n) {
+  GRID_STRIDE(i, n) {
+    double z[Q + 1];
+#pragma unroll
+    for (int j = 0; j <= Q; j++) z[j] = Z.p[j][i];
+#pragma unroll
+    for (int k = 1; k <= Q; k++)
+#pragma unroll
+      for (int j = Q; j >= k; j--) z[j - 1] = 1.0 * z[j - 1] + sign * z[j];
+#pragma unroll
+    for (int j = 0; j < Q; j++) Z.p[j][i] = z[j];
+  }
+}
+
+__global__ void k_ewt(const double* __restrict__ y, double rtol, double atol,
+                      const double* __restrict__ av, double* __restrict__ w, sunindextype n) {
+  GRID_STRIDE(i, n) {
+    const double t = fabs(y[i]);
+    const double d = av ? rtol * t + 1.0 * av[i] : rtol * t + atol;
+    w[i] = 1.0 / d;
+  }
+}
+
 // ------------------------------------------------------------ reductions
 enum RedKind { RED_SUM = 0, RED_MAX = 1, RED_MIN = 2 };
 
@@ -240,7 +271,7 @@ __device__ void block_reduce(int kind, double (&v)[K], double (*sm)[kWaves]) {
 // Final pass run by the last workgroup: partials[b*K + k] for b < nblk.
 template <int K>
 __device__ void finish(int kind, int nblk, const double* partials, double* out,
-                       double (*sm)[kWaves]) {
+                       double (*sm)[kWaves], unsigned seq) {
   double v[K];
 #pragma unroll
   for (int k = 0; k < K; k++) v[k] = identity(kind);
@@ -258,13 +289,17 @@ __device__ void finish(int kind, int nblk, const double* partials, double* out,
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; k++) out[k] = v[k];
+    // results first, then the sequence word the host may spin on
+    __threadfence_system();
+    __hip_atomic_store(reinterpret_cast<unsigned*>(out + kMaxK), seq, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
 template <class F>
 __global__ __launch_bounds__(kBlock) void k_reduce1(int kind, F f, sunindextype n,
                                                     double* partials, unsigned* ticket,
-                                                    double* out) {
+                                                    double* out, unsigned seq) {
   __shared__ double sm[1][kWaves];
   __shared__ bool last;
   double v[1] = {identity(kind)};
@@ -279,7 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce1(int kind, F f, sunindextype 
   __syncthreads();
   if (last) {
     __threadfence();
-    finish<1>(kind, gridDim.x, partials, out, sm);
+    finish<1>(kind, gridDim.x, partials, out, sm, seq);
     if (threadIdx.x == 0) *ticket = 0u;
   }
 }
@@ -288,7 +323,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce1(int kind, F f, sunindextype 
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_dotmulti(const double* __restrict__ x, PtrPack Y,
                                                      sunindextype n, double* partials,
-                                                     unsigned* ticket, double* out) {
+                                                     unsigned* ticket, double* out,
+                                                     unsigned seq) {
   __shared__ double sm[K][kWaves];
   __shared__ bool last;
   double v[K];
@@ -310,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void k_dotmulti(const double* __restrict__ 
   __syncthreads();
   if (last) {
     __threadfence();
-    finish<K>(RED_SUM, gridDim.x, partials, out, sm);
+    finish<K>(RED_SUM, gridDim.x, partials, out, sm, seq);
     if (threadIdx.x == 0) *ticket = 0u;
   }
 }
@@ -318,7 +354,8 @@ __global__ __launch_bounds__(kBlock) void k_dotmulti(const double* __restrict__ 
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_wsqarray(PtrPack X, PtrPack W, const double* id,
                                                      sunindextype n, double* partials,
-                                                     unsigned* ticket, double* out) {
+                                                     unsigned* ticket, double* out,
+                                                     unsigned seq) {
   __shared__ double sm[K][kWaves];
   __shared__ bool last;
   double v[K];
@@ -343,9 +380,27 @@ __global__ __launch_bounds__(kBlock) void k_wsqarray(PtrPack X, PtrPack W, const
   __syncthreads();
   if (last) {
     __threadfence();
-    finish<K>(RED_SUM, gridDim.x, partials, out, sm);
+    finish<K>(RED_SUM, gridDim.x, partials, out, sm, seq);
     if (threadIdx.x == 0) *ticket = 0u;
   }
+}
+
+// Wait for the reduction tagged `seq` on stream s.  Spin mode polls the
+// pinned sequence word (bounded: after 2 s it falls back to a stream sync,
+// which also surfaces any device error); otherwise hipStreamSynchronize.
+void wait_result(hipStream_t s, RedWs* w, unsigned seq) {
+  g_syncs++;
+  if (g_sync_mode == 1) {
+    const unsigned* word = reinterpret_cast<const unsigned*>(w->host + kMaxK);
+    auto t0 = std::chrono::steady_clock::now();
+    for (unsigned it = 0;; it++) {
+      if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) return;
+      if ((it & 4095u) == 4095u &&
+          std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2))
+        break;
+    }
+  }
+  hipStreamSynchronize(s);
 }
 
 template <class F>
@@ -354,11 +409,11 @@ double reduce1(N_Vector v, int kind, F f) {
   hipStream_t s = S(v);
   RedWs* w = workspace(s);
   const int g = red_grid(n);
+  const unsigned seq = ++w->seq;
   hipLaunchKernelGGL(k_reduce1<F>, dim3(g), dim3(kBlock), 0, s, kind, f, n, w->partials,
-                     w->ticket, w->host_dev);
+                     w->ticket, w->host_dev, seq);
   check_launch("reduce");
-  hipStreamSynchronize(s);
-  g_syncs++;
+  wait_result(s, w, seq);
   return w->host[0];
 }
 
@@ -366,7 +421,7 @@ template <int K>
 void launch_dotmulti(hipStream_t s, const double* x, const PtrPack& Y, sunindextype n,
                      RedWs* w) {
   hipLaunchKernelGGL(k_dotmulti<K>, dim3(red_grid(n)), dim3(kBlock), 0, s, x, Y, n,
-                     w->partials, w->ticket, w->host_dev);
+                     w->partials, w->ticket, w->host_dev, w->seq);
 }
 
 void dotmulti_dispatch(int K, hipStream_t s, const double* x, const PtrPack& Y, sunindextype n,
@@ -395,7 +450,7 @@ template <int K>
 void launch_wsq(hipStream_t s, const PtrPack& X, const PtrPack& W, const double* id,
                 sunindextype n, RedWs* w) {
   hipLaunchKernelGGL(k_wsqarray<K>, dim3(red_grid(n)), dim3(kBlock), 0, s, X, W, id, n,
-                     w->partials, w->ticket, w->host_dev);
+                     w->partials, w->ticket, w->host_dev, w->seq);
 }
 void wsq_dispatch(int K, hipStream_t s, const PtrPack& X, const PtrPack& W, const double* id,
                   sunindextype n, RedWs* w) {
@@ -452,6 +507,12 @@ void* NVHip_DefaultStream(void) {
 }
 
 long NVHip_NumSyncs(void) { return g_syncs; }
+
+int NVHip_SetSyncMode(int mode) {
+  if (mode != 0 && mode != 1) return -1;
+  g_sync_mode = mode;
+  return 0;
+}
 
 N_Vector N_VNewWithStream_Hip(sunindextype length, void* stream, SUNContext sunctx) {
   hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)NVHip_DefaultStream();
@@ -637,10 +698,10 @@ SUNErrCode N_VDotProdMulti_Hip(int nvec, N_Vector x, N_Vector* Y, sunrealtype* d
     const int k = (nvec - base < kMaxK) ? nvec - base : kMaxK;
     PtrPack yp{};
     for (int j = 0; j < k; j++) yp.p[j] = D(Y[base + j]);
+    ++w->seq;
     dotmulti_dispatch(k, s, D(x), yp, LEN(x), w);
     check_launch("dotmulti");
-    hipStreamSynchronize(s);
-    g_syncs++;
+    wait_result(s, w, w->seq);
     for (int j = 0; j < k; j++) dotprods[base + j] = w->host[j];
   }
   return SUN_SUCCESS;
@@ -700,10 +761,10 @@ static SUNErrCode wsq_array(int nvec, N_Vector* X, N_Vector* W, N_Vector id, sun
       xp.p[j] = D(X[base + j]);
       wp.p[j] = D(W[base + j]);
     }
+    ++w->seq;
     wsq_dispatch(k, s, xp, wp, id ? D(id) : nullptr, n, w);
     check_launch("wsqarray");
-    hipStreamSynchronize(s);
-    g_syncs++;
+    wait_result(s, w, w->seq);
     for (int j = 0; j < k; j++) nrm[base + j] = std::sqrt(w->host[j] / (double)n);
   }
   return SUN_SUCCESS;
@@ -741,6 +802,31 @@ int N_VEnableFusedOps_Hip(N_Vector v, sunbooleantype tf) {
   o->nvconstvectorarray = tf ? N_VConstVectorArray_Hip : nullptr;
   o->nvwrmsnormvectorarray = tf ? N_VWrmsNormVectorArray_Hip : nullptr;
   o->nvwrmsnormmaskvectorarray = tf ? N_VWrmsNormMaskVectorArray_Hip : nullptr;
+  return 0;
+}
+
+int NVHip_Pascal(int q, N_Vector* zn, sunrealtype sign) {
+  if (q < 1 || q > 5) return -1;
+  for (int j = 0; j <= q; j++)
+    if (!zn[j] || zn[j]->ops->nvgetvectorid != N_VGetVectorID_Hip) return -1;
+  OutPack zp{};
+  for (int j = 0; j <= q; j++) zp.p[j] = D(zn[j]);
+  const sunindextype n = LEN(zn[0]);
+  hipStream_t s = S(zn[0]);
+  switch (q) {
+    case 1: hipLaunchKernelGGL(k_pascal<1>, dim3(ew_grid(n)), dim3(kBlock), 0, s, zp, sign, n); break;
+    case 2: hipLaunchKernelGGL(k_pascal<2>, dim3(ew_grid(n)), dim3(kBlock), 0, s, zp, sign, n); break;
+    case 3: hipLaunchKernelGGL(k_pascal<3>, dim3(ew_grid(n)), dim3(kBlock), 0, s, zp, sign, n); break;
+    case 4: hipLaunchKernelGGL(k_pascal<4>, dim3(ew_grid(n)), dim3(kBlock), 0, s, zp, sign, n); break;
+    default: hipLaunchKernelGGL(k_pascal<5>, dim3(ew_grid(n)), dim3(kBlock), 0, s, zp, sign, n); break;
+  }
+  check_launch("pascal");
+  return 0;
+}
+
+int NVHip_EwtSet(N_Vector y, sunrealtype rtol, sunrealtype atol, N_Vector abstol, N_Vector w) {
+  if (y->ops->nvgetvectorid != N_VGetVectorID_Hip) return -1;
+  EW_LAUNCH(k_ewt, w, D(y), rtol, atol, abstol ? (const double*)D(abstol) : nullptr, D(w), LEN(w));
   return 0;
 }
 

@@ -25,6 +25,7 @@ SOURCES = [
     "cvode_bdf.cpp",
     "geometry.cpp",
     "llg_driver.cpp",
+    "output.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

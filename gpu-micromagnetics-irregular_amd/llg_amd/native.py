@@ -103,6 +103,7 @@ SIGNATURES = {
     "N_VGetToHost_Hip": (C.c_int, [VP, DP]),
     "NVHip_DefaultStream": (VP, []),
     "NVHip_NumSyncs": (C.c_long, []),
+    "NVHip_SetSyncMode": (C.c_int, [C.c_int]),
     "N_VGetVectorID_Hip": (C.c_int, [VP]),
     "N_VClone_Hip": (VP, [VP]),
     "N_VCloneEmpty_Hip": (VP, [VP]),
@@ -138,6 +139,8 @@ SIGNATURES = {
     "N_VConstVectorArray_Hip": (C.c_int, [C.c_int, C.c_double, C.POINTER(VP)]),
     "N_VWrmsNormVectorArray_Hip": (C.c_int, [C.c_int, C.POINTER(VP), C.POINTER(VP), DP]),
     "N_VWrmsNormMaskVectorArray_Hip": (C.c_int, [C.c_int, C.POINTER(VP), C.POINTER(VP), VP, DP]),
+    "NVHip_Pascal": (C.c_int, [C.c_int, C.POINTER(VP), C.c_double]),
+    "NVHip_EwtSet": (C.c_int, [VP, C.c_double, C.c_double, VP, VP]),
     "FusedNVec_Init": (None, [VP]),
     "FusedNVec_FreePool": (None, []),
     # demag_fft.h
@@ -189,8 +192,16 @@ SIGNATURES = {
     "LLG_ICTiltedZ": (C.c_int, [C.c_int, C.c_int, UCP, C.c_double, C.c_uint64, DP]),
     "LLG_ICSplitX": (C.c_int, [C.c_int, C.c_int, UCP, C.c_double, C.c_uint64, DP]),
     "LLG_ICRandomSphere": (C.c_int, [C.c_int, C.c_int, UCP, C.c_uint64, DP]),
+    # llg_output.h
+    "LLG_FrameWriterOpen": (VP, [C.c_char_p]),
+    "LLG_FrameWriterWrite": (C.c_int, [VP, C.c_double, C.c_int, C.c_int, C.c_int, DP, DP, C.c_int]),
+    "LLG_FrameWriterClose": (C.c_int, [VP]),
+    "LLG_ShouldWriteFrame": (C.c_int, [C.c_long, C.c_double, C.c_double, C.c_int, C.c_int]),
+    "LLG_WriteMaskFile": (C.c_int, [C.c_char_p, C.c_int, C.c_int, DP]),
+    "LLG_WriteGrainIdFile": (C.c_int, [C.c_char_p, C.c_int, C.c_int, IP]),
     # llg_cvode.h
     "LLGCVodeCreate": (VP, [C.c_int, VP]),
+    "LLGCVodeSetFusedOps": (C.c_int, [VP, VP]),
     "LLGCVodeInit": (C.c_int, [VP, VP, C.c_double, VP]),
     "LLGCVodeReInit": (C.c_int, [VP, C.c_double, VP]),
     "LLGCVodeSetUserData": (C.c_int, [VP, VP]),

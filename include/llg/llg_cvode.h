@@ -55,7 +55,19 @@ typedef struct llgcv_stats {
   long nsyncs;            /* host synchronisations made by vector reductions */
 } llgcv_stats;
 
+/* Optional one-pass kernels for integrator-internal vector sequences.  Each
+ * must produce exactly the values of the standard-op sequence it replaces:
+ *   pascal(q, zn, sign): for k=1..q, j=q..k: zn[j-1] += sign*zn[j]
+ *                        (the Nordsieck predict, sign=+1, and restore, -1)
+ *   ewt(y, rtol, atol, abstol, w): w = 1/(rtol*|y| + atol) with the vector
+ *                        abstol when non-NULL, else the scalar atol   */
+typedef struct llgcv_fused_ops {
+  int (*pascal)(int q, N_Vector* zn, sunrealtype sign);
+  int (*ewt)(N_Vector y, sunrealtype rtol, sunrealtype atol, N_Vector abstol, N_Vector w);
+} llgcv_fused_ops;
+
 void* LLGCVodeCreate(int lmm, SUNContext sunctx);
+int   LLGCVodeSetFusedOps(void* mem, const llgcv_fused_ops* ops);
 int   LLGCVodeInit(void* mem, CVRhsFn f, sunrealtype t0, N_Vector y0);
 int   LLGCVodeReInit(void* mem, sunrealtype t0, N_Vector y0);
 int   LLGCVodeSetUserData(void* mem, void* user_data);

@@ -55,6 +55,9 @@ int          N_VGetToHost_Hip(N_Vector v, sunrealtype* h_dst);
 void* NVHip_DefaultStream(void);
 /* counters of host synchronisations made by reductions (diagnostics) */
 long  NVHip_NumSyncs(void);
+/* how the host waits for a reduction result: 0 = hipStreamSynchronize,
+ * 1 = poll the pinned result's sequence word (bounded, falls back to 0) */
+int   NVHip_SetSyncMode(int mode);
 
 /* standard ops */
 N_Vector_ID  N_VGetVectorID_Hip(N_Vector v);
@@ -97,6 +100,13 @@ SUNErrCode N_VConstVectorArray_Hip(int nvec, sunrealtype c, N_Vector* Z);
 SUNErrCode N_VWrmsNormVectorArray_Hip(int nvec, N_Vector* X, N_Vector* W, sunrealtype* nrm);
 SUNErrCode N_VWrmsNormMaskVectorArray_Hip(int nvec, N_Vector* X, N_Vector* W, N_Vector id,
                                           sunrealtype* nrm);
+
+/* One-pass kernels for the integrator's vector sequences (llgcv_fused_ops):
+ * the Nordsieck predict/restore over q+1 history vectors and the error
+ * weights.  Values are identical to the equivalent N_VLinearSum / N_VAbs /
+ * N_VInv sequences. */
+int NVHip_Pascal(int q, N_Vector* zn, sunrealtype sign);
+int NVHip_EwtSet(N_Vector y, sunrealtype rtol, sunrealtype atol, N_Vector abstol, N_Vector w);
 
 /* Fused-ops layer with the reference's entry-point names.
  * FusedNVec_Init installs the one-launch multi-dot and linear-combination

@@ -224,6 +224,12 @@ static inline SUNErrCode N_VScaleVectorArray(int nv, sunrealtype* c, N_Vector* X
   for (int i = 0; i < nv; i++) N_VScale(c[i], X[i], Z[i]);
   return SUN_SUCCESS;
 }
+static inline SUNErrCode N_VWrmsNormVectorArray(int nv, N_Vector* X, N_Vector* W, sunrealtype* nrm) {
+  if (nv < 1) return SUN_SUCCESS;
+  if (X[0]->ops->nvwrmsnormvectorarray) return X[0]->ops->nvwrmsnormvectorarray(nv, X, W, nrm);
+  for (int i = 0; i < nv; i++) nrm[i] = N_VWrmsNorm(X[i], W[i]);
+  return SUN_SUCCESS;
+}
 static inline SUNErrCode N_VLinearSumVectorArray(int nv, sunrealtype a, N_Vector* X, sunrealtype b,
                                                  N_Vector* Y, N_Vector* Z) {
   if (nv < 1) return SUN_SUCCESS;

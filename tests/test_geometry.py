@@ -172,3 +172,36 @@ def test_seeded_ics_are_unit_and_masked():
     m = g.ic_split_x(0.05).reshape(3, 32, 64)
     assert (m[0, :, :32][g.active.reshape(32, 64)[:, :32] == 1] > 0.99).all()
     assert (m[0, :, 32:][g.active.reshape(32, 64)[:, 32:] == 1] < -0.99).all()
+
+
+def test_frame_and_aux_file_formats(tmp_path):
+    """output.txt frames, mask.txt and grain_id.txt in the reference's text
+    formats (2d_i4/2d_fft.cu:393-411, 2d_i3/2d_fft.cu:390-408)."""
+    L = la.lib()
+    ng, ny = 4, 3
+    y = np.arange(3 * ng * ny, dtype=float) / 7.0
+    w = np.linspace(0, 1, ng * ny)
+    p = str(tmp_path / "output.txt").encode()
+    h = L.LLG_FrameWriterOpen(p)
+    assert L.LLG_FrameWriterWrite(h, 0.5, 3 * ng, ny, ng, y.ctypes.data_as(C.POINTER(C.c_double)),
+                                  None, 0) == 0
+    assert L.LLG_FrameWriterWrite(h, 1.0, 3 * ng, ny, ng, y.ctypes.data_as(C.POINTER(C.c_double)),
+                                  w.ctypes.data_as(C.POINTER(C.c_double)), 0) == 0
+    assert L.LLG_FrameWriterClose(h) == 0
+    lines = open(p.decode()).read().split("\n")
+    assert lines[0] == "0.500000 12 3"
+    assert lines[1] == "%f %f %f" % (y[0], y[12], y[24])
+    assert lines[1 + ng * ny] == ""
+    k = 2 + ng * ny
+    assert lines[k] == "1.000000 12 3"
+    assert lines[k + 5] == "%f %f %f" % (w[4] * y[4], w[4] * y[16], w[4] * y[28])
+    assert L.LLG_ShouldWriteFrame(50, 10.0, 100.0, 50, 500) == 1
+    assert L.LLG_ShouldWriteFrame(51, 10.0, 100.0, 50, 500) == 0
+    assert L.LLG_ShouldWriteFrame(1000, 200.0, 100.0, 50, 500) == 1
+    mp = str(tmp_path / "mask.txt").encode()
+    assert L.LLG_WriteMaskFile(mp, ng, ny, w.ctypes.data_as(C.POINTER(C.c_double))) == 0
+    assert open(mp.decode()).read().split("\n")[:3] == ["4 3", "%f" % w[0], "%f" % w[1]]
+    gid = np.arange(ng * ny, dtype=np.int32)
+    gp = str(tmp_path / "grain_id.txt").encode()
+    assert L.LLG_WriteGrainIdFile(gp, ng, ny, gid.ctypes.data_as(C.POINTER(C.c_int))) == 0
+    assert open(gp.decode()).read().split("\n")[:3] == ["4 3", "0", "1"]

@@ -45,6 +45,10 @@ CASES = [
      lambda: la.Geometry.full(48, 48), "stripes"),
     ("plain128_neumann", dict(name="plain128", ng=40, ny=24),
      lambda: la.Geometry.full(40, 24), "tilted_z"),
+    ("i3_polycrystal_windowed", dict(name="i3", ng=48, ny=32, demag=1.0),
+     lambda: la.Geometry.polycrystal(48, 32, num_grains=8, dead_frac=0.25), "grain_bumps"),
+    ("i4_jok_two_phase", dict(name="i4", ng=32, ny=24, demag=4.0, pc_mode=la.PC_JOK),
+     lambda: la.Geometry.square_hole(32, 24, halfside_frac=0.2), "two_domain"),
 ]
 
 
@@ -55,7 +59,7 @@ def make(case):
     cfg = la.preset(kw.pop("name"), **kw)
     y0 = {"two_domain": geom.ic_two_domain, "split_x": geom.ic_split_x,
           "uniform": geom.ic_uniform, "stripes": geom.ic_stripes,
-          "tilted_z": geom.ic_tilted_z}[ic]()
+          "tilted_z": geom.ic_tilted_z, "grain_bumps": geom.ic_grain_bumps}[ic]()
     return cfg, geom, y0
 
 
@@ -80,6 +84,9 @@ def test_rhs_jv_precond_parity(case):
     assert rel(sim.jtv(yn_ref, v), ref.jtv(yn_ref, v)) < TOL
     gamma = 0.037
     assert sim.psetup(yn_ref, gamma) == ref.psetup(yn_ref, gamma)
+    if cfg.pc_mode == la.PC_JOK:
+        # jok=1 reuses the stored J and only refactors with the new gamma
+        assert sim.psetup(yn_ref * 0.5, 0.02, jok=1) == ref.psetup(yn_ref * 0.5, 0.02, jok=1) == 0
     assert rel(sim.pinv(), ref.pinv()) < TOL or cfg.exec_ == la.EXEC_COMPACT
     r = np.random.default_rng(3).standard_normal(y.size)
     assert rel(sim.psolve(r), ref.psolve(r)) < TOL
@@ -115,6 +122,27 @@ def test_demag_periodic_matches_direct_sum_odd_rows():
     la.lib().Demag_Destroy(d)
     h_ref = orc.demag_direct(nx, ny, 1.0, 2.0, y, padded=False)
     assert rel(h, h_ref) < TOL
+
+
+@pytest.mark.parametrize("padded", [False, True])
+def test_demag_fp32_spectral_error(padded):
+    """FP32 spectral demag against the FP64 pipeline: per-field max error
+    relative to the field's max magnitude stays at FP32 level."""
+    nx, ny = 96, 64
+    geom = la.Geometry.square_hole(nx, ny, halfside_frac=0.2)
+    y = geom.ic_split_x()
+    out = {}
+    for fp32 in (0, 1):
+        cfg = la.preset("square_hole" if padded else "i4", ng=nx, ny=ny, demag=4.0)
+        cfg.demag_fp32 = fp32
+        sim = la.Simulation(cfg, geom)
+        sim.rhs(y)
+        out[fp32] = sim.hdemag().reshape(3, -1)
+    for c in range(3):
+        scale = np.max(np.abs(out[0][c])) + 1e-30
+        err = np.max(np.abs(out[1][c] - out[0][c])) / scale
+        print(f"[gpu] fp32 demag comp {c} rel err {err:.2e}")
+        assert err < 2e-5
 
 
 def test_integrate_square_hole_small():

@@ -661,8 +661,7 @@ int nls_convtest(Integrator* m, N_Vector ycor, N_Vector del, double tol) {
   const int mit = m->nls_curiter;
   double dnorm, ynorm = -1.0;
   if (mit > 0 && del->ops->nvwrmsnormvectorarray) {
-    // both norms in one reduction (the second is only used on convergence);
-    // each value equals the separate N_VWrmsNorm result
+    // both norms in one reduction (the second is only used on convergence)
     N_Vector X[2] = {del, ycor};
     N_Vector W[2] = {m->ewt, m->ewt};
     double nr[2];

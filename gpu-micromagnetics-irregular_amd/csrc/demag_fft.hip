@@ -161,8 +161,14 @@ struct Dft<5, T> {
   }
 };
 
+// LDS index padding: one spare slot per 8 complex values, so the stride-R
+// writes of the early Stockham passes spread over the banks.
+__device__ __host__ inline int lpad(int i) { return i + (i >> 3); }
+__device__ __host__ inline int lpad_len(int L) { return L + (L >> 3) + 1; }
+
 // One Stockham pass of radix R over ncomp transforms of length L stored at
-// buf + comp*cstride (in place: all reads, barrier, all writes, barrier).
+// buf + comp*cstride (padded index, in place: all reads, barrier, all
+// writes, barrier).
 template <int R, int B, class T>
 __device__ void pass(V2<T>* buf, int L, int Ns, T sign, const V2<T>* __restrict__ tw, int ncomp,
                      int cstride) {
@@ -179,7 +185,7 @@ __device__ void pass(V2<T>* buf, int L, int Ns, T sign, const V2<T>* __restrict_
       const int k = j % Ns;
 #pragma unroll
       for (int t = 0; t < R; t++) {
-        V2<T> x = src[j + t * nb];
+        V2<T> x = src[lpad(j + t * nb)];
         if (t > 0 && k > 0) {
           V2<T> w = tw[t * k * tstep];
           if (sign > 0) w.y = -w.y;
@@ -200,7 +206,7 @@ __device__ void pass(V2<T>* buf, int L, int Ns, T sign, const V2<T>* __restrict_
       Dft<R, T>::run(v[r], sign);
       const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-      for (int t = 0; t < R; t++) dst[base + t * Ns] = v[r][t];
+      for (int t = 0; t < R; t++) dst[lpad(base + t * Ns)] = v[r][t];
     }
   }
   __syncthreads();
@@ -263,14 +269,14 @@ __global__ __launch_bounds__(1024) void k_row_fwd(const double* __restrict__ in,
     double a = 0.0, b = 0.0;
     if (n0 < nx_in) a = wr ? wr[n0] * src[n0] : src[n0];
     if (n1 < nx_in) b = wr ? wr[n1] * src[n1] : src[n1];
-    lds[m] = mk<T>((T)a, (T)b);
+    lds[lpad(m)] = mk<T>((T)a, (T)b);
   }
   __syncthreads();
-  fft_lds<T>(lds, plan, T(-1), 1, L);
+  fft_lds<T>(lds, plan, T(-1), 1, lpad_len(L));
   V2<T>* out = spec + ((long)comp * rows + row) * Hx;
   for (int k = threadIdx.x; k <= L; k += blockDim.x) {
-    const V2<T> zk = lds[k == L ? 0 : k];
-    const V2<T> zc = cconj(lds[k == 0 ? 0 : L - k]);
+    const V2<T> zk = lds[lpad(k == L ? 0 : k)];
+    const V2<T> zc = cconj(lds[lpad(k == 0 ? 0 : L - k)]);
     const V2<T> e = mk<T>(T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y + zc.y));
     const V2<T> d = csub(zk, zc);
     const V2<T> o = mk<T>(T(0.5) * d.y, T(-0.5) * d.x);
@@ -294,14 +300,14 @@ __global__ __launch_bounds__(1024) void k_row_inv(const V2<T>* __restrict__ spec
     const V2<T> xc = cconj(in[L - k]);
     const V2<T> a = cadd(xk, xc);
     const V2<T> b = cmul(csub(xk, xc), cconj(post[k]));
-    lds[k] = mk<T>(a.x - b.y, a.y + b.x);   // a + i b
+    lds[lpad(k)] = mk<T>(a.x - b.y, a.y + b.x);   // a + i b
   }
   __syncthreads();
-  fft_lds<T>(lds, plan, T(1), 1, L);
+  fft_lds<T>(lds, plan, T(1), 1, lpad_len(L));
   double* dst = out + comp * comp_stride + (long)row * nx_out;
   for (int m = threadIdx.x; m < L; m += blockDim.x) {
     const int n0 = 2 * m;
-    const V2<T> z = lds[m];
+    const V2<T> z = lds[lpad(m)];
     if (n0 < nx_out) dst[n0] = (double)z.x;
     if (n0 + 1 < nx_out) dst[n0 + 1] = (double)z.y;
   }
@@ -325,11 +331,12 @@ __global__ __launch_bounds__(1024) void k_col(V2<T>* __restrict__ spec, int rows
   const int kx = xcd_remap(blockIdx.x, gridDim.x);
   const int Py = plan.L;
   const int ncomp = mode == 0 ? 3 : 1;
+  const int cs = lpad_len(Py);
   for (int c = 0; c < ncomp; c++)
     for (int y = threadIdx.x; y < Py; y += blockDim.x)
-      lds[c * Py + y] = y < rows_in ? spec[((long)c * rows_in + y) * Hx + kx] : mk<T>(T(0), T(0));
+      lds[c * cs + lpad(y)] = y < rows_in ? spec[((long)c * rows_in + y) * Hx + kx] : mk<T>(T(0), T(0));
   __syncthreads();
-  fft_lds<T>(lds, plan, T(-1), ncomp, Py);
+  fft_lds<T>(lds, plan, T(-1), ncomp, cs);
   if (mode == 0) {
     const long P = (long)Hx * Py;
     const T* kxx = K + (long)kx * Py;
@@ -338,21 +345,22 @@ __global__ __launch_bounds__(1024) void k_col(V2<T>* __restrict__ spec, int rows
     const T* kxyr = kzz + P;
     const T* kxyi = kxyr + P;
     for (int ky = threadIdx.x; ky < Py; ky += blockDim.x) {
-      const V2<T> mx = lds[ky], my = lds[Py + ky], mz = lds[2 * Py + ky];
+      const int q = lpad(ky);
+      const V2<T> mx = lds[q], my = lds[cs + q], mz = lds[2 * cs + q];
       const T a = kxx[ky], b = kyy[ky], cz = kzz[ky];
       const V2<T> xy = mk<T>(kxyr[ky], has_xy_im ? kxyi[ky] : T(0));
       const V2<T> hx = cadd(mk<T>(a * mx.x, a * mx.y), cmul(xy, my));
       const V2<T> hy = cadd(cmul(xy, mx), mk<T>(b * my.x, b * my.y));
-      lds[ky] = hx;
-      lds[Py + ky] = hy;
-      lds[2 * Py + ky] = mk<T>(cz * mz.x, cz * mz.y);
+      lds[q] = hx;
+      lds[cs + q] = hy;
+      lds[2 * cs + q] = mk<T>(cz * mz.x, cz * mz.y);
     }
     __syncthreads();
-    fft_lds<T>(lds, plan, T(1), 3, Py);
+    fft_lds<T>(lds, plan, T(1), 3, cs);
   }
   for (int c = 0; c < ncomp; c++)
     for (int y = threadIdx.x; y < rows_out; y += blockDim.x)
-      spec[((long)c * rows_out + y) * Hx + kx] = lds[c * Py + y];
+      spec[((long)c * rows_out + y) * Hx + kx] = lds[c * cs + lpad(y)];
 }
 
 __global__ void k_to_float(const double* __restrict__ a, float* __restrict__ b, long n) {
@@ -680,10 +688,10 @@ DemagData* init_common(int nx, int ny, double thick, double strength, int padded
   const double scale = strength / ((double)Px * (double)Py);
   for (int c = 0; c < 4; c++) {
     hipLaunchKernelGGL(k_row_fwd<double>, dim3(Py, 1), dim3(d->row.block),
-                       sizeof(double2) * d->row.p.L, d->stream,
+                       sizeof(double2) * lpad_len(d->row.p.L), d->stream,
                        (const double*)(dKr + (size_t)c * Px * Py), (const double*)nullptr,
                        (long)0, Px, Py, d->row.p, (const double2*)d->d_post, d->d_spec);
-    hipLaunchKernelGGL(k_col<double>, dim3(Hx), dim3(d->col1.block), sizeof(double2) * Py,
+    hipLaunchKernelGGL(k_col<double>, dim3(Hx), dim3(d->col1.block), sizeof(double2) * lpad_len(Py),
                        d->stream, d->d_spec, Py, Py, Hx, d->col1.p, (const double*)nullptr, 0, 1);
     const long P = (long)Hx * Py;
     hipLaunchKernelGGL(k_extract, dim3((Hx + 255) / 256, Py), dim3(256), 0, d->stream,
@@ -708,15 +716,15 @@ void apply_t(DemagData* d, const HostPlan<T>& row, const HostPlan<T>& col, const
   V2<T>* spec = reinterpret_cast<V2<T>*>(d->d_spec);
   kprof::begin(kprof::DEMAG_ALL, s);
   kprof::begin(kprof::DEMAG_ROW_FWD, s);
-  hipLaunchKernelGGL(k_row_fwd<T>, dim3(ny, 3), dim3(row.block), sizeof(V2<T>) * row.p.L, s, y,
+  hipLaunchKernelGGL(k_row_fwd<T>, dim3(ny, 3), dim3(row.block), sizeof(V2<T>) * lpad_len(row.p.L), s, y,
                      w, N, nx, ny, row.p, post, spec);
   kprof::end(kprof::DEMAG_ROW_FWD, s);
   kprof::begin(kprof::DEMAG_COL, s);
-  hipLaunchKernelGGL(k_col<T>, dim3(Hx), dim3(col.block), sizeof(V2<T>) * 3 * Py, s, spec, ny, ny,
+  hipLaunchKernelGGL(k_col<T>, dim3(Hx), dim3(col.block), sizeof(V2<T>) * 3 * lpad_len(Py), s, spec, ny, ny,
                      Hx, col.p, K, d->has_xy_im, 0);
   kprof::end(kprof::DEMAG_COL, s);
   kprof::begin(kprof::DEMAG_ROW_INV, s);
-  hipLaunchKernelGGL(k_row_inv<T>, dim3(ny, 3), dim3(row.block), sizeof(V2<T>) * row.p.L, s,
+  hipLaunchKernelGGL(k_row_inv<T>, dim3(ny, 3), dim3(row.block), sizeof(V2<T>) * lpad_len(row.p.L), s,
                      (const V2<T>*)spec, ny, row.p, post, h, N, nx);
   kprof::end(kprof::DEMAG_ROW_INV, s);
   kprof::end(kprof::DEMAG_ALL, s);

@@ -525,6 +525,22 @@ int LLG_ProfileStart(int kernel_id, int capacity) {
   return 0;
 }
 
+void* LLG_EventCreate(void) {
+  hipEvent_t e = nullptr;
+  return hipEventCreate(&e) == hipSuccess ? (void*)e : nullptr;
+}
+int LLG_EventRecord(void* ev, void* stream) {
+  return hipEventRecord((hipEvent_t)ev, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+double LLG_EventElapsedMs(void* e0, void* e1) {
+  float ms = -1.f;
+  if (hipEventSynchronize((hipEvent_t)e1) != hipSuccess) return -1.0;
+  hipEventElapsedTime(&ms, (hipEvent_t)e0, (hipEvent_t)e1);
+  return ms;
+}
+void LLG_EventDestroy(void* ev) { hipEventDestroy((hipEvent_t)ev); }
+int LLG_DeviceSynchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -1; }
+
 int LLG_ProfileStop(double* total_ms, long* launches, long* dropped) {
   kprof::State& st = kprof::state();
   st.target = kprof::NONE;

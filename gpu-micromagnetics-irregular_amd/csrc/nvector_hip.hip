@@ -30,7 +30,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-constexpr int kMaxRedBlocks = 1024;
+constexpr int kMaxRedBlocks = 2048;
 constexpr int kMaxK = NVHIP_FUSED_MAX;
 
 struct NvContent {
@@ -87,6 +87,7 @@ inline int ew_grid(sunindextype n) {
   return (int)(g < 1 ? 1 : g);
 }
 inline int red_grid(sunindextype n) {
+  // ~4 elements per thread: enough loads in flight to cover HBM latency
   sunindextype g = (n + 4 * kBlock - 1) / (4 * kBlock);
   if (g > kMaxRedBlocks) g = kMaxRedBlocks;
   return (int)(g < 1 ? 1 : g);
@@ -302,8 +303,19 @@ __global__ __launch_bounds__(kBlock) void k_reduce1(int kind, F f, sunindextype 
                                                     double* out, unsigned seq) {
   __shared__ double sm[1][kWaves];
   __shared__ bool last;
-  double v[1] = {identity(kind)};
-  GRID_STRIDE(i, n) v[0] = combine(kind, v[0], f(i));
+  // four independent accumulators so four loads per thread are in flight
+  double a0 = identity(kind), a1 = a0, a2 = a0, a3 = a0;
+  const sunindextype stride = (sunindextype)gridDim.x * blockDim.x;
+  sunindextype i = (sunindextype)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const double x0 = f(i), x1 = f(i + stride), x2 = f(i + 2 * stride), x3 = f(i + 3 * stride);
+    a0 = combine(kind, a0, x0);
+    a1 = combine(kind, a1, x1);
+    a2 = combine(kind, a2, x2);
+    a3 = combine(kind, a3, x3);
+  }
+  for (; i < n; i += stride) a0 = combine(kind, a0, f(i));
+  double v[1] = {combine(kind, combine(kind, a0, a1), combine(kind, a2, a3))};
   block_reduce<1>(kind, v, sm);
   if (threadIdx.x == 0) {
     partials[blockIdx.x] = v[0];
@@ -327,14 +339,26 @@ __global__ __launch_bounds__(kBlock) void k_dotmulti(const double* __restrict__ 
                                                      unsigned seq) {
   __shared__ double sm[K][kWaves];
   __shared__ bool last;
-  double v[K];
+  double v[K], u[K];
 #pragma unroll
-  for (int k = 0; k < K; k++) v[k] = 0.0;
-  GRID_STRIDE(i, n) {
-    const double xi = x[i];
+  for (int k = 0; k < K; k++) v[k] = u[k] = 0.0;
+  const sunindextype stride = (sunindextype)gridDim.x * blockDim.x;
+  sunindextype i = (sunindextype)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + stride < n; i += 2 * stride) {
+    const double x0 = x[i], x1 = x[i + stride];
 #pragma unroll
-    for (int k = 0; k < K; k++) v[k] += xi * Y.p[k][i];
+    for (int k = 0; k < K; k++) {
+      v[k] += x0 * Y.p[k][i];
+      u[k] += x1 * Y.p[k][i + stride];
+    }
   }
+  for (; i < n; i += stride) {
+    const double x0 = x[i];
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] += x0 * Y.p[k][i];
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] += u[k];
   block_reduce<K>(RED_SUM, v, sm);
   if (threadIdx.x == 0) {
 #pragma unroll
@@ -358,10 +382,23 @@ __global__ __launch_bounds__(kBlock) void k_wsqarray(PtrPack X, PtrPack W, const
                                                      unsigned seq) {
   __shared__ double sm[K][kWaves];
   __shared__ bool last;
-  double v[K];
+  double v[K], u[K];
 #pragma unroll
-  for (int k = 0; k < K; k++) v[k] = 0.0;
-  GRID_STRIDE(i, n) {
+  for (int k = 0; k < K; k++) v[k] = u[k] = 0.0;
+  const sunindextype stride = (sunindextype)gridDim.x * blockDim.x;
+  sunindextype i = (sunindextype)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + stride < n; i += 2 * stride) {
+    const bool on0 = id ? (id[i] > 0.0) : true;
+    const bool on1 = id ? (id[i + stride] > 0.0) : true;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const double p0 = X.p[k][i] * W.p[k][i];
+      const double p1 = X.p[k][i + stride] * W.p[k][i + stride];
+      v[k] += on0 ? p0 * p0 : 0.0;
+      u[k] += on1 ? p1 * p1 : 0.0;
+    }
+  }
+  for (; i < n; i += stride) {
     const bool on = id ? (id[i] > 0.0) : true;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -369,6 +406,8 @@ __global__ __launch_bounds__(kBlock) void k_wsqarray(PtrPack X, PtrPack W, const
       v[k] += on ? p * p : 0.0;
     }
   }
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] += u[k];
   block_reduce<K>(RED_SUM, v, sm);
   if (threadIdx.x == 0) {
 #pragma unroll

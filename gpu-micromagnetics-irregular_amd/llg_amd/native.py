@@ -171,6 +171,11 @@ SIGNATURES = {
                                     C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     "LLG_ProfileStart": (C.c_int, [C.c_int, C.c_int]),
     "LLG_ProfileStop": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    "LLG_EventCreate": (VP, []),
+    "LLG_EventRecord": (C.c_int, [VP, VP]),
+    "LLG_EventElapsedMs": (C.c_double, [VP, VP]),
+    "LLG_EventDestroy": (None, [VP]),
+    "LLG_DeviceSynchronize": (C.c_int, []),
     "LLG_Rhs": (C.c_int, [C.c_double, VP, VP, VP]),
     "JtvProduct": (C.c_int, [VP, VP, C.c_double, VP, VP, VP, VP]),
     "PrecondSetup": (C.c_int, [C.c_double, VP, VP, C.c_int, C.POINTER(C.c_int), C.c_double, VP]),

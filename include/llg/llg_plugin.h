@@ -120,6 +120,12 @@ int      LLG_GetCallCounts(llg_ctx* ctx, long* nf, long* njv, long* npsetup, lon
  * inverse, 8 whole demag.  capacity = launches to record. */
 int LLG_ProfileStart(int kernel_id, int capacity);
 int LLG_ProfileStop(double* total_ms, long* launches, long* dropped);
+/* HIP event helpers for timing from a host binding (diagnostics) */
+void*  LLG_EventCreate(void);
+int    LLG_EventRecord(void* ev, void* stream);
+double LLG_EventElapsedMs(void* e0, void* e1);
+void   LLG_EventDestroy(void* ev);
+int    LLG_DeviceSynchronize(void);
 
 /* CVODE plug-ins (user_data = llg_ctx*) */
 int LLG_Rhs(sunrealtype t, N_Vector y, N_Vector ydot, void* user_data);

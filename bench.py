@@ -107,9 +107,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--gstype", type=int, default=0, help="0 auto, 1 modified, 2 classical GS")
-    ap.add_argument("--sync-mode", type=int, default=0,
-                    help="reduction wait: 0 stream sync, 1 poll the pinned result word")
+    ap.add_argument("--gstype", type=int, default=2,
+                    help="Gram-Schmidt: 2 classical (default: one host wait per Krylov "
+                         "iteration), 1 modified (the reference's choice at this size), 0 auto")
+    ap.add_argument("--sync-mode", type=int, default=1,
+                    help="reduction wait: 1 poll the pinned result word (default), 0 stream sync")
     ap.add_argument("--fp32-demag", action="store_true", help="FP32 spectral demag form")
     args = ap.parse_args()
 

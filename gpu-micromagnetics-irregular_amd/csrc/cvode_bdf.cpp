@@ -67,6 +67,7 @@ constexpr double kLsDgMax = 0.2;
 constexpr double kEpLin = 0.05;
 // SPGMR
 constexpr double kGsFactor = 1000.0;
+constexpr int kFusedMax = 16;        // widest fused multi-dot / combination
 
 // internal flags
 enum {
@@ -133,6 +134,8 @@ struct Integrator {
   double givens[2 * kMaxKrylov] = {};
   double yg[kMaxKrylov + 1] = {};
   double gs_c[kMaxKrylov + 2] = {};
+  double gs_d[kMaxKrylov + 2] = {};
+  bool y_current = false;   // y == zn[0] + ycor already (fused Newton update)
   N_Vector gs_X[kMaxKrylov + 2] = {};
   int ls_iters = 0;
   double ls_resnorm = 0;
@@ -418,13 +421,22 @@ void modified_gs(Integrator* m, int k, double* new_norm) {
   }
 }
 
-void classical_gs(Integrator* m, int k, double* new_norm) {
+// Classical Gram-Schmidt with one conditional reorthogonalisation pass
+// (SUNDIALS SUNClassicalGS).  When `dots` is given, the first multi-dot
+// (including <v_k, v_k>) has already been formed by the caller's fused
+// product + multi-dot pass; when `zz` is given too, the projection and its
+// squared norm have been applied as well (one-wait projection).
+void classical_gs(Integrator* m, int k, double* new_norm, const double* dots = nullptr,
+                  const double* zz_done = nullptr) {
   N_Vector* v = m->V;
   const int km1 = k - 1;
   const int i0 = std::max(k - m->maxl, 0);
   double* s = m->gs_c;
   N_Vector* X = m->gs_X;
-  N_VDotProdMulti(k - i0 + 1, v[k], v + i0, s);
+  if (dots)
+    for (int i = 0; i <= k - i0; i++) s[i] = dots[i];
+  else
+    N_VDotProdMulti(k - i0 + 1, v[k], v + i0, s);
   const double vk_norm = std::sqrt(s[k - i0]);
   for (int i = k - i0 - 1; i >= 0; i--) {
     m->Hes[i + i0][km1] = s[i];
@@ -433,8 +445,16 @@ void classical_gs(Integrator* m, int k, double* new_norm) {
   }
   s[0] = 1.0;
   X[0] = v[k];
-  N_VLinearCombination(k - i0 + 1, s, X, v[k]);
-  *new_norm = std::sqrt(N_VDotProd(v[k], v[k]));
+  double zz = 0.0;
+  if (zz_done) {
+    *new_norm = std::sqrt(*zz_done);
+  } else if (m->fused.lincombdot && k - i0 + 1 <= kFusedMax &&
+             m->fused.lincombdot(k - i0 + 1, s, X, v[k], &zz) == 0) {
+    *new_norm = std::sqrt(zz);
+  } else {
+    N_VLinearCombination(k - i0 + 1, s, X, v[k]);
+    *new_norm = std::sqrt(N_VDotProd(v[k], v[k]));
+  }
   if ((kGsFactor * (*new_norm)) < vk_norm) {
     N_VDotProdMulti(k - i0, v[k], v + i0, s + 1);
     s[0] = 1.0;
@@ -452,6 +472,11 @@ void classical_gs(Integrator* m, int k, double* new_norm) {
 // CVLS interface: z = (I - gamma J) v
 int ls_atimes(Integrator* m, N_Vector v, N_Vector z) {
   int rv;
+  if (m->jtimes && m->fused.atimes &&
+      m->fused.atimes(v, z, m->gamma, m->tn, m->ycur, m->fcur, m->user_data) == 0) {
+    m->njtimes++;
+    return 0;
+  }
   if (m->jtimes) {
     rv = m->jtimes(v, z, m->tn, m->ycur, m->fcur, m->user_data, m->ytemp);
   } else {
@@ -492,16 +517,20 @@ int spgmr_solve(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s)
   bool converged = false;
   m->ls_iters = 0;
 
-  // r0 = b (zero initial guess)
-  N_VScale(1.0, b, m->vtemp);
+  // r0 = b (zero initial guess); b is read here only, so the solution may
+  // be returned in place of b (x == b) and r0 needs no copy
+  double rr = 0.0;
+  N_Vector t0 = b;
   if (pre_left) {
-    rv = ls_psolve(m, m->vtemp, V[0], delta, SUN_PREC_LEFT);
+    rv = ls_psolve(m, b, V[0], delta, SUN_PREC_LEFT);
     if (rv != 0) return rv < 0 ? LS_PSOLVE_FAIL_UNREC : LS_PSOLVE_FAIL_REC;
-    N_VProd(s, V[0], V[0]);
-  } else {
-    N_VProd(s, m->vtemp, V[0]);
+    t0 = V[0];
   }
-  double r_norm = std::sqrt(N_VDotProd(V[0], V[0]));
+  if (!(m->fused.proddotmulti && m->fused.proddotmulti(s, t0, V[0], 1, &V[0], &rr) == 0)) {
+    N_VProd(s, t0, V[0]);
+    rr = N_VDotProd(V[0], V[0]);
+  }
+  double r_norm = std::sqrt(rr);
   double beta = r_norm;
   m->ls_resnorm = r_norm;
   if (r_norm <= delta) {
@@ -509,17 +538,20 @@ int spgmr_solve(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s)
     return LS_SUCCESS;
   }
   double rho = beta;
-  N_VConst(0.0, m->xcor);
   for (int i = 0; i <= l_max; i++)
     for (int j = 0; j < l_max; j++) m->Hes[i][j] = 0.0;
   double rotation_product = 1.0;
-  N_VScale(1.0 / r_norm, V[0], V[0]);
+  // V[l] is normalised by vscale at the top of iteration l (fused with
+  // the unscaling vtemp = s2^-1 V[l] when available)
+  double vscale = 1.0 / r_norm;
 
   for (int l = 0; l < l_max; l++) {
     m->ls_iters++;
     krydim = l + 1;
-    // vtemp = s2^-1 V[l]
-    N_VDiv(V[l], s, m->vtemp);
+    if (!(m->fused.scalediv && m->fused.scalediv(vscale, V[l], s, V[l], m->vtemp) == 0)) {
+      N_VScale(vscale, V[l], V[l]);
+      N_VDiv(V[l], s, m->vtemp);
+    }
     if (pre_right) {
       rv = ls_psolve(m, m->vtemp, V[l + 1], delta, SUN_PREC_RIGHT);
       if (rv != 0) return rv < 0 ? LS_PSOLVE_FAIL_UNREC : LS_PSOLVE_FAIL_REC;
@@ -527,42 +559,60 @@ int spgmr_solve(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s)
     }
     rv = ls_atimes(m, m->vtemp, V[l + 1]);
     if (rv != 0) return rv < 0 ? LS_ATIMES_FAIL_UNREC : LS_ATIMES_FAIL_REC;
+    N_Vector tl = V[l + 1];
     if (pre_left) {
       rv = ls_psolve(m, V[l + 1], m->vtemp, delta, SUN_PREC_LEFT);
       if (rv != 0) return rv < 0 ? LS_PSOLVE_FAIL_UNREC : LS_PSOLVE_FAIL_REC;
-      N_VProd(s, m->vtemp, V[l + 1]);
-    } else {
-      N_VProd(s, V[l + 1], V[l + 1]);
+      tl = m->vtemp;
     }
     double hnorm = 0.0;
-    if (m->gstype == SUN_CLASSICAL_GS) classical_gs(m, l + 1, &hnorm);
-    else modified_gs(m, l + 1, &hnorm);
+    const int k = l + 1, i0 = std::max(k - m->maxl, 0);
+    double* dots = m->gs_d;
+    double zz = 0.0;
+    if (m->gstype == SUN_CLASSICAL_GS && m->fused.gsproject && k - i0 + 1 <= kFusedMax &&
+        m->fused.gsproject(s, tl, V[k], k - i0 + 1, V + i0, dots, &zz) == 0) {
+      classical_gs(m, k, &hnorm, dots, &zz);
+    } else if (m->gstype == SUN_CLASSICAL_GS && m->fused.proddotmulti &&
+               k - i0 + 1 <= kFusedMax &&
+               m->fused.proddotmulti(s, tl, V[k], k - i0 + 1, V + i0, dots) == 0) {
+      classical_gs(m, k, &hnorm, dots);
+    } else {
+      N_VProd(s, tl, V[k]);
+      if (m->gstype == SUN_CLASSICAL_GS) classical_gs(m, k, &hnorm);
+      else modified_gs(m, k, &hnorm);
+    }
     m->Hes[l + 1][l] = hnorm;
     if (qr_fact(krydim, m->Hes, m->givens, l) != 0) return LS_QRFACT_FAIL;
     rotation_product *= m->givens[2 * l + 1];
     m->ls_resnorm = rho = std::fabs(rotation_product * r_norm);
     if (rho <= delta) { converged = true; break; }
-    N_VScale(1.0 / m->Hes[l + 1][l], V[l + 1], V[l + 1]);
+    vscale = 1.0 / m->Hes[l + 1][l];
   }
 
   m->yg[0] = r_norm;
   for (int i = 1; i <= krydim; i++) m->yg[i] = 0.0;
   if (qr_sol(krydim, m->Hes, m->givens, m->yg) != 0) return LS_QRSOL_FAIL;
+  // xcor = 0 + sum_k yg_k V_k, then x = s2^-1 xcor (and P2^-1 for right
+  // preconditioning).  The fused form starts its sum from +0 as well.
   m->gs_c[0] = 1.0;
   m->gs_X[0] = m->xcor;
   for (int k = 0; k < krydim; k++) {
     m->gs_c[k + 1] = m->yg[k];
     m->gs_X[k + 1] = V[k];
   }
-  N_VLinearCombination(krydim + 1, m->gs_c, m->gs_X, m->xcor);
 
   if (converged || rho < beta) {
-    N_VDiv(m->xcor, s, m->xcor);
-    if (pre_right) {
-      rv = ls_psolve(m, m->xcor, x, delta, SUN_PREC_RIGHT);
-      if (rv != 0) return rv < 0 ? LS_PSOLVE_FAIL_UNREC : LS_PSOLVE_FAIL_REC;
-    } else {
-      N_VScale(1.0, m->xcor, x);
+    if (!(!pre_right && m->fused.lincombdiv && krydim <= kFusedMax &&
+          m->fused.lincombdiv(krydim, m->gs_c + 1, m->gs_X + 1, s, x) == 0)) {
+      N_VConst(0.0, m->xcor);
+      N_VLinearCombination(krydim + 1, m->gs_c, m->gs_X, m->xcor);
+      if (pre_right) {
+        N_VDiv(m->xcor, s, m->xcor);
+        rv = ls_psolve(m, m->xcor, x, delta, SUN_PREC_RIGHT);
+        if (rv != 0) return rv < 0 ? LS_PSOLVE_FAIL_UNREC : LS_PSOLVE_FAIL_REC;
+      } else {
+        N_VDiv(m->xcor, s, x);
+      }
     }
     return converged ? LS_SUCCESS : LS_RES_REDUCED;
   }
@@ -604,8 +654,8 @@ int ls_solve(Integrator* m, N_Vector b, N_Vector weight, N_Vector ynow, N_Vector
     const int rv = m->jtsetup(m->tn, ynow, fnow, m->user_data);
     if (rv != 0) return rv;
   }
-  const int rv = spgmr_solve(m, m->lsx, b, delta, weight);
-  N_VScale(1.0, m->lsx, b);
+  // the solution overwrites b (SPGMR reads b before it writes x)
+  const int rv = spgmr_solve(m, b, b, delta, weight);
   m->nli += m->ls_iters;
   if (rv != LS_SUCCESS) m->ncfl++;
   switch (rv) {
@@ -621,15 +671,20 @@ int ls_solve(Integrator* m, N_Vector b, N_Vector weight, N_Vector ynow, N_Vector
 }
 
 // ------------------------------------------------------- nonlinear solve
-int nls_residual(Integrator* m, N_Vector ycor, N_Vector res) {
-  N_VLinearSum(1.0, m->zn[0], 1.0, ycor, m->y);
+// Newton residual, returned NEGATED: -res = -(rl1 zn[1] + ycor - gamma f).
+// The Newton solver needs -res as the linear system's right-hand side;
+// negating the coefficients gives exactly the negated values, so the
+// separate N_VScale(-1) pass is not needed.
+int nls_residual(Integrator* m, N_Vector ycor, N_Vector negres) {
+  if (!m->y_current) N_VLinearSum(1.0, m->zn[0], 1.0, ycor, m->y);
+  m->y_current = false;
   const int rv = m->f(m->tn, m->y, m->ftemp, m->user_data);
   m->nfe++;
   if (rv < 0) return LLGCV_RHSFUNC_FAIL;
   if (rv > 0) return RHSFUNC_RECVR;
-  double c[3] = {m->rl1, 1.0, -m->gamma};
+  double c[3] = {-m->rl1, -1.0, m->gamma};
   N_Vector X[3] = {m->zn[1], ycor, m->ftemp};
-  N_VLinearCombination(3, c, X, res);
+  N_VLinearCombination(3, c, X, negres);
   return 0;
 }
 
@@ -657,6 +712,20 @@ int nls_lsolve(Integrator* m, N_Vector delta) {
   return 0;
 }
 
+// convergence test on precomputed norms (ynorm < 0: not available)
+int nls_convtest_norms(Integrator* m, N_Vector ycor, double dnorm, double ynorm, double tol) {
+  const int mit = m->nls_curiter;
+  if (mit > 0) m->crate = std::max(kCrDown * m->crate, dnorm / m->delp);
+  const double dcon = dnorm * std::min(1.0, m->crate) / tol;
+  if (dcon <= 1.0) {
+    m->acnrm = (mit == 0) ? dnorm : (ynorm >= 0.0 ? ynorm : N_VWrmsNorm(ycor, m->ewt));
+    return 0;
+  }
+  if (mit >= 1 && dnorm > kRDiv * m->delp) return NLS_CONV_RECVR;
+  m->delp = dnorm;
+  return NLS_CONTINUE;
+}
+
 int nls_convtest(Integrator* m, N_Vector ycor, N_Vector del, double tol) {
   const int mit = m->nls_curiter;
   double dnorm, ynorm = -1.0;
@@ -671,15 +740,7 @@ int nls_convtest(Integrator* m, N_Vector ycor, N_Vector del, double tol) {
   } else {
     dnorm = N_VWrmsNorm(del, m->ewt);
   }
-  if (mit > 0) m->crate = std::max(kCrDown * m->crate, dnorm / m->delp);
-  const double dcon = dnorm * std::min(1.0, m->crate) / tol;
-  if (dcon <= 1.0) {
-    m->acnrm = (mit == 0) ? dnorm : (ynorm >= 0.0 ? ynorm : N_VWrmsNorm(ycor, m->ewt));
-    return 0;
-  }
-  if (mit >= 1 && dnorm > kRDiv * m->delp) return NLS_CONV_RECVR;
-  m->delp = dnorm;
-  return NLS_CONTINUE;
+  return nls_convtest_norms(m, ycor, dnorm, ynorm, tol);
 }
 
 // SUNNonlinSolSolve_Newton
@@ -688,6 +749,7 @@ int newton_solve(Integrator* m, N_Vector ycor, double tol, bool call_lsetup, lon
   bool jbad = false;
   int rv = 0;
   N_Vector del = m->delta;
+  m->y_current = false;
   for (;;) {
     rv = nls_residual(m, ycor, del);
     if (rv != 0) break;
@@ -700,11 +762,19 @@ int newton_solve(Integrator* m, N_Vector ycor, double tol, bool call_lsetup, lon
     m->nls_curiter = 0;
     for (;;) {
       (*niters)++;
-      N_VScale(-1.0, del, del);
-      rv = nls_lsolve(m, del);
+      rv = nls_lsolve(m, del);   // del holds -res: solve (I - gamma J) del = -res
       if (rv != 0) break;
-      N_VLinearSum(1.0, ycor, 1.0, del, ycor);
-      rv = nls_convtest(m, ycor, del, tol);
+      double dn = 0.0, yn = -1.0;
+      const bool vec2 = m->nls_curiter > 0 && del->ops->nvwrmsnormvectorarray;
+      if ((m->nls_curiter == 0 || vec2) && m->fused.newtonupd &&
+          m->fused.newtonupd(m->nls_curiter, ycor, del, m->zn[0], m->ewt, m->y, &dn, &yn) == 0) {
+        // ycor += del, y = zn[0] + ycor and the test norms in one pass
+        m->y_current = true;
+        rv = nls_convtest_norms(m, ycor, dn, yn, tol);
+      } else {
+        N_VLinearSum(1.0, ycor, 1.0, del, ycor);
+        rv = nls_convtest(m, ycor, del, tol);
+      }
       if (rv == 0) {
         m->nls_jcur = false;
         return 0;
@@ -720,6 +790,7 @@ int newton_solve(Integrator* m, N_Vector ycor, double tol, bool call_lsetup, lon
       call_lsetup = true;
       jbad = true;
       N_VConst(0.0, ycor);
+      m->y_current = false;
       continue;
     }
     break;
@@ -744,8 +815,9 @@ int cv_nls(Integrator* m, int nflag) {
   const int flag = newton_solve(m, m->acor, m->tq[4], call_setup, &ni, &nf);
   m->nni += ni;
   m->nnf += nf;
-  if (flag != 0) return flag;
-  N_VLinearSum(1.0, m->zn[0], 1.0, m->acor, m->y);
+  if (flag != 0) { m->y_current = false; return flag; }
+  if (!m->y_current) N_VLinearSum(1.0, m->zn[0], 1.0, m->acor, m->y);
+  m->y_current = false;
   return 0;
 }
 

@@ -65,6 +65,7 @@ struct Plan1D {
   int npass;
   int radix[kMaxPasses];
   int bpt[kMaxPasses];      // butterflies per thread per pass (<= 8)
+  int psh;                  // LDS padding: one spare slot per 2^psh values (31: none)
   const V2<T>* tw;          // exp(-2 pi i q / L), q in [0, L)
 };
 
@@ -161,17 +162,18 @@ struct Dft<5, T> {
   }
 };
 
-// LDS index padding: one spare slot per 8 complex values, so the stride-R
-// writes of the early Stockham passes spread over the banks.
-__device__ __host__ inline int lpad(int i) { return i + (i >> 3); }
-__device__ __host__ inline int lpad_len(int L) { return L + (L >> 3) + 1; }
+// LDS index padding: one spare slot per 2^sh complex values, so the
+// stride-R accesses of the Stockham passes spread over the banks
+// (sh = 31 disables it).  Padding costs LDS, hence occupancy.
+__device__ __host__ inline int lpad(int i, int sh) { return i + (i >> sh); }
+__device__ __host__ inline int lpad_len(int L, int sh) { return L + (L >> sh) + 1; }
 
 // One Stockham pass of radix R over ncomp transforms of length L stored at
 // buf + comp*cstride (padded index, in place: all reads, barrier, all
 // writes, barrier).
 template <int R, int B, class T>
 __device__ void pass(V2<T>* buf, int L, int Ns, T sign, const V2<T>* __restrict__ tw, int ncomp,
-                     int cstride) {
+                     int cstride, int sh) {
   const int nb = L / R;
   const int total = nb * ncomp;
   const int tstep = L / (Ns * R);
@@ -185,7 +187,7 @@ __device__ void pass(V2<T>* buf, int L, int Ns, T sign, const V2<T>* __restrict_
       const int k = j % Ns;
 #pragma unroll
       for (int t = 0; t < R; t++) {
-        V2<T> x = src[lpad(j + t * nb)];
+        V2<T> x = src[lpad(j + t * nb, sh)];
         if (t > 0 && k > 0) {
           V2<T> w = tw[t * k * tstep];
           if (sign > 0) w.y = -w.y;
@@ -206,7 +208,7 @@ __device__ void pass(V2<T>* buf, int L, int Ns, T sign, const V2<T>* __restrict_
       Dft<R, T>::run(v[r], sign);
       const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-      for (int t = 0; t < R; t++) dst[lpad(base + t * Ns)] = v[r][t];
+      for (int t = 0; t < R; t++) dst[lpad(base + t * Ns, sh)] = v[r][t];
     }
   }
   __syncthreads();
@@ -218,19 +220,19 @@ struct MaxB { static constexpr int value = R == 8 ? 2 : (R == 5 ? 3 : (R == 2 ? 
 
 template <int R, int B, class T>
 __device__ inline void pass_if(V2<T>* buf, int L, int Ns, T sign, const V2<T>* tw, int ncomp,
-                               int cstride) {
-  if constexpr (B <= MaxB<R>::value) pass<R, B, T>(buf, L, Ns, sign, tw, ncomp, cstride);
+                               int cstride, int sh) {
+  if constexpr (B <= MaxB<R>::value) pass<R, B, T>(buf, L, Ns, sign, tw, ncomp, cstride, sh);
 }
 
 template <int R, class T>
 __device__ void pass_b(V2<T>* buf, int L, int Ns, T sign, const V2<T>* tw, int ncomp,
-                       int cstride, int B) {
+                       int cstride, int B, int sh) {
   switch (B) {
-    case 1: pass_if<R, 1, T>(buf, L, Ns, sign, tw, ncomp, cstride); break;
-    case 2: pass_if<R, 2, T>(buf, L, Ns, sign, tw, ncomp, cstride); break;
-    case 3: pass_if<R, 3, T>(buf, L, Ns, sign, tw, ncomp, cstride); break;
-    case 4: pass_if<R, 4, T>(buf, L, Ns, sign, tw, ncomp, cstride); break;
-    default: pass_if<R, 8, T>(buf, L, Ns, sign, tw, ncomp, cstride); break;
+    case 1: pass_if<R, 1, T>(buf, L, Ns, sign, tw, ncomp, cstride, sh); break;
+    case 2: pass_if<R, 2, T>(buf, L, Ns, sign, tw, ncomp, cstride, sh); break;
+    case 3: pass_if<R, 3, T>(buf, L, Ns, sign, tw, ncomp, cstride, sh); break;
+    case 4: pass_if<R, 4, T>(buf, L, Ns, sign, tw, ncomp, cstride, sh); break;
+    default: pass_if<R, 8, T>(buf, L, Ns, sign, tw, ncomp, cstride, sh); break;
   }
 }
 
@@ -240,11 +242,11 @@ __device__ void fft_lds(V2<T>* buf, const Plan1D<T>& p, T sign, int ncomp, int c
   for (int i = 0; i < p.npass; i++) {
     const int R = p.radix[i], B = p.bpt[i];
     switch (R) {
-      case 8: pass_b<8, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
-      case 4: pass_b<4, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
-      case 2: pass_b<2, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
-      case 3: pass_b<3, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
-      default: pass_b<5, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B); break;
+      case 8: pass_b<8, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B, p.psh); break;
+      case 4: pass_b<4, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B, p.psh); break;
+      case 2: pass_b<2, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B, p.psh); break;
+      case 3: pass_b<3, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B, p.psh); break;
+      default: pass_b<5, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B, p.psh); break;
     }
     Ns *= R;
   }
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(1024) void k_row_fwd(const double* __restrict__ in,
   extern __shared__ char lds_raw[];
   V2<T>* lds = reinterpret_cast<V2<T>*>(lds_raw);
   const int row = blockIdx.x, comp = blockIdx.y;
-  const int L = plan.L, Hx = L + 1;
+  const int L = plan.L, Hx = L + 1, sh = plan.psh;
   const double* src = in + comp * comp_stride + (long)row * nx_in;
   const double* wr = w ? w + (long)row * nx_in : nullptr;
   for (int m = threadIdx.x; m < L; m += blockDim.x) {
@@ -269,14 +271,14 @@ __global__ __launch_bounds__(1024) void k_row_fwd(const double* __restrict__ in,
     double a = 0.0, b = 0.0;
     if (n0 < nx_in) a = wr ? wr[n0] * src[n0] : src[n0];
     if (n1 < nx_in) b = wr ? wr[n1] * src[n1] : src[n1];
-    lds[lpad(m)] = mk<T>((T)a, (T)b);
+    lds[lpad(m, sh)] = mk<T>((T)a, (T)b);
   }
   __syncthreads();
-  fft_lds<T>(lds, plan, T(-1), 1, lpad_len(L));
+  fft_lds<T>(lds, plan, T(-1), 1, lpad_len(L, sh));
   V2<T>* out = spec + ((long)comp * rows + row) * Hx;
   for (int k = threadIdx.x; k <= L; k += blockDim.x) {
-    const V2<T> zk = lds[lpad(k == L ? 0 : k)];
-    const V2<T> zc = cconj(lds[lpad(k == 0 ? 0 : L - k)]);
+    const V2<T> zk = lds[lpad(k == L ? 0 : k, sh)];
+    const V2<T> zc = cconj(lds[lpad(k == 0 ? 0 : L - k, sh)]);
     const V2<T> e = mk<T>(T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y + zc.y));
     const V2<T> d = csub(zk, zc);
     const V2<T> o = mk<T>(T(0.5) * d.y, T(-0.5) * d.x);
@@ -293,21 +295,21 @@ __global__ __launch_bounds__(1024) void k_row_inv(const V2<T>* __restrict__ spec
   extern __shared__ char lds_raw[];
   V2<T>* lds = reinterpret_cast<V2<T>*>(lds_raw);
   const int row = blockIdx.x, comp = blockIdx.y;
-  const int L = plan.L, Hx = L + 1;
+  const int L = plan.L, Hx = L + 1, sh = plan.psh;
   const V2<T>* in = spec + ((long)comp * rows + row) * Hx;
   for (int k = threadIdx.x; k < L; k += blockDim.x) {
     const V2<T> xk = in[k];
     const V2<T> xc = cconj(in[L - k]);
     const V2<T> a = cadd(xk, xc);
     const V2<T> b = cmul(csub(xk, xc), cconj(post[k]));
-    lds[lpad(k)] = mk<T>(a.x - b.y, a.y + b.x);   // a + i b
+    lds[lpad(k, sh)] = mk<T>(a.x - b.y, a.y + b.x);   // a + i b
   }
   __syncthreads();
-  fft_lds<T>(lds, plan, T(1), 1, lpad_len(L));
+  fft_lds<T>(lds, plan, T(1), 1, lpad_len(L, sh));
   double* dst = out + comp * comp_stride + (long)row * nx_out;
   for (int m = threadIdx.x; m < L; m += blockDim.x) {
     const int n0 = 2 * m;
-    const V2<T> z = lds[lpad(m)];
+    const V2<T> z = lds[lpad(m, sh)];
     if (n0 < nx_out) dst[n0] = (double)z.x;
     if (n0 + 1 < nx_out) dst[n0 + 1] = (double)z.y;
   }
@@ -331,10 +333,11 @@ __global__ __launch_bounds__(1024) void k_col(V2<T>* __restrict__ spec, int rows
   const int kx = xcd_remap(blockIdx.x, gridDim.x);
   const int Py = plan.L;
   const int ncomp = mode == 0 ? 3 : 1;
-  const int cs = lpad_len(Py);
+  const int sh = plan.psh;
+  const int cs = lpad_len(Py, sh);
   for (int c = 0; c < ncomp; c++)
     for (int y = threadIdx.x; y < Py; y += blockDim.x)
-      lds[c * cs + lpad(y)] = y < rows_in ? spec[((long)c * rows_in + y) * Hx + kx] : mk<T>(T(0), T(0));
+      lds[c * cs + lpad(y, sh)] = y < rows_in ? spec[((long)c * rows_in + y) * Hx + kx] : mk<T>(T(0), T(0));
   __syncthreads();
   fft_lds<T>(lds, plan, T(-1), ncomp, cs);
   if (mode == 0) {
@@ -345,7 +348,7 @@ __global__ __launch_bounds__(1024) void k_col(V2<T>* __restrict__ spec, int rows
     const T* kxyr = kzz + P;
     const T* kxyi = kxyr + P;
     for (int ky = threadIdx.x; ky < Py; ky += blockDim.x) {
-      const int q = lpad(ky);
+      const int q = lpad(ky, sh);
       const V2<T> mx = lds[q], my = lds[cs + q], mz = lds[2 * cs + q];
       const T a = kxx[ky], b = kyy[ky], cz = kzz[ky];
       const V2<T> xy = mk<T>(kxyr[ky], has_xy_im ? kxyi[ky] : T(0));
@@ -360,7 +363,7 @@ __global__ __launch_bounds__(1024) void k_col(V2<T>* __restrict__ spec, int rows
   }
   for (int c = 0; c < ncomp; c++)
     for (int y = threadIdx.x; y < rows_out; y += blockDim.x)
-      spec[((long)c * rows_out + y) * Hx + kx] = lds[c * cs + lpad(y)];
+      spec[((long)c * rows_out + y) * Hx + kx] = lds[c * cs + lpad(y, sh)];
 }
 
 __global__ void k_to_float(const double* __restrict__ a, float* __restrict__ b, long n) {
@@ -495,6 +498,10 @@ bool make_plan(int L, int ncomp, HostPlan<T>& hp) {
   }
   hp.p.L = L;
   hp.p.npass = (int)rad.size();
+  // row plans (one transform per workgroup, small LDS) pad every 8 values;
+  // the 3-component column plan pads every 16 so three workgroups still
+  // fit one CU's 160 KB of LDS at Py = 1024 in FP64
+  hp.p.psh = ncomp == 3 ? 4 : 3;
   // smallest multiple-of-64 workgroup (>= 64) for which every pass fits its
   // per-thread register budget
   int block = 64;
@@ -688,10 +695,10 @@ DemagData* init_common(int nx, int ny, double thick, double strength, int padded
   const double scale = strength / ((double)Px * (double)Py);
   for (int c = 0; c < 4; c++) {
     hipLaunchKernelGGL(k_row_fwd<double>, dim3(Py, 1), dim3(d->row.block),
-                       sizeof(double2) * lpad_len(d->row.p.L), d->stream,
+                       sizeof(double2) * lpad_len(d->row.p.L, d->row.p.psh), d->stream,
                        (const double*)(dKr + (size_t)c * Px * Py), (const double*)nullptr,
                        (long)0, Px, Py, d->row.p, (const double2*)d->d_post, d->d_spec);
-    hipLaunchKernelGGL(k_col<double>, dim3(Hx), dim3(d->col1.block), sizeof(double2) * lpad_len(Py),
+    hipLaunchKernelGGL(k_col<double>, dim3(Hx), dim3(d->col1.block), sizeof(double2) * lpad_len(Py, d->col1.p.psh),
                        d->stream, d->d_spec, Py, Py, Hx, d->col1.p, (const double*)nullptr, 0, 1);
     const long P = (long)Hx * Py;
     hipLaunchKernelGGL(k_extract, dim3((Hx + 255) / 256, Py), dim3(256), 0, d->stream,
@@ -716,15 +723,15 @@ void apply_t(DemagData* d, const HostPlan<T>& row, const HostPlan<T>& col, const
   V2<T>* spec = reinterpret_cast<V2<T>*>(d->d_spec);
   kprof::begin(kprof::DEMAG_ALL, s);
   kprof::begin(kprof::DEMAG_ROW_FWD, s);
-  hipLaunchKernelGGL(k_row_fwd<T>, dim3(ny, 3), dim3(row.block), sizeof(V2<T>) * lpad_len(row.p.L), s, y,
+  hipLaunchKernelGGL(k_row_fwd<T>, dim3(ny, 3), dim3(row.block), sizeof(V2<T>) * lpad_len(row.p.L, row.p.psh), s, y,
                      w, N, nx, ny, row.p, post, spec);
   kprof::end(kprof::DEMAG_ROW_FWD, s);
   kprof::begin(kprof::DEMAG_COL, s);
-  hipLaunchKernelGGL(k_col<T>, dim3(Hx), dim3(col.block), sizeof(V2<T>) * 3 * lpad_len(Py), s, spec, ny, ny,
+  hipLaunchKernelGGL(k_col<T>, dim3(Hx), dim3(col.block), sizeof(V2<T>) * 3 * lpad_len(Py, col.p.psh), s, spec, ny, ny,
                      Hx, col.p, K, d->has_xy_im, 0);
   kprof::end(kprof::DEMAG_COL, s);
   kprof::begin(kprof::DEMAG_ROW_INV, s);
-  hipLaunchKernelGGL(k_row_inv<T>, dim3(ny, 3), dim3(row.block), sizeof(V2<T>) * lpad_len(row.p.L), s,
+  hipLaunchKernelGGL(k_row_inv<T>, dim3(ny, 3), dim3(row.block), sizeof(V2<T>) * lpad_len(row.p.L, row.p.psh), s,
                      (const V2<T>*)spec, ny, row.p, post, h, N, nx);
   kprof::end(kprof::DEMAG_ROW_INV, s);
   kprof::end(kprof::DEMAG_ALL, s);
@@ -791,6 +798,8 @@ int Demag_SetPrecision(DemagData* d, int fp32) {
     const int Hx = d->hx, Py = d->py;
     const size_t P5 = 5 * (size_t)Hx * Py;
     if (!make_plan(d->px / 2, 1, d->row32) || !make_plan(d->py, 3, d->col32)) return -1;
+    d->row32.p.psh = d->row.p.psh;
+    d->col32.p.psh = d->col.p.psh;
     std::vector<float2> post(Hx);
     for (int k = 0; k < Hx; k++) {
       const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)d->px;
@@ -808,6 +817,13 @@ int Demag_SetPrecision(DemagData* d, int fp32) {
     if (!launch_ok("Demag_SetPrecision")) return -1;
   }
   d->fp32 = 1;
+  return 0;
+}
+
+int Demag_SetLdsPadding(DemagData* d, int row_shift, int col_shift) {
+  if (!d || row_shift < 1 || row_shift > 31 || col_shift < 1 || col_shift > 31) return -1;
+  d->row.p.psh = d->row32.p.psh = row_shift;
+  d->col.p.psh = d->col32.p.psh = col_shift;
   return 0;
 }
 

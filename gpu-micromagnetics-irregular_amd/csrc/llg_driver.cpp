@@ -42,7 +42,7 @@ void LLG_DefaultRunOpts(llg_run_opts* o) {
   o->gstype = 0;        // auto: classical iff neq < 500000 (2d_i4/2d_fft.cu:645-650)
   o->max_order = 5;
   o->mxstep = 500;
-  o->fused = 1;
+  o->fused = 3;
   o->use_jtv = 1;
   o->use_prec = 1;
   o->pretype = SUN_PREC_LEFT;
@@ -60,7 +60,7 @@ void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_hos
   r->y = N_VNewWithStream_Hip(neq, stream, &r->sctx);
   r->abstol = N_VNewWithStream_Hip(neq, stream, &r->sctx);
   if (!r->y || !r->abstol) { LLG_RunDestroy(r); return nullptr; }
-  if (opts->fused) FusedNVec_Init(r->y);
+  if (opts->fused & 1) FusedNVec_Init(r->y);
   N_VSetFromHost_Hip(r->y, y0_host);
   N_VConst(opts->atol, r->abstol);
   r->cv = LLGCVodeCreate(LLGCV_BDF, &r->sctx);
@@ -78,10 +78,17 @@ void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_hos
     return nullptr;
   }
   if (opts->use_prec) LLGCVodeSetPreconditioner(r->cv, PrecondSetup, PrecondSolve);
-  if (opts->fused) {
+  if (opts->fused & 2) {
     llgcv_fused_ops fo{};
     fo.pascal = NVHip_Pascal;
     fo.ewt = NVHip_EwtSet;
+    fo.scalediv = NVHip_ScaleDiv;
+    fo.proddotmulti = NVHip_ProdDotMulti;
+    fo.lincombdot = NVHip_LinCombDot;
+    fo.gsproject = NVHip_GSProject;
+    fo.lincombdiv = NVHip_LinCombDiv;
+    fo.newtonupd = NVHip_NewtonUpdate;
+    if (opts->use_jtv) fo.atimes = LLG_Atimes;
     LLGCVodeSetFusedOps(r->cv, &fo);
   }
   r->t = opts->t0;

@@ -251,9 +251,26 @@ __global__ __launch_bounds__(BX* BY) void k_rhs_tiled(GeoD g, TermsD T,
 }
 
 // ------------------------------------------------------------------- Jv
+// Jv = J(y) v; with AT the SPGMR operator z = v + (-gamma) Jv is written
+// instead, rounded exactly as N_VLinearSum(1, v, -gamma, Jv) rounds it
+template <bool AT>
 __global__ __launch_bounds__(256) void k_jtv(GeoD g, TermsD T, const double* __restrict__ y,
-                                             const double* __restrict__ v, double* __restrict__ Jv) {
+                                             const double* __restrict__ v, double* __restrict__ Jv,
+                                             double mgamma) {
   int cell, gx, gy;
+  if (AT && g.compact) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int hidx = t - g.n_active;
+    if (hidx >= 0 && hidx < g.n_hole) {
+#pragma clang fp contract(off)
+      const int hc = g.hole_ids[hidx];
+      for (int c = 0; c < 3; c++) {
+        const double p = mgamma * 0.0;
+        Jv[c * g.ncell + hc] = v[c * g.ncell + hc] + p;
+      }
+      return;
+    }
+  }
   if (!map_cell(g, Jv, cell, gx, gy)) return;
   const int N = g.ncell;
   double h[3], dh[3];
@@ -268,12 +285,17 @@ __global__ __launch_bounds__(256) void k_jtv(GeoD g, TermsD T, const double* __r
                     g.alpha * (dh[1] - dmh * m2 - mh * v2);
   const double j3 = g.chg * (v2 * h[0] + m2 * dh[0] - v1 * h[1] - m1 * dh[1]) +
                     g.alpha * (dh[2] - dmh * m3 - mh * v3);
-  if (g.compact) {
-    Jv[cell] = j1; Jv[N + cell] = j2; Jv[2 * N + cell] = j3;
-  } else {
+  double o1 = j1, o2 = j2, o3 = j3;
+  if (!g.compact) {
     const double gt = gate_of(g, T.gate, cell);
-    Jv[cell] = gt * j1; Jv[N + cell] = gt * j2; Jv[2 * N + cell] = gt * j3;
+    o1 = gt * j1; o2 = gt * j2; o3 = gt * j3;
   }
+  if (AT) {
+#pragma clang fp contract(off)
+    const double p1 = mgamma * o1, p2 = mgamma * o2, p3 = mgamma * o3;
+    o1 = v1 + p1; o2 = v2 + p2; o3 = v3 + p3;
+  }
+  Jv[cell] = o1; Jv[N + cell] = o2; Jv[2 * N + cell] = o3;
 }
 
 // ---------------------------------------------------------- block-Jacobi
@@ -731,11 +753,31 @@ int JtvProduct(N_Vector v, N_Vector Jv, sunrealtype t, N_Vector y, N_Vector fy, 
   dim3 grid, block;
   grid_for(c, grid, block);
   kprof::begin(kprof::JTV, s);
-  hipLaunchKernelGGL(k_jtv, grid, block, 0, s, geo_of(c), to_dev(c->cfg.jv),
+  hipLaunchKernelGGL(k_jtv<false>, grid, block, 0, s, geo_of(c), to_dev(c->cfg.jv),
                      (const double*)N_VGetDeviceArrayPointer(y),
-                     (const double*)N_VGetDeviceArrayPointer(v), N_VGetDeviceArrayPointer(Jv));
+                     (const double*)N_VGetDeviceArrayPointer(v), N_VGetDeviceArrayPointer(Jv),
+                     0.0);
   kprof::end(kprof::JTV, s);
   return launch_status("jtv");
+}
+
+int LLG_Atimes(N_Vector v, N_Vector z, sunrealtype gamma, sunrealtype t, N_Vector y,
+               N_Vector fy, void* user_data) {
+  (void)t; (void)fy;
+  llg_ctx* c = (llg_ctx*)user_data;
+  if (!c || !v || !z || !y || N_VGetDeviceArrayPointer(v) == N_VGetDeviceArrayPointer(z))
+    return -1;
+  c->njv++;
+  hipStream_t s = stream_of(z, c);
+  dim3 grid, block;
+  grid_for(c, grid, block);
+  kprof::begin(kprof::JTV, s);
+  hipLaunchKernelGGL(k_jtv<true>, grid, block, 0, s, geo_of(c), to_dev(c->cfg.jv),
+                     (const double*)N_VGetDeviceArrayPointer(y),
+                     (const double*)N_VGetDeviceArrayPointer(v), N_VGetDeviceArrayPointer(z),
+                     -gamma);
+  kprof::end(kprof::JTV, s);
+  return launch_status("atimes");
 }
 
 int PrecondSetup(sunrealtype t, N_Vector y, N_Vector fy, sunbooleantype jok,

@@ -1,5 +1,5 @@
 // nvector_hip.hip -- HIP N_Vector for gfx950 with fused ops and
-// deterministic single-launch reductions.
+// deterministic reductions.
 //
 // Replaces NVECTOR_CUDA and the reference's fused layer
 // (src/irregular/2d_fft/2d_i4/deferred_nvector.cu:95-235,
@@ -7,12 +7,16 @@
 //
 // Design (MI355X): every vector carries the stream its ops launch on.
 // Element-wise ops are grid-stride kernels over 64-wide wavefronts.
-// A reduction is ONE launch: each workgroup reduces its slice (wave
-// shuffles, then LDS across the 4 waves), writes a partial in a fixed
-// slot, and the last workgroup to finish (atomic ticket) sums the
+// A reduction is a streaming launch whose workgroups each reduce a slice
+// (independent accumulators, wave shuffles, LDS across the 4 waves) into
+// a fixed partial slot, followed by a one-workgroup launch that sums the
 // partials in slot order and stores the K results straight into pinned
-// host memory.  The host then waits on the stream once.  The result does
-// not depend on workgroup scheduling, unlike the reference's atomicAdd
+// host memory; the host then waits once.  The streaming launch has no
+// device-scope fence or atomic: on the multi-XCD part those write back
+// and invalidate the XCD's L2, which costs more than the second launch.
+// (A single-launch form, where the last workgroup to take an atomic
+// ticket does the final sum, is kept selectable.)  The result does not
+// depend on workgroup scheduling, unlike the reference's atomicAdd
 // accumulation (deferred_nvector.cu:127-129).
 #include <hip/hip_runtime.h>
 
@@ -26,11 +30,17 @@
 
 #include "llg/nvector_hip.h"
 
+// Floating-point contraction is off in this file: every a*b+c that should
+// round once is written as fma(), everything else rounds each operation.
+// The arithmetic of each op is therefore fixed by the source, and a fused
+// kernel that repeats an op sequence reproduces its bits.
+#pragma clang fp contract(off)
+
 namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-constexpr int kMaxRedBlocks = 2048;
+constexpr int kMaxRedBlocks = 4096;
 constexpr int kMaxK = NVHIP_FUSED_MAX;
 
 struct NvContent {
@@ -47,11 +57,18 @@ inline sunindextype LEN(N_Vector v) { return C(v)->n; }
 inline hipStream_t S(N_Vector v) { return C(v)->stream; }
 
 // ----------------------------------------------------- reduction scratch
+// Result slots: pinned host memory holding kMaxK results + a sequence
+// word each.  Slot 0 serves every reduction; slot 1 holds the multi-dot of
+// a Gram-Schmidt projection while its combination + norm uses slot 0, so
+// one host wait covers both.  `dres` is a device copy of slot 1 that the
+// combination kernel reads its coefficients from.
+constexpr int kSlot = kMaxK + 1;
 struct RedWs {
   double* partials = nullptr;   // kMaxRedBlocks * kMaxK
   unsigned* ticket = nullptr;   // zero between launches
-  double* host = nullptr;       // pinned, kMaxK results + 1 sequence slot
+  double* host = nullptr;       // pinned, 2 slots of kSlot doubles
   double* host_dev = nullptr;   // device alias of host
+  double* dres = nullptr;       // device memory, kMaxK doubles
   unsigned seq = 0;             // last sequence number issued on this stream
 };
 
@@ -61,6 +78,8 @@ hipStream_t g_default_stream = nullptr;
 bool g_default_stream_made = false;
 long g_syncs = 0;
 int g_sync_mode = 0;   // 0: hipStreamSynchronize, 1: spin on the pinned sequence word
+int g_red_blocks = 512;   // partial workgroups per reduction (upper bound)
+int g_red_mode = 1;       // 0: single launch with ticket, 1: partial + finish launches
 
 RedWs* workspace(hipStream_t s) {
   std::lock_guard<std::mutex> lk(g_mu);
@@ -69,13 +88,14 @@ RedWs* workspace(hipStream_t s) {
   RedWs* w = new RedWs();
   if (hipMalloc(&w->partials, sizeof(double) * kMaxRedBlocks * kMaxK) != hipSuccess ||
       hipMalloc(&w->ticket, sizeof(unsigned)) != hipSuccess ||
-      hipHostMalloc(&w->host, sizeof(double) * (kMaxK + 1), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipMalloc(&w->dres, sizeof(double) * kMaxK) != hipSuccess ||
+      hipHostMalloc(&w->host, sizeof(double) * 2 * kSlot, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
       hipHostGetDevicePointer((void**)&w->host_dev, w->host, 0) != hipSuccess) {
     fprintf(stderr, "[nvector_hip] reduction workspace allocation failed\n");
     abort();
   }
   hipMemset(w->ticket, 0, sizeof(unsigned));
-  std::memset(w->host, 0, sizeof(double) * (kMaxK + 1));
+  std::memset(w->host, 0, sizeof(double) * 2 * kSlot);
   hipDeviceSynchronize();
   g_ws[s] = w;
   return w;
@@ -87,9 +107,9 @@ inline int ew_grid(sunindextype n) {
   return (int)(g < 1 ? 1 : g);
 }
 inline int red_grid(sunindextype n) {
-  // ~4 elements per thread: enough loads in flight to cover HBM latency
-  sunindextype g = (n + 4 * kBlock - 1) / (4 * kBlock);
-  if (g > kMaxRedBlocks) g = kMaxRedBlocks;
+  // at least ~8 elements per thread, at most g_red_blocks workgroups
+  sunindextype g = (n + 8 * kBlock - 1) / (8 * kBlock);
+  if (g > g_red_blocks) g = g_red_blocks;
   return (int)(g < 1 ? 1 : g);
 }
 
@@ -107,7 +127,7 @@ void check_launch(const char* what) {
 
 __global__ void k_linearsum(double a, const double* __restrict__ x, double b,
                             const double* __restrict__ y, double* z, sunindextype n) {
-  GRID_STRIDE(i, n) z[i] = a * x[i] + b * y[i];
+  GRID_STRIDE(i, n) z[i] = fma(a, x[i], b * y[i]);
 }
 __global__ void k_const(double c, double* z, sunindextype n) { GRID_STRIDE(i, n) z[i] = c; }
 __global__ void k_prod(const double* x, const double* y, double* z, sunindextype n) {
@@ -136,12 +156,35 @@ struct PtrPack { const double* p[kMaxK]; };
 struct OutPack { double* p[kMaxK]; };
 struct CoefPack { double c[kMaxK]; };
 
+// sum_j c_j X_j[i] in the order every linear-combination kernel uses
+__device__ inline double lincomb_elem(int nv, const CoefPack& c, const PtrPack& X,
+                                      sunindextype i) {
+  double s = c.c[0] * X.p[0][i];
+  for (int j = 1; j < nv; j++) s = fma(c.c[j], X.p[j][i], s);
+  return s;
+}
 // z = sum_j c_j X_j   (z may alias any X_j: each element read before write)
 __global__ void k_lincomb(int nv, CoefPack c, PtrPack X, double* z, sunindextype n) {
+  GRID_STRIDE(i, n) z[i] = lincomb_elem(nv, c, X, i);
+}
+// x = (0 + sum_j c_j X_j) / s: a linear combination into a zeroed vector
+// followed by N_VDiv, the SPGMR solution update
+__global__ void k_lincombdiv(int nv, CoefPack c, PtrPack X, const double* __restrict__ sv,
+                             double* x, sunindextype n) {
   GRID_STRIDE(i, n) {
-    double s = c.c[0] * X.p[0][i];
-    for (int j = 1; j < nv; j++) s += c.c[j] * X.p[j][i];
-    z[i] = s;
+    double acc = 1.0 * 0.0;
+    for (int j = 0; j < nv; j++) acc = fma(c.c[j], X.p[j][i], acc);
+    x[i] = acc / sv[i];
+  }
+}
+// xo = c x, vo = xo / s  (the Krylov basis normalisation fused with the
+// next iteration's unscaling)
+__global__ void k_scalediv(double c, const double* x, const double* __restrict__ sv, double* xo,
+                           double* vo, sunindextype n) {
+  GRID_STRIDE(i, n) {
+    const double a = c * x[i];
+    xo[i] = a;
+    vo[i] = a / sv[i];
   }
 }
 // Z_j = a_j x + Y_j
@@ -149,14 +192,14 @@ __global__ void k_scaleaddmulti(int nv, CoefPack a, const double* x, PtrPack Y, 
                                 sunindextype n) {
   GRID_STRIDE(i, n) {
     const double xi = x[i];
-    for (int j = 0; j < nv; j++) Z.p[j][i] = a.c[j] * xi + Y.p[j][i];
+    for (int j = 0; j < nv; j++) Z.p[j][i] = fma(a.c[j], xi, Y.p[j][i]);
   }
 }
 // Z_j = a X_j + b Y_j
 __global__ void k_lsumarray(int nv, double a, PtrPack X, double b, PtrPack Y, OutPack Z,
                             sunindextype n) {
   GRID_STRIDE(i, n) {
-    for (int j = 0; j < nv; j++) Z.p[j][i] = a * X.p[j][i] + b * Y.p[j][i];
+    for (int j = 0; j < nv; j++) Z.p[j][i] = fma(a, X.p[j][i], b * Y.p[j][i]);
   }
 }
 // Z_j = c_j X_j
@@ -183,7 +226,7 @@ __global__ void k_pascal(OutPack Z, double sign, sunindextype n) {
 #pragma unroll
     for (int k = 1; k <= Q; k++)
 #pragma unroll
-      for (int j = Q; j >= k; j--) z[j - 1] = 1.0 * z[j - 1] + sign * z[j];
+      for (int j = Q; j >= k; j--) z[j - 1] = fma(sign, z[j], z[j - 1]);
 #pragma unroll
     for (int j = 0; j < Q; j++) Z.p[j][i] = z[j];
   }
@@ -193,7 +236,9 @@ __global__ void k_ewt(const double* __restrict__ y, double rtol, double atol,
                       const double* __restrict__ av, double* __restrict__ w, sunindextype n) {
   GRID_STRIDE(i, n) {
     const double t = fabs(y[i]);
-    const double d = av ? rtol * t + 1.0 * av[i] : rtol * t + atol;
+    // vector atol: N_VLinearSum(rtol, |y|, 1, atol) = fma(rtol, |y|, atol);
+    // scalar atol: N_VScale then N_VAddConst (two roundings)
+    const double d = av ? fma(rtol, t, av[i]) : rtol * t + atol;
     w[i] = 1.0 / d;
   }
 }
@@ -248,6 +293,58 @@ struct FConstr {
   }
 };
 
+// z = sum_j c_j X_j (written), contribution z_i^2: the linear combination
+// and the squared norm of its result in one pass
+struct FLincombSq {
+  int nv; CoefPack c; PtrPack X; double* z;
+  __device__ double operator()(sunindextype i) const {
+    const double zi = lincomb_elem(nv, c, X, i);
+    z[i] = zi;
+    return zi * zi;
+  }
+};
+
+// Reductions accumulate rounded terms with separate adds, so a fused
+// producer that hands the same terms to the same accumulation loop gets
+// the same bits.
+
+// z = s .* t (written), contribution z_i * y_i (y = z when self): the
+// product and one dot in one pass, in the single-dot accumulation order
+struct FProdDot {
+  const double *s, *t, *y; double* z; bool self;
+  __device__ double operator()(sunindextype i) const {
+    const double zi = s[i] * t[i];
+    z[i] = zi;
+    return zi * (self ? zi : y[i]);
+  }
+};
+// z = X_0 - sum_{j>=1} d_{j-1} X_j with the coefficients d read from device
+// memory (a Gram-Schmidt multi-dot's results), contribution z_i^2.  Same
+// arithmetic as lincomb_elem with c = {1, -d_0, -d_1, ...}.
+struct FGsLincombSq {
+  int nv; const double* d; PtrPack X; double* z;
+  __device__ double operator()(sunindextype i) const {
+    double zi = 1.0 * X.p[0][i];
+    for (int j = 1; j < nv; j++) zi = fma(-d[j - 1], X.p[j][i], zi);
+    z[i] = zi;
+    return zi * zi;
+  }
+};
+
+// first Newton iteration: ycor += del, y = zn0 + ycor, contribution
+// (del w)^2 -- the FWsq term of N_VWrmsNorm(del, w)
+struct FNewtonUpd0 {
+  const double *del, *w, *zn0; double *ycor, *y;
+  __device__ double operator()(sunindextype i) const {
+    const double d = del[i];
+    const double c = fma(1.0, ycor[i], 1.0 * d);
+    ycor[i] = c;
+    y[i] = fma(1.0, zn0[i], 1.0 * c);
+    const double p = d * w[i];
+    return p * p;
+  }
+};
+
 // Block-wide reduction of K values per thread; result valid in thread 0.
 template <int K>
 __device__ void block_reduce(int kind, double (&v)[K], double (*sm)[kWaves]) {
@@ -271,7 +368,7 @@ __device__ void block_reduce(int kind, double (&v)[K], double (*sm)[kWaves]) {
 
 // Final pass run by the last workgroup: partials[b*K + k] for b < nblk.
 template <int K>
-__device__ void finish(int kind, int nblk, const double* partials, double* out,
+__device__ void finish(int kind, int nblk, const double* partials, double* out, double* dout,
                        double (*sm)[kWaves], unsigned seq) {
   double v[K];
 #pragma unroll
@@ -290,6 +387,10 @@ __device__ void finish(int kind, int nblk, const double* partials, double* out,
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; k++) out[k] = v[k];
+    if (dout) {
+#pragma unroll
+      for (int k = 0; k < K; k++) dout[k] = v[k];
+    }
     // results first, then the sequence word the host may spin on
     __threadfence_system();
     __hip_atomic_store(reinterpret_cast<unsigned*>(out + kMaxK), seq, __ATOMIC_RELEASE,
@@ -300,7 +401,7 @@ __device__ void finish(int kind, int nblk, const double* partials, double* out,
 template <class F>
 __global__ __launch_bounds__(kBlock) void k_reduce1(int kind, F f, sunindextype n,
                                                     double* partials, unsigned* ticket,
-                                                    double* out, unsigned seq) {
+                                                    double* out, double* dout, unsigned seq) {
   __shared__ double sm[1][kWaves];
   __shared__ bool last;
   // four independent accumulators so four loads per thread are in flight
@@ -317,6 +418,10 @@ __global__ __launch_bounds__(kBlock) void k_reduce1(int kind, F f, sunindextype 
   for (; i < n; i += stride) a0 = combine(kind, a0, f(i));
   double v[1] = {combine(kind, combine(kind, a0, a1), combine(kind, a2, a3))};
   block_reduce<1>(kind, v, sm);
+  if (!ticket) {
+    if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
+    return;
+  }
   if (threadIdx.x == 0) {
     partials[blockIdx.x] = v[0];
     __threadfence();
@@ -326,17 +431,21 @@ __global__ __launch_bounds__(kBlock) void k_reduce1(int kind, F f, sunindextype 
   __syncthreads();
   if (last) {
     __threadfence();
-    finish<1>(kind, gridDim.x, partials, out, sm, seq);
+    finish<1>(kind, gridDim.x, partials, out, dout, sm, seq);
     if (threadIdx.x == 0) *ticket = 0u;
   }
 }
 
 // K dot products x . Y_k (and K weighted squares when W given)
+// x_i = (prod ? s_i * t_i : x_i); dots_k = sum_i x_i Y_k[i], where a Y_k
+// flagged in `self` is x itself.  With prod set, x is also written (the
+// Krylov vector z = s .* t and its Gram-Schmidt dots in one pass).
 template <int K>
-__global__ __launch_bounds__(kBlock) void k_dotmulti(const double* __restrict__ x, PtrPack Y,
-                                                     sunindextype n, double* partials,
-                                                     unsigned* ticket, double* out,
-                                                     unsigned seq) {
+__global__ __launch_bounds__(kBlock) void k_dotmulti(const double* x, PtrPack Y, sunindextype n,
+                                                     double* partials, unsigned* ticket,
+                                                     double* out, double* dout, unsigned seq,
+                                                     const double* sv, const double* tv,
+                                                     double* xw, unsigned self) {
   __shared__ double sm[K][kWaves];
   __shared__ bool last;
   double v[K], u[K];
@@ -344,22 +453,40 @@ __global__ __launch_bounds__(kBlock) void k_dotmulti(const double* __restrict__ 
   for (int k = 0; k < K; k++) v[k] = u[k] = 0.0;
   const sunindextype stride = (sunindextype)gridDim.x * blockDim.x;
   sunindextype i = (sunindextype)blockIdx.x * blockDim.x + threadIdx.x;
+  auto xval = [&](sunindextype j) -> double {
+    if (sv) {
+      const double a = sv[j] * tv[j];
+      xw[j] = a;
+      return a;
+    }
+    return x[j];
+  };
+  auto yval = [&](int k, sunindextype j, double xj) -> double {
+    return ((self >> k) & 1u) ? xj : Y.p[k][j];
+  };
   for (; i + stride < n; i += 2 * stride) {
-    const double x0 = x[i], x1 = x[i + stride];
+    const double x0 = xval(i), x1 = xval(i + stride);
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      v[k] += x0 * Y.p[k][i];
-      u[k] += x1 * Y.p[k][i + stride];
+      v[k] += x0 * yval(k, i, x0);
+      u[k] += x1 * yval(k, i + stride, x1);
     }
   }
   for (; i < n; i += stride) {
-    const double x0 = x[i];
+    const double x0 = xval(i);
 #pragma unroll
-    for (int k = 0; k < K; k++) v[k] += x0 * Y.p[k][i];
+    for (int k = 0; k < K; k++) v[k] += x0 * yval(k, i, x0);
   }
 #pragma unroll
   for (int k = 0; k < K; k++) v[k] += u[k];
   block_reduce<K>(RED_SUM, v, sm);
+  if (!ticket) {
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < K; k++) partials[blockIdx.x * K + k] = v[k];
+    }
+    return;
+  }
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; k++) partials[blockIdx.x * K + k] = v[k];
@@ -370,16 +497,25 @@ __global__ __launch_bounds__(kBlock) void k_dotmulti(const double* __restrict__ 
   __syncthreads();
   if (last) {
     __threadfence();
-    finish<K>(RED_SUM, gridDim.x, partials, out, sm, seq);
+    finish<K>(RED_SUM, gridDim.x, partials, out, dout, sm, seq);
     if (threadIdx.x == 0) *ticket = 0u;
   }
 }
+
+// Newton update fused into a 2-vector WRMS pass: X_1 = ycor + X_0 is
+// formed (ycor overwritten) and y = zn0 + X_1 written, as the two
+// N_VLinearSum calls it replaces would.
+struct NewtonUpd {
+  double* ycor = nullptr;
+  const double* zn0 = nullptr;
+  double* y = nullptr;
+};
 
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_wsqarray(PtrPack X, PtrPack W, const double* id,
                                                      sunindextype n, double* partials,
                                                      unsigned* ticket, double* out,
-                                                     unsigned seq) {
+                                                     double* dout, unsigned seq, NewtonUpd nu) {
   __shared__ double sm[K][kWaves];
   __shared__ bool last;
   double v[K], u[K];
@@ -387,13 +523,22 @@ __global__ __launch_bounds__(kBlock) void k_wsqarray(PtrPack X, PtrPack W, const
   for (int k = 0; k < K; k++) v[k] = u[k] = 0.0;
   const sunindextype stride = (sunindextype)gridDim.x * blockDim.x;
   sunindextype i = (sunindextype)blockIdx.x * blockDim.x + threadIdx.x;
+  auto xv = [&](int k, sunindextype j) -> double {
+    if (nu.ycor && k == 1) {
+      const double c = fma(1.0, nu.ycor[j], 1.0 * X.p[0][j]);
+      nu.ycor[j] = c;
+      nu.y[j] = fma(1.0, nu.zn0[j], 1.0 * c);
+      return c;
+    }
+    return X.p[k][j];
+  };
   for (; i + stride < n; i += 2 * stride) {
     const bool on0 = id ? (id[i] > 0.0) : true;
     const bool on1 = id ? (id[i + stride] > 0.0) : true;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      const double p0 = X.p[k][i] * W.p[k][i];
-      const double p1 = X.p[k][i + stride] * W.p[k][i + stride];
+      const double p0 = xv(k, i) * W.p[k][i];
+      const double p1 = xv(k, i + stride) * W.p[k][i + stride];
       v[k] += on0 ? p0 * p0 : 0.0;
       u[k] += on1 ? p1 * p1 : 0.0;
     }
@@ -402,13 +547,20 @@ __global__ __launch_bounds__(kBlock) void k_wsqarray(PtrPack X, PtrPack W, const
     const bool on = id ? (id[i] > 0.0) : true;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      const double p = X.p[k][i] * W.p[k][i];
+      const double p = xv(k, i) * W.p[k][i];
       v[k] += on ? p * p : 0.0;
     }
   }
 #pragma unroll
   for (int k = 0; k < K; k++) v[k] += u[k];
   block_reduce<K>(RED_SUM, v, sm);
+  if (!ticket) {
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < K; k++) partials[blockIdx.x * K + k] = v[k];
+    }
+    return;
+  }
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; k++) partials[blockIdx.x * K + k] = v[k];
@@ -419,8 +571,44 @@ __global__ __launch_bounds__(kBlock) void k_wsqarray(PtrPack X, PtrPack W, const
   __syncthreads();
   if (last) {
     __threadfence();
-    finish<K>(RED_SUM, gridDim.x, partials, out, sm, seq);
+    finish<K>(RED_SUM, gridDim.x, partials, out, dout, sm, seq);
     if (threadIdx.x == 0) *ticket = 0u;
+  }
+}
+
+// Second launch of the two-launch form: one workgroup sums the nblk
+// partials (slot order) and publishes the K results.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_finish(int kind, int nblk, const double* partials,
+                                                   double* out, double* dout, unsigned seq) {
+  __shared__ double sm[K][kWaves];
+  finish<K>(kind, nblk, partials, out, dout, sm, seq);
+}
+
+template <int K>
+void launch_finish(hipStream_t s, int kind, int nblk, const RedWs* w, unsigned seq,
+                   int slot = 0) {
+  hipLaunchKernelGGL(k_finish<K>, dim3(1), dim3(kBlock), 0, s, kind, nblk, w->partials,
+                     w->host_dev + slot * kSlot, slot ? w->dres : nullptr, seq);
+}
+void finish_dispatch(int K, hipStream_t s, int kind, int nblk, const RedWs* w, unsigned seq) {
+  switch (K) {
+    case 1: launch_finish<1>(s, kind, nblk, w, seq); break;
+    case 2: launch_finish<2>(s, kind, nblk, w, seq); break;
+    case 3: launch_finish<3>(s, kind, nblk, w, seq); break;
+    case 4: launch_finish<4>(s, kind, nblk, w, seq); break;
+    case 5: launch_finish<5>(s, kind, nblk, w, seq); break;
+    case 6: launch_finish<6>(s, kind, nblk, w, seq); break;
+    case 7: launch_finish<7>(s, kind, nblk, w, seq); break;
+    case 8: launch_finish<8>(s, kind, nblk, w, seq); break;
+    case 9: launch_finish<9>(s, kind, nblk, w, seq); break;
+    case 10: launch_finish<10>(s, kind, nblk, w, seq); break;
+    case 11: launch_finish<11>(s, kind, nblk, w, seq); break;
+    case 12: launch_finish<12>(s, kind, nblk, w, seq); break;
+    case 13: launch_finish<13>(s, kind, nblk, w, seq); break;
+    case 14: launch_finish<14>(s, kind, nblk, w, seq); break;
+    case 15: launch_finish<15>(s, kind, nblk, w, seq); break;
+    default: launch_finish<16>(s, kind, nblk, w, seq); break;
   }
 }
 
@@ -430,7 +618,7 @@ __global__ __launch_bounds__(kBlock) void k_wsqarray(PtrPack X, PtrPack W, const
 void wait_result(hipStream_t s, RedWs* w, unsigned seq) {
   g_syncs++;
   if (g_sync_mode == 1) {
-    const unsigned* word = reinterpret_cast<const unsigned*>(w->host + kMaxK);
+    const unsigned* word = reinterpret_cast<const unsigned*>(w->host + kMaxK);   // slot 0
     auto t0 = std::chrono::steady_clock::now();
     for (unsigned it = 0;; it++) {
       if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) return;
@@ -450,46 +638,62 @@ double reduce1(N_Vector v, int kind, F f) {
   const int g = red_grid(n);
   const unsigned seq = ++w->seq;
   hipLaunchKernelGGL(k_reduce1<F>, dim3(g), dim3(kBlock), 0, s, kind, f, n, w->partials,
-                     w->ticket, w->host_dev, seq);
+                     g_red_mode == 0 ? w->ticket : nullptr, w->host_dev, (double*)nullptr, seq);
+  if (g_red_mode != 0) finish_dispatch(1, s, kind, g, w, seq);
   check_launch("reduce");
   wait_result(s, w, seq);
   return w->host[0];
 }
 
+struct ProdIn {
+  const double* s = nullptr;   // when set, x = s .* t is formed and written to xw
+  const double* t = nullptr;
+  double* xw = nullptr;
+  unsigned self = 0;           // bit k: Y_k is x itself
+  int slot = 0;                // result slot (1 also fills the device copy)
+};
+
 template <int K>
 void launch_dotmulti(hipStream_t s, const double* x, const PtrPack& Y, sunindextype n,
-                     RedWs* w) {
-  hipLaunchKernelGGL(k_dotmulti<K>, dim3(red_grid(n)), dim3(kBlock), 0, s, x, Y, n,
-                     w->partials, w->ticket, w->host_dev, w->seq);
+                     RedWs* w, const ProdIn& pi) {
+  const int g = red_grid(n);
+  hipLaunchKernelGGL(k_dotmulti<K>, dim3(g), dim3(kBlock), 0, s, x, Y, n,
+                     w->partials, g_red_mode == 0 ? w->ticket : nullptr,
+                     w->host_dev + pi.slot * kSlot, pi.slot ? w->dres : nullptr, w->seq,
+                     pi.s, pi.t, pi.xw, pi.self);
+  if (g_red_mode != 0) launch_finish<K>(s, RED_SUM, g, w, w->seq, pi.slot);
 }
 
 void dotmulti_dispatch(int K, hipStream_t s, const double* x, const PtrPack& Y, sunindextype n,
-                       RedWs* w) {
+                       RedWs* w, const ProdIn& pi = ProdIn{}) {
   switch (K) {
-    case 1: launch_dotmulti<1>(s, x, Y, n, w); break;
-    case 2: launch_dotmulti<2>(s, x, Y, n, w); break;
-    case 3: launch_dotmulti<3>(s, x, Y, n, w); break;
-    case 4: launch_dotmulti<4>(s, x, Y, n, w); break;
-    case 5: launch_dotmulti<5>(s, x, Y, n, w); break;
-    case 6: launch_dotmulti<6>(s, x, Y, n, w); break;
-    case 7: launch_dotmulti<7>(s, x, Y, n, w); break;
-    case 8: launch_dotmulti<8>(s, x, Y, n, w); break;
-    case 9: launch_dotmulti<9>(s, x, Y, n, w); break;
-    case 10: launch_dotmulti<10>(s, x, Y, n, w); break;
-    case 11: launch_dotmulti<11>(s, x, Y, n, w); break;
-    case 12: launch_dotmulti<12>(s, x, Y, n, w); break;
-    case 13: launch_dotmulti<13>(s, x, Y, n, w); break;
-    case 14: launch_dotmulti<14>(s, x, Y, n, w); break;
-    case 15: launch_dotmulti<15>(s, x, Y, n, w); break;
-    default: launch_dotmulti<16>(s, x, Y, n, w); break;
+    case 1: launch_dotmulti<1>(s, x, Y, n, w, pi); break;
+    case 2: launch_dotmulti<2>(s, x, Y, n, w, pi); break;
+    case 3: launch_dotmulti<3>(s, x, Y, n, w, pi); break;
+    case 4: launch_dotmulti<4>(s, x, Y, n, w, pi); break;
+    case 5: launch_dotmulti<5>(s, x, Y, n, w, pi); break;
+    case 6: launch_dotmulti<6>(s, x, Y, n, w, pi); break;
+    case 7: launch_dotmulti<7>(s, x, Y, n, w, pi); break;
+    case 8: launch_dotmulti<8>(s, x, Y, n, w, pi); break;
+    case 9: launch_dotmulti<9>(s, x, Y, n, w, pi); break;
+    case 10: launch_dotmulti<10>(s, x, Y, n, w, pi); break;
+    case 11: launch_dotmulti<11>(s, x, Y, n, w, pi); break;
+    case 12: launch_dotmulti<12>(s, x, Y, n, w, pi); break;
+    case 13: launch_dotmulti<13>(s, x, Y, n, w, pi); break;
+    case 14: launch_dotmulti<14>(s, x, Y, n, w, pi); break;
+    case 15: launch_dotmulti<15>(s, x, Y, n, w, pi); break;
+    default: launch_dotmulti<16>(s, x, Y, n, w, pi); break;
   }
 }
 
 template <int K>
 void launch_wsq(hipStream_t s, const PtrPack& X, const PtrPack& W, const double* id,
-                sunindextype n, RedWs* w) {
-  hipLaunchKernelGGL(k_wsqarray<K>, dim3(red_grid(n)), dim3(kBlock), 0, s, X, W, id, n,
-                     w->partials, w->ticket, w->host_dev, w->seq);
+                sunindextype n, RedWs* w, const NewtonUpd& nu = NewtonUpd{}) {
+  const int g = red_grid(n);
+  hipLaunchKernelGGL(k_wsqarray<K>, dim3(g), dim3(kBlock), 0, s, X, W, id, n,
+                     w->partials, g_red_mode == 0 ? w->ticket : nullptr, w->host_dev,
+                     (double*)nullptr, w->seq, nu);
+  if (g_red_mode != 0) launch_finish<K>(s, RED_SUM, g, w, w->seq);
 }
 void wsq_dispatch(int K, hipStream_t s, const PtrPack& X, const PtrPack& W, const double* id,
                   sunindextype n, RedWs* w) {
@@ -550,6 +754,13 @@ long NVHip_NumSyncs(void) { return g_syncs; }
 int NVHip_SetSyncMode(int mode) {
   if (mode != 0 && mode != 1) return -1;
   g_sync_mode = mode;
+  return 0;
+}
+
+int NVHip_SetReduceConfig(int max_blocks, int mode) {
+  if (max_blocks < 1 || max_blocks > kMaxRedBlocks || (mode != 0 && mode != 1)) return -1;
+  g_red_blocks = max_blocks;
+  g_red_mode = mode;
   return 0;
 }
 
@@ -731,6 +942,10 @@ SUNErrCode N_VScaleAddMulti_Hip(int nvec, sunrealtype* a, N_Vector x, N_Vector* 
 
 SUNErrCode N_VDotProdMulti_Hip(int nvec, N_Vector x, N_Vector* Y, sunrealtype* dotprods) {
   if (nvec < 1) return SUN_SUCCESS;
+  if (nvec == 1) {   // one dot: the single-dot kernel (as SUNDIALS' generic layer)
+    dotprods[0] = N_VDotProd_Hip(x, Y[0]);
+    return SUN_SUCCESS;
+  }
   hipStream_t s = S(x);
   RedWs* w = workspace(s);
   for (int base = 0; base < nvec; base += kMaxK) {
@@ -866,6 +1081,148 @@ int NVHip_Pascal(int q, N_Vector* zn, sunrealtype sign) {
 int NVHip_EwtSet(N_Vector y, sunrealtype rtol, sunrealtype atol, N_Vector abstol, N_Vector w) {
   if (y->ops->nvgetvectorid != N_VGetVectorID_Hip) return -1;
   EW_LAUNCH(k_ewt, w, D(y), rtol, atol, abstol ? (const double*)D(abstol) : nullptr, D(w), LEN(w));
+  return 0;
+}
+
+int NVHip_ScaleDiv(sunrealtype c, N_Vector x, N_Vector sv, N_Vector xout, N_Vector vout) {
+  if (!x || !sv || !xout || !vout || LEN(x) != LEN(sv) || LEN(x) != LEN(xout) ||
+      LEN(x) != LEN(vout) || D(sv) == D(vout) || D(x) == D(vout))
+    return -1;
+  EW_LAUNCH(k_scalediv, x, c, D(x), D(sv), D(xout), D(vout), LEN(x));
+  return 0;
+}
+
+int NVHip_ProdDotMulti(N_Vector sv, N_Vector t, N_Vector z, int nvec, N_Vector* Y,
+                       sunrealtype* dots) {
+  if (!sv || !t || !z || nvec < 1 || nvec > kMaxK || LEN(sv) != LEN(z) || LEN(t) != LEN(z))
+    return -1;
+  PtrPack yp{};
+  ProdIn pi;
+  pi.s = D(sv);
+  pi.t = D(t);
+  pi.xw = D(z);
+  for (int j = 0; j < nvec; j++) {
+    if (!Y[j] || LEN(Y[j]) != LEN(z)) return -1;
+    yp.p[j] = D(Y[j]);
+    if (D(Y[j]) == D(z)) pi.self |= 1u << j;
+  }
+  if (nvec == 1) {
+    dots[0] = reduce1(z, RED_SUM, FProdDot{pi.s, pi.t, yp.p[0], pi.xw, pi.self != 0});
+    return 0;
+  }
+  hipStream_t s = S(z);
+  RedWs* w = workspace(s);
+  ++w->seq;
+  dotmulti_dispatch(nvec, s, D(z), yp, LEN(z), w, pi);
+  check_launch("proddotmulti");
+  wait_result(s, w, w->seq);
+  for (int j = 0; j < nvec; j++) dots[j] = w->host[j];
+  return 0;
+}
+
+int NVHip_LinCombDiv(int nvec, sunrealtype* c, N_Vector* X, N_Vector sv, N_Vector x) {
+  if (nvec < 1 || nvec > kMaxK || !sv || !x || LEN(sv) != LEN(x)) return -1;
+  CoefPack cp{};
+  PtrPack xp{};
+  for (int j = 0; j < nvec; j++) {
+    if (!X[j] || LEN(X[j]) != LEN(x)) return -1;
+    cp.c[j] = c[j];
+    xp.p[j] = D(X[j]);
+  }
+  EW_LAUNCH(k_lincombdiv, x, nvec, cp, xp, D(sv), D(x), LEN(x));
+  return 0;
+}
+
+int NVHip_NewtonUpdate(int iter, N_Vector ycor, N_Vector del, N_Vector zn0, N_Vector w,
+                       N_Vector y, sunrealtype* dnorm, sunrealtype* ynorm) {
+  if (!ycor || !del || !zn0 || !w || !y || !dnorm) return -1;
+  const sunindextype n = LEN(ycor);
+  if (LEN(del) != n || LEN(zn0) != n || LEN(w) != n || LEN(y) != n) return -1;
+  if (D(y) == D(ycor) || D(y) == D(del) || D(y) == D(zn0) || D(ycor) == D(del)) return -1;
+  if (iter == 0) {
+    const double sq = reduce1(ycor, RED_SUM, FNewtonUpd0{D(del), D(w), D(zn0), D(ycor), D(y)});
+    *dnorm = std::sqrt(sq / (double)n);
+    return 0;
+  }
+  if (!ynorm) return -1;
+  hipStream_t s = S(ycor);
+  RedWs* wk = workspace(s);
+  PtrPack xp{}, wp{};
+  xp.p[0] = D(del);
+  xp.p[1] = D(ycor);
+  wp.p[0] = wp.p[1] = D(w);
+  NewtonUpd nu;
+  nu.ycor = D(ycor);
+  nu.zn0 = D(zn0);
+  nu.y = D(y);
+  ++wk->seq;
+  launch_wsq<2>(s, xp, wp, nullptr, n, wk, nu);
+  check_launch("newtonupdate");
+  wait_result(s, wk, wk->seq);
+  *dnorm = std::sqrt(wk->host[0] / (double)n);
+  *ynorm = std::sqrt(wk->host[1] / (double)n);
+  return 0;
+}
+
+int NVHip_GSProject(N_Vector sv, N_Vector t, N_Vector z, int nvec, N_Vector* Y,
+                    sunrealtype* dots, sunrealtype* zz) {
+  if (!sv || !t || !z || nvec < 2 || nvec > kMaxK || !dots || !zz || Y[nvec - 1] != z ||
+      LEN(sv) != LEN(z) || LEN(t) != LEN(z))
+    return -1;
+  PtrPack yp{};
+  ProdIn pi;
+  pi.s = D(sv);
+  pi.t = D(t);
+  pi.xw = D(z);
+  pi.slot = 1;
+  FGsLincombSq f{};
+  f.nv = nvec;
+  f.z = D(z);
+  f.X.p[0] = D(z);
+  for (int j = 0; j < nvec; j++) {
+    if (!Y[j] || LEN(Y[j]) != LEN(z)) return -1;
+    yp.p[j] = D(Y[j]);
+    if (D(Y[j]) == D(z)) pi.self |= 1u << j;
+    if (j < nvec - 1) {
+      if (D(Y[j]) == D(z)) return -1;
+      f.X.p[j + 1] = D(Y[j]);
+    }
+  }
+  const sunindextype n = LEN(z);
+  hipStream_t s = S(z);
+  RedWs* w = workspace(s);
+  // 1) z = s .* t and its dots with Y (slot 1 + device copy), no host wait
+  ++w->seq;
+  dotmulti_dispatch(nvec, s, D(z), yp, n, w, pi);
+  const unsigned seq_dots = w->seq;
+  f.d = w->dres;
+  // 2) z -= sum d_j Y_j and <z, z> (slot 0); the one wait covers both
+  const int g = red_grid(n);
+  const unsigned seq = ++w->seq;
+  hipLaunchKernelGGL(k_reduce1<FGsLincombSq>, dim3(g), dim3(kBlock), 0, s, RED_SUM, f, n,
+                     w->partials, g_red_mode == 0 ? w->ticket : nullptr, w->host_dev,
+                     (double*)nullptr, seq);
+  if (g_red_mode != 0) finish_dispatch(1, s, RED_SUM, g, w, seq);
+  check_launch("gsproject");
+  wait_result(s, w, seq);
+  const unsigned got = *reinterpret_cast<volatile unsigned*>(w->host + kSlot + kMaxK);
+  if (got != seq_dots) hipStreamSynchronize(s);
+  for (int j = 0; j < nvec; j++) dots[j] = w->host[kSlot + j];
+  *zz = w->host[0];
+  return 0;
+}
+
+int NVHip_LinCombDot(int nvec, sunrealtype* c, N_Vector* X, N_Vector z, sunrealtype* zz) {
+  if (nvec < 1 || nvec > kMaxK || !z || !zz) return -1;
+  FLincombSq f{};
+  f.nv = nvec;
+  f.z = D(z);
+  for (int j = 0; j < nvec; j++) {
+    if (!X[j] || LEN(X[j]) != LEN(z)) return -1;
+    f.c.c[j] = c[j];
+    f.X.p[j] = D(X[j]);
+  }
+  *zz = reduce1(z, RED_SUM, f);
   return 0;
 }
 

@@ -103,6 +103,7 @@ SIGNATURES = {
     "N_VGetToHost_Hip": (C.c_int, [VP, DP]),
     "NVHip_DefaultStream": (VP, []),
     "NVHip_NumSyncs": (C.c_long, []),
+    "NVHip_SetReduceConfig": (C.c_int, [C.c_int, C.c_int]),
     "NVHip_SetSyncMode": (C.c_int, [C.c_int]),
     "N_VGetVectorID_Hip": (C.c_int, [VP]),
     "N_VClone_Hip": (VP, [VP]),
@@ -141,6 +142,12 @@ SIGNATURES = {
     "N_VWrmsNormMaskVectorArray_Hip": (C.c_int, [C.c_int, C.POINTER(VP), C.POINTER(VP), VP, DP]),
     "NVHip_Pascal": (C.c_int, [C.c_int, C.POINTER(VP), C.c_double]),
     "NVHip_EwtSet": (C.c_int, [VP, C.c_double, C.c_double, VP, VP]),
+    "NVHip_ScaleDiv": (C.c_int, [C.c_double, VP, VP, VP, VP]),
+    "NVHip_ProdDotMulti": (C.c_int, [VP, VP, VP, C.c_int, C.POINTER(VP), DP]),
+    "NVHip_LinCombDot": (C.c_int, [C.c_int, DP, C.POINTER(VP), VP, DP]),
+    "NVHip_GSProject": (C.c_int, [VP, VP, VP, C.c_int, C.POINTER(VP), DP, DP]),
+    "NVHip_LinCombDiv": (C.c_int, [C.c_int, DP, C.POINTER(VP), VP, VP]),
+    "NVHip_NewtonUpdate": (C.c_int, [C.c_int, VP, VP, VP, VP, VP, DP, DP]),
     "FusedNVec_Init": (None, [VP]),
     "FusedNVec_FreePool": (None, []),
     # demag_fft.h
@@ -151,6 +158,7 @@ SIGNATURES = {
     "Demag_SetStream": (None, [VP, VP]),
     "Demag_GetSelfCoupling": (None, [VP, DP, DP, DP]),
     "Demag_Destroy": (None, [VP]),
+    "Demag_SetLdsPadding": (C.c_int, [VP, C.c_int, C.c_int]),
     "Demag_SetPrecision": (C.c_int, [VP, C.c_int]),
     "Demag_GetPlanShape": (C.c_int, [VP, IP, IP, IP]),
     "Demag_GetSpectrum": (C.c_int, [VP, DP]),
@@ -178,6 +186,7 @@ SIGNATURES = {
     "LLG_DeviceSynchronize": (C.c_int, []),
     "LLG_Rhs": (C.c_int, [C.c_double, VP, VP, VP]),
     "JtvProduct": (C.c_int, [VP, VP, C.c_double, VP, VP, VP, VP]),
+    "LLG_Atimes": (C.c_int, [VP, VP, C.c_double, C.c_double, VP, VP, VP]),
     "PrecondSetup": (C.c_int, [C.c_double, VP, VP, C.c_int, C.POINTER(C.c_int), C.c_double, VP]),
     "PrecondSolve": (C.c_int, [C.c_double, VP, VP, VP, VP, C.c_double, C.c_double, C.c_int, VP]),
     "Precond_Create": (VP, [C.c_int, C.c_int, C.c_int]),

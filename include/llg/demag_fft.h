@@ -45,6 +45,10 @@ void Demag_Destroy(DemagData* d);
  * input gather and output field stay FP64).  fp32 = 0 restores FP64. */
 int  Demag_SetPrecision(DemagData* d, int fp32);
 
+/* LDS padding of the FFT passes: one spare complex slot per 2^shift values
+ * (31 disables it) for the row and the column plans.  A tuning knob:
+ * results do not depend on it. */
+int  Demag_SetLdsPadding(DemagData* d, int row_shift, int col_shift);
 /* introspection used by tests and the bench roofline */
 int  Demag_GetPlanShape(DemagData* d, int* px, int* py, int* padded);
 /* copy the (strength-scaled, 1/(px*py)-normalised) tensor spectrum to host:

@@ -60,10 +60,36 @@ typedef struct llgcv_stats {
  *   pascal(q, zn, sign): for k=1..q, j=q..k: zn[j-1] += sign*zn[j]
  *                        (the Nordsieck predict, sign=+1, and restore, -1)
  *   ewt(y, rtol, atol, abstol, w): w = 1/(rtol*|y| + atol) with the vector
- *                        abstol when non-NULL, else the scalar atol   */
+ *                        abstol when non-NULL, else the scalar atol
+ * and, inside SPGMR (each may be NULL; a non-zero return falls back to the
+ * standard ops):
+ *   scalediv(c, x, s, xout, vout): xout = c x; vout = xout / s
+ *   proddotmulti(s, t, z, k, Y, d): z = s .* t; d_j = <z, Y_j>, j < k
+ *   lincombdot(k, c, X, z, zz):   z = sum_j c_j X_j; *zz = <z, z>
+ *   gsproject(s, t, z, k, Y, d, zz): z = s .* t; d_j = <z, Y_j> (Y_{k-1}
+ *                        is z); z -= sum_{j<k-1} d_j Y_j; *zz = <z, z>
+ *   lincombdiv(k, c, X, s, x): x = (0 + sum_j c_j X_j) / s
+ * and in the Newton iteration:
+ *   newtonupd(iter, ycor, del, zn0, w, y, dn, yn): ycor += del;
+ *                        y = zn0 + ycor; *dn = WRMS(del, w) and, for
+ *                        iter > 0, *yn = WRMS(ycor, w) computed as
+ *                        N_VWrmsNormVectorArray does (iter 0: N_VWrmsNorm)
+ *   atimes(v, z, gamma, t, y, fy, user_data): z = v - gamma J v in one
+ *                        call, for use with the same jtimes callback (equal
+ *                        to jtimes + N_VLinearSum(1, v, -gamma, Jv, z))   */
 typedef struct llgcv_fused_ops {
   int (*pascal)(int q, N_Vector* zn, sunrealtype sign);
   int (*ewt)(N_Vector y, sunrealtype rtol, sunrealtype atol, N_Vector abstol, N_Vector w);
+  int (*scalediv)(sunrealtype c, N_Vector x, N_Vector s, N_Vector xout, N_Vector vout);
+  int (*proddotmulti)(N_Vector s, N_Vector t, N_Vector z, int k, N_Vector* Y, sunrealtype* d);
+  int (*lincombdot)(int k, sunrealtype* c, N_Vector* X, N_Vector z, sunrealtype* zz);
+  int (*gsproject)(N_Vector s, N_Vector t, N_Vector z, int k, N_Vector* Y, sunrealtype* d,
+                   sunrealtype* zz);
+  int (*lincombdiv)(int k, sunrealtype* c, N_Vector* X, N_Vector s, N_Vector x);
+  int (*newtonupd)(int iter, N_Vector ycor, N_Vector del, N_Vector zn0, N_Vector w, N_Vector y,
+                   sunrealtype* dn, sunrealtype* yn);
+  int (*atimes)(N_Vector v, N_Vector z, sunrealtype gamma, sunrealtype t, N_Vector y,
+                N_Vector fy, void* user_data);
 } llgcv_fused_ops;
 
 void* LLGCVodeCreate(int lmm, SUNContext sunctx);

@@ -23,7 +23,10 @@ typedef struct llg_run_opts {
   int    maxl;        /* KRYLOV_DIM */
   int    gstype;      /* 0 = auto (classical iff neq < 500000), else SUN_*_GS */
   int    max_order;   /* MAX_BDF_ORDER */
-  int    fused;       /* FusedNVec_Init on y */
+  int    fused;       /* bit 0: FusedNVec_Init on y (one-launch multi-dot and
+                         linear combination); bit 1: the integrator's one-pass
+                         kernels (Nordsieck, error weights, SPGMR products,
+                         Gram-Schmidt); default 3.  Bit 1 changes no value. */
   long   mxstep;      /* max steps per CVode call (CVODE default 500) */
   int    use_jtv;     /* analytic Jv (else difference quotient) */
   int    use_prec;    /* block-Jacobi preconditioner */

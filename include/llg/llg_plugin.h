@@ -131,6 +131,13 @@ int    LLG_DeviceSynchronize(void);
 int LLG_Rhs(sunrealtype t, N_Vector y, N_Vector ydot, void* user_data);
 int JtvProduct(N_Vector v, N_Vector Jv, sunrealtype t, N_Vector y, N_Vector fy,
                void* user_data, N_Vector tmp);
+/* The SPGMR operator of the Newton system in one launch:
+ * z = v - gamma * J(y) v, equal bit for bit to JtvProduct followed by
+ * N_VLinearSum(1, v, -gamma, Jv, z) (CVODE's cvLsATimes).  z must not
+ * alias v.  No reference counterpart: installed as the integrator's fused
+ * atimes by the driver (llgcv_fused_ops.atimes). */
+int LLG_Atimes(N_Vector v, N_Vector z, sunrealtype gamma, sunrealtype t, N_Vector y,
+               N_Vector fy, void* user_data);
 int PrecondSetup(sunrealtype t, N_Vector y, N_Vector fy, sunbooleantype jok,
                  sunbooleantype* jcurPtr, sunrealtype gamma, void* user_data);
 int PrecondSolve(sunrealtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z,

@@ -14,9 +14,9 @@
  *
  * Layout: one contiguous FP64 device array per vector (SoA [mx|my|mz] for
  * the LLG state), an optional host mirror, and the HIP stream every op of
- * the vector is launched on.  Reductions are deterministic: per-block
- * partials in fixed order, finished by the last block of the same launch,
- * result written to pinned host memory (one launch + one stream sync).
+ * the vector is launched on.  Reductions are deterministic: per-workgroup
+ * partials in fixed slots, summed in slot order by a one-workgroup finish
+ * launch that writes the result to pinned host memory (one host wait).
  */
 #ifndef LLG_NVECTOR_HIP_H
 #define LLG_NVECTOR_HIP_H
@@ -58,6 +58,11 @@ long  NVHip_NumSyncs(void);
 /* how the host waits for a reduction result: 0 = hipStreamSynchronize,
  * 1 = poll the pinned result's sequence word (bounded, falls back to 0) */
 int   NVHip_SetSyncMode(int mode);
+/* reduction launch shape: at most max_blocks (1..4096) partial workgroups;
+ * mode 0 = one launch (last workgroup sums the partials), 1 = partial
+ * launch + one-workgroup finish launch (default: 512, 1).  Results are
+ * deterministic for a fixed configuration and vector length. */
+int   NVHip_SetReduceConfig(int max_blocks, int mode);
 
 /* standard ops */
 N_Vector_ID  N_VGetVectorID_Hip(N_Vector v);
@@ -107,6 +112,37 @@ SUNErrCode N_VWrmsNormMaskVectorArray_Hip(int nvec, N_Vector* X, N_Vector* W, N_
  * N_VInv sequences. */
 int NVHip_Pascal(int q, N_Vector* zn, sunrealtype sign);
 int NVHip_EwtSet(N_Vector y, sunrealtype rtol, sunrealtype atol, N_Vector abstol, N_Vector w);
+
+/* One-pass kernels for the SPGMR loop (llgcv_fused_ops), each equal
+ * element for element, and reduction for reduction, to the N_Vector
+ * sequence it replaces:
+ *   NVHip_ScaleDiv     xout = c x; vout = xout / s
+ *                      (N_VScale then N_VDiv; vout may not alias x or s)
+ *   NVHip_ProdDotMulti z = s .* t; dots_j = <z, Y_j>, j < nvec <= 16,
+ *                      where Y_j may be z itself
+ *                      (N_VProd then N_VDotProdMulti)
+ *   NVHip_LinCombDot   z = sum_j c_j X_j (X_j may alias z); *zz = <z, z>
+ *                      (N_VLinearCombination then N_VDotProd), nvec <= 16
+ *   NVHip_GSProject    one classical Gram-Schmidt projection with ONE host
+ *                      wait: z = s .* t; dots_j = <z, Y_j> (j < nvec, and
+ *                      Y_{nvec-1} must be z); z -= sum_{j<nvec-1} dots_j Y_j;
+ *                      *zz = <z, z>.  The combination reads its coefficients
+ *                      from device memory, so it is queued behind the
+ *                      multi-dot without a round trip.  Equal to N_VProd,
+ *                      N_VDotProdMulti, N_VLinearCombination, N_VDotProd. */
+int NVHip_ScaleDiv(sunrealtype c, N_Vector x, N_Vector s, N_Vector xout, N_Vector vout);
+int NVHip_ProdDotMulti(N_Vector s, N_Vector t, N_Vector z, int nvec, N_Vector* Y,
+                       sunrealtype* dots);
+int NVHip_LinCombDot(int nvec, sunrealtype* c, N_Vector* X, N_Vector z, sunrealtype* zz);
+int NVHip_GSProject(N_Vector s, N_Vector t, N_Vector z, int nvec, N_Vector* Y,
+                    sunrealtype* dots, sunrealtype* zz);
+/* x = (0 + sum_j c_j X_j) / s (N_VConst 0, N_VLinearCombination, N_VDiv) */
+int NVHip_LinCombDiv(int nvec, sunrealtype* c, N_Vector* X, N_Vector s, N_Vector x);
+/* Newton update: ycor += del; y = zn0 + ycor; *dnorm = WRMS(del, w); for
+ * iter > 0 also *ynorm = WRMS(ycor, w) (as N_VWrmsNormVectorArray of the
+ * pair; iter 0 as N_VWrmsNorm).  y must not alias the inputs. */
+int NVHip_NewtonUpdate(int iter, N_Vector ycor, N_Vector del, N_Vector zn0, N_Vector w,
+                       N_Vector y, sunrealtype* dnorm, sunrealtype* ynorm);
 
 /* Fused-ops layer with the reference's entry-point names.
  * FusedNVec_Init installs the one-launch multi-dot and linear-combination

@@ -156,3 +156,20 @@ def test_integrate_square_hole_small():
     y = run.state()
     assert np.all(np.isfinite(y))
     assert np.all(y.reshape(3, -1)[:, geom.active == 0] == 0.0)
+
+
+@pytest.mark.parametrize("case", [CASES[1], CASES[2], CASES[7]], ids=["padded", "compact", "i3"])
+def test_atimes_equals_jtv_then_linear_sum(case):
+    """LLG_Atimes == JtvProduct + N_VLinearSum(1, v, -gamma, Jv), bit for bit"""
+    cfg, geom, y0 = make(case)
+    sim = la.Simulation(cfg, geom)
+    L = la.lib()
+    rng = np.random.default_rng(5)
+    vy = sim.vec(perturbed(y0, geom.active, 3))
+    vv = sim.vec(rng.standard_normal(sim.neq))
+    za, zb = sim.vec(np.zeros(sim.neq)), sim.vec(np.zeros(sim.neq))
+    gamma = 0.0371
+    assert L.JtvProduct(vv.v, za.v, 0.0, vy.v, None, sim.ctx, None) == 0
+    L.N_VLinearSum_Hip(1.0, vv.v, -gamma, za.v, za.v)
+    assert L.LLG_Atimes(vv.v, zb.v, gamma, 0.0, vy.v, None, sim.ctx) == 0
+    assert np.array_equal(za.get(), zb.get())

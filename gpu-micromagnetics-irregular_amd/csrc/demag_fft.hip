@@ -46,9 +46,13 @@ constexpr int kMaxPasses = 16;
 constexpr int kXcd = 8;
 
 // complex types of the two pipeline precisions
-template <class T> struct V2T;
-template <> struct V2T<double> { using type = double2; };
-template <> struct V2T<float> { using type = float2; };
+// a plain complex pair (no member functions, so it lives in any address
+// space); same size and alignment as double2 / float2
+template <class T>
+struct alignas(2 * sizeof(T)) Cplx {
+  T x, y;
+};
+template <class T> struct V2T { using type = Cplx<T>; };
 template <class T> using V2 = typename V2T<T>::type;
 
 template <class T>
@@ -65,9 +69,16 @@ struct Plan1D {
   int npass;
   int radix[kMaxPasses];
   int bpt[kMaxPasses];      // butterflies per thread per pass (<= 8)
+  int nsh[kMaxPasses];      // log2(Ns) of the pass when Ns is a power of two, else -1
+  int nbsh[kMaxPasses];     // log2(L / radix) when a power of two, else -1
   int psh;                  // LDS padding: one spare slot per 2^psh values (31: none)
   const V2<T>* tw;          // exp(-2 pi i q / L), q in [0, L)
 };
+
+// a complex value in LDS (the FFT helpers are always inlined into the
+// kernels, so these pointers resolve to the LDS address space)
+template <class T>
+using LdsV2 = V2<T>;
 
 template <class C2> __device__ inline C2 cmul(C2 a, C2 b) {
   C2 r;
@@ -170,27 +181,52 @@ __device__ __host__ inline int lpad_len(int L, int sh) { return L + (L >> sh) + 
 
 // One Stockham pass of radix R over ncomp transforms of length L stored at
 // buf + comp*cstride (padded index, in place: all reads, barrier, all
-// writes, barrier).
-template <int R, int B, class T>
-__device__ void pass(V2<T>* buf, int L, int Ns, T sign, const V2<T>* __restrict__ tw, int ncomp,
-                     int cstride, int sh) {
+// writes, barrier).  Butterfly b of the pass is (comp, j), j = grp*Ns + k;
+// the divisions become shifts when Ns and L/R are powers of two (nsh,
+// nbsh >= 0), which they are for every power-of-two length.
+template <int R, int B, int SIGN, class T>
+__device__ __forceinline__ void pass(LdsV2<T>* buf, int L, int Ns, const V2<T>* __restrict__ tw, int ncomp,
+                     int cstride, int sh, int nsh, int nbsh) {
+  const T sign = T(SIGN);
   const int nb = L / R;
   const int total = nb * ncomp;
   const int tstep = L / (Ns * R);
   V2<T> v[B][R];
+  int dbase[B];
 #pragma unroll
   for (int r = 0; r < B; r++) {
     const int b = threadIdx.x + r * blockDim.x;
+    dbase[r] = -1;
     if (b < total) {
-      const int comp = b / nb, j = b - comp * nb;
-      const V2<T>* src = buf + comp * cstride;
-      const int k = j % Ns;
+      int comp, j;
+      if (ncomp == 1) {
+        comp = 0;
+        j = b;
+      } else if (nbsh >= 0) {
+        comp = b >> nbsh;
+        j = b & (nb - 1);
+      } else {
+        comp = b / nb;
+        j = b - comp * nb;
+      }
+      int k, grp;
+      if (nsh >= 0) {
+        k = j & (Ns - 1);
+        grp = j >> nsh;
+      } else {
+        grp = j / Ns;
+        k = j - grp * Ns;
+      }
+      const LdsV2<T>* src = buf + comp * cstride;
+      dbase[r] = comp * cstride + grp * Ns * R + k;
+      const int tk = k * tstep;
 #pragma unroll
       for (int t = 0; t < R; t++) {
         V2<T> x = src[lpad(j + t * nb, sh)];
-        if (t > 0 && k > 0) {
-          V2<T> w = tw[t * k * tstep];
-          if (sign > 0) w.y = -w.y;
+        if (t > 0) {
+          // branch-free: tw[0] = (1, 0) leaves x unchanged when k == 0
+          V2<T> w = tw[t * tk];
+          if (SIGN > 0) w.y = -w.y;
           x = cmul(x, w);
         }
         v[r][t] = x;
@@ -200,13 +236,17 @@ __device__ void pass(V2<T>* buf, int L, int Ns, T sign, const V2<T>* __restrict_
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < B; r++) {
-    const int b = threadIdx.x + r * blockDim.x;
-    if (b < total) {
-      const int comp = b / nb, j = b - comp * nb;
-      V2<T>* dst = buf + comp * cstride;
-      const int k = j % Ns;
+    if (dbase[r] >= 0) {
       Dft<R, T>::run(v[r], sign);
-      const int base = (j / Ns) * Ns * R + k;
+      // dbase holds comp*cstride + the unpadded in-transform index: pad
+      // the in-transform part only
+      const int b = threadIdx.x + r * blockDim.x;
+      int comp;
+      if (ncomp == 1) comp = 0;
+      else if (nbsh >= 0) comp = b >> nbsh;
+      else comp = b / nb;
+      LdsV2<T>* dst = buf + comp * cstride;
+      const int base = dbase[r] - comp * cstride;
 #pragma unroll
       for (int t = 0; t < R; t++) dst[lpad(base + t * Ns, sh)] = v[r][t];
     }
@@ -218,35 +258,40 @@ __device__ void pass(V2<T>* buf, int L, int Ns, T sign, const V2<T>* __restrict_
 template <int R>
 struct MaxB { static constexpr int value = R == 8 ? 2 : (R == 5 ? 3 : (R == 2 ? 8 : 4)); };
 
-template <int R, int B, class T>
-__device__ inline void pass_if(V2<T>* buf, int L, int Ns, T sign, const V2<T>* tw, int ncomp,
-                               int cstride, int sh) {
-  if constexpr (B <= MaxB<R>::value) pass<R, B, T>(buf, L, Ns, sign, tw, ncomp, cstride, sh);
+template <int R, int B, int SIGN, class T>
+__device__ __forceinline__ void pass_if(LdsV2<T>* buf, int L, int Ns, const V2<T>* tw, int ncomp,
+                               int cstride, int sh, int nsh, int nbsh) {
+  if constexpr (B <= MaxB<R>::value)
+    pass<R, B, SIGN, T>(buf, L, Ns, tw, ncomp, cstride, sh, nsh, nbsh);
 }
 
-template <int R, class T>
-__device__ void pass_b(V2<T>* buf, int L, int Ns, T sign, const V2<T>* tw, int ncomp,
-                       int cstride, int B, int sh) {
+template <int R, int SIGN, class T>
+__device__ __forceinline__ void pass_b(LdsV2<T>* buf, int L, int Ns, const V2<T>* tw, int ncomp, int cstride,
+                       int B, int sh, int nsh, int nbsh) {
   switch (B) {
-    case 1: pass_if<R, 1, T>(buf, L, Ns, sign, tw, ncomp, cstride, sh); break;
-    case 2: pass_if<R, 2, T>(buf, L, Ns, sign, tw, ncomp, cstride, sh); break;
-    case 3: pass_if<R, 3, T>(buf, L, Ns, sign, tw, ncomp, cstride, sh); break;
-    case 4: pass_if<R, 4, T>(buf, L, Ns, sign, tw, ncomp, cstride, sh); break;
-    default: pass_if<R, 8, T>(buf, L, Ns, sign, tw, ncomp, cstride, sh); break;
+    case 1: pass_if<R, 1, SIGN, T>(buf, L, Ns, tw, ncomp, cstride, sh, nsh, nbsh); break;
+    case 2: pass_if<R, 2, SIGN, T>(buf, L, Ns, tw, ncomp, cstride, sh, nsh, nbsh); break;
+    case 3: pass_if<R, 3, SIGN, T>(buf, L, Ns, tw, ncomp, cstride, sh, nsh, nbsh); break;
+    case 4: pass_if<R, 4, SIGN, T>(buf, L, Ns, tw, ncomp, cstride, sh, nsh, nbsh); break;
+    default: pass_if<R, 8, SIGN, T>(buf, L, Ns, tw, ncomp, cstride, sh, nsh, nbsh); break;
   }
 }
 
-template <class T>
-__device__ void fft_lds(V2<T>* buf, const Plan1D<T>& p, T sign, int ncomp, int cstride) {
+// in-place FFT of ncomp transforms in LDS; SIGN -1 forward, +1 inverse.
+// Always inlined: a call would spill the plan (a kernel argument) to the
+// stack and turn the twiddle loads into flat loads.
+template <int SIGN, class T>
+__device__ __forceinline__ void fft_lds(LdsV2<T>* buf, const Plan1D<T>& p, int ncomp, int cstride) {
   int Ns = 1;
   for (int i = 0; i < p.npass; i++) {
     const int R = p.radix[i], B = p.bpt[i];
+    const int ns = p.nsh[i], nbs = p.nbsh[i];
     switch (R) {
-      case 8: pass_b<8, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B, p.psh); break;
-      case 4: pass_b<4, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B, p.psh); break;
-      case 2: pass_b<2, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B, p.psh); break;
-      case 3: pass_b<3, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B, p.psh); break;
-      default: pass_b<5, T>(buf, p.L, Ns, sign, p.tw, ncomp, cstride, B, p.psh); break;
+      case 8: pass_b<8, SIGN, T>(buf, p.L, Ns, p.tw, ncomp, cstride, B, p.psh, ns, nbs); break;
+      case 4: pass_b<4, SIGN, T>(buf, p.L, Ns, p.tw, ncomp, cstride, B, p.psh, ns, nbs); break;
+      case 2: pass_b<2, SIGN, T>(buf, p.L, Ns, p.tw, ncomp, cstride, B, p.psh, ns, nbs); break;
+      case 3: pass_b<3, SIGN, T>(buf, p.L, Ns, p.tw, ncomp, cstride, B, p.psh, ns, nbs); break;
+      default: pass_b<5, SIGN, T>(buf, p.L, Ns, p.tw, ncomp, cstride, B, p.psh, ns, nbs); break;
     }
     Ns *= R;
   }
@@ -255,13 +300,13 @@ __device__ void fft_lds(V2<T>* buf, const Plan1D<T>& p, T sign, int ncomp, int c
 // ----------------------------------------------------------- row forward
 // grid (rows, ncomp); in = base + comp*comp_stride + row*nx_in (FP64 input)
 template <class T>
-__global__ __launch_bounds__(1024) void k_row_fwd(const double* __restrict__ in,
+__global__ __launch_bounds__(512) void k_row_fwd(const double* __restrict__ in,
                                                   const double* __restrict__ w, long comp_stride,
                                                   int nx_in, int rows, Plan1D<T> plan,
                                                   const V2<T>* __restrict__ post,
                                                   V2<T>* __restrict__ spec) {
   extern __shared__ char lds_raw[];
-  V2<T>* lds = reinterpret_cast<V2<T>*>(lds_raw);
+  LdsV2<T>* lds = reinterpret_cast<LdsV2<T>*>(lds_raw);
   const int row = blockIdx.x, comp = blockIdx.y;
   const int L = plan.L, Hx = L + 1, sh = plan.psh;
   const double* src = in + comp * comp_stride + (long)row * nx_in;
@@ -274,7 +319,7 @@ __global__ __launch_bounds__(1024) void k_row_fwd(const double* __restrict__ in,
     lds[lpad(m, sh)] = mk<T>((T)a, (T)b);
   }
   __syncthreads();
-  fft_lds<T>(lds, plan, T(-1), 1, lpad_len(L, sh));
+  fft_lds<-1, T>(lds, plan, 1, lpad_len(L, sh));
   V2<T>* out = spec + ((long)comp * rows + row) * Hx;
   for (int k = threadIdx.x; k <= L; k += blockDim.x) {
     const V2<T> zk = lds[lpad(k == L ? 0 : k, sh)];
@@ -288,12 +333,12 @@ __global__ __launch_bounds__(1024) void k_row_fwd(const double* __restrict__ in,
 
 // ---------------------------------------------------------- row inverse
 template <class T>
-__global__ __launch_bounds__(1024) void k_row_inv(const V2<T>* __restrict__ spec, int rows,
+__global__ __launch_bounds__(512) void k_row_inv(const V2<T>* __restrict__ spec, int rows,
                                                   Plan1D<T> plan, const V2<T>* __restrict__ post,
                                                   double* __restrict__ out, long comp_stride,
                                                   int nx_out) {
   extern __shared__ char lds_raw[];
-  V2<T>* lds = reinterpret_cast<V2<T>*>(lds_raw);
+  LdsV2<T>* lds = reinterpret_cast<LdsV2<T>*>(lds_raw);
   const int row = blockIdx.x, comp = blockIdx.y;
   const int L = plan.L, Hx = L + 1, sh = plan.psh;
   const V2<T>* in = spec + ((long)comp * rows + row) * Hx;
@@ -305,7 +350,7 @@ __global__ __launch_bounds__(1024) void k_row_inv(const V2<T>* __restrict__ spec
     lds[lpad(k, sh)] = mk<T>(a.x - b.y, a.y + b.x);   // a + i b
   }
   __syncthreads();
-  fft_lds<T>(lds, plan, T(1), 1, lpad_len(L, sh));
+  fft_lds<1, T>(lds, plan, 1, lpad_len(L, sh));
   double* dst = out + comp * comp_stride + (long)row * nx_out;
   for (int m = threadIdx.x; m < L; m += blockDim.x) {
     const int n0 = 2 * m;
@@ -325,11 +370,11 @@ __device__ inline int xcd_remap(int bid, int G) {
 // mode 0: fwd FFT, 3x3 tensor product, inverse FFT (3 components)
 // mode 1: fwd FFT only (1 component; tensor spectra at init)
 template <class T>
-__global__ __launch_bounds__(1024) void k_col(V2<T>* __restrict__ spec, int rows_in,
+__global__ __launch_bounds__(512) void k_col(V2<T>* __restrict__ spec, int rows_in,
                                               int rows_out, int Hx, Plan1D<T> plan,
                                               const T* __restrict__ K, int has_xy_im, int mode) {
   extern __shared__ char lds_raw[];
-  V2<T>* lds = reinterpret_cast<V2<T>*>(lds_raw);
+  LdsV2<T>* lds = reinterpret_cast<LdsV2<T>*>(lds_raw);
   const int kx = xcd_remap(blockIdx.x, gridDim.x);
   const int Py = plan.L;
   const int ncomp = mode == 0 ? 3 : 1;
@@ -339,7 +384,7 @@ __global__ __launch_bounds__(1024) void k_col(V2<T>* __restrict__ spec, int rows
     for (int y = threadIdx.x; y < Py; y += blockDim.x)
       lds[c * cs + lpad(y, sh)] = y < rows_in ? spec[((long)c * rows_in + y) * Hx + kx] : mk<T>(T(0), T(0));
   __syncthreads();
-  fft_lds<T>(lds, plan, T(-1), ncomp, cs);
+  fft_lds<-1, T>(lds, plan, ncomp, cs);
   if (mode == 0) {
     const long P = (long)Hx * Py;
     const T* kxx = K + (long)kx * Py;
@@ -359,7 +404,7 @@ __global__ __launch_bounds__(1024) void k_col(V2<T>* __restrict__ spec, int rows
       lds[2 * cs + q] = mk<T>(cz * mz.x, cz * mz.y);
     }
     __syncthreads();
-    fft_lds<T>(lds, plan, T(1), 3, cs);
+    fft_lds<1, T>(lds, plan, 3, cs);
   }
   for (int c = 0; c < ncomp; c++)
     for (int y = threadIdx.x; y < rows_out; y += blockDim.x)
@@ -483,6 +528,7 @@ bool factorize(int L, std::vector<int>& rad) {
 template <class T>
 struct HostPlan {
   Plan1D<T> p{};
+  int ncomp = 1;
   int block = 64;
   V2<T>* d_tw = nullptr;
 };
@@ -502,8 +548,8 @@ bool make_plan(int L, int ncomp, HostPlan<T>& hp) {
   // the 3-component column plan pads every 16 so three workgroups still
   // fit one CU's 160 KB of LDS at Py = 1024 in FP64
   hp.p.psh = ncomp == 3 ? 4 : 3;
-  // smallest multiple-of-64 workgroup (>= 64) for which every pass fits its
-  // per-thread register budget
+  // default workgroup: the smallest multiple of 64 for which every pass
+  // fits its per-thread register budget, at least min(widest pass, 256)
   int block = 64;
   for (int r : rad) {
     const int tot = ncomp * L / r;
@@ -513,9 +559,20 @@ bool make_plan(int L, int ncomp, HostPlan<T>& hp) {
   int maxb = 1;
   for (int r : rad) maxb = std::max(maxb, ncomp * L / r);
   block = std::max(block, std::min(((maxb + 63) / 64) * 64, 256));
-  if (block > 1024) { derr("FFT length too large for one workgroup: " + std::to_string(L)); return false; }
+  if (block > 512) { derr("FFT length too large for one workgroup: " + std::to_string(L)); return false; }
+  hp.ncomp = ncomp;
   hp.block = block;
+  auto log2_if_pow2 = [](int x) {
+    if (x < 1 || (x & (x - 1))) return -1;
+    int l = 0;
+    while ((1 << l) < x) l++;
+    return l;
+  };
+  int ns_run = 1;
   for (int i = 0; i < hp.p.npass; i++) {
+    hp.p.nsh[i] = log2_if_pow2(ns_run);
+    hp.p.nbsh[i] = log2_if_pow2(L / rad[i]);
+    ns_run *= rad[i];
     hp.p.radix[i] = rad[i];
     const int tot = ncomp * L / rad[i];
     int b = (tot + block - 1) / block;
@@ -530,6 +587,24 @@ bool make_plan(int L, int ncomp, HostPlan<T>& hp) {
   if (hipMalloc(&hp.d_tw, sizeof(V2<T>) * L) != hipSuccess) { derr("hipMalloc twiddles"); return false; }
   hipMemcpy(hp.d_tw, tw.data(), sizeof(V2<T>) * L, hipMemcpyHostToDevice);
   hp.p.tw = hp.d_tw;
+  return true;
+}
+
+// workgroup size override (multiple of 64, <= 1024); every pass must fit
+// its register budget at the resulting butterflies per thread
+template <class T>
+bool set_block(HostPlan<T>& hp, int block) {
+  if (block < 64 || block > 512 || block % 64) return false;
+  int bpt[kMaxPasses];
+  for (int i = 0; i < hp.p.npass; i++) {
+    const int r = hp.p.radix[i];
+    int b = (hp.ncomp * hp.p.L / r + block - 1) / block;
+    if (b > 4) b = 8;
+    if (b > max_b(r)) return false;
+    bpt[i] = b;
+  }
+  for (int i = 0; i < hp.p.npass; i++) hp.p.bpt[i] = bpt[i];
+  hp.block = block;
   return true;
 }
 
@@ -697,9 +772,11 @@ DemagData* init_common(int nx, int ny, double thick, double strength, int padded
     hipLaunchKernelGGL(k_row_fwd<double>, dim3(Py, 1), dim3(d->row.block),
                        sizeof(double2) * lpad_len(d->row.p.L, d->row.p.psh), d->stream,
                        (const double*)(dKr + (size_t)c * Px * Py), (const double*)nullptr,
-                       (long)0, Px, Py, d->row.p, (const double2*)d->d_post, d->d_spec);
+                       (long)0, Px, Py, d->row.p, (const V2<double>*)d->d_post,
+                       (V2<double>*)d->d_spec);
     hipLaunchKernelGGL(k_col<double>, dim3(Hx), dim3(d->col1.block), sizeof(double2) * lpad_len(Py, d->col1.p.psh),
-                       d->stream, d->d_spec, Py, Py, Hx, d->col1.p, (const double*)nullptr, 0, 1);
+                       d->stream, (V2<double>*)d->d_spec, Py, Py, Hx, d->col1.p,
+                       (const double*)nullptr, 0, 1);
     const long P = (long)Hx * Py;
     hipLaunchKernelGGL(k_extract, dim3((Hx + 255) / 256, Py), dim3(256), 0, d->stream,
                        (const double2*)d->d_spec, Py, Hx, scale, 0, d->d_K + c * P);
@@ -740,9 +817,9 @@ void apply_t(DemagData* d, const HostPlan<T>& row, const HostPlan<T>& col, const
 
 void apply(DemagData* d, const double* y, const double* w, double* h) {
   if (d->fp32)
-    apply_t<float>(d, d->row32, d->col32, d->d_post32, d->d_K32, y, w, h);
+    apply_t<float>(d, d->row32, d->col32, (const V2<float>*)d->d_post32, d->d_K32, y, w, h);
   else
-    apply_t<double>(d, d->row, d->col, d->d_post, d->d_K, y, w, h);
+    apply_t<double>(d, d->row, d->col, (const V2<double>*)d->d_post, d->d_K, y, w, h);
 }
 
 }  // namespace
@@ -817,6 +894,17 @@ int Demag_SetPrecision(DemagData* d, int fp32) {
     if (!launch_ok("Demag_SetPrecision")) return -1;
   }
   d->fp32 = 1;
+  return 0;
+}
+
+int Demag_SetWorkgroups(DemagData* d, int row_block, int col_block) {
+  if (!d) return -1;
+  if (row_block && (!set_block(d->row, row_block) ||
+                    (d->d_K32 && !set_block(d->row32, row_block))))
+    return -1;
+  if (col_block && (!set_block(d->col, col_block) ||
+                    (d->d_K32 && !set_block(d->col32, col_block))))
+    return -1;
   return 0;
 }
 

@@ -25,7 +25,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
 
 #include "llg/nvector_hip.h"
@@ -76,7 +78,7 @@ std::mutex g_mu;
 std::unordered_map<hipStream_t, RedWs*> g_ws;
 hipStream_t g_default_stream = nullptr;
 bool g_default_stream_made = false;
-long g_syncs = 0;
+std::atomic<long> g_syncs{0};
 int g_sync_mode = 0;   // 0: hipStreamSynchronize, 1: spin on the pinned sequence word
 int g_red_blocks = 512;   // partial workgroups per reduction (upper bound)
 int g_red_mode = 1;       // 0: single launch with ticket, 1: partial + finish launches
@@ -617,11 +619,12 @@ void finish_dispatch(int K, hipStream_t s, int kind, int nblk, const RedWs* w, u
 // which also surfaces any device error); otherwise hipStreamSynchronize.
 void wait_result(hipStream_t s, RedWs* w, unsigned seq) {
   g_syncs++;
-  if (g_sync_mode == 1) {
+  if (g_sync_mode == 1 || g_sync_mode == 2) {
     const unsigned* word = reinterpret_cast<const unsigned*>(w->host + kMaxK);   // slot 0
     auto t0 = std::chrono::steady_clock::now();
     for (unsigned it = 0;; it++) {
       if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) return;
+      if (g_sync_mode == 2 && (it & 15u) == 15u) std::this_thread::yield();
       if ((it & 4095u) == 4095u &&
           std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2))
         break;
@@ -749,10 +752,10 @@ void* NVHip_DefaultStream(void) {
   return (void*)g_default_stream;
 }
 
-long NVHip_NumSyncs(void) { return g_syncs; }
+long NVHip_NumSyncs(void) { return g_syncs.load(); }
 
 int NVHip_SetSyncMode(int mode) {
-  if (mode != 0 && mode != 1) return -1;
+  if (mode < 0 || mode > 2) return -1;
   g_sync_mode = mode;
   return 0;
 }

@@ -16,6 +16,7 @@ import numpy as np
 
 from . import native
 from .native import (llg_config, llg_terms, llg_run_opts, llg_run_result, llg_grain,
+                     llg_instance, llg_ensemble_result,
                      EXEC_YMSK, EXEC_COMPACT, ANISO_LINEAR, ANISO_CUBIC, NORM_NONE,
                      NORM_EXACT, NORM_REGULARIZED, BC_PERIODIC, BC_NEUMANN, DEMAG_OFF,
                      DEMAG_PERIODIC, DEMAG_PADDED, GATE_NONE, GATE_MASK, GATE_WINDOW,
@@ -421,6 +422,96 @@ class Run:
     def close(self):
         if self.h:
             lib().LLG_RunDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------- ensembles
+ENSEMBLE_GRIDS = ((256, 256), (512, 256), (768, 512), (1024, 512))
+
+
+def ensemble_members(n=64, seed=4, demag=2.0, grids=ENSEMBLE_GRIDS, frac=(0.2, 0.8),
+                     hmax=0.1):
+    """The batched-ensemble workload: n instances, grid drawn from `grids`,
+    an ellipse or a rectangular ring of seeded active fraction in `frac`,
+    in-plane h_ext of magnitude U[0, hmax] along a random direction, FFT
+    demag `demag` (periodic), i4 physics, two-domain initial state.
+    Returns a list of (llg_config, Geometry, y0)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        ng, ny = grids[int(rng.integers(len(grids)))]
+        f = float(rng.uniform(*frac))
+        if rng.random() < 0.5:
+            aspect = float(rng.uniform(0.7, 1.3))
+            rx = min(0.5, math.sqrt(f * aspect / math.pi))
+            ry = min(0.5, f / (math.pi * rx))
+            geom = Geometry.ellipse(ng, ny, rx, ry)
+        else:
+            wf, hf = float(rng.uniform(0.9, 1.0)), float(rng.uniform(0.9, 1.0))
+            inner = math.sqrt(max(0.0, 1.0 - min(f, 0.99 * wf * hf) / (wf * hf)))
+            geom = Geometry.ring(ng, ny, wf, hf, inner, inner)
+        h, phi = float(rng.uniform(0.0, hmax)), float(rng.uniform(0.0, 2.0 * math.pi))
+        cfg = preset("i4", ng=ng, ny=ny, demag=demag)
+        cfg.rhs.happ[0], cfg.rhs.happ[1], cfg.rhs.happ[2] = h * math.cos(phi), h * math.sin(phi), 0.0
+        y0 = geom.ic_split_x(max_angle=0.05, seed=1000 + i)
+        out.append((cfg, geom, y0))
+    return out
+
+
+class Ensemble:
+    """Independent instances (own grid, geometry, fields, step control)
+    integrated concurrently on one GPU: one context, stream, integrator and
+    host thread per instance (LLG_EnsembleCreate).  members: list of
+    (llg_config, Geometry, y0)."""
+
+    def __init__(self, members, nthreads=0, **run_kw):
+        L = lib()
+        self.n = len(members)
+        self._keep = []
+        arr = (llg_instance * self.n)()
+        for i, (cfg, geom, y0) in enumerate(members):
+            y0 = np.ascontiguousarray(y0, dtype=np.float64)
+            self._keep += [geom, y0]
+            arr[i].cfg = cfg
+            arr[i].active = _ucp(geom.active)
+            arr[i].w = _dp(geom.w)
+            arr[i].cell_msk = _dp(geom.msk)
+            arr[i].cell_nsk = _dp(geom.nsk)
+            arr[i].y0 = _dp(y0)
+        self.sizes = [3 * cfg.ng * cfg.ny for cfg, _, _ in members]
+        opts = Simulation.run_opts(**run_kw)
+        self.h = L.LLG_EnsembleCreate(self.n, arr, C.byref(opts), nthreads)
+        if not self.h:
+            raise RuntimeError("LLG_EnsembleCreate failed: " + L.LLG_LastError().decode())
+
+    def advance(self, t_end):
+        res = llg_ensemble_result()
+        rc = lib().LLG_EnsembleAdvance(self.h, t_end, C.byref(res))
+        out = {k: getattr(res, k) for k, _ in res._fields_ if k != "pad"}
+        if rc != 0:
+            raise RuntimeError(f"ensemble integration failed (rc {rc}, {res.failed} instances)")
+        return out
+
+    def state(self, i):
+        y = np.empty(self.sizes[i])
+        if lib().LLG_EnsembleGetState(self.h, i, _dp(y)) != 0:
+            raise RuntimeError("LLG_EnsembleGetState failed")
+        return y
+
+    def result(self, i):
+        r = llg_run_result()
+        lib().LLG_EnsembleGetResult(self.h, i, C.byref(r))
+        return {"t": r.t_reached, "wall_s": r.wall_s, **r.st.as_dict()}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().LLG_EnsembleDestroy(self.h)
             self.h = None
 
     def __del__(self):

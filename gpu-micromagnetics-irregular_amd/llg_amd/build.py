@@ -26,6 +26,7 @@ SOURCES = [
     "geometry.cpp",
     "llg_driver.cpp",
     "output.cpp",
+    "llg_ensemble.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

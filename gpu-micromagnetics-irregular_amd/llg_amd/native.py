@@ -80,6 +80,22 @@ class llg_grain(C.Structure):
     ]
 
 
+class llg_instance(C.Structure):
+    _fields_ = [
+        ("cfg", llg_config), ("active", C.POINTER(C.c_ubyte)), ("w", C.POINTER(C.c_double)),
+        ("cell_msk", C.POINTER(C.c_double)), ("cell_nsk", C.POINTER(C.c_double)),
+        ("y0", C.POINTER(C.c_double)),
+    ]
+
+
+class llg_ensemble_result(C.Structure):
+    _fields_ = [
+        ("wall_s", C.c_double), ("inst_time", C.c_double), ("nst", C.c_long),
+        ("nfe", C.c_long), ("nli", C.c_long), ("nni", C.c_long), ("failed", C.c_int),
+        ("pad", C.c_int),
+    ]
+
+
 DP = C.POINTER(C.c_double)
 UCP = C.POINTER(C.c_ubyte)
 IP = C.POINTER(C.c_int)
@@ -159,6 +175,7 @@ SIGNATURES = {
     "Demag_GetSelfCoupling": (None, [VP, DP, DP, DP]),
     "Demag_Destroy": (None, [VP]),
     "Demag_SetLdsPadding": (C.c_int, [VP, C.c_int, C.c_int]),
+    "Demag_SetWorkgroups": (C.c_int, [VP, C.c_int, C.c_int]),
     "Demag_SetPrecision": (C.c_int, [VP, C.c_int]),
     "Demag_GetPlanShape": (C.c_int, [VP, IP, IP, IP]),
     "Demag_GetSpectrum": (C.c_int, [VP, DP]),
@@ -216,6 +233,14 @@ SIGNATURES = {
     # llg_cvode.h
     "LLGCVodeCreate": (VP, [C.c_int, VP]),
     "LLGCVodeSetFusedOps": (C.c_int, [VP, VP]),
+    # llg_ensemble.h
+    "LLG_EnsembleCreate": (VP, [C.c_int, C.POINTER(llg_instance), C.POINTER(llg_run_opts),
+                                C.c_int]),
+    "LLG_EnsembleAdvance": (C.c_int, [VP, C.c_double, C.POINTER(llg_ensemble_result)]),
+    "LLG_EnsembleSize": (C.c_int, [VP]),
+    "LLG_EnsembleGetState": (C.c_int, [VP, C.c_int, DP]),
+    "LLG_EnsembleGetResult": (C.c_int, [VP, C.c_int, C.POINTER(llg_run_result)]),
+    "LLG_EnsembleDestroy": (None, [VP]),
     "LLGCVodeInit": (C.c_int, [VP, VP, C.c_double, VP]),
     "LLGCVodeReInit": (C.c_int, [VP, C.c_double, VP]),
     "LLGCVodeSetUserData": (C.c_int, [VP, VP]),

@@ -49,6 +49,11 @@ int  Demag_SetPrecision(DemagData* d, int fp32);
  * (31 disables it) for the row and the column plans.  A tuning knob:
  * results do not depend on it. */
 int  Demag_SetLdsPadding(DemagData* d, int row_shift, int col_shift);
+/* Workgroup sizes of the row and column FFT kernels (multiples of 64 up to
+ * 1024; 0 keeps the current one).  Fails (-1, nothing changed for that
+ * plan) when a pass would exceed its per-thread register budget.  A
+ * tuning knob: results do not depend on it. */
+int  Demag_SetWorkgroups(DemagData* d, int row_block, int col_block);
 /* introspection used by tests and the bench roofline */
 int  Demag_GetPlanShape(DemagData* d, int* px, int* py, int* padded);
 /* copy the (strength-scaled, 1/(px*py)-normalised) tensor spectrum to host:

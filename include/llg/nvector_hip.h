@@ -56,7 +56,8 @@ void* NVHip_DefaultStream(void);
 /* counters of host synchronisations made by reductions (diagnostics) */
 long  NVHip_NumSyncs(void);
 /* how the host waits for a reduction result: 0 = hipStreamSynchronize,
- * 1 = poll the pinned result's sequence word (bounded, falls back to 0) */
+ * 1 = poll the pinned result's sequence word (bounded, falls back to 0),
+ * 2 = as 1 but yielding the core between polls (many host threads) */
 int   NVHip_SetSyncMode(int mode);
 /* reduction launch shape: at most max_blocks (1..4096) partial workgroups;
  * mode 0 = one launch (last workgroup sums the partials), 1 = partial

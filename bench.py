@@ -103,6 +103,63 @@ def cpu_baseline(la, cfg, geom, y0, budget_s, threads):
                        f"oracle setup (tensor + plans) {init_s:.1f} s excluded")}
 
 
+def ensemble_bench(la, args):
+    """64 independent instances (grids 256x256..1024x512, ellipse/ring
+    geometries with active fractions 0.2-0.8, in-plane h_ext up to 0.1,
+    periodic FFT demag 2.0), each with its own BDF step control, one host
+    thread and stream per instance.  value = instance-simulated time per
+    wall second, summed over instances; the sequential comparison runs a
+    few instances one after another on the same GPU."""
+    la.lib().NVHip_SetSyncMode(2)   # many host threads: poll and yield
+    members = la.ensemble_members(n=args.instances, seed=4)
+    ncell = sum(c.ng * c.ny for c, _, _ in members)
+    print(f"[bench] ensemble of {len(members)} instances, {ncell / 1e6:.2f} M cells",
+          file=sys.stderr, flush=True)
+    kw = dict(rtol=1e-4, atol=1e-4, maxl=5, gstype=args.gstype, max_order=5, t_out=0.1,
+              mxstep=100000)
+    step = 1.0
+    if args.warmup > 0:
+        w = la.Ensemble(members, nthreads=0, **kw)
+        w.advance(args.warmup * step)
+        w.close()
+        print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+    ens = la.Ensemble(members, nthreads=0, **kw)
+    out = ens.advance(args.steps * step)
+    ens.close()
+    value = out["inst_time"] / out["wall_s"]
+    print(f"[bench] ensemble advanced {out['inst_time']:.1f} instance-time units in "
+          f"{out['wall_s']:.3f} s: nst={out['nst']} nli={out['nli']}", file=sys.stderr, flush=True)
+    # the same instances one at a time (a sample of 4, a short interval each)
+    seq_t, seq_wall = 0.0, 0.0
+    for cfg, geom, y0 in members[:4]:
+        sim = la.Simulation(cfg, geom)
+        run = sim.integrator(y0, **kw)
+        r = run.advance(0.5 * step)
+        seq_t += r["t"]
+        seq_wall += r["wall_s"]
+        run.close()
+        sim.close()
+    la.lib().NVHip_SetSyncMode(args.sync_mode)
+    seq = seq_t / seq_wall if seq_wall > 0 else None
+    rec = {
+        "metric": "instance-sim-time/wall-s (64-instance ensemble)",
+        "value": round(value, 4), "unit": "instance-sim-time/wall-s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * out["wall_s"] / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded ensemble: grids, geometries, fields and two-domain states)",
+        "config": {"workload": f"{len(members)} independent instances, grids "
+                               "256x256/512x256/768x512/1024x512, ellipse/ring 0.2-0.8 active, "
+                               "in-plane h_ext <= 0.1, periodic FFT demag 2.0, i4 physics, "
+                               "rtol=atol=1e-4, own step control each",
+                   "cells": ncell, "nst": out["nst"], "nli": out["nli"], "nfe": out["nfe"],
+                   "failed": out["failed"],
+                   "sequential_sample_inst_time_per_wall_s": round(seq, 4) if seq else None},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,9 +178,12 @@ def main():
     ap.add_argument("--sync-mode", type=int, default=1,
                     help="reduction wait: 1 poll the pinned result word (default), 0 stream sync")
     ap.add_argument("--fp32-demag", action="store_true", help="FP32 spectral demag form")
-    ap.add_argument("--workload", default="square_hole", choices=["square_hole", "ref512"],
+    ap.add_argument("--workload", default="square_hole",
+                    choices=["square_hole", "ref512", "ensemble"],
                     help="square_hole: 1536x512 cells, padded demag (default); ref512: the "
-                         "reference's published 512x512 periodic i1 run (use --steps 1000)")
+                         "reference's published 512x512 periodic i1 run (use --steps 1000); "
+                         "ensemble: 64 independent instances run concurrently")
+    ap.add_argument("--instances", type=int, default=64, help="ensemble size")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -134,6 +194,8 @@ def main():
     import llg_amd as la
     la.lib()   # raises when the native library is missing
     la.lib().NVHip_SetSyncMode(args.sync_mode)
+    if args.workload == "ensemble":
+        return ensemble_bench(la, args)
 
     ref = args.workload == "ref512"
     if ref:

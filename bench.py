@@ -35,6 +35,18 @@ for p in (str(ROOT / "gpu-micromagnetics-irregular_amd"), str(ROOT)):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = ROOT / "profiles" / "square_hole_1536x512_hbm_traffic.md"
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of this workload (None when not measured)."""
+    if kernel != "demag_col" or workload != "square_hole" or not TRAFFIC_PROFILE.exists():
+        return None
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        if line.startswith("demag_col_traffic_bytes:"):
+            return int(line.split(":")[1])
+    return None
 REF512_PUBLISHED = 1000.0 / 29.644070   # sim-time/wall-s of the reference's own log
 KERNELS = {"rhs": 1, "jtv": 2, "build_p": 3, "apply_p": 4, "demag_row_fwd": 5,
            "demag_col": 6, "demag_row_inv": 7, "demag": 8}
@@ -245,7 +257,11 @@ def main():
     achieved = nbytes / avg_s / 1e9 if nl.value else 0.0
     roof = {"bound": "hbm", "kernel": args.profile_kernel, "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None, "bytes_per_launch": nbytes, "avg_launch_us": round(avg_s * 1e6, 2),
+            "traffic": measured_traffic(args.profile_kernel,
+                                        "square_hole" if (not ref and padded and ng == 1536
+                                                          and ny == 512 and not args.fp32_demag)
+                                        else "other"),
+            "bytes_per_launch": nbytes, "avg_launch_us": round(avg_s * 1e6, 2),
             "launches": nl.value}
 
     cpu = None

@@ -27,6 +27,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
@@ -196,6 +198,10 @@ def main():
                          "reference's published 512x512 periodic i1 run (use --steps 1000); "
                          "ensemble: 64 independent instances run concurrently")
     ap.add_argument("--instances", type=int, default=64, help="ensemble size")
+    ap.add_argument("--learned-pc", action="store_true",
+                    help="train the learned block-Jacobi correction (on states of a "
+                         "trajectory from another seed, untimed) and use it in the timed run")
+    ap.add_argument("--lpc-steps", type=int, default=400, help="training steps")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -232,6 +238,28 @@ def main():
     step = 1.0
     ropts = dict(rtol=1e-4, atol=1e-4, maxl=5, gstype=args.gstype, max_order=5, t_out=0.1,
                  mxstep=100000)
+
+    lpc_info = None
+    if args.learned_pc:
+        t_tr = time.perf_counter()
+        ytr = (geom.ic_split_x(max_angle=0.05, seed=2) if not ref
+               else geom.ic_stripes(left=0.25, right=0.75, eps=0.01, seed=777))
+        tr = sim.integrator(ytr, **ropts)
+        states = [ytr]
+        for t in (1.0, 2.0, 3.0, 4.0, 5.0):
+            tr.advance(t)
+            states.append(tr.state())
+        tr.close()
+        lpc = la.LearnedPrecond(sim)
+        lpc.init_random(seed=0, std=1e-3)
+        hist = lpc.train(states, steps=args.lpc_steps // 2, lr=2e-3, gamma_range=(1e-3, 2e-1),
+                         seed=1)
+        hist += lpc.train(states, steps=args.lpc_steps - args.lpc_steps // 2, lr=5e-4,
+                          gamma_range=(1e-3, 2e-1), seed=2)
+        lpc.enable(True)
+        lpc_info = {"train_steps": args.lpc_steps, "train_s": round(time.perf_counter() - t_tr, 1),
+                    "loss_first": float(np.mean(hist[:10])), "loss_last": float(np.mean(hist[-10:]))}
+        print(f"[bench] learned correction trained: {lpc_info}", file=sys.stderr, flush=True)
 
     if args.warmup > 0:
         w = sim.integrator(y0, **ropts)
@@ -300,7 +328,9 @@ def main():
                    "nsetups": out["nsetups"], "netf": out["netf"], "ncfn": out["ncfn"],
                    "host_syncs": out["nsyncs"], "setup_s": round(setup_s, 2),
                    "gs": {0: "auto", 1: "modified", 2: "classical"}[args.gstype],
-                   "sync_mode": args.sync_mode},
+                   "sync_mode": args.sync_mode,
+                   "preconditioner": "block-Jacobi + learned correction" if args.learned_pc
+                   else "block-Jacobi", "learned_pc": lpc_info},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

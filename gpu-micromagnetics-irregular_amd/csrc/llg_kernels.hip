@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "llg/llg_plugin.h"
+#include "llg/learned_pc.h"
 #include "llg/nvector_hip.h"
 #include "kprof.h"
 
@@ -462,6 +463,7 @@ struct llg_ctx {
   std::vector<unsigned char> h_active;
   std::vector<int> h_ids, h_hole_ids;
   long nf, njv, npset, npsol;
+  llg_lpc* lpc = nullptr;   // learned correction applied by PrecondSolve
 };
 
 namespace {
@@ -820,7 +822,25 @@ int PrecondSolve(sunrealtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z,
                      c->cfg.pc.gate, (const double*)c->pd->d_P,
                      (const double*)N_VGetDeviceArrayPointer(r), N_VGetDeviceArrayPointer(z));
   kprof::end(kprof::APPLY_P, s);
+  if (c->lpc &&
+      LLG_LpcCorrect(c->lpc, N_VGetDeviceArrayPointer(y), gamma, N_VGetDeviceArrayPointer(z)) != 0)
+    return -1;
   return launch_status("PrecondSolve");
 }
+
+int LLG_SetLearnedPrecond(llg_ctx* ctx, llg_lpc* p) {
+  if (!ctx) return -1;
+  ctx->lpc = p;
+  return 0;
+}
+
+int LLG_GetGrid(llg_ctx* ctx, int* ng, int* ny) {
+  if (!ctx) return -1;
+  if (ng) *ng = ctx->cfg.ng;
+  if (ny) *ny = ctx->cfg.ny;
+  return 0;
+}
+
+const unsigned char* LLG_GetActiveDevice(llg_ctx* ctx) { return ctx ? ctx->d_active : nullptr; }
 
 }  // extern "C"

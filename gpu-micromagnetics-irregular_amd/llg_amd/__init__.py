@@ -519,3 +519,136 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------- learned preconditioner
+class LearnedPrecond:
+    """The learned correction to block-Jacobi (include/llg/learned_pc.h) on a
+    Simulation's grid.  Training pairs come from the solver's own FP64
+    operator: x is a sampled smooth field on the active cells, b = (I -
+    gamma J(y)) x by LLG_Atimes, u = P_bj^-1 b by PrecondSolve, so x solves
+    the Newton system exactly."""
+
+    def __init__(self, sim, periodic=None):
+        L = lib()
+        self.sim = sim
+        if periodic is None:
+            periodic = sim.cfg.boundary == BC_PERIODIC and sim.cfg.demag != DEMAG_PADDED
+        self.h = L.LLG_LpcCreate(sim.ctx, 1 if periodic else 0)
+        if not self.h:
+            raise RuntimeError("LLG_LpcCreate failed: " + L.LLG_LpcLastError().decode())
+        self.nw = L.LLG_LpcNumWeights(self.h)
+        n = sim.neq
+        self._y, self._x, self._b, self._u = sim.vec(), sim.vec(), sim.vec(), sim.vec()
+
+    # weights
+    def weights(self):
+        w = np.zeros(self.nw, np.float32)
+        lib().LLG_LpcGetWeights(self.h, w.ctypes.data_as(C.POINTER(C.c_float)))
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.nw
+        if lib().LLG_LpcSetWeights(self.h, w.ctypes.data_as(C.POINTER(C.c_float))) != 0:
+            raise RuntimeError("LLG_LpcSetWeights failed")
+
+    def grad(self):
+        g = np.zeros(self.nw, np.float32)
+        lib().LLG_LpcGetGrad(self.h, g.ctypes.data_as(C.POINTER(C.c_float)))
+        return g
+
+    def init_random(self, seed=0, std=None):
+        rng = np.random.default_rng(seed)
+        sizes = [9 * 16 * 32, 9 * 32 * 32, 9 * 32 * 16]
+        parts = []
+        for k, nk in enumerate(sizes):
+            fan_in = 9 * (16 if k == 0 else 32)
+            sd = std if std is not None else 1.0 / math.sqrt(fan_in)
+            parts.append(rng.normal(0.0, sd, nk))
+        self.set_weights(np.concatenate(parts).astype(np.float32))
+
+    # pairs
+    def smooth_field(self, rng, nmodes=6, kmax=6, noise=None):
+        """a few random Fourier modes plus cell-wise noise of random weight
+        (Newton corrections carry both smooth and grid-scale content),
+        zero off the active cells"""
+        g = self.sim.geom
+        ng, ny = g.ng, g.ny
+        X, Y = np.meshgrid(np.arange(ng) / ng, np.arange(ny) / ny)
+        out = np.zeros((3, ny, ng))
+        for c in range(3):
+            for _ in range(nmodes):
+                kx, ky = rng.integers(0, kmax + 1, 2)
+                ph = rng.uniform(0, 2 * math.pi)
+                out[c] += rng.normal() * np.cos(2 * math.pi * (kx * X + ky * Y) + ph)
+        if noise is None:
+            noise = float(rng.uniform(0.0, 1.0))
+        out += noise * math.sqrt(nmodes) * rng.standard_normal(out.shape)
+        out *= np.tile(g.active.reshape(ny, ng).astype(float), (3, 1, 1))
+        return out.ravel()
+
+    def make_pair(self, y, gamma, rng):
+        """device buffers (u, y, x) for one exact pair at state y, gamma"""
+        L, sim = lib(), self.sim
+        L.LLG_SetLearnedPrecond(sim.ctx, None)
+        self._y.set(y)
+        vf = sim.vec()
+        L.LLG_Rhs(0.0, self._y.v, vf.v, sim.ctx)          # demag field for PrecondSetup
+        jc = C.c_int()
+        L.PrecondSetup(0.0, self._y.v, vf.v, 0, C.byref(jc), gamma, sim.ctx)
+        self._x.set(self.smooth_field(rng))
+        L.LLG_Atimes(self._x.v, self._b.v, gamma, 0.0, self._y.v, None, sim.ctx)
+        L.PrecondSolve(0.0, self._y.v, None, self._b.v, self._u.v, gamma, 0.0, 1, sim.ctx)
+        xs = float(np.sqrt(np.mean(self._x.get() ** 2)) + 1e-30)
+        return 1.0 / xs
+
+    def dptr(self, v):
+        return lib().N_VGetDeviceArrayPointer_Hip(v.v)
+
+    def loss_grad(self, gamma, scale):
+        loss = C.c_double()
+        if lib().LLG_LpcLossGrad(self.h, self.dptr(self._u), self.dptr(self._y),
+                                 self.dptr(self._x), gamma, scale, C.byref(loss)) != 0:
+            raise RuntimeError("LLG_LpcLossGrad failed: " + lib().LLG_LpcLastError().decode())
+        return loss.value
+
+    def forward(self, gamma, scale):
+        o = np.zeros(4 * self.sim.ncell, np.float32)
+        lib().LLG_LpcForward(self.h, self.dptr(self._u), self.dptr(self._y), gamma, scale,
+                             o.ctypes.data_as(C.POINTER(C.c_float)))
+        return o.reshape(-1, 4)
+
+    def train(self, states, steps=200, lr=1e-3, gamma_range=(1e-3, 1e-1), seed=0,
+              log_every=0):
+        rng = np.random.default_rng(seed)
+        hist = []
+        for it in range(steps):
+            y = states[int(rng.integers(len(states)))]
+            gamma = float(math.exp(rng.uniform(math.log(gamma_range[0]),
+                                               math.log(gamma_range[1]))))
+            scale = self.make_pair(y, gamma, rng)
+            loss = C.c_double()
+            if lib().LLG_LpcTrainStep(self.h, self.dptr(self._u), self.dptr(self._y),
+                                      self.dptr(self._x), gamma, scale, lr,
+                                      C.byref(loss)) != 0:
+                raise RuntimeError("LLG_LpcTrainStep failed")
+            hist.append(loss.value)
+            if log_every and it % log_every == 0:
+                print(f"[lpc] step {it} loss {loss.value:.4e}", flush=True)
+        return hist
+
+    def enable(self, on=True):
+        lib().LLG_SetLearnedPrecond(self.sim.ctx, self.h if on else None)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().LLG_SetLearnedPrecond(self.sim.ctx, None)
+            lib().LLG_LpcDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

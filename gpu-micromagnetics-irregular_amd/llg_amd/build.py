@@ -27,6 +27,7 @@ SOURCES = [
     "llg_driver.cpp",
     "output.cpp",
     "llg_ensemble.cpp",
+    "learned_pc.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

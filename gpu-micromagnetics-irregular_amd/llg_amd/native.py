@@ -233,6 +233,22 @@ SIGNATURES = {
     # llg_cvode.h
     "LLGCVodeCreate": (VP, [C.c_int, VP]),
     "LLGCVodeSetFusedOps": (C.c_int, [VP, VP]),
+    # learned_pc.h
+    "LLG_LpcCreate": (VP, [VP, C.c_int]),
+    "LLG_LpcDestroy": (None, [VP]),
+    "LLG_LpcLastError": (C.c_char_p, []),
+    "LLG_LpcNumWeights": (C.c_int, [VP]),
+    "LLG_LpcSetWeights": (C.c_int, [VP, C.POINTER(C.c_float)]),
+    "LLG_LpcGetWeights": (C.c_int, [VP, C.POINTER(C.c_float)]),
+    "LLG_LpcGetGrad": (C.c_int, [VP, C.POINTER(C.c_float)]),
+    "LLG_LpcForward": (C.c_int, [VP, DP, DP, C.c_double, C.c_double, C.POINTER(C.c_float)]),
+    "LLG_LpcLossGrad": (C.c_int, [VP, DP, DP, DP, C.c_double, C.c_double, DP]),
+    "LLG_LpcAdamStep": (C.c_int, [VP, C.c_float]),
+    "LLG_LpcTrainStep": (C.c_int, [VP, DP, DP, DP, C.c_double, C.c_double, C.c_float, DP]),
+    "LLG_LpcCorrect": (C.c_int, [VP, DP, C.c_double, DP]),
+    "LLG_SetLearnedPrecond": (C.c_int, [VP, VP]),
+    "LLG_GetGrid": (C.c_int, [VP, IP, IP]),
+    "LLG_GetActiveDevice": (VP, [VP]),
     # llg_ensemble.h
     "LLG_EnsembleCreate": (VP, [C.c_int, C.POINTER(llg_instance), C.POINTER(llg_run_opts),
                                 C.c_int]),

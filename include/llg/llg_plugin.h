@@ -111,6 +111,9 @@ int      LLG_GetHdemag(llg_ctx* ctx, double* h_hd3);            /* last f()'s fi
 int      LLG_GetPinv(llg_ctx* ctx, double* h_p9);               /* 9*ncell SoA */
 DemagData* LLG_GetDemag(llg_ctx* ctx);
 void*    LLG_GetStream(llg_ctx* ctx);
+int      LLG_GetGrid(llg_ctx* ctx, int* ng, int* ny);
+/* device copy of the active flags (one byte per cell) */
+const unsigned char* LLG_GetActiveDevice(llg_ctx* ctx);
 /* per-callback launch counters (diagnostics) */
 int      LLG_GetCallCounts(llg_ctx* ctx, long* nf, long* njv, long* npsetup, long* npsolve);
 

@@ -1,0 +1,130 @@
+"""The learned block-Jacobi correction on the GPU (MFMA BF16 kernels)
+against its CPU oracle (oracle/learned_pc_oracle.py), and what training it
+gains the solver.
+
+Tolerances: the device accumulates BF16 products in FP32 and rounds the
+hidden layers to BF16, the oracle accumulates in float64 and rounds the
+same way, so a few BF16 roundings can fall on the other side: forward
+output within 2e-2 of max|o|, weight gradients within 3e-2 in norm per
+layer.  With zero weights the preconditioner must equal block-Jacobi bit
+for bit.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import llg_amd as la
+from oracle import learned_pc_oracle as lo
+
+pytestmark = pytest.mark.gpu
+
+
+def _sim(ng=64, ny=48):
+    cfg = la.preset("square_hole", ng=ng, ny=ny)
+    geom = la.Geometry.square_hole(ng, ny, halfside_frac=0.2)
+    y0 = geom.ic_split_x()
+    return la.Simulation(cfg, geom), geom, y0
+
+
+def _states(sim, y0, times=(0.05, 0.2, 0.5)):
+    run = sim.integrator(y0, rtol=1e-4, atol=1e-4)
+    out = [y0]
+    for t in times:
+        run.advance(t)
+        out.append(run.state())
+    run.close()
+    return out
+
+
+def test_forward_and_gradient_match_oracle():
+    sim, geom, y0 = _sim()
+    lpc = la.LearnedPrecond(sim)
+    lpc.init_random(seed=5, std=0.08)
+    rng = np.random.default_rng(7)
+    gamma = 0.02
+    scale = lpc.make_pair(y0, gamma, rng)
+    o_dev = lpc.forward(gamma, scale)[:, :3]
+    loss = lpc.loss_grad(gamma, scale)
+    g_dev = lpc.grad().astype(np.float64)
+    w = lpc.weights().astype(np.float64)
+    u, y, x = lpc._u.get(), lpc._y.get(), lpc._x.get()
+    lo_loss, g_ref, st = lo.loss_grad(w, u, y, x, geom.active, gamma, scale, geom.ng, geom.ny,
+                                      periodic=False)
+    o_ref = st["o"][..., :3].reshape(-1, 3)
+    err = np.max(np.abs(o_dev - o_ref)) / np.max(np.abs(o_ref))
+    print(f"[gpu] lpc forward rel err {err:.2e}, loss dev {loss:.6e} oracle {lo_loss:.6e}")
+    assert err < 2e-2
+    assert abs(loss - lo_loss) <= 2e-2 * lo_loss
+    o = 0
+    for k, nk in enumerate([9 * 16 * 32, 9 * 32 * 32, 9 * 32 * 16]):
+        gd, gr = g_dev[o:o + nk], g_ref[o:o + nk]
+        rel = np.linalg.norm(gd - gr) / np.linalg.norm(gr)
+        print(f"[gpu] lpc dW{k + 1} rel err {rel:.2e}")
+        assert rel < 3e-2
+        o += nk
+    lpc.close()
+
+
+def test_zero_weights_is_block_jacobi():
+    sim, geom, y0 = _sim()
+    lpc = la.LearnedPrecond(sim)
+    rng = np.random.default_rng(1)
+    lpc.make_pair(y0, 0.03, rng)
+    L = la.lib()
+    zb, zl = sim.vec(), sim.vec()
+    L.PrecondSolve(0.0, lpc._y.v, None, lpc._b.v, zb.v, 0.03, 0.0, 1, sim.ctx)
+    lpc.enable(True)
+    L.PrecondSolve(0.0, lpc._y.v, None, lpc._b.v, zl.v, 0.03, 0.0, 1, sim.ctx)
+    lpc.enable(False)
+    assert np.array_equal(zb.get(), zl.get())
+    lpc.close()
+
+
+def _heldout_errors(lpc, states, rng, n=6, gamma_range=(2e-3, 5e-2)):
+    """relative error of block-Jacobi alone and of the corrected
+    preconditioner against the exact solution of fresh pairs"""
+    e_bj, e_lpc = [], []
+    for i in range(n):
+        y = states[i % len(states)]
+        gamma = float(math.exp(rng.uniform(math.log(gamma_range[0]), math.log(gamma_range[1]))))
+        scale = lpc.make_pair(y, gamma, rng)
+        o = lpc.forward(gamma, scale)[:, :3].astype(np.float64)
+        u = lpc._u.get().reshape(3, -1).T * scale
+        x = lpc._x.get().reshape(3, -1).T * scale
+        nx = np.linalg.norm(x)
+        e_bj.append(np.linalg.norm(u - x) / nx)
+        e_lpc.append(np.linalg.norm(u + o - x) / nx)
+    return float(np.mean(e_bj)), float(np.mean(e_lpc))
+
+
+def test_training_lowers_loss_and_preconditioner_error():
+    """what the trained network gains over block-Jacobi: the error of the
+    preconditioned solution on held-out exact pairs, and (printed) the
+    Krylov iterations of a short integration"""
+    sim, geom, y0 = _sim()
+    states = _states(sim, y0)
+    lpc = la.LearnedPrecond(sim)
+    lpc.init_random(seed=0, std=1e-3)
+    hist = lpc.train(states, steps=300, lr=2e-3, gamma_range=(2e-3, 5e-2), seed=3)
+    first, last = np.mean(hist[:20]), np.mean(hist[-20:])
+    e_bj, e_lpc = _heldout_errors(lpc, states, np.random.default_rng(99))
+    print(f"[gpu] lpc training loss {first:.4e} -> {last:.4e}; held-out relative error "
+          f"block-Jacobi {e_bj:.3f}, learned {e_lpc:.3f}")
+    assert last < 0.5 * first
+    assert e_lpc < 0.5 * e_bj
+    res = {}
+    for on in (False, True):
+        lpc.enable(on)
+        run = sim.integrator(y0, rtol=1e-5, atol=1e-5)
+        out = run.advance(1.0)
+        res[on] = (out, run.state())
+        run.close()
+    lpc.enable(False)
+    (ob, yb), (ol, yl) = res[False], res[True]
+    print(f"[gpu] block-Jacobi nst {ob['nst']} nli {ob['nli']} | learned nst {ol['nst']} "
+          f"nli {ol['nli']}  max|dm| {np.max(np.abs(yb - yl)):.2e}")
+    # the integration with the correction converges to the same tolerance
+    assert ol["nli"] <= 1.05 * ob["nli"]
+    assert np.max(np.abs(yb - yl)) < 1e-3
+    lpc.close()

@@ -202,6 +202,8 @@ def main():
                     help="train the learned block-Jacobi correction (on states of a "
                          "trajectory from another seed, untimed) and use it in the timed run")
     ap.add_argument("--lpc-steps", type=int, default=400, help="training steps")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the launch-shape autotuner at setup (untimed)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -234,6 +236,12 @@ def main():
           file=sys.stderr, flush=True)
     t_setup = time.perf_counter()
     sim = la.Simulation(cfg, geom)
+    tune = None
+    if not args.no_autotune:
+        ch = (C.c_int * 6)()
+        if la.lib().LLG_Autotune(sim.ctx, 5, ch) == 0:
+            tune = {"stencil_block": [ch[0], ch[1]], "demag_row_block": ch[2],
+                    "demag_col_block": ch[3], "demag_row_pad": ch[4], "demag_col_pad": ch[5]}
     setup_s = time.perf_counter() - t_setup
     step = 1.0
     ropts = dict(rtol=1e-4, atol=1e-4, maxl=5, gstype=args.gstype, max_order=5, t_out=0.1,
@@ -330,7 +338,7 @@ def main():
                    "gs": {0: "auto", 1: "modified", 2: "classical"}[args.gstype],
                    "sync_mode": args.sync_mode,
                    "preconditioner": "block-Jacobi + learned correction" if args.learned_pc
-                   else "block-Jacobi", "learned_pc": lpc_info},
+                   else "block-Jacobi", "learned_pc": lpc_info, "autotune": tune},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -915,6 +915,60 @@ int Demag_SetLdsPadding(DemagData* d, int row_shift, int col_shift) {
   return 0;
 }
 
+int Demag_Autotune(DemagData* d, int reps, int* choice) {
+  if (!d || reps < 1) return -1;
+  const long n3 = 3L * d->nx * d->ny;
+  double *y = nullptr, *h = nullptr;
+  if (hipMalloc(&y, sizeof(double) * n3) != hipSuccess) return -1;
+  if (hipMalloc(&h, sizeof(double) * n3) != hipSuccess) { hipFree(y); return -1; }
+  hipMemsetAsync(y, 0, sizeof(double) * n3, d->stream);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  const int row_blocks[] = {128, 192, 256, 320}, col_blocks[] = {192, 256, 320};
+  const int row_pads[] = {3, 31}, col_pads[] = {4, 31};
+  const int rb0 = d->row.block, cb0 = d->col.block;
+  const int rp0 = d->row.p.psh, cp0 = d->col.p.psh;
+  float best = 1e30f;
+  int best_cfg[4] = {rb0, cb0, rp0, cp0};
+  for (int rb : row_blocks)
+    for (int cb : col_blocks)
+      for (int rp : row_pads)
+        for (int cp : col_pads) {
+          if (!set_block(d->row, rb) || !set_block(d->col, cb)) continue;
+          // the column plan's three-component LDS must fit three workgroups
+          d->row.p.psh = rp;
+          d->col.p.psh = cp;
+          apply(d, y, nullptr, h);   // warm
+          hipEventRecord(e0, d->stream);
+          for (int i = 0; i < reps; i++) apply(d, y, nullptr, h);
+          hipEventRecord(e1, d->stream);
+          hipEventSynchronize(e1);
+          float ms = 0.f;
+          hipEventElapsedTime(&ms, e0, e1);
+          if (ms < best) {
+            best = ms;
+            best_cfg[0] = rb; best_cfg[1] = cb; best_cfg[2] = rp; best_cfg[3] = cp;
+          }
+        }
+  set_block(d->row, best_cfg[0]);
+  set_block(d->col, best_cfg[1]);
+  d->row.p.psh = best_cfg[2];
+  d->col.p.psh = best_cfg[3];
+  if (d->d_K32) {
+    set_block(d->row32, best_cfg[0]);
+    set_block(d->col32, best_cfg[1]);
+  }
+  (void)rb0; (void)cb0; (void)rp0; (void)cp0;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(y);
+  hipFree(h);
+  if (choice)
+    for (int i = 0; i < 4; i++) choice[i] = best_cfg[i];
+  return launch_ok("Demag_Autotune") ? 0 : -1;
+}
+
 int Demag_GetPlanShape(DemagData* d, int* px, int* py, int* padded) {
   if (!d) return -1;
   if (px) *px = d->px;

@@ -464,6 +464,7 @@ struct llg_ctx {
   std::vector<int> h_ids, h_hole_ids;
   long nf, njv, npset, npsol;
   llg_lpc* lpc = nullptr;   // learned correction applied by PrecondSolve
+  int bx = kBX, by = kBY;   // 2D stencil workgroup shape (LLG_SetStencilBlock)
 };
 
 namespace {
@@ -486,8 +487,8 @@ void grid_for(const llg_ctx* c, dim3& grid, dim3& block) {
     block = dim3(k1D);
     grid = dim3((n + k1D - 1) / k1D);
   } else {
-    block = dim3(kBX, kBY);
-    grid = dim3((c->cfg.ng + kBX - 1) / kBX, (c->cfg.ny + kBY - 1) / kBY);
+    block = dim3(c->bx, c->by);
+    grid = dim3((c->cfg.ng + c->bx - 1) / c->bx, (c->cfg.ny + c->by - 1) / c->by);
   }
 }
 
@@ -826,6 +827,67 @@ int PrecondSolve(sunrealtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z,
       LLG_LpcCorrect(c->lpc, N_VGetDeviceArrayPointer(y), gamma, N_VGetDeviceArrayPointer(z)) != 0)
     return -1;
   return launch_status("PrecondSolve");
+}
+
+int LLG_SetStencilBlock(llg_ctx* ctx, int bx, int by) {
+  if (!ctx || bx < 1 || by < 1 || bx * by > 256 || (bx * by) % 64) return -1;
+  ctx->bx = bx;
+  ctx->by = by;
+  return 0;
+}
+
+int LLG_Autotune(llg_ctx* ctx, int reps, int* choice) {
+  if (!ctx || reps < 1) return -1;
+  if (ctx->cfg.exec == LLG_EXEC_COMPACT) {   // 1D launches: nothing to tune
+    if (choice) { choice[0] = k1D; choice[1] = 1; }
+    return 0;
+  }
+  const int shapes[][2] = {{64, 4}, {32, 8}, {16, 16}, {128, 2}, {64, 2}, {32, 4}, {256, 1}};
+  double *y = nullptr, *v = nullptr, *o = nullptr;
+  const size_t nb = sizeof(double) * 3 * (size_t)ctx->ncell;
+  if (hipMalloc(&y, nb) != hipSuccess) return -1;
+  if (hipMalloc(&v, nb) != hipSuccess) { hipFree(y); return -1; }
+  if (hipMalloc(&o, nb) != hipSuccess) { hipFree(y); hipFree(v); return -1; }
+  hipMemsetAsync(y, 0, nb, ctx->stream);
+  hipMemsetAsync(v, 0, nb, ctx->stream);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  float best = 1e30f;
+  int bbx = ctx->bx, bby = ctx->by;
+  for (const auto& sh : shapes) {
+    ctx->bx = sh[0];
+    ctx->by = sh[1];
+    dim3 grid, block;
+    grid_for(ctx, grid, block);
+    // the Jv stencil stands for the stencil family (RHS, Jv, build/apply P)
+    hipLaunchKernelGGL(k_jtv<false>, grid, block, 0, ctx->stream, geo_of(ctx),
+                       to_dev(ctx->cfg.jv), (const double*)y, (const double*)v, o, 0.0);
+    hipEventRecord(e0, ctx->stream);
+    for (int i = 0; i < reps; i++)
+      hipLaunchKernelGGL(k_jtv<false>, grid, block, 0, ctx->stream, geo_of(ctx),
+                         to_dev(ctx->cfg.jv), (const double*)y, (const double*)v, o, 0.0);
+    hipEventRecord(e1, ctx->stream);
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best) { best = ms; bbx = sh[0]; bby = sh[1]; }
+  }
+  ctx->bx = bbx;
+  ctx->by = bby;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(y);
+  hipFree(v);
+  hipFree(o);
+  if (choice) { choice[0] = bbx; choice[1] = bby; }
+  if (ctx->demag) {
+    int dc[4];
+    if (Demag_Autotune(ctx->demag, reps, dc) != 0) return -1;
+    if (choice)
+      for (int i = 0; i < 4; i++) choice[2 + i] = dc[i];
+  }
+  return launch_status("LLG_Autotune");
 }
 
 int LLG_SetLearnedPrecond(llg_ctx* ctx, llg_lpc* p) {

@@ -176,6 +176,7 @@ SIGNATURES = {
     "Demag_Destroy": (None, [VP]),
     "Demag_SetLdsPadding": (C.c_int, [VP, C.c_int, C.c_int]),
     "Demag_SetWorkgroups": (C.c_int, [VP, C.c_int, C.c_int]),
+    "Demag_Autotune": (C.c_int, [VP, C.c_int, IP]),
     "Demag_SetPrecision": (C.c_int, [VP, C.c_int]),
     "Demag_GetPlanShape": (C.c_int, [VP, IP, IP, IP]),
     "Demag_GetSpectrum": (C.c_int, [VP, DP]),
@@ -248,6 +249,8 @@ SIGNATURES = {
     "LLG_LpcCorrect": (C.c_int, [VP, DP, C.c_double, DP]),
     "LLG_SetLearnedPrecond": (C.c_int, [VP, VP]),
     "LLG_GetGrid": (C.c_int, [VP, IP, IP]),
+    "LLG_SetStencilBlock": (C.c_int, [VP, C.c_int, C.c_int]),
+    "LLG_Autotune": (C.c_int, [VP, C.c_int, IP]),
     "LLG_GetActiveDevice": (VP, [VP]),
     # llg_ensemble.h
     "LLG_EnsembleCreate": (VP, [C.c_int, C.POINTER(llg_instance), C.POINTER(llg_run_opts),

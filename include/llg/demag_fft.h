@@ -54,6 +54,10 @@ int  Demag_SetLdsPadding(DemagData* d, int row_shift, int col_shift);
  * plan) when a pass would exceed its per-thread register budget.  A
  * tuning knob: results do not depend on it. */
 int  Demag_SetWorkgroups(DemagData* d, int row_block, int col_block);
+/* Autotuner: times Demag_Apply (reps launches each) over row/column
+ * workgroup sizes and LDS paddings, keeps the fastest and returns it in
+ * choice[4] = {row block, col block, row pad shift, col pad shift}. */
+int  Demag_Autotune(DemagData* d, int reps, int* choice);
 /* introspection used by tests and the bench roofline */
 int  Demag_GetPlanShape(DemagData* d, int* px, int* py, int* padded);
 /* copy the (strength-scaled, 1/(px*py)-normalised) tensor spectrum to host:

@@ -112,6 +112,15 @@ int      LLG_GetPinv(llg_ctx* ctx, double* h_p9);               /* 9*ncell SoA *
 DemagData* LLG_GetDemag(llg_ctx* ctx);
 void*    LLG_GetStream(llg_ctx* ctx);
 int      LLG_GetGrid(llg_ctx* ctx, int* ng, int* ny);
+/* 2D workgroup shape of the stencil kernels (RHS, Jv, P): bx*by a multiple
+ * of 64, at most 256.  Results do not depend on it. */
+int      LLG_SetStencilBlock(llg_ctx* ctx, int bx, int by);
+/* Autotuner: times the Jv stencil over workgroup shapes and the demag
+ * passes over workgroup sizes and LDS paddings (Demag_Autotune), keeps the
+ * fastest.  choice[6] = {bx, by, demag row block, col block, row pad, col
+ * pad}.  No reference counterpart (the reference fixes 256-thread 1D
+ * launches at compile time). */
+int      LLG_Autotune(llg_ctx* ctx, int reps, int* choice);
 /* device copy of the active flags (one byte per cell) */
 const unsigned char* LLG_GetActiveDevice(llg_ctx* ctx);
 /* per-callback launch counters (diagnostics) */

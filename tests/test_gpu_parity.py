@@ -5,6 +5,8 @@ Tolerance: 1e-12 relative (max |diff| / max |ref|) for single FP64 f / Jv /
 Psetup / Psolve evaluations -- the bar stated for FP64 RHS/Jv; masks and
 compact id lists bit-exact.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -173,3 +175,40 @@ def test_atimes_equals_jtv_then_linear_sum(case):
     L.N_VLinearSum_Hip(1.0, vv.v, -gamma, za.v, za.v)
     assert L.LLG_Atimes(vv.v, zb.v, gamma, 0.0, vy.v, None, sim.ctx) == 0
     assert np.array_equal(za.get(), zb.get())
+
+
+def test_demag_autotune_keeps_the_field_bitwise():
+    """the autotuner only changes launch shapes and LDS layout: the field
+    must not change by a single bit"""
+    cfg = la.preset("square_hole", ng=96, ny=64)
+    geom = la.Geometry.square_hole(96, 64, halfside_frac=0.2)
+    sim = la.Simulation(cfg, geom)
+    L = la.lib()
+    d = L.LLG_GetDemag(sim.ctx)
+    y = sim.vec(geom.ic_split_x())
+    h0, h1 = sim.vec(), sim.vec()
+    L.Demag_Apply(d, L.N_VGetDeviceArrayPointer_Hip(y.v), L.N_VGetDeviceArrayPointer_Hip(h0.v))
+    choice = (C.c_int * 4)()
+    assert L.Demag_Autotune(d, 3, choice) == 0
+    print(f"[gpu] autotune choice {list(choice)}")
+    L.Demag_Apply(d, L.N_VGetDeviceArrayPointer_Hip(y.v), L.N_VGetDeviceArrayPointer_Hip(h1.v))
+    assert np.array_equal(h0.get(), h1.get())
+
+
+def test_context_autotune_keeps_callbacks_bitwise():
+    cfg, geom, y0 = make(CASES[1])
+    sim = la.Simulation(cfg, geom)
+    L = la.lib()
+    rng = np.random.default_rng(4)
+    y = perturbed(y0, geom.active, 8)
+    v = rng.standard_normal(sim.neq)
+    f0, _ = sim.rhs(y)
+    j0 = sim.jtv(y, v)
+    choice = (C.c_int * 6)()
+    assert L.LLG_Autotune(sim.ctx, 3, choice) == 0
+    print(f"[gpu] context autotune choice {list(choice)}")
+    f1, _ = sim.rhs(y)
+    j1 = sim.jtv(y, v)
+    assert np.array_equal(f0, f1)
+    assert np.array_equal(np.asarray(j0), np.asarray(j1))
+    assert L.LLG_SetStencilBlock(sim.ctx, 50, 4) == -1   # 200 threads: not whole waves

@@ -37,7 +37,16 @@
 #include "llg/nvector_hip.h"
 #include "kprof.h"
 
+// No implicit multiply-add contraction in this file: every fused
+// multiply-add is written out (fmad below), so a kernel's rounding does
+// not depend on how the compiler schedules it.  The general and the
+// compile-time FFT plans therefore give identical bits.
+#pragma clang fp contract(off)
+
 namespace {
+
+__device__ __host__ inline double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __host__ inline float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 thread_local std::string g_derr;
 void derr(const std::string& e) { g_derr = e; }
@@ -82,8 +91,8 @@ using LdsV2 = V2<T>;
 
 template <class C2> __device__ inline C2 cmul(C2 a, C2 b) {
   C2 r;
-  r.x = a.x * b.x - a.y * b.y;
-  r.y = a.x * b.y + a.y * b.x;
+  r.x = fmad(a.x, b.x, -(a.y * b.y));
+  r.y = fmad(a.x, b.y, a.y * b.x);
   return r;
 }
 template <class C2> __device__ inline C2 cadd(C2 a, C2 b) { C2 r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
@@ -145,7 +154,7 @@ struct Dft<3, T> {
   __device__ static inline void run(V2<T> (&v)[3], T sign) {
     const T c = T(-0.5), s = sign * T(0.86602540378443864676);
     const V2<T> t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-    const V2<T> a = mk<T>(v[0].x + c * t1.x, v[0].y + c * t1.y);
+    const V2<T> a = mk<T>(fmad(c, t1.x, v[0].x), fmad(c, t1.y, v[0].y));
     const V2<T> b = mk<T>(-s * t2.y, s * t2.x);   // i*s*t2
     v[0] = cadd(v[0], t1);
     v[1] = cadd(a, b);
@@ -159,10 +168,10 @@ struct Dft<5, T> {
     const T s1 = sign * T(0.95105651629515357212), s2 = sign * T(0.58778525229247312917);
     const V2<T> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
     const V2<T> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-    const V2<T> a1 = mk<T>(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
-    const V2<T> a2 = mk<T>(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
-    const V2<T> b1r = mk<T>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
-    const V2<T> b2r = mk<T>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    const V2<T> a1 = mk<T>(fmad(c2, t2.x, fmad(c1, t1.x, v[0].x)), fmad(c2, t2.y, fmad(c1, t1.y, v[0].y)));
+    const V2<T> a2 = mk<T>(fmad(c1, t2.x, fmad(c2, t1.x, v[0].x)), fmad(c1, t2.y, fmad(c2, t1.y, v[0].y)));
+    const V2<T> b1r = mk<T>(fmad(s1, t3.x, s2 * t4.x), fmad(s1, t3.y, s2 * t4.y));
+    const V2<T> b2r = mk<T>(fmad(s2, t3.x, -(s1 * t4.x)), fmad(s2, t3.y, -(s1 * t4.y)));
     const V2<T> b1 = mk<T>(-b1r.y, b1r.x);   // i * b1r
     const V2<T> b2 = mk<T>(-b2r.y, b2r.x);
     v[0] = cadd(v[0], cadd(t1, t2));
@@ -297,18 +306,104 @@ __device__ __forceinline__ void fft_lds(LdsV2<T>* buf, const Plan1D<T>& p, int n
   }
 }
 
+// ------------------------------------------------ compile-time plans
+// The same Stockham passes with the length, radices, workgroup size and LDS
+// padding as template constants: every index division folds to a shift or
+// a multiply, and butterflies per thread are exact.  Same arithmetic as
+// fft_lds, so the results are identical.
+template <int L, int R, int Ns, int NCOMP, int BLOCK, int SH, int SIGN, class T>
+__device__ __forceinline__ void pass_ct(LdsV2<T>* buf, const V2<T>* __restrict__ tw, int cs) {
+  constexpr int nb = L / R, total = nb * NCOMP, B = (total + BLOCK - 1) / BLOCK;
+  constexpr int tstep = L / (Ns * R);
+  const T sign = T(SIGN);
+  V2<T> v[B][R];
+  int base[B], coff[B];
+#pragma unroll
+  for (int r = 0; r < B; r++) {
+    const int b = threadIdx.x + r * BLOCK;
+    coff[r] = -1;
+    if (total % BLOCK == 0 || b < total) {
+      const int comp = b / nb, j = b - comp * nb;
+      const int grp = j / Ns, k = j - grp * Ns;
+      coff[r] = comp * cs;
+      base[r] = grp * Ns * R + k;
+      const LdsV2<T>* src = buf + coff[r];
+      const int tk = k * tstep;
+#pragma unroll
+      for (int t = 0; t < R; t++) {
+        V2<T> x = src[lpad(j + t * nb, SH)];
+        if (t > 0) {
+          V2<T> w = tw[t * tk];
+          if (SIGN > 0) w.y = -w.y;
+          x = cmul(x, w);
+        }
+        v[r][t] = x;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < B; r++) {
+    if (coff[r] >= 0) {
+      Dft<R, T>::run(v[r], sign);
+      LdsV2<T>* dst = buf + coff[r];
+#pragma unroll
+      for (int t = 0; t < R; t++) dst[lpad(base[r] + t * Ns, SH)] = v[r][t];
+    }
+  }
+  __syncthreads();
+}
+
+template <int SIGN, class T, int L, int NCOMP, int BLOCK, int SH, int Ns>
+__device__ __forceinline__ void fft_ct_rec(LdsV2<T>*, const V2<T>*, int) {}
+
+template <int SIGN, class T, int L, int NCOMP, int BLOCK, int SH, int Ns, int R, int... Rest>
+__device__ __forceinline__ void fft_ct_rec(LdsV2<T>* buf, const V2<T>* tw, int cs) {
+  pass_ct<L, R, Ns, NCOMP, BLOCK, SH, SIGN, T>(buf, tw, cs);
+  fft_ct_rec<SIGN, T, L, NCOMP, BLOCK, SH, Ns * R, Rest...>(buf, tw, cs);
+}
+
+// FFT policies of the kernels: the runtime plan, or a compile-time one
+template <class T>
+struct GenFft {
+  static constexpr bool kCompileTime = false;
+  Plan1D<T> p;
+  __device__ int len() const { return p.L; }
+  __device__ int shift() const { return p.psh; }
+  template <int SIGN>
+  __device__ __forceinline__ void run(LdsV2<T>* buf, int ncomp, int cs) const {
+    fft_lds<SIGN, T>(buf, p, ncomp, cs);
+  }
+};
+
+template <class T, int L, int BLOCK, int SH, int... Rs>
+struct CtFft {
+  static constexpr bool kCompileTime = true;
+  const V2<T>* tw;
+  static constexpr int kL = L, kBlock = BLOCK, kSh = SH;
+  static constexpr int kRad[] = {Rs...};
+  static constexpr int kNpass = sizeof...(Rs);
+  __device__ int len() const { return L; }
+  __device__ int shift() const { return SH; }
+  template <int SIGN>
+  __device__ __forceinline__ void run(LdsV2<T>* buf, int ncomp, int cs) const {
+    if (ncomp == 3) fft_ct_rec<SIGN, T, L, 3, BLOCK, SH, 1, Rs...>(buf, tw, cs);
+    else fft_ct_rec<SIGN, T, L, 1, BLOCK, SH, 1, Rs...>(buf, tw, cs);
+  }
+};
+
 // ----------------------------------------------------------- row forward
 // grid (rows, ncomp); in = base + comp*comp_stride + row*nx_in (FP64 input)
-template <class T>
+template <class T, class F>
 __global__ __launch_bounds__(512) void k_row_fwd(const double* __restrict__ in,
                                                   const double* __restrict__ w, long comp_stride,
-                                                  int nx_in, int rows, Plan1D<T> plan,
+                                                  int nx_in, int rows, F plan,
                                                   const V2<T>* __restrict__ post,
                                                   V2<T>* __restrict__ spec) {
   extern __shared__ char lds_raw[];
   LdsV2<T>* lds = reinterpret_cast<LdsV2<T>*>(lds_raw);
   const int row = blockIdx.x, comp = blockIdx.y;
-  const int L = plan.L, Hx = L + 1, sh = plan.psh;
+  const int L = plan.len(), Hx = L + 1, sh = plan.shift();
   const double* src = in + comp * comp_stride + (long)row * nx_in;
   const double* wr = w ? w + (long)row * nx_in : nullptr;
   for (int m = threadIdx.x; m < L; m += blockDim.x) {
@@ -319,7 +414,7 @@ __global__ __launch_bounds__(512) void k_row_fwd(const double* __restrict__ in,
     lds[lpad(m, sh)] = mk<T>((T)a, (T)b);
   }
   __syncthreads();
-  fft_lds<-1, T>(lds, plan, 1, lpad_len(L, sh));
+  plan.template run<-1>(lds, 1, lpad_len(L, sh));
   V2<T>* out = spec + ((long)comp * rows + row) * Hx;
   for (int k = threadIdx.x; k <= L; k += blockDim.x) {
     const V2<T> zk = lds[lpad(k == L ? 0 : k, sh)];
@@ -332,15 +427,15 @@ __global__ __launch_bounds__(512) void k_row_fwd(const double* __restrict__ in,
 }
 
 // ---------------------------------------------------------- row inverse
-template <class T>
+template <class T, class F>
 __global__ __launch_bounds__(512) void k_row_inv(const V2<T>* __restrict__ spec, int rows,
-                                                  Plan1D<T> plan, const V2<T>* __restrict__ post,
+                                                  F plan, const V2<T>* __restrict__ post,
                                                   double* __restrict__ out, long comp_stride,
                                                   int nx_out) {
   extern __shared__ char lds_raw[];
   LdsV2<T>* lds = reinterpret_cast<LdsV2<T>*>(lds_raw);
   const int row = blockIdx.x, comp = blockIdx.y;
-  const int L = plan.L, Hx = L + 1, sh = plan.psh;
+  const int L = plan.len(), Hx = L + 1, sh = plan.shift();
   const V2<T>* in = spec + ((long)comp * rows + row) * Hx;
   for (int k = threadIdx.x; k < L; k += blockDim.x) {
     const V2<T> xk = in[k];
@@ -350,7 +445,7 @@ __global__ __launch_bounds__(512) void k_row_inv(const V2<T>* __restrict__ spec,
     lds[lpad(k, sh)] = mk<T>(a.x - b.y, a.y + b.x);   // a + i b
   }
   __syncthreads();
-  fft_lds<1, T>(lds, plan, 1, lpad_len(L, sh));
+  plan.template run<1>(lds, 1, lpad_len(L, sh));
   double* dst = out + comp * comp_stride + (long)row * nx_out;
   for (int m = threadIdx.x; m < L; m += blockDim.x) {
     const int n0 = 2 * m;
@@ -369,22 +464,22 @@ __device__ inline int xcd_remap(int bid, int G) {
 // --------------------------------------------------------------- column
 // mode 0: fwd FFT, 3x3 tensor product, inverse FFT (3 components)
 // mode 1: fwd FFT only (1 component; tensor spectra at init)
-template <class T>
+template <class T, class F>
 __global__ __launch_bounds__(512) void k_col(V2<T>* __restrict__ spec, int rows_in,
-                                              int rows_out, int Hx, Plan1D<T> plan,
+                                              int rows_out, int Hx, F plan,
                                               const T* __restrict__ K, int has_xy_im, int mode) {
   extern __shared__ char lds_raw[];
   LdsV2<T>* lds = reinterpret_cast<LdsV2<T>*>(lds_raw);
   const int kx = xcd_remap(blockIdx.x, gridDim.x);
-  const int Py = plan.L;
+  const int Py = plan.len();
   const int ncomp = mode == 0 ? 3 : 1;
-  const int sh = plan.psh;
+  const int sh = plan.shift();
   const int cs = lpad_len(Py, sh);
   for (int c = 0; c < ncomp; c++)
     for (int y = threadIdx.x; y < Py; y += blockDim.x)
       lds[c * cs + lpad(y, sh)] = y < rows_in ? spec[((long)c * rows_in + y) * Hx + kx] : mk<T>(T(0), T(0));
   __syncthreads();
-  fft_lds<-1, T>(lds, plan, ncomp, cs);
+  plan.template run<-1>(lds, ncomp, cs);
   if (mode == 0) {
     const long P = (long)Hx * Py;
     const T* kxx = K + (long)kx * Py;
@@ -397,14 +492,15 @@ __global__ __launch_bounds__(512) void k_col(V2<T>* __restrict__ spec, int rows_
       const V2<T> mx = lds[q], my = lds[cs + q], mz = lds[2 * cs + q];
       const T a = kxx[ky], b = kyy[ky], cz = kzz[ky];
       const V2<T> xy = mk<T>(kxyr[ky], has_xy_im ? kxyi[ky] : T(0));
-      const V2<T> hx = cadd(mk<T>(a * mx.x, a * mx.y), cmul(xy, my));
-      const V2<T> hy = cadd(cmul(xy, mx), mk<T>(b * my.x, b * my.y));
+      const V2<T> cxy = cmul(xy, my), cyx = cmul(xy, mx);
+      const V2<T> hx = mk<T>(fmad(a, mx.x, cxy.x), fmad(a, mx.y, cxy.y));
+      const V2<T> hy = mk<T>(fmad(b, my.x, cyx.x), fmad(b, my.y, cyx.y));
       lds[q] = hx;
       lds[cs + q] = hy;
       lds[2 * cs + q] = mk<T>(cz * mz.x, cz * mz.y);
     }
     __syncthreads();
-    fft_lds<1, T>(lds, plan, 3, cs);
+    plan.template run<1>(lds, 3, cs);
   }
   for (int c = 0; c < ncomp; c++)
     for (int y = threadIdx.x; y < rows_out; y += blockDim.x)
@@ -684,6 +780,7 @@ struct DemagData {
   float* d_K32;
   int has_xy_im;
   int fp32;
+  int ct = 1;         // compile-time FFT plans where one matches the plan
 };
 
 namespace {
@@ -769,13 +866,14 @@ DemagData* init_common(int nx, int ny, double thick, double strength, int padded
   // spectra of the 4 real planes: row R2C + column forward FFT
   const double scale = strength / ((double)Px * (double)Py);
   for (int c = 0; c < 4; c++) {
-    hipLaunchKernelGGL(k_row_fwd<double>, dim3(Py, 1), dim3(d->row.block),
+    hipLaunchKernelGGL((k_row_fwd<double, GenFft<double>>), dim3(Py, 1), dim3(d->row.block),
                        sizeof(double2) * lpad_len(d->row.p.L, d->row.p.psh), d->stream,
                        (const double*)(dKr + (size_t)c * Px * Py), (const double*)nullptr,
-                       (long)0, Px, Py, d->row.p, (const V2<double>*)d->d_post,
+                       (long)0, Px, Py, GenFft<double>{d->row.p}, (const V2<double>*)d->d_post,
                        (V2<double>*)d->d_spec);
-    hipLaunchKernelGGL(k_col<double>, dim3(Hx), dim3(d->col1.block), sizeof(double2) * lpad_len(Py, d->col1.p.psh),
-                       d->stream, (V2<double>*)d->d_spec, Py, Py, Hx, d->col1.p,
+    hipLaunchKernelGGL((k_col<double, GenFft<double>>), dim3(Hx), dim3(d->col1.block),
+                       sizeof(double2) * lpad_len(Py, d->col1.p.psh), d->stream,
+                       (V2<double>*)d->d_spec, Py, Py, Hx, GenFft<double>{d->col1.p},
                        (const double*)nullptr, 0, 1);
     const long P = (long)Hx * Py;
     hipLaunchKernelGGL(k_extract, dim3((Hx + 255) / 256, Py), dim3(256), 0, d->stream,
@@ -791,6 +889,39 @@ DemagData* init_common(int nx, int ny, double thick, double strength, int padded
   return d;
 }
 
+// compile-time plans of the benchmarked shapes, each at the default
+// workgroup size and LDS padding of its plan: the zero-padded 3072 x 1024
+// grid (row transforms of 1536 points, columns of 1024) and the periodic
+// 512 x 512 grid (rows of 256, columns of 512)
+template <class T> using CtRow1536 = CtFft<T, 1536, 256, 3, 8, 8, 8, 3>;
+template <class T> using CtRow256 = CtFft<T, 256, 64, 3, 8, 8, 4>;
+template <class T> using CtCol1024 = CtFft<T, 1024, 256, 4, 8, 8, 4, 4>;
+template <class T> using CtCol512 = CtFft<T, 512, 192, 4, 8, 8, 8>;
+
+template <class F, class T>
+bool ct_fits(const HostPlan<T>& hp) {
+  if (hp.p.L != F::kL || hp.block != F::kBlock || hp.p.psh != F::kSh || hp.p.npass != F::kNpass)
+    return false;
+  for (int i = 0; i < F::kNpass; i++)
+    if (hp.p.radix[i] != F::kRad[i]) return false;
+  return true;
+}
+
+// calls fn with the FFT policy for a row plan (compile-time when one fits)
+template <class T, class Fn>
+void with_row_plan(const HostPlan<T>& hp, bool ct, Fn&& fn) {
+  if (ct && ct_fits<CtRow1536<T>>(hp)) fn(CtRow1536<T>{hp.p.tw});
+  else if (ct && ct_fits<CtRow256<T>>(hp)) fn(CtRow256<T>{hp.p.tw});
+  else fn(GenFft<T>{hp.p});
+}
+
+template <class T, class Fn>
+void with_col_plan(const HostPlan<T>& hp, bool ct, Fn&& fn) {
+  if (ct && ct_fits<CtCol1024<T>>(hp)) fn(CtCol1024<T>{hp.p.tw});
+  else if (ct && ct_fits<CtCol512<T>>(hp)) fn(CtCol512<T>{hp.p.tw});
+  else fn(GenFft<T>{hp.p});
+}
+
 template <class T>
 void apply_t(DemagData* d, const HostPlan<T>& row, const HostPlan<T>& col, const V2<T>* post,
              const T* K, const double* y, const double* w, double* h) {
@@ -798,18 +929,26 @@ void apply_t(DemagData* d, const HostPlan<T>& row, const HostPlan<T>& col, const
   const long N = (long)nx * ny;
   hipStream_t s = d->stream;
   V2<T>* spec = reinterpret_cast<V2<T>*>(d->d_spec);
+  const size_t row_lds = sizeof(V2<T>) * lpad_len(row.p.L, row.p.psh);
   kprof::begin(kprof::DEMAG_ALL, s);
   kprof::begin(kprof::DEMAG_ROW_FWD, s);
-  hipLaunchKernelGGL(k_row_fwd<T>, dim3(ny, 3), dim3(row.block), sizeof(V2<T>) * lpad_len(row.p.L, row.p.psh), s, y,
-                     w, N, nx, ny, row.p, post, spec);
+  with_row_plan(row, d->ct, [&](auto f) {
+    hipLaunchKernelGGL((k_row_fwd<T, decltype(f)>), dim3(ny, 3), dim3(row.block), row_lds, s, y,
+                       w, N, nx, ny, f, post, spec);
+  });
   kprof::end(kprof::DEMAG_ROW_FWD, s);
   kprof::begin(kprof::DEMAG_COL, s);
-  hipLaunchKernelGGL(k_col<T>, dim3(Hx), dim3(col.block), sizeof(V2<T>) * 3 * lpad_len(Py, col.p.psh), s, spec, ny, ny,
-                     Hx, col.p, K, d->has_xy_im, 0);
+  with_col_plan(col, d->ct, [&](auto f) {
+    hipLaunchKernelGGL((k_col<T, decltype(f)>), dim3(Hx), dim3(col.block),
+                       sizeof(V2<T>) * 3 * lpad_len(Py, col.p.psh), s, spec, ny, ny, Hx, f, K,
+                       d->has_xy_im, 0);
+  });
   kprof::end(kprof::DEMAG_COL, s);
   kprof::begin(kprof::DEMAG_ROW_INV, s);
-  hipLaunchKernelGGL(k_row_inv<T>, dim3(ny, 3), dim3(row.block), sizeof(V2<T>) * lpad_len(row.p.L, row.p.psh), s,
-                     (const V2<T>*)spec, ny, row.p, post, h, N, nx);
+  with_row_plan(row, d->ct, [&](auto f) {
+    hipLaunchKernelGGL((k_row_inv<T, decltype(f)>), dim3(ny, 3), dim3(row.block), row_lds, s,
+                       (const V2<T>*)spec, ny, f, post, h, N, nx);
+  });
   kprof::end(kprof::DEMAG_ROW_INV, s);
   kprof::end(kprof::DEMAG_ALL, s);
   launch_ok("Demag_Apply");
@@ -908,6 +1047,21 @@ int Demag_SetWorkgroups(DemagData* d, int row_block, int col_block) {
   return 0;
 }
 
+int Demag_SetSpecialized(DemagData* d, int on) {
+  if (!d) return -1;
+  d->ct = on ? 1 : 0;
+  int n = 0;
+  auto count = [&](auto f) { n += decltype(f)::kCompileTime ? 1 : 0; };
+  if (d->fp32) {
+    with_row_plan(d->row32, d->ct, count);
+    with_col_plan(d->col32, d->ct, count);
+  } else {
+    with_row_plan(d->row, d->ct, count);
+    with_col_plan(d->col, d->ct, count);
+  }
+  return n;
+}
+
 int Demag_SetLdsPadding(DemagData* d, int row_shift, int col_shift) {
   if (!d || row_shift < 1 || row_shift > 31 || col_shift < 1 || col_shift > 31) return -1;
   d->row.p.psh = d->row32.p.psh = row_shift;
@@ -925,10 +1079,18 @@ int Demag_Autotune(DemagData* d, int reps, int* choice) {
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
-  const int row_blocks[] = {128, 192, 256, 320}, col_blocks[] = {192, 256, 320};
-  const int row_pads[] = {3, 31}, col_pads[] = {4, 31};
   const int rb0 = d->row.block, cb0 = d->col.block;
   const int rp0 = d->row.p.psh, cp0 = d->col.p.psh;
+  // candidates: the current settings first, then the usual alternatives
+  auto cands = [](int cur, std::initializer_list<int> alt) {
+    std::vector<int> v{cur};
+    for (int a : alt)
+      if (a != cur) v.push_back(a);
+    return v;
+  };
+  const std::vector<int> row_blocks = cands(rb0, {64, 128, 192, 256, 320});
+  const std::vector<int> col_blocks = cands(cb0, {128, 192, 256, 320});
+  const std::vector<int> row_pads = cands(rp0, {3, 31}), col_pads = cands(cp0, {4, 31});
   float best = 1e30f;
   int best_cfg[4] = {rb0, cb0, rp0, cp0};
   for (int rb : row_blocks)

@@ -175,6 +175,7 @@ SIGNATURES = {
     "Demag_GetSelfCoupling": (None, [VP, DP, DP, DP]),
     "Demag_Destroy": (None, [VP]),
     "Demag_SetLdsPadding": (C.c_int, [VP, C.c_int, C.c_int]),
+    "Demag_SetSpecialized": (C.c_int, [VP, C.c_int]),
     "Demag_SetWorkgroups": (C.c_int, [VP, C.c_int, C.c_int]),
     "Demag_Autotune": (C.c_int, [VP, C.c_int, IP]),
     "Demag_SetPrecision": (C.c_int, [VP, C.c_int]),

@@ -50,10 +50,19 @@ int  Demag_SetPrecision(DemagData* d, int fp32);
  * results do not depend on it. */
 int  Demag_SetLdsPadding(DemagData* d, int row_shift, int col_shift);
 /* Workgroup sizes of the row and column FFT kernels (multiples of 64 up to
- * 1024; 0 keeps the current one).  Fails (-1, nothing changed for that
+ * 512; 0 keeps the current one).  Fails (-1, nothing changed for that
  * plan) when a pass would exceed its per-thread register budget.  A
  * tuning knob: results do not depend on it. */
 int  Demag_SetWorkgroups(DemagData* d, int row_block, int col_block);
+/* Compile-time FFT plans (on by default): kernels specialised for the
+ * zero-padded 3072 x 1024 grid (rows of 1536 points, columns of 1024) and
+ * the periodic 512 x 512 grid (rows of 256, columns of 512), at the
+ * default workgroup size and padding of each plan, run whenever the plan
+ * matches; other plans take the general kernels.  Same arithmetic, so
+ * results do not depend on it.  Returns the number of the two transform
+ * directions (row, column) of the current precision that take a
+ * compile-time plan, or -1. */
+int  Demag_SetSpecialized(DemagData* d, int on);
 /* Autotuner: times Demag_Apply (reps launches each) over row/column
  * workgroup sizes and LDS paddings, keeps the fastest and returns it in
  * choice[4] = {row block, col block, row pad shift, col pad shift}. */

@@ -545,9 +545,36 @@ int spgmr_solve(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s)
   // the unscaling vtemp = s2^-1 V[l] when available)
   double vscale = 1.0 / r_norm;
 
+  // the fused operator chain (left preconditioning, classical GS): one
+  // launch forms V[l+1] from the unscaled V[l]; the projection pass then
+  // normalises V[l] as its multi-dot reads it
+  const bool chain = pre_left && !pre_right && m->gstype == SUN_CLASSICAL_GS && m->jtimes &&
+                     m->fused.krylov && m->fused.gsscaled && l_max + 1 <= kFusedMax;
+
   for (int l = 0; l < l_max; l++) {
     m->ls_iters++;
     krydim = l + 1;
+    if (chain && m->fused.krylov(vscale, V[l], s, V[l + 1], m->gamma, m->tn, m->ycur, m->fcur,
+                                 m->user_data) == 0) {
+      m->njtimes++;
+      m->nps++;
+      const int k = l + 1;
+      double hnorm = 0.0, zz = 0.0;
+      double* dots = m->gs_d;
+      if (m->fused.gsscaled(V[k], k + 1, V, l, vscale, dots, &zz) == 0) {
+        classical_gs(m, k, &hnorm, dots, &zz);
+      } else {
+        N_VScale(vscale, V[l], V[l]);
+        classical_gs(m, k, &hnorm);
+      }
+      m->Hes[l + 1][l] = hnorm;
+      if (qr_fact(krydim, m->Hes, m->givens, l) != 0) return LS_QRFACT_FAIL;
+      rotation_product *= m->givens[2 * l + 1];
+      m->ls_resnorm = rho = std::fabs(rotation_product * r_norm);
+      if (rho <= delta) { converged = true; break; }
+      vscale = 1.0 / m->Hes[l + 1][l];
+      continue;
+    }
     if (!(m->fused.scalediv && m->fused.scalediv(vscale, V[l], s, V[l], m->vtemp) == 0)) {
       N_VScale(vscale, V[l], V[l]);
       N_VDiv(V[l], s, m->vtemp);

@@ -33,6 +33,11 @@
 #include "llg/nvector_hip.h"
 #include "kprof.h"
 
+// No implicit multiply-add contraction: every product and sum is rounded
+// on its own, so a value does not depend on which kernel forms it (the
+// fused Krylov operator below reproduces the separate launches bit for bit).
+#pragma clang fp contract(off)
+
 namespace {
 
 thread_local std::string g_err;
@@ -99,10 +104,27 @@ __device__ inline void axes(const GeoD& g, const TermsD& T, int cell, double msk
   }
 }
 
+// Read access to the direction vector v of a Jacobian product: stored
+// (VPlain), or formed on the fly as (c x) / s, the SPGMR basis vector
+// normalised and unscaled exactly as N_VScale then N_VDiv round it.
+struct VPlain {
+  const double* __restrict__ v;
+  __device__ double operator()(int i) const { return v[i]; }
+};
+struct VScaled {
+  const double* __restrict__ x;
+  const double* __restrict__ s;
+  double c;
+  __device__ double operator()(int i) const {
+    const double a = c * x[i];
+    return a / s[i];
+  }
+};
+
 // h (without demag) from terms T; optional linearised partner dh for v
-template <bool LIN>
+template <bool LIN, class VA = VPlain>
 __device__ inline void field(const GeoD& g, const TermsD& T, const double* __restrict__ y,
-                             const double* __restrict__ v, int cell, int gx, int gy,
+                             const VA& v, int cell, int gx, int gy,
                              double h[3], double dh[3]) {
   const int N = g.ncell;
   int lc, rc, uc, dc;
@@ -116,7 +138,7 @@ __device__ inline void field(const GeoD& g, const TermsD& T, const double* __res
   const double mdotmsk = m[0] * msk[0] + m[1] * msk[1] + m[2] * msk[2];
   double vv[3] = {0, 0, 0}, vdotmsk = 0.0;
   if (LIN) {
-    vv[0] = v[cell]; vv[1] = v[N + cell]; vv[2] = v[2 * N + cell];
+    vv[0] = v(cell); vv[1] = v(N + cell); vv[2] = v(2 * N + cell);
     vdotmsk = vv[0] * msk[0] + vv[1] * msk[1] + vv[2] * msk[2];
   }
 #pragma unroll
@@ -130,8 +152,8 @@ __device__ inline void field(const GeoD& g, const TermsD& T, const double* __res
     else an = msk[a] * (T.chk * mdotmsk + T.cha);
     h[a] = T.happ[a] + T.che * ex + an + T.chb * nsk[a] * (L + R);
     if (LIN) {
-      const double vL = wl * v[a * N + lc], vR = wr * v[a * N + rc];
-      const double vU = wu * v[a * N + uc], vD = wd * v[a * N + dc];
+      const double vL = wl * v(a * N + lc), vR = wr * v(a * N + rc);
+      const double vU = wu * v(a * N + uc), vD = wd * v(a * N + dc);
       double vex = vL + vR + vU + vD;
       if (g.exch_self) vex = vex - 4.0 * vv[a];
       double dan;
@@ -182,7 +204,7 @@ __global__ __launch_bounds__(256) void k_rhs(GeoD g, TermsD T, const double* __r
   if (!map_cell(g, yd, cell, gx, gy)) return;
   const int N = g.ncell;
   double h[3], dummy[3];
-  field<false>(g, T, y, nullptr, cell, gx, gy, h, dummy);
+  field<false>(g, T, y, VPlain{nullptr}, cell, gx, gy, h, dummy);
   if (hd) { h[0] += hd[cell]; h[1] += hd[N + cell]; h[2] += hd[2 * N + cell]; }
   const double m1 = y[cell], m2 = y[N + cell], m3 = y[2 * N + cell];
   const double mh = m1 * h[0] + m2 * h[1] + m3 * h[2];
@@ -254,30 +276,15 @@ __global__ __launch_bounds__(BX* BY) void k_rhs_tiled(GeoD g, TermsD T,
 // ------------------------------------------------------------------- Jv
 // Jv = J(y) v; with AT the SPGMR operator z = v + (-gamma) Jv is written
 // instead, rounded exactly as N_VLinearSum(1, v, -gamma, Jv) rounds it
-template <bool AT>
-__global__ __launch_bounds__(256) void k_jtv(GeoD g, TermsD T, const double* __restrict__ y,
-                                             const double* __restrict__ v, double* __restrict__ Jv,
-                                             double mgamma) {
-  int cell, gx, gy;
-  if (AT && g.compact) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int hidx = t - g.n_active;
-    if (hidx >= 0 && hidx < g.n_hole) {
-#pragma clang fp contract(off)
-      const int hc = g.hole_ids[hidx];
-      for (int c = 0; c < 3; c++) {
-        const double p = mgamma * 0.0;
-        Jv[c * g.ncell + hc] = v[c * g.ncell + hc] + p;
-      }
-      return;
-    }
-  }
-  if (!map_cell(g, Jv, cell, gx, gy)) return;
+template <bool AT, class VA>
+__device__ inline void jtv_cell(const GeoD& g, const TermsD& T, const double* __restrict__ y,
+                                const VA& v, int cell, int gx, int gy, double mgamma,
+                                double o[3]) {
   const int N = g.ncell;
   double h[3], dh[3];
   field<true>(g, T, y, v, cell, gx, gy, h, dh);
   const double m1 = y[cell], m2 = y[N + cell], m3 = y[2 * N + cell];
-  const double v1 = v[cell], v2 = v[N + cell], v3 = v[2 * N + cell];
+  const double v1 = v(cell), v2 = v(N + cell), v3 = v(2 * N + cell);
   const double mh = m1 * h[0] + m2 * h[1] + m3 * h[2];
   const double dmh = (v1 * h[0] + v2 * h[1] + v3 * h[2]) + (m1 * dh[0] + m2 * dh[1] + m3 * dh[2]);
   const double j1 = g.chg * (v3 * h[1] + m3 * dh[1] - v2 * h[2] - m2 * dh[2]) +
@@ -292,11 +299,34 @@ __global__ __launch_bounds__(256) void k_jtv(GeoD g, TermsD T, const double* __r
     o1 = gt * j1; o2 = gt * j2; o3 = gt * j3;
   }
   if (AT) {
-#pragma clang fp contract(off)
     const double p1 = mgamma * o1, p2 = mgamma * o2, p3 = mgamma * o3;
     o1 = v1 + p1; o2 = v2 + p2; o3 = v3 + p3;
   }
-  Jv[cell] = o1; Jv[N + cell] = o2; Jv[2 * N + cell] = o3;
+  o[0] = o1; o[1] = o2; o[2] = o3;
+}
+
+template <bool AT>
+__global__ __launch_bounds__(256) void k_jtv(GeoD g, TermsD T, const double* __restrict__ y,
+                                             const double* __restrict__ v, double* __restrict__ Jv,
+                                             double mgamma) {
+  int cell, gx, gy;
+  if (AT && g.compact) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int hidx = t - g.n_active;
+    if (hidx >= 0 && hidx < g.n_hole) {
+      const int hc = g.hole_ids[hidx];
+      for (int c = 0; c < 3; c++) {
+        const double p = mgamma * 0.0;
+        Jv[c * g.ncell + hc] = v[c * g.ncell + hc] + p;
+      }
+      return;
+    }
+  }
+  if (!map_cell(g, Jv, cell, gx, gy)) return;
+  const int N = g.ncell;
+  double o[3];
+  jtv_cell<AT>(g, T, y, VPlain{v}, cell, gx, gy, mgamma, o);
+  Jv[cell] = o[0]; Jv[N + cell] = o[1]; Jv[2 * N + cell] = o[2];
 }
 
 // ---------------------------------------------------------- block-Jacobi
@@ -338,7 +368,7 @@ __global__ __launch_bounds__(256) void k_build_P(GeoD g, TermsD T, const double*
   double J[9];
   if (rebuild_J) {
     double h[3], dummy[3];
-    field<false>(g, T, y, nullptr, cell, gx, gy, h, dummy);
+    field<false>(g, T, y, VPlain{nullptr}, cell, gx, gy, h, dummy);
     if (hd) { h[0] += hd[cell]; h[1] += hd[N + cell]; h[2] += hd[2 * N + cell]; }
     double msk[3], nsk[3];
     axes(g, T, cell, msk, nsk);
@@ -386,6 +416,54 @@ __global__ __launch_bounds__(256) void k_build_P(GeoD g, TermsD T, const double*
   for (int s = 0; s < 9; s++) P[s * N + cell] = Pi[s];
 }
 
+// z = gate * P_cell r for one cell (P stored SoA, 9 planes)
+__device__ inline void apply_P_cell(const GeoD& g, int gate, const double* __restrict__ P, int cell,
+                                    const double r[3], double z[3]) {
+  const int N = g.ncell;
+  const double z0 = P[0 * N + cell] * r[0] + P[1 * N + cell] * r[1] + P[2 * N + cell] * r[2];
+  const double z1 = P[3 * N + cell] * r[0] + P[4 * N + cell] * r[1] + P[5 * N + cell] * r[2];
+  const double z2 = P[6 * N + cell] * r[0] + P[7 * N + cell] * r[1] + P[8 * N + cell] * r[2];
+  if (g.compact) {
+    z[0] = z0; z[1] = z1; z[2] = z2;
+  } else {
+    const double gt = gate_of(g, gate, cell);
+    z[0] = gt * z0; z[1] = gt * z1; z[2] = gt * z2;
+  }
+}
+
+// One SPGMR iteration's operator chain with left block-Jacobi
+// preconditioning, in one pass:
+//   u = (c vl) / s          (formed where read; vl itself is not written)
+//   z = u + (-gamma) J u,   vk = s .* (gate P z)
+// Every value is rounded as the separate N_VScale + N_VDiv, atimes,
+// PrecondSolve and N_VProd launches round it.  Compact hole cells get
+// vk = s * 0, as PrecondSolve's zeroed hole followed by the product.
+__global__ __launch_bounds__(256) void k_krylov(GeoD g, TermsD T, int pgate,
+                                                const double* __restrict__ P,
+                                                const double* __restrict__ y,
+                                                const double* __restrict__ vl,
+                                                const double* __restrict__ s, double c,
+                                                double mgamma, double* __restrict__ vk) {
+  int cell, gx, gy;
+  const int N = g.ncell;
+  if (g.compact) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int hidx = t - g.n_active;
+    if (hidx >= 0 && hidx < g.n_hole) {
+      const int hc = g.hole_ids[hidx];
+      for (int a = 0; a < 3; a++) vk[a * N + hc] = s[a * N + hc] * 0.0;
+      return;
+    }
+  }
+  if (!map_cell(g, vk, cell, gx, gy)) return;
+  double o[3], z[3];
+  jtv_cell<true>(g, T, y, VScaled{vl, s, c}, cell, gx, gy, mgamma, o);
+  apply_P_cell(g, pgate, P, cell, o, z);
+  vk[cell] = s[cell] * z[0];
+  vk[N + cell] = s[N + cell] * z[1];
+  vk[2 * N + cell] = s[2 * N + cell] * z[2];
+}
+
 __global__ __launch_bounds__(256) void k_apply_P(GeoD g, int gate, const double* __restrict__ P,
                                                  const double* __restrict__ r,
                                                  double* __restrict__ z) {
@@ -406,16 +484,10 @@ __global__ __launch_bounds__(256) void k_apply_P(GeoD g, int gate, const double*
     cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= N) return;
   }
-  const double r0 = r[cell], r1 = r[N + cell], r2 = r[2 * N + cell];
-  const double z0 = P[0 * N + cell] * r0 + P[1 * N + cell] * r1 + P[2 * N + cell] * r2;
-  const double z1 = P[3 * N + cell] * r0 + P[4 * N + cell] * r1 + P[5 * N + cell] * r2;
-  const double z2 = P[6 * N + cell] * r0 + P[7 * N + cell] * r1 + P[8 * N + cell] * r2;
-  if (g.compact) {
-    z[cell] = z0; z[N + cell] = z1; z[2 * N + cell] = z2;
-  } else {
-    const double gt = gate_of(g, gate, cell);
-    z[cell] = gt * z0; z[N + cell] = gt * z1; z[2 * N + cell] = gt * z2;
-  }
+  const double rr[3] = {r[cell], r[N + cell], r[2 * N + cell]};
+  double zz[3];
+  apply_P_cell(g, gate, P, cell, rr, zz);
+  z[cell] = zz[0]; z[N + cell] = zz[1]; z[2 * N + cell] = zz[2];
 }
 
 // --------------------------------------------------------------- normalise
@@ -827,6 +899,29 @@ int PrecondSolve(sunrealtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z,
       LLG_LpcCorrect(c->lpc, N_VGetDeviceArrayPointer(y), gamma, N_VGetDeviceArrayPointer(z)) != 0)
     return -1;
   return launch_status("PrecondSolve");
+}
+
+int LLG_Krylov(sunrealtype c, N_Vector vl, N_Vector s, N_Vector vk, sunrealtype gamma,
+               sunrealtype t, N_Vector y, N_Vector fy, void* user_data) {
+  (void)t; (void)fy;
+  llg_ctx* ctx = (llg_ctx*)user_data;
+  // the in-kernel preconditioner is the plain block-Jacobi one
+  if (!ctx || ctx->lpc || !ctx->pd || !vl || !s || !vk || !y) return -1;
+  const double* dvl = N_VGetDeviceArrayPointer(vl);
+  const double* ds = N_VGetDeviceArrayPointer(s);
+  double* dvk = N_VGetDeviceArrayPointer(vk);
+  if (dvk == dvl || dvk == ds || dvk == N_VGetDeviceArrayPointer(y)) return -1;
+  ctx->njv++;
+  ctx->npsol++;
+  hipStream_t st = stream_of(vk, ctx);
+  dim3 grid, block;
+  grid_for(ctx, grid, block);
+  kprof::begin(kprof::JTV, st);
+  hipLaunchKernelGGL(k_krylov, grid, block, 0, st, geo_of(ctx), to_dev(ctx->cfg.jv),
+                     ctx->cfg.pc.gate, (const double*)ctx->pd->d_P,
+                     (const double*)N_VGetDeviceArrayPointer(y), dvl, ds, c, -gamma, dvk);
+  kprof::end(kprof::JTV, st);
+  return launch_status("krylov");
 }
 
 int LLG_SetStencilBlock(llg_ctx* ctx, int bx, int by) {

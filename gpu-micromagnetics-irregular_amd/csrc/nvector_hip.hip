@@ -441,13 +441,16 @@ __global__ __launch_bounds__(kBlock) void k_reduce1(int kind, F f, sunindextype 
 // K dot products x . Y_k (and K weighted squares when W given)
 // x_i = (prod ? s_i * t_i : x_i); dots_k = sum_i x_i Y_k[i], where a Y_k
 // flagged in `self` is x itself.  With prod set, x is also written (the
-// Krylov vector z = s .* t and its Gram-Schmidt dots in one pass).
+// Krylov vector z = s .* t and its Gram-Schmidt dots in one pass).  With
+// sidx >= 0, Y_sidx is scaled on read, a = sc * Y_sidx[i], and a written
+// back: the normalisation of the newest basis vector done in this pass.
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_dotmulti(const double* x, PtrPack Y, sunindextype n,
                                                      double* partials, unsigned* ticket,
                                                      double* out, double* dout, unsigned seq,
                                                      const double* sv, const double* tv,
-                                                     double* xw, unsigned self) {
+                                                     double* xw, unsigned self, int sidx,
+                                                     double sc) {
   __shared__ double sm[K][kWaves];
   __shared__ bool last;
   double v[K], u[K];
@@ -464,7 +467,13 @@ __global__ __launch_bounds__(kBlock) void k_dotmulti(const double* x, PtrPack Y,
     return x[j];
   };
   auto yval = [&](int k, sunindextype j, double xj) -> double {
-    return ((self >> k) & 1u) ? xj : Y.p[k][j];
+    if ((self >> k) & 1u) return xj;
+    if (k == sidx) {
+      const double a = sc * Y.p[k][j];
+      const_cast<double*>(Y.p[k])[j] = a;
+      return a;
+    }
+    return Y.p[k][j];
   };
   for (; i + stride < n; i += 2 * stride) {
     const double x0 = xval(i), x1 = xval(i + stride);
@@ -654,6 +663,8 @@ struct ProdIn {
   double* xw = nullptr;
   unsigned self = 0;           // bit k: Y_k is x itself
   int slot = 0;                // result slot (1 also fills the device copy)
+  int sidx = -1;               // Y_sidx scaled by sc on read and written back
+  double sc = 1.0;
 };
 
 template <int K>
@@ -663,7 +674,7 @@ void launch_dotmulti(hipStream_t s, const double* x, const PtrPack& Y, sunindext
   hipLaunchKernelGGL(k_dotmulti<K>, dim3(g), dim3(kBlock), 0, s, x, Y, n,
                      w->partials, g_red_mode == 0 ? w->ticket : nullptr,
                      w->host_dev + pi.slot * kSlot, pi.slot ? w->dres : nullptr, w->seq,
-                     pi.s, pi.t, pi.xw, pi.self);
+                     pi.s, pi.t, pi.xw, pi.self, pi.sidx, pi.sc);
   if (g_red_mode != 0) launch_finish<K>(s, RED_SUM, g, w, w->seq, pi.slot);
 }
 
@@ -1167,17 +1178,23 @@ int NVHip_NewtonUpdate(int iter, N_Vector ycor, N_Vector del, N_Vector zn0, N_Ve
   return 0;
 }
 
-int NVHip_GSProject(N_Vector sv, N_Vector t, N_Vector z, int nvec, N_Vector* Y,
-                    sunrealtype* dots, sunrealtype* zz) {
-  if (!sv || !t || !z || nvec < 2 || nvec > kMaxK || !dots || !zz || Y[nvec - 1] != z ||
-      LEN(sv) != LEN(z) || LEN(t) != LEN(z))
-    return -1;
+}  // extern "C"
+
+namespace {
+// Gram-Schmidt projection of z against Y (Y[nvec-1] == z): dots, then
+// z -= sum d_j Y_j with its squared norm, one host wait.  With sv set,
+// z = sv .* t is formed first; with sidx >= 0, Y_sidx is scaled by sc
+// (and written back) as the dots read it.
+int gs_project(N_Vector sv, N_Vector t, N_Vector z, int nvec, N_Vector* Y, int sidx, double sc,
+               sunrealtype* dots, sunrealtype* zz) {
   PtrPack yp{};
   ProdIn pi;
-  pi.s = D(sv);
-  pi.t = D(t);
-  pi.xw = D(z);
+  pi.s = sv ? D(sv) : nullptr;
+  pi.t = sv ? D(t) : nullptr;
+  pi.xw = sv ? D(z) : nullptr;
   pi.slot = 1;
+  pi.sidx = sidx;
+  pi.sc = sc;
   FGsLincombSq f{};
   f.nv = nvec;
   f.z = D(z);
@@ -1213,6 +1230,25 @@ int NVHip_GSProject(N_Vector sv, N_Vector t, N_Vector z, int nvec, N_Vector* Y,
   for (int j = 0; j < nvec; j++) dots[j] = w->host[kSlot + j];
   *zz = w->host[0];
   return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int NVHip_GSProject(N_Vector sv, N_Vector t, N_Vector z, int nvec, N_Vector* Y,
+                    sunrealtype* dots, sunrealtype* zz) {
+  if (!sv || !t || !z || nvec < 2 || nvec > kMaxK || !dots || !zz || Y[nvec - 1] != z ||
+      LEN(sv) != LEN(z) || LEN(t) != LEN(z))
+    return -1;
+  return gs_project(sv, t, z, nvec, Y, -1, 1.0, dots, zz);
+}
+
+int NVHip_GSProjectScaled(N_Vector z, int nvec, N_Vector* Y, int sidx, sunrealtype sc,
+                          sunrealtype* dots, sunrealtype* zz) {
+  if (!z || nvec < 2 || nvec > kMaxK || !dots || !zz || Y[nvec - 1] != z || sidx < 0 ||
+      sidx >= nvec - 1)
+    return -1;
+  return gs_project(nullptr, nullptr, z, nvec, Y, sidx, sc, dots, zz);
 }
 
 int NVHip_LinCombDot(int nvec, sunrealtype* c, N_Vector* X, N_Vector z, sunrealtype* zz) {

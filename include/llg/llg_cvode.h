@@ -76,7 +76,14 @@ typedef struct llgcv_stats {
  *                        N_VWrmsNormVectorArray does (iter 0: N_VWrmsNorm)
  *   atimes(v, z, gamma, t, y, fy, user_data): z = v - gamma J v in one
  *                        call, for use with the same jtimes callback (equal
- *                        to jtimes + N_VLinearSum(1, v, -gamma, Jv, z))   */
+ *                        to jtimes + N_VLinearSum(1, v, -gamma, Jv, z))
+ * and, for left preconditioning with classical Gram-Schmidt, the whole
+ * operator chain of one SPGMR iteration (both needed; used together):
+ *   krylov(c, vl, s, vk, gamma, t, y, fy, user_data): with u = (c vl) / s,
+ *                        vk = s .* psolve(u - gamma J u), vl left unscaled
+ *                        (the registered jtimes and psolve callbacks, fused)
+ *   gsscaled(z, k, Y, i, c, d, zz): Y_i = c Y_i, then the gsproject
+ *                        projection of z (already formed) against Y      */
 typedef struct llgcv_fused_ops {
   int (*pascal)(int q, N_Vector* zn, sunrealtype sign);
   int (*ewt)(N_Vector y, sunrealtype rtol, sunrealtype atol, N_Vector abstol, N_Vector w);
@@ -90,6 +97,10 @@ typedef struct llgcv_fused_ops {
                    sunrealtype* dn, sunrealtype* yn);
   int (*atimes)(N_Vector v, N_Vector z, sunrealtype gamma, sunrealtype t, N_Vector y,
                 N_Vector fy, void* user_data);
+  int (*krylov)(sunrealtype c, N_Vector vl, N_Vector s, N_Vector vk, sunrealtype gamma,
+                sunrealtype t, N_Vector y, N_Vector fy, void* user_data);
+  int (*gsscaled)(N_Vector z, int k, N_Vector* Y, int i, sunrealtype c, sunrealtype* d,
+                  sunrealtype* zz);
 } llgcv_fused_ops;
 
 void* LLGCVodeCreate(int lmm, SUNContext sunctx);

@@ -26,7 +26,10 @@ typedef struct llg_run_opts {
   int    fused;       /* bit 0: FusedNVec_Init on y (one-launch multi-dot and
                          linear combination); bit 1: the integrator's one-pass
                          kernels (Nordsieck, error weights, SPGMR products,
-                         Gram-Schmidt); default 3.  Bit 1 changes no value. */
+                         Gram-Schmidt); bit 2 (with bit 1, left block-Jacobi
+                         and analytic Jv): each SPGMR iteration's operator
+                         chain in one launch (LLG_Krylov); default 7.  Bits
+                         1 and 2 change no value. */
   long   mxstep;      /* max steps per CVode call (CVODE default 500) */
   int    use_jtv;     /* analytic Jv (else difference quotient) */
   int    use_prec;    /* block-Jacobi preconditioner */

@@ -150,6 +150,16 @@ int JtvProduct(N_Vector v, N_Vector Jv, sunrealtype t, N_Vector y, N_Vector fy,
  * atimes by the driver (llgcv_fused_ops.atimes). */
 int LLG_Atimes(N_Vector v, N_Vector z, sunrealtype gamma, sunrealtype t, N_Vector y,
                N_Vector fy, void* user_data);
+/* One SPGMR iteration's operator chain with left block-Jacobi
+ * preconditioning, in one launch: with u = (c vl) / s (formed where read;
+ * vl is not written), vk = s .* P^{-1}(u - gamma J(y) u).  Equal bit for
+ * bit to N_VScale(c, vl, vl), N_VDiv(vl, s, u), LLG_Atimes(u, z),
+ * PrecondSolve(z, t) and N_VProd(s, t, vk) (the CVLS/SPGMR sequence,
+ * apart from vl staying unscaled).  Returns -1 without launching when a
+ * learned correction is active or vk aliases vl, s or y.  No reference
+ * counterpart: installed as llgcv_fused_ops.krylov by the driver. */
+int LLG_Krylov(sunrealtype c, N_Vector vl, N_Vector s, N_Vector vk, sunrealtype gamma,
+               sunrealtype t, N_Vector y, N_Vector fy, void* user_data);
 int PrecondSetup(sunrealtype t, N_Vector y, N_Vector fy, sunbooleantype jok,
                  sunbooleantype* jcurPtr, sunrealtype gamma, void* user_data);
 int PrecondSolve(sunrealtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z,

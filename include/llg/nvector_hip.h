@@ -137,6 +137,12 @@ int NVHip_ProdDotMulti(N_Vector s, N_Vector t, N_Vector z, int nvec, N_Vector* Y
 int NVHip_LinCombDot(int nvec, sunrealtype* c, N_Vector* X, N_Vector z, sunrealtype* zz);
 int NVHip_GSProject(N_Vector s, N_Vector t, N_Vector z, int nvec, N_Vector* Y,
                     sunrealtype* dots, sunrealtype* zz);
+/* NVHip_GSProject for a z already formed, with the basis vector Y_sidx
+ * (sidx < nvec - 1) normalised in the same pass: Y_sidx = sc Y_sidx is
+ * written back as the multi-dot reads it.  Equal to N_VScale(sc, Y_sidx)
+ * followed by the projection of z. */
+int NVHip_GSProjectScaled(N_Vector z, int nvec, N_Vector* Y, int sidx, sunrealtype sc,
+                          sunrealtype* dots, sunrealtype* zz);
 /* x = (0 + sum_j c_j X_j) / s (N_VConst 0, N_VLinearCombination, N_VDiv) */
 int NVHip_LinCombDiv(int nvec, sunrealtype* c, N_Vector* X, N_Vector s, N_Vector x);
 /* Newton update: ycor += del; y = zn0 + ycor; *dnorm = WRMS(del, w); for

@@ -6,8 +6,9 @@ Trajectory bar: at rtol = atol = 1e-6 the GPU state stays within 1e-5
 max |dm| of the oracle's; |m| drift is printed.  The integrator's one-pass
 kernels (Nordsieck predict/restore, error weights, SPGMR scale/divide,
 product + Gram-Schmidt multi-dot, combination + norm) must reproduce the
-standard N_Vector op sequence bit for bit, so the fused and unfused runs
-are compared for equality.
+standard N_Vector op sequence bit for bit, as must the one-launch Krylov
+operator chain (scale/divide, J v, block-Jacobi solve and product), so
+the fused and unfused runs are compared for equality.
 """
 import numpy as np
 import pytest
@@ -24,21 +25,34 @@ def _case(name="square_hole", ng=48, ny=32, **kw):
     return cfg, geom, geom.ic_split_x()
 
 
-@pytest.mark.parametrize("gstype", [1, 2])
-def test_fused_integrator_kernels_are_bitwise_neutral(gstype):
-    cfg, geom, y0 = _case()
+def _compact_case():
+    cfg = la.preset("i6", ng=64, ny=64, demag=4.0)
+    geom = la.Geometry.ellipse(64, 64, 0.3, 0.3)
+    return cfg, geom, geom.ic_two_domain()
+
+
+@pytest.mark.parametrize("gstype,case", [(1, "square_hole"), (2, "square_hole"),
+                                         (2, "i6_compact")])
+def test_fused_integrator_kernels_are_bitwise_neutral(gstype, case):
+    """fused = 1: standard op sequences; 3: the one-pass integrator kernels;
+    7: also the one-launch Krylov operator chain.  All three must agree."""
+    cfg, geom, y0 = _case() if case == "square_hole" else _compact_case()
     sim = la.Simulation(cfg, geom)
     res = {}
-    for fused in (1, 3):
+    for fused in (1, 3, 7):
         run = sim.integrator(y0, rtol=1e-5, atol=1e-5, gstype=gstype, fused=fused)
         out = run.advance(1.0)
         res[fused] = (run.state(), out)
         run.close()
-    (ya, sa), (yb, sb) = res[1], res[3]
-    print(f"[gpu] gs={gstype} nst={sa['nst']} nli={sa['nli']} vs nst={sb['nst']} nli={sb['nli']}")
-    for k in ("nst", "nfe", "nni", "nli", "netf", "ncfn", "nsetups"):
-        assert sa[k] == sb[k], k
-    assert np.array_equal(ya, yb)
+    ya, sa = res[1]
+    for fused in (3, 7):
+        yb, sb = res[fused]
+        print(f"[gpu] {case} gs={gstype} fused 1: nst={sa['nst']} nli={sa['nli']} "
+              f"syncs={sa['nsyncs']}; fused {fused}: nst={sb['nst']} nli={sb['nli']} "
+              f"syncs={sb['nsyncs']}")
+        for k in ("nst", "nfe", "nni", "nli", "netf", "ncfn", "nsetups"):
+            assert sa[k] == sb[k], (fused, k)
+        assert np.array_equal(ya, yb), fused
 
 
 @pytest.mark.parametrize("name,kw", [("square_hole", {}), ("i6", {"demag": 4.0}),

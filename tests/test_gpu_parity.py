@@ -177,6 +177,50 @@ def test_atimes_equals_jtv_then_linear_sum(case):
     assert np.array_equal(za.get(), zb.get())
 
 
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_krylov_chain_equals_separate_calls(case):
+    """LLG_Krylov == NVHip_ScaleDiv + LLG_Atimes + PrecondSolve + N_VProd and
+    NVHip_GSProjectScaled == (N_VScale of the basis vector) + NVHip_GSProject,
+    bit for bit; the chain leaves the basis vector it reads unscaled"""
+    cfg, geom, y0 = make(case)
+    sim = la.Simulation(cfg, geom)
+    L = la.lib()
+    rng = np.random.default_rng(6)
+    n = sim.neq
+    y = perturbed(y0, geom.active, 4)
+    sim.rhs(y)
+    gamma, c = 0.0371, 0.4375
+    sim.psetup(y, gamma)
+    vy = sim.vec(y)
+    s = sim.vec(1.0 / (1e-4 * np.abs(rng.standard_normal(n)) + 1e-4))
+    x = rng.standard_normal(n) * np.tile(geom.active.astype(float), 3)
+    # separate calls (the unfused SPGMR iteration)
+    vl_a, u, z, t, vk_a = sim.vec(x), sim.vec(), sim.vec(), sim.vec(), sim.vec()
+    assert L.NVHip_ScaleDiv(c, vl_a.v, s.v, vl_a.v, u.v) == 0
+    assert L.LLG_Atimes(u.v, z.v, gamma, 0.0, vy.v, None, sim.ctx) == 0
+    assert L.PrecondSolve(0.0, vy.v, None, z.v, t.v, gamma, 0.0, la.PREC_LEFT, sim.ctx) == 0
+    L.N_VProd_Hip(s.v, t.v, vk_a.v)
+    # the chain
+    vl_b, vk_b = sim.vec(x), sim.vec()
+    assert L.LLG_Krylov(c, vl_b.v, s.v, vk_b.v, gamma, 0.0, vy.v, None, sim.ctx) == 0
+    assert np.array_equal(vl_b.get(), x)
+    assert np.array_equal(vk_a.get(), vk_b.get())
+    assert L.LLG_Krylov(c, vl_b.v, s.v, vl_b.v, gamma, 0.0, vy.v, None, sim.ctx) == -1
+    # Gram-Schmidt against [w0, w1, vl, vk], vl normalised in the pass
+    w = [rng.standard_normal(n) for _ in range(2)]
+    wa, wb = [sim.vec(a) for a in w], [sim.vec(a) for a in w]
+    Ya = (C.c_void_p * 4)(wa[0].v, wa[1].v, vl_a.v, vk_a.v)
+    Yb = (C.c_void_p * 4)(wb[0].v, wb[1].v, vl_b.v, vk_b.v)
+    da, db = (C.c_double * 4)(), (C.c_double * 4)()
+    za, zb = C.c_double(), C.c_double()
+    assert L.NVHip_GSProject(s.v, t.v, vk_a.v, 4, Ya, da, C.byref(za)) == 0
+    assert L.NVHip_GSProjectScaled(vk_b.v, 4, Yb, 2, c, db, C.byref(zb)) == 0
+    assert list(da) == list(db) and za.value == zb.value
+    assert np.array_equal(vl_a.get(), vl_b.get())
+    assert np.array_equal(vk_a.get(), vk_b.get())
+    print(f"[gpu] {case[0]}: |vk|^2 after projection {zb.value:.6e}")
+
+
 def test_demag_autotune_keeps_the_field_bitwise():
     """the autotuner only changes launch shapes and LDS layout: the field
     must not change by a single bit"""

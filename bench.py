@@ -238,10 +238,11 @@ def main():
     sim = la.Simulation(cfg, geom)
     tune = None
     if not args.no_autotune:
-        ch = (C.c_int * 6)()
+        ch = (C.c_int * 8)()
         if la.lib().LLG_Autotune(sim.ctx, 5, ch) == 0:
             tune = {"stencil_block": [ch[0], ch[1]], "demag_row_block": ch[2],
-                    "demag_col_block": ch[3], "demag_row_pad": ch[4], "demag_col_pad": ch[5]}
+                    "demag_col_block": ch[3], "demag_row_pad": ch[4], "demag_col_pad": ch[5],
+                    "demag_row_kernel": ch[6], "demag_col_kernel": ch[7]}
     setup_s = time.perf_counter() - t_setup
     step = 1.0
     ropts = dict(rtol=1e-4, atol=1e-4, maxl=5, gstype=args.gstype, max_order=5, t_out=0.1,

@@ -35,6 +35,7 @@
 
 #include "llg/demag_fft.h"
 #include "llg/nvector_hip.h"
+#include "demag_reg.h"
 #include "kprof.h"
 
 // No implicit multiply-add contraction in this file: every fused
@@ -45,32 +46,32 @@
 
 namespace {
 
-__device__ __host__ inline double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __host__ inline float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+using rfft::cadd;
+using rfft::cconj;
+using rfft::cmul;
+using rfft::cmuli;
+using rfft::csub;
+using rfft::fmad;
+using rfft::mk;
 
 thread_local std::string g_derr;
 void derr(const std::string& e) { g_derr = e; }
 
 constexpr int kMaxPasses = 16;
 constexpr int kXcd = 8;
+constexpr size_t kMaxLds = 160 * 1024;   // LDS of one gfx950 CU
 
-// complex types of the two pipeline precisions
-// a plain complex pair (no member functions, so it lives in any address
-// space); same size and alignment as double2 / float2
-template <class T>
-struct alignas(2 * sizeof(T)) Cplx {
-  T x, y;
-};
-template <class T> struct V2T { using type = Cplx<T>; };
-template <class T> using V2 = typename V2T<T>::type;
-
-template <class T>
-__device__ __host__ inline V2<T> mk(T x, T y) {
-  V2<T> r;
-  r.x = x;
-  r.y = y;
-  return r;
+// dynamic LDS above 64 KB has to be allowed per kernel
+template <class K>
+bool allow_lds(K kern, size_t bytes) {
+  if (bytes <= 65536) return true;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess;
 }
+
+// complex types of the two pipeline precisions (a plain pair, same size
+// and alignment as double2 / float2)
+template <class T> using V2 = rfft::Cplx<T>;
 
 template <class T>
 struct Plan1D {
@@ -88,23 +89,6 @@ struct Plan1D {
 // kernels, so these pointers resolve to the LDS address space)
 template <class T>
 using LdsV2 = V2<T>;
-
-template <class C2> __device__ inline C2 cmul(C2 a, C2 b) {
-  C2 r;
-  r.x = fmad(a.x, b.x, -(a.y * b.y));
-  r.y = fmad(a.x, b.y, a.y * b.x);
-  return r;
-}
-template <class C2> __device__ inline C2 cadd(C2 a, C2 b) { C2 r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
-template <class C2> __device__ inline C2 csub(C2 a, C2 b) { C2 r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
-template <class C2> __device__ inline C2 cconj(C2 a) { C2 r; r.x = a.x; r.y = -a.y; return r; }
-// multiply by sign*i
-template <class C2, class T> __device__ inline C2 cmuli(C2 a, T sign) {
-  C2 r;
-  r.x = -sign * a.y;
-  r.y = sign * a.x;
-  return r;
-}
 
 // in-register DFT of length R, kernel exp(sign * 2 pi i j k / R)
 template <int R, class T>
@@ -655,7 +639,10 @@ bool make_plan(int L, int ncomp, HostPlan<T>& hp) {
   int maxb = 1;
   for (int r : rad) maxb = std::max(maxb, ncomp * L / r);
   block = std::max(block, std::min(((maxb + 63) / 64) * 64, 256));
-  if (block > 512) { derr("FFT length too large for one workgroup: " + std::to_string(L)); return false; }
+  // the Stockham kernels hold all ncomp transforms in one workgroup's LDS;
+  // a length beyond one workgroup keeps only its twiddle table (the
+  // register-resident kernels cover it) and block = 0 marks it
+  if (block > 512 || (size_t)ncomp * lpad_len(L, hp.p.psh) * sizeof(V2<T>) > kMaxLds) block = 0;
   hp.ncomp = ncomp;
   hp.block = block;
   auto log2_if_pow2 = [](int x) {
@@ -671,7 +658,7 @@ bool make_plan(int L, int ncomp, HostPlan<T>& hp) {
     ns_run *= rad[i];
     hp.p.radix[i] = rad[i];
     const int tot = ncomp * L / rad[i];
-    int b = (tot + block - 1) / block;
+    int b = block ? (tot + block - 1) / block : 8;
     if (b > 4) b = 8;
     hp.p.bpt[i] = b;
   }
@@ -690,6 +677,7 @@ bool make_plan(int L, int ncomp, HostPlan<T>& hp) {
 // its register budget at the resulting butterflies per thread
 template <class T>
 bool set_block(HostPlan<T>& hp, int block) {
+  if (hp.block == 0) return block == 0;   // no Stockham form of this plan
   if (block < 64 || block > 512 || block % 64) return false;
   int bpt[kMaxPasses];
   for (int i = 0; i < hp.p.npass; i++) {
@@ -781,6 +769,10 @@ struct DemagData {
   int has_xy_im;
   int fp32;
   int ct = 1;         // compile-time FFT plans where one matches the plan
+  // register-resident FFT kernels (demag_reg_kernels.h): variant per pass,
+  // -1 = the LDS Stockham kernels of this file
+  int reg_row = 0;
+  int reg_col = 0;
 };
 
 namespace {
@@ -831,6 +823,15 @@ DemagData* init_common(int nx, int ny, double thick, double strength, int padded
     destroy(d);
     return nullptr;
   }
+  // the tensor spectra use the one-component Stockham transforms; the field
+  // needs a row and a column kernel of either family
+  if (d->row.block == 0 || d->col1.block == 0 ||
+      (d->col.block == 0 && !demag_reg::has_col_plan<double>(d->py, 0))) {
+    derr("Demag: FFT lengths " + std::to_string(d->px / 2) + " x " + std::to_string(d->py) +
+         " exceed one workgroup");
+    destroy(d);
+    return nullptr;
+  }
   const int Px = d->px, Py = d->py, Hx = d->hx;
   std::vector<double2> post(Hx);
   for (int k = 0; k < Hx; k++) {
@@ -865,6 +866,15 @@ DemagData* init_common(int nx, int ny, double thick, double strength, int padded
                      qy, Px, Py, padded, dKr);
   // spectra of the 4 real planes: row R2C + column forward FFT
   const double scale = strength / ((double)Px * (double)Py);
+  if (!allow_lds(k_row_fwd<double, GenFft<double>>,
+                 sizeof(double2) * lpad_len(d->row.p.L, d->row.p.psh)) ||
+      !allow_lds(k_col<double, GenFft<double>>, sizeof(double2) * lpad_len(Py, d->col1.p.psh))) {
+    derr("Demag: LDS attribute");
+    hipFree(dQ);
+    hipFree(dKr);
+    destroy(d);
+    return nullptr;
+  }
   for (int c = 0; c < 4; c++) {
     hipLaunchKernelGGL((k_row_fwd<double, GenFft<double>>), dim3(Py, 1), dim3(d->row.block),
                        sizeof(double2) * lpad_len(d->row.p.L, d->row.p.psh), d->stream,
@@ -886,6 +896,9 @@ DemagData* init_common(int nx, int ny, double thick, double strength, int padded
   hipFree(dQ);
   hipFree(dKr);
   if (!launch_ok("Demag_Init")) { destroy(d); return nullptr; }
+  // register-resident FFT kernels wherever a plan covers the lengths
+  d->reg_row = demag_reg::has_row_plan<double>(d->px / 2, 0) ? 0 : -1;
+  d->reg_col = demag_reg::has_col_plan<double>(d->py, 0) ? 0 : -1;
   return d;
 }
 
@@ -930,25 +943,42 @@ void apply_t(DemagData* d, const HostPlan<T>& row, const HostPlan<T>& col, const
   hipStream_t s = d->stream;
   V2<T>* spec = reinterpret_cast<V2<T>*>(d->d_spec);
   const size_t row_lds = sizeof(V2<T>) * lpad_len(row.p.L, row.p.psh);
+  const int half = d->padded;
   kprof::begin(kprof::DEMAG_ALL, s);
   kprof::begin(kprof::DEMAG_ROW_FWD, s);
-  with_row_plan(row, d->ct, [&](auto f) {
-    hipLaunchKernelGGL((k_row_fwd<T, decltype(f)>), dim3(ny, 3), dim3(row.block), row_lds, s, y,
-                       w, N, nx, ny, f, post, spec);
-  });
+  int rc = d->reg_row < 0 ? 0
+                          : demag_reg::launch_row_fwd<T>(row.p.L, d->reg_row, half, y, w, N, nx,
+                                                          ny, row.p.tw, post, spec, s);
+  if (rc == 0)
+    with_row_plan(row, d->ct, [&](auto f) {
+      allow_lds(k_row_fwd<T, decltype(f)>, row_lds);
+      hipLaunchKernelGGL((k_row_fwd<T, decltype(f)>), dim3(ny, 3), dim3(row.block), row_lds, s,
+                         y, w, N, nx, ny, f, post, spec);
+    });
   kprof::end(kprof::DEMAG_ROW_FWD, s);
   kprof::begin(kprof::DEMAG_COL, s);
-  with_col_plan(col, d->ct, [&](auto f) {
-    hipLaunchKernelGGL((k_col<T, decltype(f)>), dim3(Hx), dim3(col.block),
-                       sizeof(V2<T>) * 3 * lpad_len(Py, col.p.psh), s, spec, ny, ny, Hx, f, K,
-                       d->has_xy_im, 0);
-  });
+  rc = d->reg_col < 0 ? 0
+                      : demag_reg::launch_col<T>(Py, d->reg_col, half, spec, Hx, col.p.tw, K,
+                                                  d->has_xy_im, s);
+  if (rc == 0)
+    with_col_plan(col, d->ct, [&](auto f) {
+      const size_t col_lds = sizeof(V2<T>) * 3 * lpad_len(Py, col.p.psh);
+      allow_lds(k_col<T, decltype(f)>, col_lds);
+      hipLaunchKernelGGL((k_col<T, decltype(f)>), dim3(Hx), dim3(col.block), col_lds, s, spec,
+                         ny, ny, Hx, f, K, d->has_xy_im, 0);
+    });
   kprof::end(kprof::DEMAG_COL, s);
   kprof::begin(kprof::DEMAG_ROW_INV, s);
-  with_row_plan(row, d->ct, [&](auto f) {
-    hipLaunchKernelGGL((k_row_inv<T, decltype(f)>), dim3(ny, 3), dim3(row.block), row_lds, s,
-                       (const V2<T>*)spec, ny, f, post, h, N, nx);
-  });
+  rc = d->reg_row < 0 ? 0
+                      : demag_reg::launch_row_inv<T>(row.p.L, d->reg_row, half,
+                                                      (const V2<T>*)spec, ny, row.p.tw, post, h,
+                                                      N, nx, s);
+  if (rc == 0)
+    with_row_plan(row, d->ct, [&](auto f) {
+      allow_lds(k_row_inv<T, decltype(f)>, row_lds);
+      hipLaunchKernelGGL((k_row_inv<T, decltype(f)>), dim3(ny, 3), dim3(row.block), row_lds, s,
+                         (const V2<T>*)spec, ny, f, post, h, N, nx);
+    });
   kprof::end(kprof::DEMAG_ROW_INV, s);
   kprof::end(kprof::DEMAG_ALL, s);
   launch_ok("Demag_Apply");
@@ -1069,6 +1099,26 @@ int Demag_SetLdsPadding(DemagData* d, int row_shift, int col_shift) {
   return 0;
 }
 
+int Demag_SetFftKernels(DemagData* d, int row_variant, int col_variant) {
+  if (!d) return -1;
+  const int Lr = d->px / 2, Lc = d->py;
+  if (row_variant >= 0 && !demag_reg::has_row_plan<double>(Lr, row_variant)) return -1;
+  if (col_variant >= 0 && !demag_reg::has_col_plan<double>(Lc, col_variant)) return -1;
+  // the Stockham kernels exist only for lengths that fit one workgroup
+  if (row_variant < 0 && (d->row.block == 0 || (d->d_K32 && d->row32.block == 0))) return -1;
+  if (col_variant < 0 && (d->col.block == 0 || (d->d_K32 && d->col32.block == 0))) return -1;
+  d->reg_row = row_variant < 0 ? -1 : row_variant;
+  d->reg_col = col_variant < 0 ? -1 : col_variant;
+  return 0;
+}
+
+int Demag_GetFftKernels(DemagData* d, int* row_variant, int* col_variant) {
+  if (!d) return -1;
+  if (row_variant) *row_variant = d->reg_row;
+  if (col_variant) *col_variant = d->reg_col;
+  return 0;
+}
+
 int Demag_Autotune(DemagData* d, int reps, int* choice) {
   if (!d || reps < 1) return -1;
   const long n3 = 3L * d->nx * d->ny;
@@ -1081,6 +1131,7 @@ int Demag_Autotune(DemagData* d, int reps, int* choice) {
   hipEventCreate(&e1);
   const int rb0 = d->row.block, cb0 = d->col.block;
   const int rp0 = d->row.p.psh, cp0 = d->col.p.psh;
+  const int rv0 = d->reg_row, cv0 = d->reg_col;
   // candidates: the current settings first, then the usual alternatives
   auto cands = [](int cur, std::initializer_list<int> alt) {
     std::vector<int> v{cur};
@@ -1088,46 +1139,72 @@ int Demag_Autotune(DemagData* d, int reps, int* choice) {
       if (a != cur) v.push_back(a);
     return v;
   };
+  std::vector<int> row_vars, col_vars;
+  for (int v : cands(rv0, {0, -1}))
+    if (v < 0 ? d->row.block > 0 && (!d->d_K32 || d->row32.block > 0)
+              : demag_reg::has_row_plan<double>(d->px / 2, v))
+      row_vars.push_back(v);
+  for (int v : cands(cv0, {0, 1, -1}))
+    if (v < 0 ? d->col.block > 0 && (!d->d_K32 || d->col32.block > 0)
+              : demag_reg::has_col_plan<double>(d->py, v))
+      col_vars.push_back(v);
   const std::vector<int> row_blocks = cands(rb0, {64, 128, 192, 256, 320});
   const std::vector<int> col_blocks = cands(cb0, {128, 192, 256, 320});
   const std::vector<int> row_pads = cands(rp0, {3, 31}), col_pads = cands(cp0, {4, 31});
   float best = 1e30f;
-  int best_cfg[4] = {rb0, cb0, rp0, cp0};
-  for (int rb : row_blocks)
-    for (int cb : col_blocks)
-      for (int rp : row_pads)
-        for (int cp : col_pads) {
-          if (!set_block(d->row, rb) || !set_block(d->col, cb)) continue;
-          // the column plan's three-component LDS must fit three workgroups
-          d->row.p.psh = rp;
-          d->col.p.psh = cp;
-          apply(d, y, nullptr, h);   // warm
-          hipEventRecord(e0, d->stream);
-          for (int i = 0; i < reps; i++) apply(d, y, nullptr, h);
-          hipEventRecord(e1, d->stream);
-          hipEventSynchronize(e1);
-          float ms = 0.f;
-          hipEventElapsedTime(&ms, e0, e1);
-          if (ms < best) {
-            best = ms;
-            best_cfg[0] = rb; best_cfg[1] = cb; best_cfg[2] = rp; best_cfg[3] = cp;
-          }
-        }
+  int best_cfg[6] = {rb0, cb0, rp0, cp0, rv0, cv0};
+  auto time_it = [&]() {
+    apply(d, y, nullptr, h);   // warm
+    hipEventRecord(e0, d->stream);
+    for (int i = 0; i < reps; i++) apply(d, y, nullptr, h);
+    hipEventRecord(e1, d->stream);
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    return ms;
+  };
+  for (int rv : row_vars)
+    for (int cv : col_vars) {
+      d->reg_row = rv;
+      d->reg_col = cv;
+      // workgroup sizes and LDS paddings only matter to the Stockham kernels
+      const std::vector<int> rbs = rv < 0 ? row_blocks : std::vector<int>{rb0};
+      const std::vector<int> rps = rv < 0 ? row_pads : std::vector<int>{rp0};
+      const std::vector<int> cbs = cv < 0 ? col_blocks : std::vector<int>{cb0};
+      const std::vector<int> cps = cv < 0 ? col_pads : std::vector<int>{cp0};
+      for (int rb : rbs)
+        for (int cb : cbs)
+          for (int rp : rps)
+            for (int cp : cps) {
+              if (!set_block(d->row, rb) || !set_block(d->col, cb)) continue;
+              d->row.p.psh = rp;
+              d->col.p.psh = cp;
+              const float ms = time_it();
+              if (ms < best) {
+                best = ms;
+                best_cfg[0] = rb; best_cfg[1] = cb; best_cfg[2] = rp; best_cfg[3] = cp;
+                best_cfg[4] = rv; best_cfg[5] = cv;
+              }
+            }
+    }
   set_block(d->row, best_cfg[0]);
   set_block(d->col, best_cfg[1]);
   d->row.p.psh = best_cfg[2];
   d->col.p.psh = best_cfg[3];
+  d->reg_row = best_cfg[4];
+  d->reg_col = best_cfg[5];
   if (d->d_K32) {
     set_block(d->row32, best_cfg[0]);
     set_block(d->col32, best_cfg[1]);
+    d->row32.p.psh = best_cfg[2];
+    d->col32.p.psh = best_cfg[3];
   }
-  (void)rb0; (void)cb0; (void)rp0; (void)cp0;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   hipFree(y);
   hipFree(h);
   if (choice)
-    for (int i = 0; i < 4; i++) choice[i] = best_cfg[i];
+    for (int i = 0; i < 6; i++) choice[i] = best_cfg[i];
   return launch_ok("Demag_Autotune") ? 0 : -1;
 }
 

@@ -1,0 +1,381 @@
+// demag_reg_kernels.h -- the three demag passes on the register-resident
+// FFT (fft_regs.h).  Included by one translation unit per precision.
+//
+// Pipeline (same spectrum layout and tensor planes as the LDS Stockham
+// kernels of demag_fft.hip):
+//   row forward : lane u of a (row, component) transform loads the complex
+//                 pairs z[m] = (x[2m], x[2m+1]) of the gathered row for
+//                 m = u + U b straight into registers (coalesced), runs the
+//                 forward FFT of length L = Px/2, and the real-FFT split
+//                 writes bins 0..L of the row.
+//   column      : lane u of a column loads rows u + U b of the three
+//                 components, runs the three forward FFTs, multiplies by the
+//                 3x3 tensor in registers (each lane holds all three
+//                 components at the same ky), runs the inverse FFTs and
+//                 stores only the rows the field needs.
+//   row inverse : the C2R pre-processing reads bins k and L-k straight from
+//                 the spectrum into the registers of the last FFT stage,
+//                 the inverse FFT ends in lane u holding z[u + U b], stored
+//                 as pairs into the SoA field (overwrite).
+// Zero-padded grids (half): rows and columns load only their non-zero first
+// half and store only the first half of the inverse outputs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "demag_reg.h"
+#include "fft_regs.h"
+
+// no implicit multiply-add contraction: every fused multiply-add is an
+// explicit fma, so the rounding of each pass is fixed by its source
+#pragma clang fp contract(off)
+
+namespace demag_reg {
+
+using rfft::cadd;
+using rfft::cconj;
+using rfft::cmul;
+using rfft::cmulc;
+using rfft::csub;
+using rfft::fmad;
+using rfft::ldc;
+using rfft::mk;
+using rfft::stc;
+using rfft::sfor;
+
+// consecutive workgroup ids -> the same XCD (of 8) gets a contiguous range
+__device__ inline int reg_xcd_remap(int bid, int G) {
+  const int per = G / 8, rem = G % 8;
+  const int x = bid % 8, idx = bid / 8;
+  return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
+}
+
+struct Bar {
+  __device__ __forceinline__ void operator()() const { __syncthreads(); }
+};
+
+// ------------------------------------------------------------ row forward
+template <class T, class F, bool HALF>
+__global__ __launch_bounds__(F::kThreads) void k_row_fwd_reg(
+    const double* __restrict__ in, const double* __restrict__ w, long comp_stride, int nx_in,
+    int rows, const C2<T>* __restrict__ tw, const C2<T>* __restrict__ post,
+    C2<T>* __restrict__ spec) {
+  using C = C2<T>;
+  constexpr int U = F::U, V = F::V, L = F::L, G = F::kPerGroup;
+  constexpr int NB = HALF ? V / 2 : V;
+  extern __shared__ char lds_raw[];
+  const int lane = threadIdx.x & (U - 1);
+  const int slot = threadIdx.x / U;
+  const int total = 3 * rows;
+  int id = reg_xcd_remap(blockIdx.x, gridDim.x) * G + slot;
+  const bool live = id < total;
+  if (!live) id = total - 1;
+  const int comp = id / rows, row = id - comp * rows;
+  C* lds = reinterpret_cast<C*>(lds_raw) + slot * F::kSlots;
+  const double* src = in + comp * comp_stride + (long)row * nx_in;
+  const double* wr = w ? w + (long)row * nx_in : nullptr;
+  C r[V];
+  sfor<0, NB>([&](auto B) {
+    const int m = lane + U * B;
+    double2 v = *reinterpret_cast<const double2*>(src + 2 * m);
+    if (wr) {
+      const double2 ww = *reinterpret_cast<const double2*>(wr + 2 * m);
+      v.x = ww.x * v.x;
+      v.y = ww.y * v.y;
+    }
+    r[B] = mk<T>((T)v.x, (T)v.y);
+  });
+  F::template forward<HALF>(r, lds, lane, tw, Bar{});
+  using Last = typename F::Last;
+  __syncthreads();
+  Last::put_out(lds, r, lane);   // natural order: slot(k) = k + k / V
+  __syncthreads();
+  if (!live) return;
+  C* out = spec + ((long)comp * rows + row) * (L + 1);
+  auto split = [&](C zk, C zc, C p) {
+    const C e = mk<T>(T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y + zc.y));
+    const C d = csub(zk, zc);
+    const C o = mk<T>(T(0.5) * d.y, T(-0.5) * d.x);
+    return cadd(e, cmul(p, o));
+  };
+  sfor<0, V>([&](auto I) {
+    const int k = lane + U * I;
+    const int km = k == 0 ? 0 : L - k;
+    const C zk = lds[k + k / V];
+    const C zc = cconj(lds[km + km / V]);
+    stc(out + k, split(zk, zc, ldc(post + k)));
+  });
+  if (lane == 0) {
+    const C z0 = lds[0];
+    stc(out + L, split(z0, cconj(z0), ldc(post + L)));
+  }
+}
+
+// ------------------------------------------------------------ row inverse
+template <class T, class F, bool HALF>
+__global__ __launch_bounds__(F::kThreads) void k_row_inv_reg(
+    const C2<T>* __restrict__ spec, int rows, const C2<T>* __restrict__ tw,
+    const C2<T>* __restrict__ post, double* __restrict__ out, long comp_stride, int nx_out) {
+  using C = C2<T>;
+  constexpr int U = F::U, V = F::V, L = F::L, G = F::kPerGroup;
+  constexpr int NB = HALF ? V / 2 : V;
+  using Last = typename F::Last;
+  extern __shared__ char lds_raw[];
+  const int lane = threadIdx.x & (U - 1);
+  const int slot = threadIdx.x / U;
+  const int total = 3 * rows;
+  int id = reg_xcd_remap(blockIdx.x, gridDim.x) * G + slot;
+  const bool live = id < total;
+  if (!live) id = total - 1;
+  const int comp = id / rows, row = id - comp * rows;
+  C* lds = reinterpret_cast<C*>(lds_raw) + slot * F::kSlots;
+  const C* in = spec + ((long)comp * rows + row) * (L + 1);
+  C r[V];
+  sfor<0, Last::NG>([&](auto J) {
+    sfor<0, Last::Q>([&](auto Tt) {
+      const int k = Last::k_of(lane, J, Tt);
+      const C xk = ldc(in + k);
+      const C xc = cconj(ldc(in + (L - k)));
+      const C a = cadd(xk, xc);
+      const C b = cmulc(csub(xk, xc), ldc(post + k));
+      r[J * Last::Q + Tt] = mk<T>(a.x - b.y, a.y + b.x);   // a + i b
+    });
+  });
+  F::template inverse<HALF>(r, lds, lane, tw, Bar{});
+  if (!live) return;
+  double* dst = out + comp * comp_stride + (long)row * nx_out;
+  sfor<0, NB>([&](auto B) {
+    const int m = lane + U * B;
+    double2 v;
+    v.x = (double)r[B].x;
+    v.y = (double)r[B].y;
+    *reinterpret_cast<double2*>(dst + 2 * m) = v;
+  });
+}
+
+// ------------------------------------------------------------------ column
+// waves per SIMD the column kernel asks the register allocator for: the
+// FP64 plans with 8 values per lane fit four (128 VGPRs), which lets a CU
+// hold 8 columns of 1024 rows at once; otherwise no constraint
+template <class T, class F>
+struct ColWaves {
+  static constexpr int value = (sizeof(T) == 8 && F::V == 8 && F::U >= 128) ? 4 : 1;
+};
+// XYI: the xy tensor plane has an imaginary part (periodic grids); the
+// zero-padded grid's xy spectrum is real (odd-odd kernel)
+template <class T, class F, bool HALF, bool XYI>
+__global__ __launch_bounds__(F::kThreads) __attribute__((amdgpu_waves_per_eu(ColWaves<T, F>::value))) void k_col_reg(C2<T>* __restrict__ spec, int Hx,
+                                                         const C2<T>* __restrict__ tw,
+                                                         const T* __restrict__ K) {
+  using C = C2<T>;
+  constexpr int U = F::U, V = F::V, L = F::L, G = F::kPerGroup;
+  constexpr int NB = HALF ? V / 2 : V;
+  constexpr int ROWS = HALF ? L / 2 : L;
+  using Last = typename F::Last;
+  constexpr int Q = Last::Q, CH = Q < 4 ? Q : 4;   // tensor values per chunk
+  extern __shared__ char lds_raw[];
+  const int lane = threadIdx.x & (U - 1);
+  const int slot = threadIdx.x / U;
+  int kx = reg_xcd_remap(blockIdx.x, gridDim.x) * G + slot;
+  const bool live = kx < Hx;
+  if (!live) kx = Hx - 1;
+  C* lds = reinterpret_cast<C*>(lds_raw) + slot * F::kSlots;
+  C r[3][V];
+  sfor<0, 3>([&](auto Cc) {
+    const C* src = spec + ((long)Cc * ROWS + rfft::opaque(lane)) * Hx + kx;
+    sfor<0, NB>([&](auto B) { r[Cc][B] = ldc(src + (long)U * B * Hx); });
+  });
+  sfor<0, 3>([&](auto Cc) { F::template forward<HALF>(r[Cc], lds, lane, tw, Bar{}); });
+  // h = K m at every (kx, ky) this lane holds, a few values at a time so the
+  // tensor loads in flight stay within the register budget; the z
+  // component (h_z = K_zz m_z) is finished and stored first, so x and y
+  // are the only components live through the rest
+  // (plane bases are uniform; the per-lane offset is a 32-bit unsigned index)
+  const long P = (long)Hx * L;
+  const T* kxx = K + (long)kx * L;
+  const T* kyy = kxx + P;
+  const T* kzz = kyy + P;
+  const T* kxyr = kzz + P;
+  const T* kxyi = kxyr + P;
+  auto each_k = [&](auto&& fn) {
+    sfor<0, Last::NG>([&](auto J) {
+      sfor<0, Q / CH>([&](auto Ch) {
+        const int ln = rfft::opaque(lane);
+        sfor<0, CH>([&](auto Tq) {
+          constexpr int t = Ch * CH + Tq;
+          fn(J * Q + t, (unsigned)Last::k_of(ln, J, t));
+        });
+      });
+    });
+  };
+  auto store = [&](auto Cc) {
+    if (live) {
+      C* dst = spec + ((long)Cc * ROWS + rfft::opaque(lane)) * Hx + kx;
+      sfor<0, NB>([&](auto B) { stc(dst + (long)U * B * Hx, r[Cc][B]); });
+    }
+  };
+  each_k([&](int i, unsigned k) {
+    const T cz = kzz[k];
+    r[2][i] = mk<T>(cz * r[2][i].x, cz * r[2][i].y);
+  });
+  F::template inverse<HALF>(r[2], lds, lane, tw, Bar{});
+  store(std::integral_constant<int, 2>{});
+  each_k([&](int i, unsigned k) {
+    const T a = kxx[k], b = kyy[k], xr = kxyr[k];
+    const C mx = r[0][i], my = r[1][i];
+    C cxy, cyx;
+    if constexpr (XYI) {
+      const C xy = mk<T>(xr, kxyi[k]);
+      cxy = cmul(xy, my);
+      cyx = cmul(xy, mx);
+    } else {
+      cxy = mk<T>(xr * my.x, xr * my.y);
+      cyx = mk<T>(xr * mx.x, xr * mx.y);
+    }
+    r[0][i] = mk<T>(fmad(a, mx.x, cxy.x), fmad(a, mx.y, cxy.y));
+    r[1][i] = mk<T>(fmad(b, my.x, cyx.x), fmad(b, my.y, cyx.y));
+  });
+  sfor<0, 2>([&](auto Cc) {
+    F::template inverse<HALF>(r[Cc], lds, lane, tw, Bar{});
+    store(Cc);
+  });
+}
+
+// ---------------------------------------------------------------- plans
+// Row transforms (complex length L = Px/2, one component per transform).
+template <class T> using RowP128 = rfft::RegFft<T, 128, 16, 4, 4>;
+template <class T> using RowP256 = rfft::RegFft<T, 256, 32, 8, 4>;
+template <class T> using RowP384 = rfft::RegFft<T, 384, 16, 4, 4>;
+template <class T> using RowP512 = rfft::RegFft<T, 512, 64, 8, 8>;
+template <class T> using RowP1024 = rfft::RegFft<T, 1024, 64, 8, 8>;
+template <class T> using RowP1536 = rfft::RegFft<T, 1536, 64, 8, 8>;
+template <class T> using RowP2048 = rfft::RegFft<T, 2048, 64, 8, 8>;
+template <class T> using RowP4096 = rfft::RegFft<T, 4096, 256, 16, 16>;
+// Column transforms (length Py, three components per lane).  Variant 0
+// holds 8 values per component and lane (about 150 VGPRs in FP64, three
+// waves per SIMD); variant 1 holds 16 (one more radix per stage, one fewer
+// LDS exchange, but over 256 registers in FP64).
+template <class T> using ColP128 = rfft::RegFft<T, 128, 16, 4, 4>;
+template <class T> using ColP256 = rfft::RegFft<T, 256, 32, 8, 4>;
+template <class T> using ColP512 = rfft::RegFft<T, 512, 64, 8, 8>;
+template <class T> using ColP1024 = rfft::RegFft<T, 1024, 128, 8, 4, 4>;
+template <class T> using ColP1024w = rfft::RegFft<T, 1024, 64, 8, 8>;
+template <class T> using ColP2048 = rfft::RegFft<T, 2048, 256, 8, 8, 4>;
+template <class T> using ColP2048w = rfft::RegFft<T, 2048, 128, 16, 8>;
+template <class T> using ColP4096 = rfft::RegFft<T, 4096, 512, 8, 8, 8>;
+template <class T> using ColP4096w = rfft::RegFft<T, 4096, 256, 16, 16>;
+template <class T> using ColP8192 = rfft::RegFft<T, 8192, 1024, 8, 8, 4, 4>;
+template <class T> using ColP8192w = rfft::RegFft<T, 8192, 512, 8, 8, 8>;
+
+template <class T, class Fn>
+bool with_row(int L, int variant, Fn&& fn) {
+  if (variant != 0) return false;
+  switch (L) {
+    case 128: fn(RowP128<T>{}); return true;
+    case 256: fn(RowP256<T>{}); return true;
+    case 384: fn(RowP384<T>{}); return true;
+    case 512: fn(RowP512<T>{}); return true;
+    case 1024: fn(RowP1024<T>{}); return true;
+    case 1536: fn(RowP1536<T>{}); return true;
+    case 2048: fn(RowP2048<T>{}); return true;
+    case 4096: fn(RowP4096<T>{}); return true;
+    default: return false;
+  }
+}
+
+template <class T, class Fn>
+bool with_col(int L, int variant, Fn&& fn) {
+  if (variant == 0) {
+    switch (L) {
+      case 128: fn(ColP128<T>{}); return true;
+      case 256: fn(ColP256<T>{}); return true;
+      case 512: fn(ColP512<T>{}); return true;
+      case 1024: fn(ColP1024<T>{}); return true;
+      case 2048: fn(ColP2048<T>{}); return true;
+      case 4096: fn(ColP4096<T>{}); return true;
+      case 8192: fn(ColP8192<T>{}); return true;
+      default: return false;
+    }
+  }
+  if (variant == 1) {
+    switch (L) {
+      case 1024: fn(ColP1024w<T>{}); return true;
+      case 2048: fn(ColP2048w<T>{}); return true;
+      case 4096: fn(ColP4096w<T>{}); return true;
+      case 8192: fn(ColP8192w<T>{}); return true;
+      default: return false;
+    }
+  }
+  return false;
+}
+
+template <class K>
+inline int prep_lds(K kern, size_t bytes) {
+  if (bytes > 65536 &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+    return -1;
+  return 0;
+}
+
+template <class T>
+bool has_row_plan(int L, int variant) {
+  return with_row<T>(L, variant, [](auto) {});
+}
+template <class T>
+bool has_col_plan(int L, int variant) {
+  return with_col<T>(L, variant, [](auto) {});
+}
+
+template <class T>
+int launch_row_fwd(int L, int variant, int half, const double* in, const double* w, long comp_stride, int nx_in,
+                   int rows, const C2<T>* tw, const C2<T>* post, C2<T>* spec, hipStream_t s) {
+  int rc = 0;
+  with_row<T>(L, variant, [&](auto f) {
+    using F = decltype(f);
+    const size_t lds = sizeof(C2<T>) * F::kSlots * F::kPerGroup;
+    const int grid = (3 * rows + F::kPerGroup - 1) / F::kPerGroup;
+    auto kern = half ? k_row_fwd_reg<T, F, true> : k_row_fwd_reg<T, F, false>;
+    if (prep_lds(kern, lds)) { rc = -1; return; }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(F::kThreads), lds, s, in, w, comp_stride, nx_in,
+                       rows, tw, post, spec);
+    rc = hipPeekAtLastError() == hipSuccess ? 1 : -1;
+  });
+  return rc;
+}
+
+template <class T>
+int launch_col(int L, int variant, int half, C2<T>* spec, int Hx, const C2<T>* tw, const T* K, int has_xy_im,
+               hipStream_t s) {
+  int rc = 0;
+  with_col<T>(L, variant, [&](auto f) {
+    using F = decltype(f);
+    const size_t lds = sizeof(C2<T>) * F::kSlots * F::kPerGroup;
+    const int grid = (Hx + F::kPerGroup - 1) / F::kPerGroup;
+    auto kern = half ? (has_xy_im ? k_col_reg<T, F, true, true> : k_col_reg<T, F, true, false>)
+                     : (has_xy_im ? k_col_reg<T, F, false, true> : k_col_reg<T, F, false, false>);
+    if (prep_lds(kern, lds)) { rc = -1; return; }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(F::kThreads), lds, s, spec, Hx, tw, K);
+    rc = hipPeekAtLastError() == hipSuccess ? 1 : -1;
+  });
+  return rc;
+}
+
+template <class T>
+int launch_row_inv(int L, int variant, int half, const C2<T>* spec, int rows, const C2<T>* tw,
+                   const C2<T>* post, double* out, long comp_stride, int nx_out, hipStream_t s) {
+  int rc = 0;
+  with_row<T>(L, variant, [&](auto f) {
+    using F = decltype(f);
+    const size_t lds = sizeof(C2<T>) * F::kSlots * F::kPerGroup;
+    const int grid = (3 * rows + F::kPerGroup - 1) / F::kPerGroup;
+    auto kern = half ? k_row_inv_reg<T, F, true> : k_row_inv_reg<T, F, false>;
+    if (prep_lds(kern, lds)) { rc = -1; return; }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(F::kThreads), lds, s, spec, rows, tw, post, out,
+                       comp_stride, nx_out);
+    rc = hipPeekAtLastError() == hipSuccess ? 1 : -1;
+  });
+  return rc;
+}
+
+}  // namespace demag_reg

@@ -977,10 +977,10 @@ int LLG_Autotune(llg_ctx* ctx, int reps, int* choice) {
   hipFree(o);
   if (choice) { choice[0] = bbx; choice[1] = bby; }
   if (ctx->demag) {
-    int dc[4];
+    int dc[6];
     if (Demag_Autotune(ctx->demag, reps, dc) != 0) return -1;
     if (choice)
-      for (int i = 0; i < 4; i++) choice[2 + i] = dc[i];
+      for (int i = 0; i < 6; i++) choice[2 + i] = dc[i];
   }
   return launch_status("LLG_Autotune");
 }

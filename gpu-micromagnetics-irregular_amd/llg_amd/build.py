@@ -22,6 +22,8 @@ SOURCES = [
     "nvector_hip.hip",
     "llg_kernels.hip",
     "demag_fft.hip",
+    "demag_reg_f64.hip",
+    "demag_reg_f32.hip",
     "cvode_bdf.cpp",
     "geometry.cpp",
     "llg_driver.cpp",

@@ -179,6 +179,8 @@ SIGNATURES = {
     "Demag_SetSpecialized": (C.c_int, [VP, C.c_int]),
     "Demag_SetWorkgroups": (C.c_int, [VP, C.c_int, C.c_int]),
     "Demag_Autotune": (C.c_int, [VP, C.c_int, IP]),
+    "Demag_SetFftKernels": (C.c_int, [VP, C.c_int, C.c_int]),
+    "Demag_GetFftKernels": (C.c_int, [VP, IP, IP]),
     "Demag_SetPrecision": (C.c_int, [VP, C.c_int]),
     "Demag_GetPlanShape": (C.c_int, [VP, IP, IP, IP]),
     "Demag_GetSpectrum": (C.c_int, [VP, DP]),

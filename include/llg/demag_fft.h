@@ -63,9 +63,23 @@ int  Demag_SetWorkgroups(DemagData* d, int row_block, int col_block);
  * directions (row, column) of the current precision that take a
  * compile-time plan, or -1. */
 int  Demag_SetSpecialized(DemagData* d, int on);
-/* Autotuner: times Demag_Apply (reps launches each) over row/column
- * workgroup sizes and LDS paddings, keeps the fastest and returns it in
- * choice[4] = {row block, col block, row pad shift, col pad shift}. */
+/* FFT kernel family per pass.  Variant >= 0 selects the register-resident
+ * kernels (each lane holds its values in registers between LDS exchanges;
+ * the tensor product happens in registers): rows have variant 0 for
+ * Px/2 in {128, 256, 384, 512, 1024, 1536, 2048, 4096}; columns have
+ * variant 0 (8 values per component and lane) for Py in {128, 256, 512,
+ * 1024, 2048, 4096, 8192} and variant 1 (16 per lane) for Py >= 1024.
+ * -1 selects the LDS Stockham kernels (any 2^a 3^b 5^c length).  Default:
+ * variant 0 where it exists.  Returns -1 (nothing changed) when a
+ * requested variant does not exist for the grid.  The two families round
+ * differently (both are within 1e-12 relative of the exact convolution). */
+int  Demag_SetFftKernels(DemagData* d, int row_variant, int col_variant);
+int  Demag_GetFftKernels(DemagData* d, int* row_variant, int* col_variant);
+/* Autotuner: times Demag_Apply (reps launches each) over the FFT kernel
+ * variants and, for the Stockham kernels, the row/column workgroup sizes
+ * and LDS paddings; keeps the fastest and returns it in choice[6] =
+ * {row block, col block, row pad shift, col pad shift, row variant,
+ * col variant}. */
 int  Demag_Autotune(DemagData* d, int reps, int* choice);
 /* introspection used by tests and the bench roofline */
 int  Demag_GetPlanShape(DemagData* d, int* px, int* py, int* padded);

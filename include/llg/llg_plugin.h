@@ -116,10 +116,11 @@ int      LLG_GetGrid(llg_ctx* ctx, int* ng, int* ny);
  * of 64, at most 256.  Results do not depend on it. */
 int      LLG_SetStencilBlock(llg_ctx* ctx, int bx, int by);
 /* Autotuner: times the Jv stencil over workgroup shapes and the demag
- * passes over workgroup sizes and LDS paddings (Demag_Autotune), keeps the
- * fastest.  choice[6] = {bx, by, demag row block, col block, row pad, col
- * pad}.  No reference counterpart (the reference fixes 256-thread 1D
- * launches at compile time). */
+ * passes over FFT kernel variants, workgroup sizes and LDS paddings
+ * (Demag_Autotune), keeps the fastest.  choice[8] = {bx, by, demag row
+ * block, col block, row pad, col pad, row variant, col variant}.  No
+ * reference counterpart (the reference fixes 256-thread 1D launches at
+ * compile time). */
 int      LLG_Autotune(llg_ctx* ctx, int reps, int* choice);
 /* device copy of the active flags (one byte per cell) */
 const unsigned char* LLG_GetActiveDevice(llg_ctx* ctx);

@@ -107,17 +107,34 @@ __device__ inline void axes(const GeoD& g, const TermsD& T, int cell, double msk
 // Read access to the direction vector v of a Jacobian product: stored
 // (VPlain), or formed on the fly as (c x) / s, the SPGMR basis vector
 // normalised and unscaled exactly as N_VScale then N_VDiv round it.
+// at(a, N, idx, nb): component a at cell idx, which is the centre (nb 0)
+// or its left / right / up / down neighbour (nb 1..4) of the cell being
+// evaluated.
 struct VPlain {
   const double* __restrict__ v;
-  __device__ double operator()(int i) const { return v[i]; }
+  __device__ double at(int a, int N, int idx, int) const { return v[a * N + idx]; }
 };
 struct VScaled {
   const double* __restrict__ x;
   const double* __restrict__ s;
   double c;
-  __device__ double operator()(int i) const {
-    const double a = c * x[i];
-    return a / s[i];
+  __device__ double at(int a, int N, int idx, int) const {
+    const int i = a * N + idx;
+    const double t = c * x[i];
+    return t / s[i];
+  }
+};
+// the same (c x) / s values, formed once per cell of an LDS tile (with a
+// one-cell halo) and read from there: tile[a][ty][tx] at the cell's tile
+// position (tx, ty), the neighbours at tx -+ 1 / ty -+ 1
+template <int SY, int SX>
+struct VTile {
+  const double (*t)[SY][SX];
+  int tx, ty;
+  __device__ double at(int a, int, int, int nb) const {
+    const int dx = nb == 1 ? -1 : (nb == 2 ? 1 : 0);
+    const int dy = nb == 3 ? -1 : (nb == 4 ? 1 : 0);
+    return t[a][ty + dy][tx + dx];
   }
 };
 
@@ -138,7 +155,7 @@ __device__ inline void field(const GeoD& g, const TermsD& T, const double* __res
   const double mdotmsk = m[0] * msk[0] + m[1] * msk[1] + m[2] * msk[2];
   double vv[3] = {0, 0, 0}, vdotmsk = 0.0;
   if (LIN) {
-    vv[0] = v(cell); vv[1] = v(N + cell); vv[2] = v(2 * N + cell);
+    vv[0] = v.at(0, N, cell, 0); vv[1] = v.at(1, N, cell, 0); vv[2] = v.at(2, N, cell, 0);
     vdotmsk = vv[0] * msk[0] + vv[1] * msk[1] + vv[2] * msk[2];
   }
 #pragma unroll
@@ -152,8 +169,8 @@ __device__ inline void field(const GeoD& g, const TermsD& T, const double* __res
     else an = msk[a] * (T.chk * mdotmsk + T.cha);
     h[a] = T.happ[a] + T.che * ex + an + T.chb * nsk[a] * (L + R);
     if (LIN) {
-      const double vL = wl * v(a * N + lc), vR = wr * v(a * N + rc);
-      const double vU = wu * v(a * N + uc), vD = wd * v(a * N + dc);
+      const double vL = wl * v.at(a, N, lc, 1), vR = wr * v.at(a, N, rc, 2);
+      const double vU = wu * v.at(a, N, uc, 3), vD = wd * v.at(a, N, dc, 4);
       double vex = vL + vR + vU + vD;
       if (g.exch_self) vex = vex - 4.0 * vv[a];
       double dan;
@@ -168,6 +185,22 @@ __device__ inline double gate_of(const GeoD& g, int gate, int cell) {
   if (gate == LLG_GATE_MASK) return g.active[cell] ? 1.0 : 0.0;
   if (gate == LLG_GATE_WINDOW) return g.w ? g.w[cell] : 1.0;
   return 1.0;
+}
+
+// Tile of a 2D stencil launch, XCD-aware: the hardware deals consecutive
+// workgroups round-robin over the 8 XCDs (each with its own L2), so in
+// launch order neighbouring tiles, which share halo rows, would sit in
+// different L2s and each fetch the halos from memory.  Renumbering gives
+// each XCD a contiguous band of tiles in row-major order; only the band
+// edges are fetched twice.
+__device__ inline void tile_of(int& tbx, int& tby) {
+  const int G = gridDim.x * gridDim.y;
+  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+  const int per = G / 8, rem = G % 8;
+  const int x = bid % 8, idx = bid / 8;
+  const int t = x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
+  tbx = t % gridDim.x;
+  tby = t / gridDim.x;
 }
 
 // thread -> cell mapping; returns false when the thread has no cell.
@@ -190,8 +223,10 @@ __device__ inline bool map_cell(const GeoD& g, double* out, int& cell, int& gx, 
     gy = cell / g.ng;
     return true;
   }
-  gx = blockIdx.x * blockDim.x + threadIdx.x;
-  gy = blockIdx.y * blockDim.y + threadIdx.y;
+  int tbx, tby;
+  tile_of(tbx, tby);
+  gx = tbx * blockDim.x + threadIdx.x;
+  gy = tby * blockDim.y + threadIdx.y;
   if (gx >= g.ng || gy >= g.ny) return false;
   cell = gy * g.ng + gx;
   return true;
@@ -230,7 +265,9 @@ __global__ __launch_bounds__(BX* BY) void k_rhs_tiled(GeoD g, TermsD T,
   constexpr int SX = BX + 2, SY = BY + 2;
   __shared__ double tile[3][SY][SX];
   const int N = g.ncell;
-  const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY;
+  int tbx, tby;
+  tile_of(tbx, tby);
+  const int x0 = tbx * BX, y0 = tby * BY;
   const bool win = T.window && g.w;
   for (int s = threadIdx.y * BX + threadIdx.x; s < SX * SY; s += BX * BY) {
     const int sx = s % SX, sy = s / SX;
@@ -284,7 +321,7 @@ __device__ inline void jtv_cell(const GeoD& g, const TermsD& T, const double* __
   double h[3], dh[3];
   field<true>(g, T, y, v, cell, gx, gy, h, dh);
   const double m1 = y[cell], m2 = y[N + cell], m3 = y[2 * N + cell];
-  const double v1 = v(cell), v2 = v(N + cell), v3 = v(2 * N + cell);
+  const double v1 = v.at(0, N, cell, 0), v2 = v.at(1, N, cell, 0), v3 = v.at(2, N, cell, 0);
   const double mh = m1 * h[0] + m2 * h[1] + m3 * h[2];
   const double dmh = (v1 * h[0] + v2 * h[1] + v3 * h[2]) + (m1 * dh[0] + m2 * dh[1] + m3 * dh[2]);
   const double j1 = g.chg * (v3 * h[1] + m3 * dh[1] - v2 * h[2] - m2 * dh[2]) +
@@ -458,6 +495,54 @@ __global__ __launch_bounds__(256) void k_krylov(GeoD g, TermsD T, int pgate,
   if (!map_cell(g, vk, cell, gx, gy)) return;
   double o[3], z[3];
   jtv_cell<true>(g, T, y, VScaled{vl, s, c}, cell, gx, gy, mgamma, o);
+  apply_P_cell(g, pgate, P, cell, o, z);
+  vk[cell] = s[cell] * z[0];
+  vk[N + cell] = s[N + cell] * z[1];
+  vk[2 * N + cell] = s[2 * N + cell] * z[2];
+}
+
+// k_krylov on the dense grid with the basis vector (c vl) / s formed once
+// per cell in an LDS tile with a one-cell halo (the halo cell of an edge is
+// the neighbour nbrs() names: wrapped when periodic, the cell itself when
+// Neumann), instead of once per neighbour read.  Same values, same rounding.
+template <int BX, int BY>
+__global__ __launch_bounds__(BX* BY) void k_krylov_tiled(GeoD g, TermsD T, int pgate,
+                                                        const double* __restrict__ P,
+                                                        const double* __restrict__ y,
+                                                        const double* __restrict__ vl,
+                                                        const double* __restrict__ s, double c,
+                                                        double mgamma, double* __restrict__ vk) {
+  constexpr int SX = BX + 2, SY = BY + 2;
+  __shared__ double tile[3][SY][SX];
+  const int N = g.ncell;
+  int tbx, tby;
+  tile_of(tbx, tby);
+  const int x0 = tbx * BX, y0 = tby * BY;
+  for (int i = threadIdx.y * BX + threadIdx.x; i < SX * SY; i += BX * BY) {
+    const int sy = i / SX, sx = i - sy * SX;
+    if ((sx == 0 || sx == SX - 1) && (sy == 0 || sy == SY - 1)) continue;   // corners unused
+    int gx = x0 + sx - 1, gy = y0 + sy - 1;
+    if (g.boundary == LLG_BC_NEUMANN) {
+      gx = gx < 0 ? 0 : (gx > g.ng - 1 ? g.ng - 1 : gx);
+      gy = gy < 0 ? 0 : (gy > g.ny - 1 ? g.ny - 1 : gy);
+    } else {
+      gx = gx < 0 ? gx + g.ng : (gx >= g.ng ? gx - g.ng : gx);
+      gy = gy < 0 ? gy + g.ny : (gy >= g.ny ? gy - g.ny : gy);
+    }
+    const int cidx = gy * g.ng + gx;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const double t = c * vl[a * N + cidx];
+      tile[a][sy][sx] = t / s[a * N + cidx];
+    }
+  }
+  __syncthreads();
+  const int gx = x0 + threadIdx.x, gy = y0 + threadIdx.y;
+  if (gx >= g.ng || gy >= g.ny) return;
+  const int cell = gy * g.ng + gx;
+  double o[3], z[3];
+  jtv_cell<true>(g, T, y, VTile<SY, SX>{tile, (int)threadIdx.x + 1, (int)threadIdx.y + 1}, cell,
+                 gx, gy, mgamma, o);
   apply_P_cell(g, pgate, P, cell, o, z);
   vk[cell] = s[cell] * z[0];
   vk[N + cell] = s[N + cell] * z[1];
@@ -914,12 +999,20 @@ int LLG_Krylov(sunrealtype c, N_Vector vl, N_Vector s, N_Vector vk, sunrealtype 
   ctx->njv++;
   ctx->npsol++;
   hipStream_t st = stream_of(vk, ctx);
-  dim3 grid, block;
-  grid_for(ctx, grid, block);
   kprof::begin(kprof::JTV, st);
-  hipLaunchKernelGGL(k_krylov, grid, block, 0, st, geo_of(ctx), to_dev(ctx->cfg.jv),
-                     ctx->cfg.pc.gate, (const double*)ctx->pd->d_P,
-                     (const double*)N_VGetDeviceArrayPointer(y), dvl, ds, c, -gamma, dvk);
+  if (ctx->cfg.exec == LLG_EXEC_YMSK) {
+    constexpr int BX = 32, BY = 8;
+    const dim3 grid((ctx->cfg.ng + BX - 1) / BX, (ctx->cfg.ny + BY - 1) / BY);
+    hipLaunchKernelGGL((k_krylov_tiled<BX, BY>), grid, dim3(BX, BY), 0, st, geo_of(ctx),
+                       to_dev(ctx->cfg.jv), ctx->cfg.pc.gate, (const double*)ctx->pd->d_P,
+                       (const double*)N_VGetDeviceArrayPointer(y), dvl, ds, c, -gamma, dvk);
+  } else {
+    dim3 grid, block;
+    grid_for(ctx, grid, block);
+    hipLaunchKernelGGL(k_krylov, grid, block, 0, st, geo_of(ctx), to_dev(ctx->cfg.jv),
+                       ctx->cfg.pc.gate, (const double*)ctx->pd->d_P,
+                       (const double*)N_VGetDeviceArrayPointer(y), dvl, ds, c, -gamma, dvk);
+  }
   kprof::end(kprof::JTV, st);
   return launch_status("krylov");
 }

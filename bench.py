@@ -202,6 +202,9 @@ def main():
                     help="train the learned block-Jacobi correction (on states of a "
                          "trajectory from another seed, untimed) and use it in the timed run")
     ap.add_argument("--lpc-steps", type=int, default=400, help="training steps")
+    ap.add_argument("--spgmr-device", action="store_true",
+                    help="queue each SPGMR solve whole on the device (one host wait per "
+                         "solve; same bits as the default host-driven loop)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the launch-shape autotuner at setup (untimed)")
     args = ap.parse_args()
@@ -247,6 +250,8 @@ def main():
     step = 1.0
     ropts = dict(rtol=1e-4, atol=1e-4, maxl=5, gstype=args.gstype, max_order=5, t_out=0.1,
                  mxstep=100000)
+    if args.spgmr_device:
+        ropts["fused"] = 15
 
     lpc_info = None
     if args.learned_pc:
@@ -339,7 +344,8 @@ def main():
                    "gs": {0: "auto", 1: "modified", 2: "classical"}[args.gstype],
                    "sync_mode": args.sync_mode,
                    "preconditioner": "block-Jacobi + learned correction" if args.learned_pc
-                   else "block-Jacobi", "learned_pc": lpc_info, "autotune": tune},
+                   else "block-Jacobi", "learned_pc": lpc_info, "autotune": tune,
+                   "spgmr": "device-resident" if args.spgmr_device else "host-driven"},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -526,6 +526,30 @@ int spgmr_solve(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s)
     if (rv != 0) return rv < 0 ? LS_PSOLVE_FAIL_UNREC : LS_PSOLVE_FAIL_REC;
     t0 = V[0];
   }
+  // the same solve queued whole on the device (one host wait); it hands
+  // back the rare re-orthogonalising solve, which runs below from b again
+  if (pre_left && !pre_right && m->gstype == SUN_CLASSICAL_GS && m->jtimes && m->fused.spgmr &&
+      l_max + 1 <= kFusedMax) {
+    int st = 0, iters = 0;
+    double r0 = 0.0, res = 0.0;
+    if (m->fused.spgmr(V[0], V, l_max, s, x, delta, m->gamma, m->tn, m->ycur, m->fcur,
+                       m->user_data, &st, &iters, &r0, &res) == 0 &&
+        st != 5) {
+      m->ls_iters = iters;
+      m->njtimes += iters;
+      m->nps += iters;
+      m->ls_resnorm = res;
+      switch (st) {
+        case 0: return LS_SUCCESS;
+        case 1: return LS_RES_REDUCED;
+        case 2: return LS_CONV_FAIL;
+        case 3: return LS_QRFACT_FAIL;
+        default: return LS_QRSOL_FAIL;
+      }
+    }
+    rv = ls_psolve(m, b, V[0], delta, SUN_PREC_LEFT);
+    if (rv != 0) return rv < 0 ? LS_PSOLVE_FAIL_UNREC : LS_PSOLVE_FAIL_REC;
+  }
   if (!(m->fused.proddotmulti && m->fused.proddotmulti(s, t0, V[0], 1, &V[0], &rr) == 0)) {
     N_VProd(s, t0, V[0]);
     rr = N_VDotProd(V[0], V[0]);

@@ -92,6 +92,7 @@ void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_hos
     if ((opts->fused & 4) && opts->use_jtv && opts->use_prec && opts->pretype == SUN_PREC_LEFT) {
       fo.krylov = LLG_Krylov;
       fo.gsscaled = NVHip_GSProjectScaled;
+      if (opts->fused & 8) fo.spgmr = LLG_SpgmrSolve;
     }
     LLGCVodeSetFusedOps(r->cv, &fo);
   }

@@ -480,7 +480,10 @@ __global__ __launch_bounds__(256) void k_krylov(GeoD g, TermsD T, int pgate,
                                                 const double* __restrict__ y,
                                                 const double* __restrict__ vl,
                                                 const double* __restrict__ s, double c,
-                                                double mgamma, double* __restrict__ vk) {
+                                                double mgamma, double* __restrict__ vk,
+                                                const double* cdev, const int* skip) {
+  if (skip && *skip) return;   // a device-resident solve that has finished
+  if (cdev) c = *cdev;         // the basis scale from device memory
   int cell, gx, gy;
   const int N = g.ncell;
   if (g.compact) {
@@ -511,7 +514,10 @@ __global__ __launch_bounds__(BX* BY) void k_krylov_tiled(GeoD g, TermsD T, int p
                                                         const double* __restrict__ y,
                                                         const double* __restrict__ vl,
                                                         const double* __restrict__ s, double c,
-                                                        double mgamma, double* __restrict__ vk) {
+                                                        double mgamma, double* __restrict__ vk,
+                                                        const double* cdev, const int* skip) {
+  if (skip && *skip) return;   // a device-resident solve that has finished
+  if (cdev) c = *cdev;         // the basis scale from device memory
   constexpr int SX = BX + 2, SY = BY + 2;
   __shared__ double tile[3][SY][SX];
   const int N = g.ncell;
@@ -986,18 +992,19 @@ int PrecondSolve(sunrealtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z,
   return launch_status("PrecondSolve");
 }
 
-int LLG_Krylov(sunrealtype c, N_Vector vl, N_Vector s, N_Vector vk, sunrealtype gamma,
-               sunrealtype t, N_Vector y, N_Vector fy, void* user_data) {
-  (void)t; (void)fy;
-  llg_ctx* ctx = (llg_ctx*)user_data;
+}  // extern "C"
+
+namespace {
+// queue the operator chain vk = s .* gate P ((v + (-gamma) J v), v = (c vl) / s,
+// with c by value or (cdev) from device memory; skip: device exit flag
+int launch_krylov(llg_ctx* ctx, sunrealtype c, const double* cdev, const int* skip, N_Vector vl,
+                  N_Vector s, N_Vector vk, sunrealtype gamma, N_Vector y) {
   // the in-kernel preconditioner is the plain block-Jacobi one
   if (!ctx || ctx->lpc || !ctx->pd || !vl || !s || !vk || !y) return -1;
   const double* dvl = N_VGetDeviceArrayPointer(vl);
   const double* ds = N_VGetDeviceArrayPointer(s);
   double* dvk = N_VGetDeviceArrayPointer(vk);
   if (dvk == dvl || dvk == ds || dvk == N_VGetDeviceArrayPointer(y)) return -1;
-  ctx->njv++;
-  ctx->npsol++;
   hipStream_t st = stream_of(vk, ctx);
   kprof::begin(kprof::JTV, st);
   if (ctx->cfg.exec == LLG_EXEC_YMSK) {
@@ -1005,16 +1012,60 @@ int LLG_Krylov(sunrealtype c, N_Vector vl, N_Vector s, N_Vector vk, sunrealtype 
     const dim3 grid((ctx->cfg.ng + BX - 1) / BX, (ctx->cfg.ny + BY - 1) / BY);
     hipLaunchKernelGGL((k_krylov_tiled<BX, BY>), grid, dim3(BX, BY), 0, st, geo_of(ctx),
                        to_dev(ctx->cfg.jv), ctx->cfg.pc.gate, (const double*)ctx->pd->d_P,
-                       (const double*)N_VGetDeviceArrayPointer(y), dvl, ds, c, -gamma, dvk);
+                       (const double*)N_VGetDeviceArrayPointer(y), dvl, ds, c, -gamma, dvk,
+                       cdev, skip);
   } else {
     dim3 grid, block;
     grid_for(ctx, grid, block);
     hipLaunchKernelGGL(k_krylov, grid, block, 0, st, geo_of(ctx), to_dev(ctx->cfg.jv),
                        ctx->cfg.pc.gate, (const double*)ctx->pd->d_P,
-                       (const double*)N_VGetDeviceArrayPointer(y), dvl, ds, c, -gamma, dvk);
+                       (const double*)N_VGetDeviceArrayPointer(y), dvl, ds, c, -gamma, dvk,
+                       cdev, skip);
   }
   kprof::end(kprof::JTV, st);
   return launch_status("krylov");
+}
+
+struct DevOpData {
+  llg_ctx* ctx;
+  double gamma;
+  N_Vector y;
+};
+int dev_op(const double* cdev, const int* skip, N_Vector vl, N_Vector s, N_Vector vk, void* d) {
+  const DevOpData* o = static_cast<const DevOpData*>(d);
+  return launch_krylov(o->ctx, 0.0, cdev, skip, vl, s, vk, o->gamma, o->y);
+}
+}  // namespace
+
+extern "C" {
+
+int LLG_Krylov(sunrealtype c, N_Vector vl, N_Vector s, N_Vector vk, sunrealtype gamma,
+               sunrealtype t, N_Vector y, N_Vector fy, void* user_data) {
+  (void)t; (void)fy;
+  llg_ctx* ctx = (llg_ctx*)user_data;
+  const int rc = launch_krylov(ctx, c, nullptr, nullptr, vl, s, vk, gamma, y);
+  if (rc == 0) {
+    ctx->njv++;
+    ctx->npsol++;
+  }
+  return rc;
+}
+
+int LLG_SpgmrSolve(N_Vector t0, N_Vector* V, int maxl, N_Vector s, N_Vector x,
+                   sunrealtype delta, sunrealtype gamma, sunrealtype t, N_Vector y, N_Vector fy,
+                   void* user_data, int* status, int* iters, sunrealtype* r_norm,
+                   sunrealtype* resnorm) {
+  (void)t; (void)fy;
+  llg_ctx* ctx = (llg_ctx*)user_data;
+  if (!ctx || ctx->lpc || !ctx->pd || !y) return -1;
+  DevOpData d{ctx, gamma, y};
+  const int rc = NVHip_SpgmrSolve(dev_op, &d, t0, V, maxl, s, x, delta, status, iters, r_norm,
+                                  resnorm);
+  if (rc == 0 && *status != NVHIP_SPGMR_FALLBACK) {
+    ctx->njv += *iters;
+    ctx->npsol += *iters;
+  }
+  return rc;
 }
 
 int LLG_SetStencilBlock(llg_ctx* ctx, int bx, int by) {

@@ -72,6 +72,7 @@ struct RedWs {
   double* host_dev = nullptr;   // device alias of host
   double* dres = nullptr;       // device memory, kMaxK doubles
   unsigned seq = 0;             // last sequence number issued on this stream
+  void* sp = nullptr;           // device state of a device-resident SPGMR solve
 };
 
 std::mutex g_mu;
@@ -403,7 +404,9 @@ __device__ void finish(int kind, int nblk, const double* partials, double* out, 
 template <class F>
 __global__ __launch_bounds__(kBlock) void k_reduce1(int kind, F f, sunindextype n,
                                                     double* partials, unsigned* ticket,
-                                                    double* out, double* dout, unsigned seq) {
+                                                    double* out, double* dout, unsigned seq,
+                                                    const int* skip) {
+  if (skip && *skip) return;   // a device-resident solve that has finished
   __shared__ double sm[1][kWaves];
   __shared__ bool last;
   // four independent accumulators so four loads per thread are in flight
@@ -450,7 +453,10 @@ __global__ __launch_bounds__(kBlock) void k_dotmulti(const double* x, PtrPack Y,
                                                      double* out, double* dout, unsigned seq,
                                                      const double* sv, const double* tv,
                                                      double* xw, unsigned self, int sidx,
-                                                     double sc) {
+                                                     double sc, const double* scd,
+                                                     const int* skip) {
+  if (skip && *skip) return;   // a device-resident solve that has finished
+  if (scd) sc = *scd;          // the scale of Y_sidx from device memory
   __shared__ double sm[K][kWaves];
   __shared__ bool last;
   double v[K], u[K];
@@ -591,16 +597,20 @@ __global__ __launch_bounds__(kBlock) void k_wsqarray(PtrPack X, PtrPack W, const
 // partials (slot order) and publishes the K results.
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_finish(int kind, int nblk, const double* partials,
-                                                   double* out, double* dout, unsigned seq) {
+                                                   double* out, double* dout, unsigned seq,
+                                                   const int* skip) {
+  if (skip && *skip) return;
   __shared__ double sm[K][kWaves];
   finish<K>(kind, nblk, partials, out, dout, sm, seq);
 }
 
+// dout: the device copy of the results (slot 1: the workspace's dres)
 template <int K>
 void launch_finish(hipStream_t s, int kind, int nblk, const RedWs* w, unsigned seq,
-                   int slot = 0) {
+                   int slot = 0, const int* skip = nullptr, double* dout = nullptr) {
   hipLaunchKernelGGL(k_finish<K>, dim3(1), dim3(kBlock), 0, s, kind, nblk, w->partials,
-                     w->host_dev + slot * kSlot, slot ? w->dres : nullptr, seq);
+                     w->host_dev + slot * kSlot, dout ? dout : (slot ? w->dres : nullptr), seq,
+                     skip);
 }
 void finish_dispatch(int K, hipStream_t s, int kind, int nblk, const RedWs* w, unsigned seq) {
   switch (K) {
@@ -650,7 +660,8 @@ double reduce1(N_Vector v, int kind, F f) {
   const int g = red_grid(n);
   const unsigned seq = ++w->seq;
   hipLaunchKernelGGL(k_reduce1<F>, dim3(g), dim3(kBlock), 0, s, kind, f, n, w->partials,
-                     g_red_mode == 0 ? w->ticket : nullptr, w->host_dev, (double*)nullptr, seq);
+                     g_red_mode == 0 ? w->ticket : nullptr, w->host_dev, (double*)nullptr, seq,
+                     (const int*)nullptr);
   if (g_red_mode != 0) finish_dispatch(1, s, kind, g, w, seq);
   check_launch("reduce");
   wait_result(s, w, seq);
@@ -665,6 +676,8 @@ struct ProdIn {
   int slot = 0;                // result slot (1 also fills the device copy)
   int sidx = -1;               // Y_sidx scaled by sc on read and written back
   double sc = 1.0;
+  const double* scd = nullptr; // when set, sc is read from device memory
+  const int* skip = nullptr;   // when set and nonzero on the device, no work
 };
 
 template <int K>
@@ -674,8 +687,8 @@ void launch_dotmulti(hipStream_t s, const double* x, const PtrPack& Y, sunindext
   hipLaunchKernelGGL(k_dotmulti<K>, dim3(g), dim3(kBlock), 0, s, x, Y, n,
                      w->partials, g_red_mode == 0 ? w->ticket : nullptr,
                      w->host_dev + pi.slot * kSlot, pi.slot ? w->dres : nullptr, w->seq,
-                     pi.s, pi.t, pi.xw, pi.self, pi.sidx, pi.sc);
-  if (g_red_mode != 0) launch_finish<K>(s, RED_SUM, g, w, w->seq, pi.slot);
+                     pi.s, pi.t, pi.xw, pi.self, pi.sidx, pi.sc, pi.scd, pi.skip);
+  if (g_red_mode != 0) launch_finish<K>(s, RED_SUM, g, w, w->seq, pi.slot, pi.skip);
 }
 
 void dotmulti_dispatch(int K, hipStream_t s, const double* x, const PtrPack& Y, sunindextype n,
@@ -1221,7 +1234,7 @@ int gs_project(N_Vector sv, N_Vector t, N_Vector z, int nvec, N_Vector* Y, int s
   const unsigned seq = ++w->seq;
   hipLaunchKernelGGL(k_reduce1<FGsLincombSq>, dim3(g), dim3(kBlock), 0, s, RED_SUM, f, n,
                      w->partials, g_red_mode == 0 ? w->ticket : nullptr, w->host_dev,
-                     (double*)nullptr, seq);
+                     (double*)nullptr, seq, (const int*)nullptr);
   if (g_red_mode != 0) finish_dispatch(1, s, RED_SUM, g, w, seq);
   check_launch("gsproject");
   wait_result(s, w, seq);
@@ -1241,6 +1254,262 @@ int NVHip_GSProject(N_Vector sv, N_Vector t, N_Vector z, int nvec, N_Vector* Y,
       LEN(sv) != LEN(z) || LEN(t) != LEN(z))
     return -1;
   return gs_project(sv, t, z, nvec, Y, -1, 1.0, dots, zz);
+}
+
+}  // extern "C"
+
+namespace {
+
+// ------------------------------------------------- device-resident SPGMR
+// The left-preconditioned SPGMR solve of SUNLinSolSolve_SPGMR (zero initial
+// guess, classical Gram-Schmidt, no restarts) with every scalar kept on the
+// device: the Hessenberg column, its Givens rotation, the residual estimate
+// and the convergence decision are made by one-thread kernels between the
+// vector passes, and every kernel of an iteration after convergence exits
+// at once (a device flag).  The host waits once, at the end.  Each scalar
+// is computed with the same operations in the same order as the host
+// integrator computes it (no contraction in this file; FP64 division and
+// square root are correctly rounded on the device as on the host), so the
+// solve is bit-identical to the host-driven one.
+constexpr double kSpGsFactor = 1000.0;   // classical GS re-orthogonalisation test
+enum SpStatus { SP_RUNNING = 0, SP_ZERO = 1, SP_CONVERGED = 2, SP_REORTH = 3, SP_QRFACT = 4,
+                SP_QRSOL = 5, SP_REDUCED = 6, SP_FAILED = 7 };
+struct SpDev {
+  double H[kMaxK][kMaxK];   // Hessenberg matrix, rows 0..maxl, columns 0..maxl-1
+  double q[2 * kMaxK];      // Givens rotations (c, s)
+  double yg[kMaxK + 1];     // least-squares right-hand side / solution
+  double zz, r_norm, beta, rho, rotprod, vscale, delta;
+  int done, status, krydim, write;
+};
+
+__global__ void k_sp_init(SpDev* st, const double* rr, double delta) {
+  if (threadIdx.x != 0) return;
+  const double r_norm = sqrt(rr[0]);
+  st->r_norm = r_norm;
+  st->beta = r_norm;
+  st->rho = r_norm;
+  st->delta = delta;
+  st->rotprod = 1.0;
+  st->krydim = 0;
+  st->write = 0;
+  st->zz = 0.0;
+  for (int i = 0; i < kMaxK; i++)
+    for (int j = 0; j < kMaxK; j++) st->H[i][j] = 0.0;
+  if (r_norm <= delta) {
+    st->done = 1;
+    st->status = SP_ZERO;
+  } else {
+    st->done = 0;
+    st->status = SP_RUNNING;
+    st->vscale = 1.0 / r_norm;
+  }
+}
+
+__device__ inline void sp_rotation(double t1, double t2, double& c, double& s) {
+  if (t2 == 0.0) {
+    c = 1.0;
+    s = 0.0;
+  } else if (fabs(t2) >= fabs(t1)) {
+    const double t3 = t1 / t2;
+    s = -1.0 / sqrt(1.0 + t3 * t3);
+    c = -s * t3;
+  } else {
+    const double t3 = t2 / t1;
+    c = 1.0 / sqrt(1.0 + t3 * t3);
+    s = -c * t3;
+  }
+}
+
+// iteration l: dots[0..l+1] = <V[l+1], V[0..l+1]> before the projection,
+// st->zz = |V[l+1]|^2 after it
+__global__ void k_sp_step(SpDev* st, int l, const double* __restrict__ dots) {
+  if (threadIdx.x != 0 || st->done) return;
+  const int k = l + 1;
+  const double vk_norm = sqrt(dots[k]);
+  for (int i = k - 1; i >= 0; i--) st->H[i][k - 1] = dots[i];
+  const double new_norm = sqrt(st->zz);
+  if ((kSpGsFactor * new_norm) < vk_norm) {   // the host path re-orthogonalises
+    st->done = 1;
+    st->status = SP_REORTH;
+    return;
+  }
+  st->H[l + 1][l] = new_norm;
+  st->krydim = k;
+  // new Hessenberg column: previous rotations, then its own
+  for (int j = 0; j < l; j++) {
+    const double t1 = st->H[j][l], t2 = st->H[j + 1][l], c = st->q[2 * j], s = st->q[2 * j + 1];
+    st->H[j][l] = c * t1 - s * t2;
+    st->H[j + 1][l] = s * t1 + c * t2;
+  }
+  double c, s;
+  const double t1 = st->H[l][l], t2 = st->H[l + 1][l];
+  sp_rotation(t1, t2, c, s);
+  st->q[2 * l] = c;
+  st->q[2 * l + 1] = s;
+  if ((st->H[l][l] = c * t1 - s * t2) == 0.0) {
+    st->done = 1;
+    st->status = SP_QRFACT;
+    return;
+  }
+  st->rotprod *= st->q[2 * l + 1];
+  st->rho = fabs(st->rotprod * st->r_norm);
+  if (st->rho <= st->delta) {
+    st->done = 1;
+    st->status = SP_CONVERGED;
+    return;
+  }
+  st->vscale = 1.0 / st->H[l + 1][l];
+}
+
+// least-squares solution and the decision to write x (write 1: the basis
+// combination, 2: zero, 0: untouched)
+__global__ void k_sp_final(SpDev* st, int maxl) {
+  if (threadIdx.x != 0) return;
+  const int status = st->status;
+  if (status == SP_ZERO) { st->write = 2; return; }
+  if (status == SP_REORTH || status == SP_QRFACT) { st->write = 0; return; }
+  const bool conv = status == SP_CONVERGED;
+  const int n = conv ? st->krydim : maxl;
+  if (!conv) st->krydim = maxl;
+  double* b = st->yg;
+  b[0] = st->r_norm;
+  for (int i = 1; i <= n; i++) b[i] = 0.0;
+  for (int k = 0; k < n; k++) {
+    const double c = st->q[2 * k], s = st->q[2 * k + 1], t1 = b[k], t2 = b[k + 1];
+    b[k] = c * t1 - s * t2;
+    b[k + 1] = s * t1 + c * t2;
+  }
+  for (int k = n - 1; k >= 0; k--) {
+    if (st->H[k][k] == 0.0) { st->status = SP_QRSOL; st->write = 0; return; }
+    b[k] /= st->H[k][k];
+    for (int i = 0; i < k; i++) b[i] -= b[k] * st->H[i][k];
+  }
+  if (conv || st->rho < st->beta) {
+    st->write = 1;
+    if (!conv) st->status = SP_REDUCED;
+  } else {
+    st->write = 0;
+    st->status = SP_FAILED;
+  }
+}
+
+// x = (0 + sum_k yg_k V_k) / s as the host's N_VLinearCombination into a
+// zeroed vector and N_VDiv (k_lincombdiv) round it; or x = 0
+__global__ void k_sp_update(const SpDev* __restrict__ st, PtrPack V,
+                            const double* __restrict__ sv, double* x, sunindextype n) {
+  const int w = st->write;
+  if (w == 0) return;
+  if (w == 2) {
+    GRID_STRIDE(i, n) x[i] = 0.0;
+    return;
+  }
+  const int nv = st->krydim;
+  GRID_STRIDE(i, n) {
+    double acc = 1.0 * 0.0;
+    for (int j = 0; j < nv; j++) acc = fma(st->yg[j], V.p[j][i], acc);
+    x[i] = acc / sv[i];
+  }
+}
+
+__global__ void k_sp_publish(const SpDev* st, double* out, unsigned seq) {
+  if (threadIdx.x != 0) return;
+  out[0] = (double)st->status;
+  out[1] = (double)st->krydim;
+  out[2] = st->rho;
+  out[3] = st->r_norm;
+  __threadfence_system();
+  __hip_atomic_store(reinterpret_cast<unsigned*>(out + kMaxK), seq, __ATOMIC_RELEASE,
+                     __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+}  // namespace
+
+extern "C" {
+
+int NVHip_SpgmrSolve(NVHipDevOp op, void* op_data, N_Vector t0, N_Vector* V, int maxl,
+                     N_Vector sv, N_Vector x, sunrealtype delta, int* status, int* iters,
+                     sunrealtype* r_norm, sunrealtype* resnorm) {
+  if (!op || !t0 || !V || !sv || !x || !status || !iters || maxl < 1 || maxl > kMaxK - 1 ||
+      g_red_mode == 0 || V[0] != t0)
+    return -1;
+  const sunindextype n = LEN(x);
+  for (int i = 0; i <= maxl; i++)
+    if (!V[i] || LEN(V[i]) != n) return -1;
+  if (LEN(sv) != n) return -1;
+  hipStream_t s = S(x);
+  RedWs* w = workspace(s);
+  if (!w->sp && hipMalloc(&w->sp, sizeof(SpDev)) != hipSuccess) {
+    w->sp = nullptr;
+    return -1;
+  }
+  SpDev* st = static_cast<SpDev*>(w->sp);
+  const int* skip = &st->done;
+  const int g = red_grid(n);
+  // V[0] = s .* t0 and its squared norm into dres[0] (the single-dot pass
+  // of NVHip_ProdDotMulti, so the same summation order)
+  {
+    const unsigned seq0 = ++w->seq;
+    const FProdDot f{D(sv), D(t0), D(V[0]), D(V[0]), true};
+    hipLaunchKernelGGL(k_reduce1<FProdDot>, dim3(g), dim3(kBlock), 0, s, RED_SUM, f, n,
+                       w->partials, (unsigned*)nullptr, w->host_dev, (double*)nullptr, seq0,
+                       (const int*)nullptr);
+    launch_finish<1>(s, RED_SUM, g, w, seq0, 0, nullptr, w->dres);
+  }
+  hipLaunchKernelGGL(k_sp_init, dim3(1), dim3(64), 0, s, st, (const double*)w->dres, delta);
+  for (int l = 0; l < maxl; l++) {
+    const int k = l + 1;
+    if (op(&st->vscale, skip, V[l], sv, V[k], op_data) != 0) return -1;
+    // dots of V[k] with V[0..k], V[l] normalised by vscale as it is read
+    PtrPack yp{};
+    for (int j = 0; j <= k; j++) yp.p[j] = D(V[j]);
+    ProdIn pi;
+    pi.self = 1u << k;
+    pi.slot = 1;
+    pi.sidx = l;
+    pi.scd = &st->vscale;
+    pi.skip = skip;
+    ++w->seq;
+    dotmulti_dispatch(k + 1, s, D(V[k]), yp, n, w, pi);
+    // V[k] -= sum_{j<k} dots_j V[j] and its squared norm into st->zz
+    FGsLincombSq f{};
+    f.nv = k + 1;
+    f.d = w->dres;
+    f.z = D(V[k]);
+    f.X.p[0] = D(V[k]);
+    for (int j = 0; j < k; j++) f.X.p[j + 1] = D(V[j]);
+    const unsigned seq = ++w->seq;
+    hipLaunchKernelGGL(k_reduce1<FGsLincombSq>, dim3(g), dim3(kBlock), 0, s, RED_SUM, f, n,
+                       w->partials, (unsigned*)nullptr, w->host_dev, (double*)nullptr, seq,
+                       skip);
+    launch_finish<1>(s, RED_SUM, g, w, seq, 0, skip, &st->zz);
+    hipLaunchKernelGGL(k_sp_step, dim3(1), dim3(64), 0, s, st, l, (const double*)w->dres);
+  }
+  hipLaunchKernelGGL(k_sp_final, dim3(1), dim3(64), 0, s, st, maxl);
+  {
+    PtrPack vp{};
+    for (int j = 0; j < maxl; j++) vp.p[j] = D(V[j]);
+    hipLaunchKernelGGL(k_sp_update, dim3(ew_grid(n)), dim3(kBlock), 0, s, (const SpDev*)st, vp,
+                       (const double*)D(sv), D(x), n);
+  }
+  const unsigned seq = ++w->seq;
+  hipLaunchKernelGGL(k_sp_publish, dim3(1), dim3(64), 0, s, (const SpDev*)st, w->host_dev, seq);
+  check_launch("spgmr");
+  wait_result(s, w, seq);
+  const int code = (int)w->host[0];
+  *iters = (int)w->host[1];
+  if (r_norm) *r_norm = w->host[3];
+  if (resnorm) *resnorm = code == SP_ZERO ? w->host[3] : w->host[2];
+  switch (code) {
+    case SP_ZERO:
+    case SP_CONVERGED: *status = NVHIP_SPGMR_SUCCESS; break;
+    case SP_REDUCED: *status = NVHIP_SPGMR_RES_REDUCED; break;
+    case SP_FAILED: *status = NVHIP_SPGMR_CONV_FAIL; break;
+    case SP_QRFACT: *status = NVHIP_SPGMR_QRFACT_FAIL; break;
+    case SP_QRSOL: *status = NVHIP_SPGMR_QRSOL_FAIL; break;
+    default: *status = NVHIP_SPGMR_FALLBACK; break;   // re-orthogonalisation needed
+  }
+  if (code == SP_ZERO) *iters = 0;
+  return 0;
 }
 
 int NVHip_GSProjectScaled(N_Vector z, int nvec, N_Vector* Y, int sidx, sunrealtype sc,

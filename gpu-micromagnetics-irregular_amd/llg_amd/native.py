@@ -163,6 +163,8 @@ SIGNATURES = {
     "NVHip_LinCombDot": (C.c_int, [C.c_int, DP, C.POINTER(VP), VP, DP]),
     "NVHip_GSProject": (C.c_int, [VP, VP, VP, C.c_int, C.POINTER(VP), DP, DP]),
     "NVHip_GSProjectScaled": (C.c_int, [VP, C.c_int, C.POINTER(VP), C.c_int, C.c_double, DP, DP]),
+    "NVHip_SpgmrSolve": (C.c_int, [VP, VP, VP, C.POINTER(VP), C.c_int, VP, VP, C.c_double, IP, IP,
+                                   DP, DP]),
     "NVHip_LinCombDiv": (C.c_int, [C.c_int, DP, C.POINTER(VP), VP, VP]),
     "NVHip_NewtonUpdate": (C.c_int, [C.c_int, VP, VP, VP, VP, VP, DP, DP]),
     "FusedNVec_Init": (None, [VP]),
@@ -210,6 +212,8 @@ SIGNATURES = {
     "JtvProduct": (C.c_int, [VP, VP, C.c_double, VP, VP, VP, VP]),
     "LLG_Atimes": (C.c_int, [VP, VP, C.c_double, C.c_double, VP, VP, VP]),
     "LLG_Krylov": (C.c_int, [C.c_double, VP, VP, VP, C.c_double, C.c_double, VP, VP, VP]),
+    "LLG_SpgmrSolve": (C.c_int, [VP, C.POINTER(VP), C.c_int, VP, VP, C.c_double, C.c_double,
+                                 C.c_double, VP, VP, VP, IP, IP, DP, DP]),
     "PrecondSetup": (C.c_int, [C.c_double, VP, VP, C.c_int, C.POINTER(C.c_int), C.c_double, VP]),
     "PrecondSolve": (C.c_int, [C.c_double, VP, VP, VP, VP, C.c_double, C.c_double, C.c_int, VP]),
     "Precond_Create": (VP, [C.c_int, C.c_int, C.c_int]),

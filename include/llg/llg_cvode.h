@@ -101,6 +101,12 @@ typedef struct llgcv_fused_ops {
                 sunrealtype t, N_Vector y, N_Vector fy, void* user_data);
   int (*gsscaled)(N_Vector z, int k, N_Vector* Y, int i, sunrealtype c, sunrealtype* d,
                   sunrealtype* zz);
+  /* the whole left-preconditioned classical-GS SPGMR solve after the first
+   * preconditioner solve (t0 = V[0] = P^-1 b), one host wait (status as
+   * NVHIP_SPGMR_*; 5 = run the host-driven solve instead) */
+  int (*spgmr)(N_Vector t0, N_Vector* V, int maxl, N_Vector s, N_Vector x, sunrealtype delta,
+               sunrealtype gamma, sunrealtype t, N_Vector y, N_Vector fy, void* user_data,
+               int* status, int* iters, sunrealtype* r_norm, sunrealtype* resnorm);
 } llgcv_fused_ops;
 
 void* LLGCVodeCreate(int lmm, SUNContext sunctx);

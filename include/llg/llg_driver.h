@@ -28,8 +28,14 @@ typedef struct llg_run_opts {
                          kernels (Nordsieck, error weights, SPGMR products,
                          Gram-Schmidt); bit 2 (with bit 1, left block-Jacobi
                          and analytic Jv): each SPGMR iteration's operator
-                         chain in one launch (LLG_Krylov); default 7.  Bits
-                         1 and 2 change no value. */
+                         chain in one launch (LLG_Krylov); bit 3 (with bits
+                         1 and 2, classical GS): the whole SPGMR solve
+                         queued on the device with one host wait
+                         (LLG_SpgmrSolve; it halves the host waits but
+                         queues every iteration up to the Krylov
+                         dimension, which costs more than the waits on
+                         the 1536x512 workload); default 7.  Bits 1 to 3
+                         change no value. */
   long   mxstep;      /* max steps per CVode call (CVODE default 500) */
   int    use_jtv;     /* analytic Jv (else difference quotient) */
   int    use_prec;    /* block-Jacobi preconditioner */

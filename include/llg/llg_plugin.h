@@ -161,6 +161,17 @@ int LLG_Atimes(N_Vector v, N_Vector z, sunrealtype gamma, sunrealtype t, N_Vecto
  * counterpart: installed as llgcv_fused_ops.krylov by the driver. */
 int LLG_Krylov(sunrealtype c, N_Vector vl, N_Vector s, N_Vector vk, sunrealtype gamma,
                sunrealtype t, N_Vector y, N_Vector fy, void* user_data);
+/* A whole left-preconditioned SPGMR solve (classical GS) queued on the
+ * device with one host wait (NVHip_SpgmrSolve with the LLG_Krylov chain as
+ * its operator): t0 = V[0] holds P^{-1} b on entry, x receives the
+ * solution, *status one of NVHIP_SPGMR_*.  Same bits as the host-driven
+ * solve.  Returns -1 without queuing anything when a learned correction is
+ * active or no block-Jacobi data exists.  No reference counterpart:
+ * installed as llgcv_fused_ops.spgmr by the driver. */
+int LLG_SpgmrSolve(N_Vector t0, N_Vector* V, int maxl, N_Vector s, N_Vector x,
+                   sunrealtype delta, sunrealtype gamma, sunrealtype t, N_Vector y, N_Vector fy,
+                   void* user_data, int* status, int* iters, sunrealtype* r_norm,
+                   sunrealtype* resnorm);
 int PrecondSetup(sunrealtype t, N_Vector y, N_Vector fy, sunbooleantype jok,
                  sunbooleantype* jcurPtr, sunrealtype gamma, void* user_data);
 int PrecondSolve(sunrealtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z,

@@ -151,6 +151,39 @@ int NVHip_LinCombDiv(int nvec, sunrealtype* c, N_Vector* X, N_Vector s, N_Vector
 int NVHip_NewtonUpdate(int iter, N_Vector ycor, N_Vector del, N_Vector zn0, N_Vector w,
                        N_Vector y, sunrealtype* dnorm, sunrealtype* ynorm);
 
+/* Device-resident SPGMR solve: SUNLinSolSolve_SPGMR with left
+ * preconditioning, zero initial guess, classical Gram-Schmidt and no
+ * restarts, queued whole on the stream of x with ONE host wait.  The
+ * Hessenberg entries, Givens rotations, residual estimate and convergence
+ * test live on the device; kernels of iterations after convergence exit at
+ * once.  Same operations in the same order as the host-driven solve, so
+ * the same bits.  No counterpart in the reference (its SUNDIALS solve waits
+ * for every Gram-Schmidt reduction).
+ *   op(c_dev, skip_dev, vl, s, vk, op_data) queues vk = s .* P^-1 (I -
+ *     gamma J) ((*c_dev) vl / s) (the operator chain with the basis scale
+ *     read from device memory); it must queue nothing that runs when
+ *     *skip_dev is nonzero and return 0, or nonzero when it cannot run.
+ *   t0 = V[0] holds P^-1 b on entry; V[0..maxl] are workspace
+ *     (maxl <= NVHIP_FUSED_MAX - 1); sv the scaling (weight) vector; x the
+ *     solution (may alias b, which is not read).
+ *   *status: one of NVHIP_SPGMR_*; FALLBACK means the classical GS step
+ *     needed a re-orthogonalisation pass: x is untouched and the caller
+ *     runs its host-driven solve.  *iters Krylov iterations, *r_norm the
+ *     initial residual norm, *resnorm the final residual estimate.
+ * Returns 0, or -1 (nothing queued that changes x) when the arguments or
+ * the reduction mode do not allow the device-resident form. */
+#define NVHIP_SPGMR_SUCCESS 0
+#define NVHIP_SPGMR_RES_REDUCED 1
+#define NVHIP_SPGMR_CONV_FAIL 2
+#define NVHIP_SPGMR_QRFACT_FAIL 3
+#define NVHIP_SPGMR_QRSOL_FAIL 4
+#define NVHIP_SPGMR_FALLBACK 5
+typedef int (*NVHipDevOp)(const double* c_dev, const int* skip_dev, N_Vector vl, N_Vector s,
+                          N_Vector vk, void* op_data);
+int NVHip_SpgmrSolve(NVHipDevOp op, void* op_data, N_Vector t0, N_Vector* V, int maxl,
+                     N_Vector sv, N_Vector x, sunrealtype delta, int* status, int* iters,
+                     sunrealtype* r_norm, sunrealtype* resnorm);
+
 /* Fused-ops layer with the reference's entry-point names.
  * FusedNVec_Init installs the one-launch multi-dot and linear-combination
  * overrides on v and makes every clone of v inherit them;

@@ -7,8 +7,10 @@ max |dm| of the oracle's; |m| drift is printed.  The integrator's one-pass
 kernels (Nordsieck predict/restore, error weights, SPGMR scale/divide,
 product + Gram-Schmidt multi-dot, combination + norm) must reproduce the
 standard N_Vector op sequence bit for bit, as must the one-launch Krylov
-operator chain (scale/divide, J v, block-Jacobi solve and product), so
-the fused and unfused runs are compared for equality.
+operator chain (scale/divide, J v, block-Jacobi solve and product) and the
+device-resident SPGMR solve (scalars, Givens rotations and the convergence
+test on the device, one host wait per solve), so the fused and unfused runs
+are compared for equality.
 """
 import numpy as np
 import pytest
@@ -35,17 +37,19 @@ def _compact_case():
                                          (2, "i6_compact")])
 def test_fused_integrator_kernels_are_bitwise_neutral(gstype, case):
     """fused = 1: standard op sequences; 3: the one-pass integrator kernels;
-    7: also the one-launch Krylov operator chain.  All three must agree."""
+    7: also the one-launch Krylov operator chain; 15: also the whole SPGMR
+    solve queued on the device with one host wait (classical GS only).
+    All must agree bit for bit, counters included."""
     cfg, geom, y0 = _case() if case == "square_hole" else _compact_case()
     sim = la.Simulation(cfg, geom)
     res = {}
-    for fused in (1, 3, 7):
+    for fused in (1, 3, 7, 15):
         run = sim.integrator(y0, rtol=1e-5, atol=1e-5, gstype=gstype, fused=fused)
         out = run.advance(1.0)
         res[fused] = (run.state(), out)
         run.close()
     ya, sa = res[1]
-    for fused in (3, 7):
+    for fused in (3, 7, 15):
         yb, sb = res[fused]
         print(f"[gpu] {case} gs={gstype} fused 1: nst={sa['nst']} nli={sa['nli']} "
               f"syncs={sa['nsyncs']}; fused {fused}: nst={sb['nst']} nli={sb['nli']} "

@@ -202,9 +202,15 @@ def main():
                     help="train the learned block-Jacobi correction (on states of a "
                          "trajectory from another seed, untimed) and use it in the timed run")
     ap.add_argument("--lpc-steps", type=int, default=400, help="training steps")
-    ap.add_argument("--spgmr-device", action="store_true",
-                    help="queue each SPGMR solve whole on the device (one host wait per "
-                         "solve; same bits as the default host-driven loop)")
+    ap.add_argument("--spgmr", default="lagged",
+                    choices=["pipelined", "lagged", "host", "device"],
+                    help="lagged: SPGMR with lagged normalisation, one pass (projection, "
+                         "operator chain and every Gram-Schmidt sum) and one host wait per "
+                         "iteration (default); pipelined: the same with each pass queued "
+                         "behind a gate before the previous pass's sums are back; host: the "
+                         "standard loop with one-pass kernels, bit-equal to the plain op "
+                         "sequence; device: the standard loop queued whole on the device, "
+                         "one host wait per solve")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the launch-shape autotuner at setup (untimed)")
     args = ap.parse_args()
@@ -250,8 +256,7 @@ def main():
     step = 1.0
     ropts = dict(rtol=1e-4, atol=1e-4, maxl=5, gstype=args.gstype, max_order=5, t_out=0.1,
                  mxstep=100000)
-    if args.spgmr_device:
-        ropts["fused"] = 15
+    ropts["fused"] = {"pipelined": 55, "lagged": 23, "host": 7, "device": 15}[args.spgmr]
 
     lpc_info = None
     if args.learned_pc:
@@ -345,7 +350,9 @@ def main():
                    "sync_mode": args.sync_mode,
                    "preconditioner": "block-Jacobi + learned correction" if args.learned_pc
                    else "block-Jacobi", "learned_pc": lpc_info, "autotune": tune,
-                   "spgmr": "device-resident" if args.spgmr_device else "host-driven"},
+                   "spgmr": {"pipelined": "lagged normalisation, gated pipeline",
+                             "lagged": "lagged normalisation", "host": "host-driven",
+                             "device": "device-resident"}[args.spgmr]},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

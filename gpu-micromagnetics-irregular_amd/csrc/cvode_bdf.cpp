@@ -128,6 +128,7 @@ struct Integrator {
   long nstlj = 0, npe = 0, nli = 0, nps = 0, ncfl = 0, njtimes = 0, nfeDQ = 0;
   N_Vector ycur = nullptr, fcur = nullptr;   // borrowed
   N_Vector lsx = nullptr;                     // linear-solver solution
+  bool newton_init = false;   // the predictor also set acor = 0, y = zn[0]
   N_Vector V[kMaxKrylov + 1] = {};
   N_Vector xcor = nullptr, vtemp = nullptr, ytemp = nullptr;
   double Hes[kMaxKrylov + 1][kMaxKrylov] = {};
@@ -258,6 +259,11 @@ void set_coeffs(Integrator* m) {
 void predict(Integrator* m) {
   m->tn += m->h;
   if (m->tstopset && (m->tn - m->tstop) * m->h > 0.0) m->tn = m->tstop;
+  // with the Newton iteration's acor = 0 and y = zn[0] + acor in the same
+  // pass (cv_nls and the first residual then skip both)
+  m->newton_init = m->fused.predict_init &&
+                   m->fused.predict_init(m->q, m->zn, m->acor, m->y) == 0;
+  if (m->newton_init) return;
   if (m->fused.pascal && m->fused.pascal(m->q, m->zn, 1.0) == 0) return;
   for (int k = 1; k <= m->q; k++)
     for (int j = m->q; j >= k; j--) N_VLinearSum(1.0, m->zn[j - 1], 1.0, m->zn[j], m->zn[j - 1]);
@@ -507,8 +513,224 @@ int ls_psolve(Integrator* m, N_Vector r, N_Vector z, double tol, int lr) {
   return rv;
 }
 
+constexpr int kLaggedMax = 8;        // widest lagged-normalisation SPGMR
+constexpr int kLaggedFallback = -100;
+constexpr int kBnormSmall = -101;   // |b| below the solve's floor: no solve
+constexpr double kEstFloor = 1e-4;   // see spgmr_lagged
+
+// The sine of the Givens rotation qr_fact would form for (t1, t2)
+double givens_sine(double t1, double t2) {
+  if (t2 == 0.0) return 0.0;
+  if (std::fabs(t2) >= std::fabs(t1)) {
+    const double t3 = t1 / t2;
+    return -1.0 / std::sqrt(1.0 + t3 * t3);
+  }
+  const double t3 = t2 / t1;
+  return -(1.0 / std::sqrt(1.0 + t3 * t3)) * t3;
+}
+
+// SUNLinSolSolve_SPGMR (left preconditioning, classical GS, zero initial
+// guess) with the normalisation of each basis vector lagged by one step:
+// the operator is linear, so the fused pass applies it to the projected,
+// not yet normalised vector u_l, and the Hessenberg column follows from
+// the same pass's sums, h_{j,l} = <A u_l, u_j> / (|u_l| |u_j|), with
+// |u_{l+1}| = h_{l+1,l} from the next pass.  The basis is kept
+// unnormalised (V[j] = u_j) and the solution is sum_j (y_j / |u_j|) u_j / s.
+// Same Hessenberg matrix, rotations and convergence test as the standard
+// loop in exact arithmetic; one pass and one host wait per iteration
+// instead of the operator, the multi-dot and the combination + norm.
+// The pass of step l+1 is only needed when column l does not converge:
+// a column that the Pythagorean estimate |A V_l|^2 - sum_j h_{j,l}^2 of
+// h_{l+1,l} shows converged (and the last column) take h_{l+1,l} from the
+// estimate when it is well conditioned, else from a projection-norm pass.
+// Returns kLaggedFallback (x untouched) when the fused passes are not
+// available or column l needs SUNDIALS' re-orthogonalisation pass.
+bool lagged_ok(const Integrator* m) {
+  const bool pre_left =
+      (m->pretype == SUN_PREC_LEFT || m->pretype == SUN_PREC_BOTH) && m->psolve;
+  const bool pre_right =
+      (m->pretype == SUN_PREC_RIGHT || m->pretype == SUN_PREC_BOTH) && m->psolve;
+  return pre_left && !pre_right && m->gstype == SUN_CLASSICAL_GS && m->jtimes &&
+         m->fused.arnoldi && m->fused.projnorm && m->maxl <= kLaggedMax;
+}
+
+int spgmr_lagged(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s,
+                 double bnorm_floor) {
+  N_Vector* V = m->V;
+  const int l_max = m->maxl;
+  N_Vector W[2] = {m->vtemp, m->xcor};
+  double r[kLaggedMax + 4];
+  double nu[kLaggedMax + 1];
+  double cb[kLaggedMax + 1];
+  const llgcv_fused_ops& f = m->fused;
+  // Pipelined form: the pass of step l+1 is queued behind a gate before
+  // step l's sums are back, and the gate is released with step l+1's
+  // coefficients (or stopped) as soon as they are known, so the launch is
+  // off the host round trip.  At most one gate is open; every exit stops it.
+  const bool pipe = f.arnoldi_queue && f.result_wait && f.gate_open && f.gate_release;
+  struct Gate {
+    const llgcv_fused_ops& f;
+    N_Vector v;
+    bool open = false;
+    unsigned ticket = 0;
+    const double* coef = nullptr;
+    const int* skip = nullptr;
+    void stop() {
+      if (open) f.gate_release(v, 1, nullptr, 0);
+      open = false;
+    }
+    ~Gate() { stop(); }
+  } gate{f, W[0]};
+  // queue step l (l >= 1) reading W[in], writing W[in ^ 1], behind a gate
+  auto queue_gated = [&](int l, int in) {
+    if (!pipe || l >= l_max || f.gate_open(W[0], &gate.coef, &gate.skip) != 0) return;
+    gate.open = true;
+    if (f.arnoldi_queue(l, 0, 0.0, nullptr, gate.coef, gate.skip, W[in], V, W[in ^ 1], s,
+                        m->gamma, m->tn, m->ycur, m->fcur, m->user_data, &gate.ticket) != 0)
+      gate.stop();
+  };
+  // step 0: u_0 = s .* P^-1 b (the initial residual), w_0 = A u_0
+  if (pipe) {
+    unsigned tk0 = 0;
+    if (f.arnoldi_queue(0, 1, 1.0, nullptr, nullptr, nullptr, b, V, W[0], s, m->gamma, m->tn,
+                        m->ycur, m->fcur, m->user_data, &tk0) != 0)
+      return kLaggedFallback;
+    queue_gated(1, 0);
+    if (f.result_wait(W[0], tk0, 4, r) != 0) return kLaggedFallback;
+  } else if (f.arnoldi(0, 1, 1.0, nullptr, b, V, W[0], s, m->gamma, m->tn, m->ycur, m->fcur,
+                       m->user_data, r) != 0) {
+    return kLaggedFallback;
+  }
+  m->nps += 2;
+  m->njtimes++;
+  if (bnorm_floor >= 0.0 &&
+      std::sqrt(r[3] / (double)N_VGetLength(b)) <= bnorm_floor)   // |b|_WRMS(s)
+    return kBnormSmall;
+  int cur = 0;   // W[cur] holds A u_l
+  const double r_norm = std::sqrt(r[0]);
+  const double beta = r_norm;
+  m->ls_resnorm = r_norm;
+  if (r_norm <= delta) {
+    gate.stop();
+    N_VConst(0.0, x);
+    return LS_SUCCESS;
+  }
+  for (int i = 0; i <= l_max; i++)
+    for (int j = 0; j < l_max; j++) m->Hes[i][j] = 0.0;
+  nu[0] = r_norm;
+  double rotation_product = 1.0, rho = beta;
+  int krydim = 0;
+  bool converged = false;
+  for (int l = 0; l < l_max; l++) {
+    m->ls_iters++;
+    krydim = l + 1;
+    // column l from the sums of step l
+    const double inv = 1.0 / nu[l];
+    double hh = 0.0;
+    for (int j = 0; j <= l; j++) {
+      m->Hes[j][l] = (r[1 + j] * inv) / nu[j];
+      hh += m->Hes[j][l] * m->Hes[j][l];
+      cb[j] = m->Hes[j][l] / nu[j];
+    }
+    const double vk_norm = std::sqrt(r[2 + l]) * inv;   // |A V_l|
+    // h_{l+1,l}^2 = |A V_l|^2 - sum_j h_{j,l}^2 (Pythagoras: A V_l minus its
+    // projection on the orthonormal V_0..V_l).  Rounding errors of the sums
+    // are ~1e-16 |A V_l|^2, so when the estimate exceeds 1e-4 |A V_l|^2 it
+    // holds h_{l+1,l} to ~1e-12 relative.  It then replaces the projection
+    // pass wherever only h_{l+1,l} is needed: for the last column, and for
+    // a column it shows converged (the rotation below is then the one the
+    // trial forms, so the test cannot disagree).
+    const double est = vk_norm * vk_norm - hh;
+    bool use_est = false;
+    if (est > kEstFloor * vk_norm * vk_norm) {
+      if (l + 1 == l_max) {
+        use_est = true;
+      } else {
+        double t[kLaggedMax + 1];
+        for (int j = 0; j <= l; j++) t[j] = m->Hes[j][l];
+        for (int k = 0; k < l; k++) {
+          const double c = m->givens[2 * k], sn = m->givens[2 * k + 1];
+          const double t1 = t[k], t2 = t[k + 1];
+          t[k] = c * t1 - sn * t2;
+          t[k + 1] = sn * t1 + c * t2;
+        }
+        const double sn = givens_sine(t[l], std::sqrt(est));
+        use_est = std::fabs(rotation_product * sn * r_norm) <= delta;
+      }
+    }
+    double nu_next = 0.0;
+    if (use_est) {
+      gate.stop();   // the queued step l+1 is not needed
+      nu_next = std::sqrt(est);
+    } else if (l + 1 == l_max) {
+      double zz = 0.0;
+      if (f.projnorm(l + 1, inv, cb, W[cur], V, nullptr, &zz) != 0) return kLaggedFallback;
+      nu_next = std::sqrt(zz);
+    } else if (gate.open) {
+      // release the queued step l+1, queue step l+2 behind a new gate,
+      // then wait for step l+1
+      double cf[kLaggedMax + 2];
+      cf[0] = inv;
+      for (int j = 0; j <= l; j++) cf[1 + j] = cb[j];
+      f.gate_release(W[0], 0, cf, l + 2);
+      gate.open = false;
+      const unsigned tk = gate.ticket;
+      queue_gated(l + 2, cur ^ 1);
+      if (f.result_wait(W[0], tk, l + 4, r) != 0) return kLaggedFallback;
+      m->nps++;
+      m->njtimes++;
+      cur ^= 1;
+      nu_next = std::sqrt(r[0]);
+    } else {
+      if (f.arnoldi(l + 1, 0, inv, cb, W[cur], V, W[cur ^ 1], s, m->gamma, m->tn, m->ycur,
+                    m->fcur, m->user_data, r) != 0)
+        return kLaggedFallback;
+      m->nps++;
+      m->njtimes++;
+      cur ^= 1;
+      nu_next = std::sqrt(r[0]);
+    }
+    if (kGsFactor * nu_next < vk_norm) return kLaggedFallback;   // re-orthogonalisation
+    nu[l + 1] = nu_next;
+    m->Hes[l + 1][l] = nu_next;
+    if (qr_fact(krydim, m->Hes, m->givens, l) != 0) return LS_QRFACT_FAIL;
+    rotation_product *= m->givens[2 * l + 1];
+    m->ls_resnorm = rho = std::fabs(rotation_product * r_norm);
+    if (rho <= delta) { converged = true; break; }
+    if (use_est && l + 1 < l_max) {
+      // not reached (the trial rotation is the one just formed); kept so
+      // that the loop stays correct if the two ever differ
+      if (f.arnoldi(l + 1, 0, inv, cb, W[cur], V, W[cur ^ 1], s, m->gamma, m->tn, m->ycur,
+                    m->fcur, m->user_data, r) != 0)
+        return kLaggedFallback;
+      m->nps++;
+      m->njtimes++;
+      cur ^= 1;
+    }
+  }
+  gate.stop();
+  m->yg[0] = r_norm;
+  for (int i = 1; i <= krydim; i++) m->yg[i] = 0.0;
+  if (qr_sol(krydim, m->Hes, m->givens, m->yg) != 0) return LS_QRSOL_FAIL;
+  if (converged || rho < beta) {
+    for (int k = 0; k < krydim; k++) {
+      m->gs_c[k] = m->yg[k] / nu[k];
+      m->gs_X[k] = V[k];
+    }
+    if (!(m->fused.lincombdiv && m->fused.lincombdiv(krydim, m->gs_c, m->gs_X, s, x) == 0)) {
+      N_VLinearCombination(krydim, m->gs_c, m->gs_X, m->xcor);
+      N_VDiv(m->xcor, s, x);
+    }
+    return converged ? LS_SUCCESS : LS_RES_REDUCED;
+  }
+  return LS_CONV_FAIL;
+}
+
 // SUNLinSolSolve_SPGMR with s1 = s2 = weight, zero initial guess
-int spgmr_solve(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s) {
+// bnorm_floor >= 0: first return kBnormSmall (b untouched) when
+// |b|_WRMS(s) <= bnorm_floor, the check cvLsSolve makes before the solve
+int spgmr_solve(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s,
+                double bnorm_floor) {
   N_Vector* V = m->V;
   const int l_max = m->maxl;
   const bool pre_left = (m->pretype == SUN_PREC_LEFT || m->pretype == SUN_PREC_BOTH) && m->psolve;
@@ -516,6 +738,12 @@ int spgmr_solve(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s)
   int krydim = 0, rv;
   bool converged = false;
   m->ls_iters = 0;
+  if (lagged_ok(m)) {
+    rv = spgmr_lagged(m, x, b, delta, s, bnorm_floor);
+    if (rv != kLaggedFallback) return rv;
+    m->ls_iters = 0;
+  }
+  if (bnorm_floor >= 0.0 && N_VWrmsNorm(b, s) <= bnorm_floor) return kBnormSmall;
 
   // r0 = b (zero initial guess); b is read here only, so the solution may
   // be returned in place of b (x == b) and r0 needs no copy
@@ -693,10 +921,15 @@ int ls_setup(Integrator* m, int convfail, N_Vector ypred, N_Vector fpred, bool* 
 // cvLsSolve: solve (I - gamma J) x = b, result in b
 int ls_solve(Integrator* m, N_Vector b, N_Vector weight, N_Vector ynow, N_Vector fnow) {
   const double deltar = m->eplifac * m->tq[4];
-  const double bnorm = N_VWrmsNorm(b, weight);
-  if (bnorm <= deltar) {
-    if (m->nls_curiter > 0) N_VConst(0.0, b);
-    return 0;
+  // with the lagged-normalisation solve (and no Jacobian-times setup to run
+  // in between) |b|_WRMS comes out of the solve's first pass
+  const bool bnorm_in_solve = lagged_ok(m) && !m->jtsetup;
+  if (!bnorm_in_solve) {
+    const double bnorm = N_VWrmsNorm(b, weight);
+    if (bnorm <= deltar) {
+      if (m->nls_curiter > 0) N_VConst(0.0, b);
+      return 0;
+    }
   }
   const double delta = deltar * m->nrmfac;
   m->ycur = ynow;
@@ -706,7 +939,11 @@ int ls_solve(Integrator* m, N_Vector b, N_Vector weight, N_Vector ynow, N_Vector
     if (rv != 0) return rv;
   }
   // the solution overwrites b (SPGMR reads b before it writes x)
-  const int rv = spgmr_solve(m, b, b, delta, weight);
+  const int rv = spgmr_solve(m, b, b, delta, weight, bnorm_in_solve ? deltar : -1.0);
+  if (rv == kBnormSmall) {
+    if (m->nls_curiter > 0) N_VConst(0.0, b);
+    return 0;
+  }
   m->nli += m->ls_iters;
   if (rv != LS_SUCCESS) m->ncfl++;
   switch (rv) {
@@ -800,7 +1037,8 @@ int newton_solve(Integrator* m, N_Vector ycor, double tol, bool call_lsetup, lon
   bool jbad = false;
   int rv = 0;
   N_Vector del = m->delta;
-  m->y_current = false;
+  m->y_current = m->newton_init;   // y = zn[0] + ycor already formed by the predictor
+  m->newton_init = false;
   for (;;) {
     rv = nls_residual(m, ycor, del);
     if (rv != 0) break;
@@ -861,7 +1099,7 @@ int cv_nls(Integrator* m, int nflag) {
     m->crate = 1.0;
     call_setup = false;
   }
-  N_VConst(0.0, m->acor);
+  if (!m->newton_init) N_VConst(0.0, m->acor);
   long ni = 0, nf = 0;
   const int flag = newton_solve(m, m->acor, m->tq[4], call_setup, &ni, &nf);
   m->nni += ni;
@@ -1027,8 +1265,22 @@ void prepare_next_step(Integrator* m, double dsm) {
     return;
   }
   m->qwait = 2;
-  compute_etaqm1(m);
-  compute_etaqp1(m);
+  double nr[2];
+  const bool both = m->q > 1 && m->q != m->qmax && m->saved_tq5 != 0.0;
+  const double cquot = both ? (m->tq[5] / m->saved_tq5) *
+                                  std::pow(m->h / m->tau[2], (double)m->L)
+                            : 0.0;
+  if (both && m->fused.wrms_pair &&
+      m->fused.wrms_pair(m->zn[m->q], -cquot, m->zn[m->qmax], 1.0, m->acor, m->ewt, nr) == 0) {
+    // both order-change norms with one wait (the same values as below)
+    const double ddn = nr[0] * m->tq[1];
+    m->etaqm1 = 1.0 / (rpow(kBias1 * ddn, 1.0 / m->q) + kAddon);
+    const double dup = nr[1] * m->tq[3];
+    m->etaqp1 = 1.0 / (rpow(kBias3 * dup, 1.0 / (m->L + 1)) + kAddon);
+  } else {
+    compute_etaqm1(m);
+    compute_etaqp1(m);
+  }
   choose_eta(m);
   set_eta(m);
 }
@@ -1399,7 +1651,10 @@ int LLGCVode(void* mem, sunrealtype tout, N_Vector yout, sunrealtype* tret, int 
       N_VScale(1.0, m->zn[0], yout);
       break;
     }
-    const double nrm = N_VWrmsNorm(m->zn[0], m->ewt);
+    // ewt was just formed from zn[0] itself, so |zn0_i| ewt_i < 1/rtol and
+    // uround |zn[0]|_WRMS < uround/rtol: with rtol >= 2 uround the test
+    // below cannot fail and its reduction (and host wait) is skipped
+    const double nrm = (m->reltol >= 2.0 * m->uround) ? 0.0 : N_VWrmsNorm(m->zn[0], m->ewt);
     m->tolsf = m->uround * nrm;
     if (m->tolsf > 1.0) {
       istate = LLGCV_TOO_MUCH_ACC;

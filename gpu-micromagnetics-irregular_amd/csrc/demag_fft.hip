@@ -24,6 +24,7 @@
 // 2d_i4/demag_fft.cu:27-91) is evaluated on the GPU over one quadrant of
 // displacements and mirrored.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -956,18 +957,20 @@ void apply_t(DemagData* d, const HostPlan<T>& row, const HostPlan<T>& col, const
                          y, w, N, nx, ny, f, post, spec);
     });
   kprof::end(kprof::DEMAG_ROW_FWD, s);
-  kprof::begin(kprof::DEMAG_COL, s);
+  // the column kernel's timing events ride on its dispatch
+  hipEvent_t ce0, ce1;
+  kprof::ext_pair(kprof::DEMAG_COL, &ce0, &ce1);
   rc = d->reg_col < 0 ? 0
                       : demag_reg::launch_col<T>(Py, d->reg_col, half, spec, Hx, col.p.tw, K,
-                                                  d->has_xy_im, s);
+                                                  d->has_xy_im, s, ce0, ce1);
   if (rc == 0)
     with_col_plan(col, d->ct, [&](auto f) {
       const size_t col_lds = sizeof(V2<T>) * 3 * lpad_len(Py, col.p.psh);
       allow_lds(k_col<T, decltype(f)>, col_lds);
-      hipLaunchKernelGGL((k_col<T, decltype(f)>), dim3(Hx), dim3(col.block), col_lds, s, spec,
-                         ny, ny, Hx, f, K, d->has_xy_im, 0);
+      hipExtLaunchKernelGGL((k_col<T, decltype(f)>), dim3(Hx), dim3(col.block),
+                            (std::uint32_t)col_lds, s, ce0, ce1, 0u, spec, ny, ny, Hx, f, K,
+                            d->has_xy_im, 0);
     });
-  kprof::end(kprof::DEMAG_COL, s);
   kprof::begin(kprof::DEMAG_ROW_INV, s);
   rc = d->reg_row < 0 ? 0
                       : demag_reg::launch_row_inv<T>(row.p.L, d->reg_row, half,

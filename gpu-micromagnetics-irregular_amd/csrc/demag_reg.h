@@ -34,10 +34,12 @@ int launch_row_fwd(int L, int variant, int half, const double* in, const double*
 
 // column pass: forward column FFTs of the three components, 3x3 tensor
 // product (planes xx, yy, zz, xy-real, xy-imag stored [kx][ky]), inverse
-// column FFTs; spec is [comp][row][kx] with row count L (L/2 when half)
+// column FFTs; spec is [comp][row][kx] with row count L (L/2 when half).
+// e0/e1 (may be null): events the dispatch itself records at the start and
+// the end of the kernel (timing without separate marker packets)
 template <class T>
 int launch_col(int L, int variant, int half, C2<T>* spec, int Hx, const C2<T>* tw, const T* K, int has_xy_im,
-               hipStream_t s);
+               hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 
 // row pass, inverse: inverse real FFT per (row, component), field written
 // into the SoA output (overwrite)
@@ -56,9 +58,9 @@ extern template int launch_row_fwd<float>(int, int, int, const double*, const do
                                           const C2<float>*, const C2<float>*, C2<float>*,
                                           hipStream_t);
 extern template int launch_col<double>(int, int, int, C2<double>*, int, const C2<double>*,
-                                       const double*, int, hipStream_t);
+                                       const double*, int, hipStream_t, hipEvent_t, hipEvent_t);
 extern template int launch_col<float>(int, int, int, C2<float>*, int, const C2<float>*, const float*,
-                                      int, hipStream_t);
+                                      int, hipStream_t, hipEvent_t, hipEvent_t);
 extern template int launch_row_inv<double>(int, int, int, const C2<double>*, int, const C2<double>*,
                                            const C2<double>*, double*, long, int, hipStream_t);
 extern template int launch_row_inv<float>(int, int, int, const C2<float>*, int, const C2<float>*,

@@ -8,7 +8,8 @@ template bool has_col_plan<double>(int, int);
 template int launch_row_fwd<double>(int, int, int, const double*, const double*, long, int, int,
                                   const C2<double>*, const C2<double>*, C2<double>*, hipStream_t);
 template int launch_col<double>(int, int, int, C2<double>*, int, const C2<double>*, const double*, int,
-                              hipStream_t);
+                              hipStream_t, hipEvent_t,
+                              hipEvent_t);
 template int launch_row_inv<double>(int, int, int, const C2<double>*, int, const C2<double>*,
                                   const C2<double>*, double*, long, int, hipStream_t);
 }  // namespace demag_reg

@@ -22,6 +22,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include "demag_reg.h"
 #include "fft_regs.h"
@@ -346,7 +347,7 @@ int launch_row_fwd(int L, int variant, int half, const double* in, const double*
 
 template <class T>
 int launch_col(int L, int variant, int half, C2<T>* spec, int Hx, const C2<T>* tw, const T* K, int has_xy_im,
-               hipStream_t s) {
+               hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   int rc = 0;
   with_col<T>(L, variant, [&](auto f) {
     using F = decltype(f);
@@ -355,7 +356,8 @@ int launch_col(int L, int variant, int half, C2<T>* spec, int Hx, const C2<T>* t
     auto kern = half ? (has_xy_im ? k_col_reg<T, F, true, true> : k_col_reg<T, F, true, false>)
                      : (has_xy_im ? k_col_reg<T, F, false, true> : k_col_reg<T, F, false, false>);
     if (prep_lds(kern, lds)) { rc = -1; return; }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(F::kThreads), lds, s, spec, Hx, tw, K);
+    hipExtLaunchKernelGGL(kern, dim3(grid), dim3(F::kThreads), (std::uint32_t)lds, s, e0, e1, 0u,
+                          spec, Hx, tw, K);
     rc = hipPeekAtLastError() == hipSuccess ? 1 : -1;
   });
   return rc;

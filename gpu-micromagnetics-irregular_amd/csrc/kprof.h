@@ -31,6 +31,21 @@ inline void begin(int id, hipStream_t s) {
   hipEventRecord(st.ev[st.used], s);
 }
 
+// An event pair for one launch of kernel `id` made with
+// hipExtLaunchKernelGGL, whose dispatch records both events itself (no
+// marker packets between the kernels); false, events null, when the id is
+// not armed or the pool is full.
+inline bool ext_pair(int id, hipEvent_t* e0, hipEvent_t* e1) {
+  *e0 = *e1 = nullptr;
+  State& st = state();
+  if (st.target != id) return false;
+  if (st.used + 2 > st.ev.size()) { st.dropped++; return false; }
+  *e0 = st.ev[st.used];
+  *e1 = st.ev[st.used + 1];
+  st.used += 2;
+  return true;
+}
+
 inline void end(int id, hipStream_t s) {
   State& st = state();
   if (st.target != id) return;

@@ -42,7 +42,7 @@ void LLG_DefaultRunOpts(llg_run_opts* o) {
   o->gstype = 0;        // auto: classical iff neq < 500000 (2d_i4/2d_fft.cu:645-650)
   o->max_order = 5;
   o->mxstep = 500;
-  o->fused = 7;
+  o->fused = 23;
   o->use_jtv = 1;
   o->use_prec = 1;
   o->pretype = SUN_PREC_LEFT;
@@ -88,11 +88,23 @@ void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_hos
     fo.gsproject = NVHip_GSProject;
     fo.lincombdiv = NVHip_LinCombDiv;
     fo.newtonupd = NVHip_NewtonUpdate;
+    fo.predict_init = NVHip_PredictInit;
+    fo.wrms_pair = NVHip_WrmsNormPair;
     if (opts->use_jtv) fo.atimes = LLG_Atimes;
     if ((opts->fused & 4) && opts->use_jtv && opts->use_prec && opts->pretype == SUN_PREC_LEFT) {
       fo.krylov = LLG_Krylov;
       fo.gsscaled = NVHip_GSProjectScaled;
       if (opts->fused & 8) fo.spgmr = LLG_SpgmrSolve;
+      if (opts->fused & 16) {
+        fo.arnoldi = LLG_Arnoldi;
+        fo.projnorm = NVHip_ProjNorm;
+      }
+      if ((opts->fused & 48) == 48) {
+        fo.arnoldi_queue = LLG_ArnoldiQueue;
+        fo.result_wait = NVHip_ReduceWait;
+        fo.gate_open = NVHip_GateOpen;
+        fo.gate_release = NVHip_GateRelease;
+      }
     }
     LLGCVodeSetFusedOps(r->cv, &fo);
   }

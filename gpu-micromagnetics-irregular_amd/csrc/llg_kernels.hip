@@ -274,6 +274,8 @@ __global__ __launch_bounds__(BX* BY) void k_rhs_tiled(GeoD g, TermsD T,
     int gx = x0 + sx - 1, gy = y0 + sy - 1;
     gx = gx < 0 ? gx + g.ng : (gx >= g.ng ? gx - g.ng : gx);
     gy = gy < 0 ? gy + g.ny : (gy >= g.ny ? gy - g.ny : gy);
+    gx = gx > g.ng - 1 ? g.ng - 1 : gx;   // never read (see k_krylov_tiled)
+    gy = gy > g.ny - 1 ? g.ny - 1 : gy;
     const int c = gy * g.ng + gx;
     const double ww = win ? g.w[c] : 1.0;
     tile[0][sy][sx] = ww * y[c];
@@ -454,18 +456,28 @@ __global__ __launch_bounds__(256) void k_build_P(GeoD g, TermsD T, const double*
 }
 
 // z = gate * P_cell r for one cell (P stored SoA, 9 planes)
-__device__ inline void apply_P_cell(const GeoD& g, int gate, const double* __restrict__ P, int cell,
+// z = gate * P_cell r for one cell, from the cell's 9 block entries
+__device__ inline void apply_P_vals(const GeoD& g, int gate, const double Pv[9], int cell,
                                     const double r[3], double z[3]) {
-  const int N = g.ncell;
-  const double z0 = P[0 * N + cell] * r[0] + P[1 * N + cell] * r[1] + P[2 * N + cell] * r[2];
-  const double z1 = P[3 * N + cell] * r[0] + P[4 * N + cell] * r[1] + P[5 * N + cell] * r[2];
-  const double z2 = P[6 * N + cell] * r[0] + P[7 * N + cell] * r[1] + P[8 * N + cell] * r[2];
+  const double z0 = Pv[0] * r[0] + Pv[1] * r[1] + Pv[2] * r[2];
+  const double z1 = Pv[3] * r[0] + Pv[4] * r[1] + Pv[5] * r[2];
+  const double z2 = Pv[6] * r[0] + Pv[7] * r[1] + Pv[8] * r[2];
   if (g.compact) {
     z[0] = z0; z[1] = z1; z[2] = z2;
   } else {
     const double gt = gate_of(g, gate, cell);
     z[0] = gt * z0; z[1] = gt * z1; z[2] = gt * z2;
   }
+}
+
+// z = gate * P_cell r for one cell (P stored SoA, 9 planes)
+__device__ inline void apply_P_cell(const GeoD& g, int gate, const double* __restrict__ P, int cell,
+                                    const double r[3], double z[3]) {
+  const int N = g.ncell;
+  double Pv[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) Pv[q] = P[q * N + cell];
+  apply_P_vals(g, gate, Pv, cell, r, z);
 }
 
 // One SPGMR iteration's operator chain with left block-Jacobi
@@ -534,6 +546,10 @@ __global__ __launch_bounds__(BX* BY) void k_krylov_tiled(GeoD g, TermsD T, int p
     } else {
       gx = gx < 0 ? gx + g.ng : (gx >= g.ng ? gx - g.ng : gx);
       gy = gy < 0 ? gy + g.ny : (gy >= g.ny ? gy - g.ny : gy);
+      // staged cells beyond the first halo column/row of a partial tile are
+      // never read; keep them in bounds on grids narrower than the tile
+      gx = gx > g.ng - 1 ? g.ng - 1 : gx;
+      gy = gy > g.ny - 1 ? g.ny - 1 : gy;
     }
     const int cidx = gy * g.ng + gx;
 #pragma unroll
@@ -553,6 +569,160 @@ __global__ __launch_bounds__(BX* BY) void k_krylov_tiled(GeoD g, TermsD T, int p
   vk[cell] = s[cell] * z[0];
   vk[N + cell] = s[N + cell] * z[1];
   vk[2 * N + cell] = s[2 * N + cell] * z[2];
+}
+
+// Inputs of one Arnoldi step with lagged normalisation (k_arnoldi).
+constexpr int kArnMax = 8;   // basis vectors before the new one (SPGMR maxl <= 8)
+struct ArnIn {
+  const double* win;          // w of the previous step; b for the first step with PRE
+  const double* U[kArnMax];   // unnormalised basis vectors u_0 .. u_{L-1}
+  double* ul;                 // u_L, written for the tile's own cells (L > 0 or PRE)
+  double a;                   // u = a win - sum_j b_j U_j
+  double b[kArnMax];
+};
+
+// One left-preconditioned SPGMR iteration with the normalisation of the
+// basis vector left to the caller (the operator is linear, so A(u/|u|) =
+// (A u)/|u|), on the dense grid:
+//   u = a win - sum_{j<L} b_j U_j     (L > 0: the classical Gram-Schmidt
+//                                      projection of the previous step)
+//     = s .* gate P win                (L == 0 with PRE: the first
+//                                      preconditioner solve and scaling)
+//     = U_0                            (L == 0 without PRE)
+//   w = s .* gate P (v + (-gamma) J v),  v = u / s
+// u is formed once per cell in an LDS tile with a one-cell halo (the halo
+// cell of an edge is the neighbour nbrs() names) and written for the
+// tile's own cells; w is written; each workgroup leaves its partial sums
+//   [|u|^2, <w,U_0>, .., <w,U_{L-1}>, <w,u>, |w|^2] (and with PRE also
+//   sum (win s)^2, the square of the WRMS norm of the right-hand side)
+// at partials[blockIdx.x * K + k], summed in workgroup order by the
+// caller's finish pass.  Workgroup b takes the tiles of the band of XCD
+// (b mod 8) in grid-stride order, so any grid size covers the grid.
+// (Keeping the own cell's U_j and P in LDS instead of re-reading them
+// removed ~25 % of the fetched bytes but no time: the kernel is bound by
+// the latency of its two dependent load phases, not by bandwidth.)
+template <int L, bool PRE, int BX, int BY>
+__global__ __launch_bounds__(BX* BY) void k_arnoldi(GeoD g, TermsD T, int pgate,
+                                                   const double* __restrict__ P,
+                                                   const double* __restrict__ y,
+                                                   const double* __restrict__ s, double mgamma,
+                                                   ArnIn in, double* __restrict__ wout,
+                                                   double* __restrict__ partials, int ntx,
+                                                   int nty, const double* coef,
+                                                   const int* skip) {
+  if (skip && *skip) return;   // queued behind a gate that stopped
+  constexpr int SX = BX + 2, SY = BY + 2, NT = BX * BY, K = L + 3 + (PRE ? 1 : 0);
+  // projection coefficients: by value, or from device memory (a gate)
+  double ca = in.a, cb[L > 0 ? L : 1];
+#pragma unroll
+  for (int j = 0; j < L; j++) cb[j] = in.b[j];
+  if (coef) {
+    ca = coef[0];
+#pragma unroll
+    for (int j = 0; j < L; j++) cb[j] = coef[1 + j];
+  }
+  __shared__ double tile[3][SY][SX];   // u / s with a one-cell halo
+  __shared__ double own[3][BY][BX];    // u of the tile's own cells
+  __shared__ double sm[K][NT / 64];
+  const int N = g.ncell;
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = 0.0;
+
+  const int G = gridDim.x, xcd = blockIdx.x & 7, idx = blockIdx.x >> 3;
+  const int nwg = G / 8 + (xcd < G % 8 ? 1 : 0);
+  const int ntile = ntx * nty, per = ntile / 8, rem = ntile % 8;
+  const int t_begin = xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
+  const int t_end = t_begin + per + (xcd < rem ? 1 : 0);
+  for (int t = t_begin + idx; t < t_end; t += nwg) {
+    const int x0 = (t % ntx) * BX, y0 = (t / ntx) * BY;
+    for (int i = tid; i < SX * SY; i += NT) {
+      const int sy = i / SX, sx = i - sy * SX;
+      if ((sx == 0 || sx == SX - 1) && (sy == 0 || sy == SY - 1)) continue;   // corners unused
+      const int ux = x0 + sx - 1, uy = y0 + sy - 1;
+      int gx = ux, gy = uy;
+      if (g.boundary == LLG_BC_NEUMANN) {
+        gx = gx < 0 ? 0 : (gx > g.ng - 1 ? g.ng - 1 : gx);
+        gy = gy < 0 ? 0 : (gy > g.ny - 1 ? g.ny - 1 : gy);
+      } else {
+        gx = gx < 0 ? gx + g.ng : (gx >= g.ng ? gx - g.ng : gx);
+        gy = gy < 0 ? gy + g.ny : (gy >= g.ny ? gy - g.ny : gy);
+        gx = gx > g.ng - 1 ? g.ng - 1 : gx;   // never read (see k_krylov_tiled)
+        gy = gy > g.ny - 1 ? g.ny - 1 : gy;
+      }
+      const int cidx = gy * g.ng + gx;
+      const bool mine = sx >= 1 && sx <= BX && sy >= 1 && sy <= BY && ux < g.ng && uy < g.ny;
+      double u[3];
+      if (L == 0 && PRE) {
+        const double r[3] = {in.win[cidx], in.win[N + cidx], in.win[2 * N + cidx]};
+        double z[3];
+        apply_P_cell(g, pgate, P, cidx, r, z);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          const double sa = s[a * N + cidx];
+          u[a] = sa * z[a];
+          if (mine) {
+            const double p = r[a] * sa;
+            acc[K - 1] = fma(p, p, acc[K - 1]);
+          }
+        }
+      } else if (L == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) u[a] = in.U[0][a * N + cidx];
+      } else {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          double v = ca * in.win[a * N + cidx];
+#pragma unroll
+          for (int j = 0; j < L; j++) v = fma(-cb[j], in.U[j][a * N + cidx], v);
+          u[a] = v;
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        tile[a][sy][sx] = u[a] / s[a * N + cidx];
+        if (mine) {
+          own[a][sy - 1][sx - 1] = u[a];
+          if (L > 0 || PRE) in.ul[a * N + cidx] = u[a];
+        }
+      }
+    }
+    __syncthreads();
+    const int gx = x0 + threadIdx.x, gy = y0 + threadIdx.y;
+    if (gx < g.ng && gy < g.ny) {
+      const int cell = gy * g.ng + gx;
+      double o[3], z[3];
+      jtv_cell<true>(g, T, y, VTile<SY, SX>{tile, (int)threadIdx.x + 1, (int)threadIdx.y + 1},
+                     cell, gx, gy, mgamma, o);
+      apply_P_cell(g, pgate, P, cell, o, z);
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        const double w = s[a * N + cell] * z[a];
+        wout[a * N + cell] = w;
+        const double u = own[a][threadIdx.y][threadIdx.x];
+        acc[0] = fma(u, u, acc[0]);
+#pragma unroll
+        for (int j = 0; j < L; j++) acc[1 + j] = fma(w, in.U[j][a * N + cell], acc[1 + j]);
+        acc[1 + L] = fma(w, u, acc[1 + L]);
+        acc[2 + L] = fma(w, w, acc[2 + L]);
+      }
+    }
+    __syncthreads();   // the tiles are restaged by the next iteration
+  }
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    double a = acc[k];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if (lane == 0) sm[k][wave] = a;
+  }
+  __syncthreads();
+  if (tid < K) {
+    double a = sm[tid][0];
+    for (int w = 1; w < NT / 64; w++) a += sm[tid][w];
+    partials[blockIdx.x * K + tid] = a;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_apply_P(GeoD g, int gate, const double* __restrict__ P,
@@ -1026,6 +1196,15 @@ int launch_krylov(llg_ctx* ctx, sunrealtype c, const double* cdev, const int* sk
   return launch_status("krylov");
 }
 
+template <int L, bool PRE>
+void launch_arnoldi_t(llg_ctx* ctx, hipStream_t st, int nwg, int ntx, int nty, const double* y,
+                      const double* s, double mgamma, const ArnIn& in, double* wout,
+                      double* partials, const double* coef, const int* skip) {
+  hipLaunchKernelGGL((k_arnoldi<L, PRE, 32, 8>), dim3(nwg), dim3(32, 8), 0, st, geo_of(ctx),
+                     to_dev(ctx->cfg.jv), ctx->cfg.pc.gate, (const double*)ctx->pd->d_P, y, s,
+                     mgamma, in, wout, partials, ntx, nty, coef, skip);
+}
+
 struct DevOpData {
   llg_ctx* ctx;
   double gamma;
@@ -1066,6 +1245,87 @@ int LLG_SpgmrSolve(N_Vector t0, N_Vector* V, int maxl, N_Vector s, N_Vector x,
     ctx->npsol += *iters;
   }
   return rc;
+}
+
+int LLG_ArnoldiQueue(int l, int pre, sunrealtype a, const sunrealtype* b, const double* coef_dev,
+                     const int* skip_dev, N_Vector win, N_Vector* U, N_Vector wout, N_Vector s,
+                     sunrealtype gamma, sunrealtype t, N_Vector y, N_Vector fy, void* user_data,
+                     unsigned* ticket) {
+  (void)t; (void)fy;
+  llg_ctx* ctx = (llg_ctx*)user_data;
+  if (!ctx || ctx->lpc || !ctx->pd || ctx->cfg.exec != LLG_EXEC_YMSK || !U || !wout || !s ||
+      !y || !ticket || l < 0 || l >= kArnMax || (pre && l != 0) ||
+      (l > 0 && (!win || (!b && !coef_dev))) || (pre && !win))
+    return -1;
+  ArnIn in{};
+  in.win = win ? N_VGetDeviceArrayPointer(win) : nullptr;
+  in.a = a;
+  for (int j = 0; j <= l; j++)
+    if (!U[j]) return -1;
+  for (int j = 0; j < l; j++) {
+    in.U[j] = N_VGetDeviceArrayPointer(U[j]);
+    in.b[j] = b ? b[j] : 0.0;
+  }
+  if (l == 0 && !pre) in.U[0] = N_VGetDeviceArrayPointer(U[0]);
+  in.ul = (l > 0 || pre) ? N_VGetDeviceArrayPointer(U[l]) : nullptr;
+  double* dw = N_VGetDeviceArrayPointer(wout);
+  const double* ds = N_VGetDeviceArrayPointer(s);
+  const double* dy = N_VGetDeviceArrayPointer(y);
+  // the kernel reads win, s, y and U[j < l] at halo cells other workgroups
+  // own, so nothing it writes may alias them
+  const double* reads[kArnMax + 3] = {in.win, ds, dy};
+  int nr = 3;
+  for (int j = 0; j < l; j++) reads[nr++] = in.U[j];
+  if (l == 0 && !pre) reads[nr++] = in.U[0];
+  for (int i = 0; i < nr; i++)
+    if (reads[i] && (reads[i] == dw || reads[i] == in.ul)) return -1;
+  if (in.ul && in.ul == dw) return -1;
+  int cap = 0;
+  double* partials = NVHip_ReducePartials(wout, &cap);
+  if (!partials) return -1;
+  constexpr int BX = 32, BY = 8;
+  const int K = l + 3 + (pre ? 1 : 0);
+  const int ntx = (ctx->cfg.ng + BX - 1) / BX, nty = (ctx->cfg.ny + BY - 1) / BY;
+  // one resident round (four 256-thread workgroups per CU on 256 CUs at
+  // this kernel's register use); larger grids take several tiles each
+  int nwg = ntx * nty;
+  if (nwg > 1024) nwg = 1024;
+  if (nwg * K > cap) nwg = cap / K;
+  hipStream_t st = stream_of(wout, ctx);
+  kprof::begin(kprof::JTV, st);
+  const double mg = -gamma;
+  const double* cd = coef_dev;
+  const int* sk = skip_dev;
+  switch (l) {
+    case 0:
+      if (pre) launch_arnoldi_t<0, true>(ctx, st, nwg, ntx, nty, dy, ds, mg, in, dw, partials, cd, sk);
+      else launch_arnoldi_t<0, false>(ctx, st, nwg, ntx, nty, dy, ds, mg, in, dw, partials, cd, sk);
+      break;
+    case 1: launch_arnoldi_t<1, false>(ctx, st, nwg, ntx, nty, dy, ds, mg, in, dw, partials, cd, sk); break;
+    case 2: launch_arnoldi_t<2, false>(ctx, st, nwg, ntx, nty, dy, ds, mg, in, dw, partials, cd, sk); break;
+    case 3: launch_arnoldi_t<3, false>(ctx, st, nwg, ntx, nty, dy, ds, mg, in, dw, partials, cd, sk); break;
+    case 4: launch_arnoldi_t<4, false>(ctx, st, nwg, ntx, nty, dy, ds, mg, in, dw, partials, cd, sk); break;
+    case 5: launch_arnoldi_t<5, false>(ctx, st, nwg, ntx, nty, dy, ds, mg, in, dw, partials, cd, sk); break;
+    case 6: launch_arnoldi_t<6, false>(ctx, st, nwg, ntx, nty, dy, ds, mg, in, dw, partials, cd, sk); break;
+    default: launch_arnoldi_t<7, false>(ctx, st, nwg, ntx, nty, dy, ds, mg, in, dw, partials, cd, sk); break;
+  }
+  kprof::end(kprof::JTV, st);
+  if (launch_status("arnoldi") != 0) return -1;
+  if (NVHip_ReduceFinishAsync(wout, K, nwg, skip_dev, ticket) != 0) return -1;
+  ctx->njv++;
+  ctx->npsol += pre ? 2 : 1;
+  return 0;
+}
+
+int LLG_Arnoldi(int l, int pre, sunrealtype a, const sunrealtype* b, N_Vector win, N_Vector* U,
+                N_Vector wout, N_Vector s, sunrealtype gamma, sunrealtype t, N_Vector y,
+                N_Vector fy, void* user_data, sunrealtype* r) {
+  if (!r) return -1;
+  unsigned tk = 0;
+  if (LLG_ArnoldiQueue(l, pre, a, b, nullptr, nullptr, win, U, wout, s, gamma, t, y, fy,
+                       user_data, &tk) != 0)
+    return -1;
+  return NVHip_ReduceWait(wout, tk, l + 3 + (pre ? 1 : 0), r);
 }
 
 int LLG_SetStencilBlock(llg_ctx* ctx, int bx, int by) {

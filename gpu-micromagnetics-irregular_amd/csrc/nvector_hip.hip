@@ -73,6 +73,21 @@ struct RedWs {
   double* dres = nullptr;       // device memory, kMaxK doubles
   unsigned seq = 0;             // last sequence number issued on this stream
   void* sp = nullptr;           // device state of a device-resident SPGMR solve
+  // gate (NVHip_GateOpen/Release): pinned mailbox, device copies of its
+  // coefficients and of its stop flag, last gate number, whether one is open
+  struct GateBox* gbox = nullptr;
+  struct GateBox* gbox_dev = nullptr;   // device alias of gbox
+  double* gcoef = nullptr;
+  int* gskip = nullptr;
+  unsigned gseq = 0;
+  bool gopen = false;
+};
+
+// Mailbox of a gate in pinned host memory, written by the host only.
+struct GateBox {
+  double coef[kMaxK];
+  int stop;
+  unsigned seq;   // written last (release): the gate number being released
 };
 
 std::mutex g_mu;
@@ -235,6 +250,27 @@ __global__ void k_pascal(OutPack Z, double sign, sunindextype n) {
   }
 }
 
+// the predictor followed by the Newton iteration's start: acor = 0 and
+// y = zn[0] + acor, as N_VConst and N_VLinearSum would write them
+template <int Q>
+__global__ void k_pascal_init(OutPack Z, double* __restrict__ acor, double* __restrict__ y,
+                              sunindextype n) {
+  GRID_STRIDE(i, n) {
+    double z[Q + 1];
+#pragma unroll
+    for (int j = 0; j <= Q; j++) z[j] = Z.p[j][i];
+#pragma unroll
+    for (int k = 1; k <= Q; k++)
+#pragma unroll
+      for (int j = Q; j >= k; j--) z[j - 1] = fma(1.0, z[j], z[j - 1]);
+#pragma unroll
+    for (int j = 0; j < Q; j++) Z.p[j][i] = z[j];
+    const double zero = 0.0;
+    acor[i] = zero;
+    y[i] = fma(1.0, z[0], 1.0 * zero);
+  }
+}
+
 __global__ void k_ewt(const double* __restrict__ y, double rtol, double atol,
                       const double* __restrict__ av, double* __restrict__ w, sunindextype n) {
   GRID_STRIDE(i, n) {
@@ -334,6 +370,19 @@ struct FGsLincombSq {
   }
 };
 
+// u = a w - sum_{j<nb} b_j U_j (written to out when set), contribution
+// u_i^2: the projection norm of a Gram-Schmidt step whose coefficients
+// come from the host
+struct FProjSq {
+  int nb; double a; CoefPack b; const double* w; PtrPack U; double* out;
+  __device__ double operator()(sunindextype i) const {
+    double u = a * w[i];
+    for (int j = 0; j < nb; j++) u = fma(-b.c[j], U.p[j][i], u);
+    if (out) out[i] = u;
+    return u * u;
+  }
+};
+
 // first Newton iteration: ycor += del, y = zn0 + ycor, contribution
 // (del w)^2 -- the FWsq term of N_VWrmsNorm(del, w)
 struct FNewtonUpd0 {
@@ -348,9 +397,10 @@ struct FNewtonUpd0 {
   }
 };
 
-// Block-wide reduction of K values per thread; result valid in thread 0.
-template <int K>
-__device__ void block_reduce(int kind, double (&v)[K], double (*sm)[kWaves]) {
+// Block-wide reduction of K values per thread over W waves; result valid
+// in thread 0.
+template <int K, int W = kWaves>
+__device__ void block_reduce(int kind, double (&v)[K], double (*sm)[W]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < K; k++) {
@@ -363,30 +413,51 @@ __device__ void block_reduce(int kind, double (&v)[K], double (*sm)[kWaves]) {
 #pragma unroll
     for (int k = 0; k < K; k++) {
       double a = sm[k][0];
-      for (int w = 1; w < kWaves; w++) a = combine(kind, a, sm[k][w]);
+      for (int w = 1; w < W; w++) a = combine(kind, a, sm[k][w]);
       v[k] = a;
     }
   }
 }
 
-// Final pass run by the last workgroup: partials[b*K + k] for b < nblk.
-template <int K>
+// Final pass: partials[b*K + k] for b < nblk.  SAME_KERNEL: run by the last
+// workgroup of the kernel that wrote them, so each load is an agent-scope
+// atomic that no stale per-CU L1 line can serve; otherwise (a kernel of its
+// own) plain loads, which the hardware keeps in flight together.  Same
+// order of summation either way.
+template <int K, bool SAME_KERNEL, int W = kWaves>
 __device__ void finish(int kind, int nblk, const double* partials, double* out, double* dout,
-                       double (*sm)[kWaves], unsigned seq) {
+                       double (*sm)[W], unsigned seq) {
   double v[K];
 #pragma unroll
   for (int k = 0; k < K; k++) v[k] = identity(kind);
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+  int b = threadIdx.x;
+  if (!SAME_KERNEL) {
+    // U blocks' loads issued before any is summed (one memory round trip
+    // per U blocks instead of per block)
+    constexpr int U = K <= 4 ? 4 : (K <= 8 ? 2 : 1);
+    for (; b + (U - 1) * (int)blockDim.x < nblk; b += U * blockDim.x) {
+      double p[U][K];
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int k = 0; k < K; k++) p[u][k] = partials[(b + u * blockDim.x) * K + k];
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = combine(kind, v[k], p[u][k]);
+    }
+  }
+  for (; b < nblk; b += blockDim.x) {
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      // agent-scope atomic load: never served from a stale per-CU L1 line
-      const double p = __hip_atomic_load(partials + b * K + k, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
+      const double p = SAME_KERNEL ? __hip_atomic_load(partials + b * K + k, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT)
+                                   : partials[b * K + k];
       v[k] = combine(kind, v[k], p);
     }
   }
   __syncthreads();
-  block_reduce<K>(kind, v, sm);
+  block_reduce<K, W>(kind, v, sm);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; k++) out[k] = v[k];
@@ -436,10 +507,59 @@ __global__ __launch_bounds__(kBlock) void k_reduce1(int kind, F f, sunindextype 
   __syncthreads();
   if (last) {
     __threadfence();
-    finish<1>(kind, gridDim.x, partials, out, dout, sm, seq);
+    finish<1, true>(kind, gridDim.x, partials, out, dout, sm, seq);
     if (threadIdx.x == 0) *ticket = 0u;
   }
 }
+
+// Two reductions in one pass, each accumulated exactly as k_reduce1
+// accumulates it (same accumulators, same order), so each result equals
+// the single reduction's bit for bit.  Two-launch form only.
+template <class F1, class F2>
+__global__ __launch_bounds__(kBlock) void k_reduce2(int kind, F1 f1, F2 f2, sunindextype n,
+                                                    double* partials) {
+  __shared__ double sm[2][kWaves];
+  double a0 = identity(kind), a1 = a0, a2 = a0, a3 = a0;
+  double b0 = a0, b1 = a0, b2 = a0, b3 = a0;
+  const sunindextype stride = (sunindextype)gridDim.x * blockDim.x;
+  sunindextype i = (sunindextype)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const double x0 = f1(i), x1 = f1(i + stride), x2 = f1(i + 2 * stride),
+                 x3 = f1(i + 3 * stride);
+    const double y0 = f2(i), y1 = f2(i + stride), y2 = f2(i + 2 * stride),
+                 y3 = f2(i + 3 * stride);
+    a0 = combine(kind, a0, x0);
+    a1 = combine(kind, a1, x1);
+    a2 = combine(kind, a2, x2);
+    a3 = combine(kind, a3, x3);
+    b0 = combine(kind, b0, y0);
+    b1 = combine(kind, b1, y1);
+    b2 = combine(kind, b2, y2);
+    b3 = combine(kind, b3, y3);
+  }
+  for (; i < n; i += stride) {
+    a0 = combine(kind, a0, f1(i));
+    b0 = combine(kind, b0, f2(i));
+  }
+  double v[2] = {combine(kind, combine(kind, a0, a1), combine(kind, a2, a3)),
+                 combine(kind, combine(kind, b0, b1), combine(kind, b2, b3))};
+  block_reduce<2>(kind, v, sm);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x * 2] = v[0];
+    partials[blockIdx.x * 2 + 1] = v[1];
+  }
+}
+
+// (a y + b z) w, squared: the FWsq term of N_VLinearSum(a, y, b, z, t)
+// followed by N_VWrmsNorm(t, w), without writing t
+struct FLinWsq {
+  double a, b; const double *y, *z, *w;
+  __device__ double operator()(sunindextype i) const {
+    const double t = fma(a, y[i], b * z[i]);
+    const double p = t * w[i];
+    return p * p;
+  }
+};
 
 // K dot products x . Y_k (and K weighted squares when W given)
 // x_i = (prod ? s_i * t_i : x_i); dots_k = sum_i x_i Y_k[i], where a Y_k
@@ -514,7 +634,7 @@ __global__ __launch_bounds__(kBlock) void k_dotmulti(const double* x, PtrPack Y,
   __syncthreads();
   if (last) {
     __threadfence();
-    finish<K>(RED_SUM, gridDim.x, partials, out, dout, sm, seq);
+    finish<K, true>(RED_SUM, gridDim.x, partials, out, dout, sm, seq);
     if (threadIdx.x == 0) *ticket = 0u;
   }
 }
@@ -588,27 +708,31 @@ __global__ __launch_bounds__(kBlock) void k_wsqarray(PtrPack X, PtrPack W, const
   __syncthreads();
   if (last) {
     __threadfence();
-    finish<K>(RED_SUM, gridDim.x, partials, out, dout, sm, seq);
+    finish<K, true>(RED_SUM, gridDim.x, partials, out, dout, sm, seq);
     if (threadIdx.x == 0) *ticket = 0u;
   }
 }
 
 // Second launch of the two-launch form: one workgroup sums the nblk
 // partials (slot order) and publishes the K results.
+// one workgroup of 256 threads (1024 measured slower: 10-16 us against
+// 6-10 us for 1024 partials of 4-7 results)
+constexpr int kFinishBlock = kBlock;
 template <int K>
-__global__ __launch_bounds__(kBlock) void k_finish(int kind, int nblk, const double* partials,
-                                                   double* out, double* dout, unsigned seq,
-                                                   const int* skip) {
+__global__ __launch_bounds__(kFinishBlock) void k_finish(int kind, int nblk,
+                                                         const double* partials, double* out,
+                                                         double* dout, unsigned seq,
+                                                         const int* skip) {
   if (skip && *skip) return;
-  __shared__ double sm[K][kWaves];
-  finish<K>(kind, nblk, partials, out, dout, sm, seq);
+  __shared__ double sm[K][kFinishBlock / 64];
+  finish<K, false, kFinishBlock / 64>(kind, nblk, partials, out, dout, sm, seq);
 }
 
 // dout: the device copy of the results (slot 1: the workspace's dres)
 template <int K>
 void launch_finish(hipStream_t s, int kind, int nblk, const RedWs* w, unsigned seq,
                    int slot = 0, const int* skip = nullptr, double* dout = nullptr) {
-  hipLaunchKernelGGL(k_finish<K>, dim3(1), dim3(kBlock), 0, s, kind, nblk, w->partials,
+  hipLaunchKernelGGL(k_finish<K>, dim3(1), dim3(kFinishBlock), 0, s, kind, nblk, w->partials,
                      w->host_dev + slot * kSlot, dout ? dout : (slot ? w->dres : nullptr), seq,
                      skip);
 }
@@ -650,6 +774,46 @@ void wait_result(hipStream_t s, RedWs* w, unsigned seq) {
     }
   }
   hipStreamSynchronize(s);
+}
+
+// One thread waits until the host releases gate `expected` (or `limit`
+// ticks of the 100 MHz wall clock pass: then it stops); then the lanes copy
+// the coefficients to device memory, one each, so the reads of host memory
+// overlap, and the skip flag that the kernels queued behind the gate test
+// is set.  Every path ends the kernel.
+__global__ __launch_bounds__(64) void k_gate(const GateBox* box, unsigned expected,
+                                              double* dcoef, int* skip,
+                                              unsigned long long limit) {
+  __shared__ int stop_s;
+  if (threadIdx.x == 0) {
+    const unsigned long long t0 = wall_clock64();
+    int stop = 1;
+    for (;;) {
+      const unsigned q =
+          __hip_atomic_load(&box->seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+      if (q == expected) {
+        stop = __hip_atomic_load(&box->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        break;
+      }
+      if (wall_clock64() - t0 > limit) break;
+      __builtin_amdgcn_s_sleep(2);
+    }
+    stop_s = stop;
+  }
+  __syncthreads();
+  const int stop = stop_s;
+  if (!stop && threadIdx.x < kMaxK)
+    dcoef[threadIdx.x] =
+        __hip_atomic_load(&box->coef[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  if (threadIdx.x == 0) *skip = stop;
+}
+
+// wait_result that reports whether the result arrived (a finish launch
+// skipped behind a stopped gate never publishes)
+bool wait_result_checked(hipStream_t s, RedWs* w, unsigned seq) {
+  wait_result(s, w, seq);
+  const unsigned* word = reinterpret_cast<const unsigned*>(w->host + kMaxK);
+  return __atomic_load_n(word, __ATOMIC_ACQUIRE) == seq;
 }
 
 template <class F>
@@ -1105,6 +1269,54 @@ int NVHip_Pascal(int q, N_Vector* zn, sunrealtype sign) {
   return 0;
 }
 
+int NVHip_PredictInit(int q, N_Vector* zn, N_Vector acor, N_Vector y) {
+  if (q < 1 || q > 5 || !acor || !y) return -1;
+  for (int j = 0; j <= q; j++)
+    if (!zn[j] || zn[j]->ops->nvgetvectorid != N_VGetVectorID_Hip) return -1;
+  if (acor->ops->nvgetvectorid != N_VGetVectorID_Hip ||
+      y->ops->nvgetvectorid != N_VGetVectorID_Hip)
+    return -1;
+  const sunindextype n = LEN(zn[0]);
+  if (LEN(acor) != n || LEN(y) != n) return -1;
+  for (int j = 0; j <= q; j++)
+    if (D(zn[j]) == D(acor) || D(zn[j]) == D(y)) return -1;
+  if (D(acor) == D(y)) return -1;
+  OutPack zp{};
+  for (int j = 0; j <= q; j++) zp.p[j] = D(zn[j]);
+  hipStream_t s = S(zn[0]);
+  const dim3 g(ew_grid(n)), b(kBlock);
+  switch (q) {
+    case 1: hipLaunchKernelGGL(k_pascal_init<1>, g, b, 0, s, zp, D(acor), D(y), n); break;
+    case 2: hipLaunchKernelGGL(k_pascal_init<2>, g, b, 0, s, zp, D(acor), D(y), n); break;
+    case 3: hipLaunchKernelGGL(k_pascal_init<3>, g, b, 0, s, zp, D(acor), D(y), n); break;
+    case 4: hipLaunchKernelGGL(k_pascal_init<4>, g, b, 0, s, zp, D(acor), D(y), n); break;
+    default: hipLaunchKernelGGL(k_pascal_init<5>, g, b, 0, s, zp, D(acor), D(y), n); break;
+  }
+  check_launch("pascal_init");
+  return 0;
+}
+
+int NVHip_WrmsNormPair(N_Vector x, sunrealtype a, N_Vector y, sunrealtype b, N_Vector z,
+                       N_Vector w, sunrealtype* nrm) {
+  if (!x || !y || !z || !w || !nrm) return -1;
+  N_Vector vs[4] = {x, y, z, w};
+  for (N_Vector v : vs)
+    if (v->ops->nvgetvectorid != N_VGetVectorID_Hip || LEN(v) != LEN(x)) return -1;
+  const sunindextype n = LEN(x);
+  hipStream_t s = S(x);
+  RedWs* ws = workspace(s);
+  const int g = red_grid(n);
+  const unsigned seq = ++ws->seq;
+  hipLaunchKernelGGL((k_reduce2<FWsq, FLinWsq>), dim3(g), dim3(kBlock), 0, s, RED_SUM,
+                     FWsq{D(x), D(w)}, FLinWsq{a, b, D(y), D(z), D(w)}, n, ws->partials);
+  finish_dispatch(2, s, RED_SUM, g, ws, seq);
+  check_launch("wrms pair");
+  wait_result(s, ws, seq);
+  nrm[0] = std::sqrt(ws->host[0] / (double)n);
+  nrm[1] = std::sqrt(ws->host[1] / (double)n);
+  return 0;
+}
+
 int NVHip_EwtSet(N_Vector y, sunrealtype rtol, sunrealtype atol, N_Vector abstol, N_Vector w) {
   if (y->ops->nvgetvectorid != N_VGetVectorID_Hip) return -1;
   EW_LAUNCH(k_ewt, w, D(y), rtol, atol, abstol ? (const double*)D(abstol) : nullptr, D(w), LEN(w));
@@ -1520,6 +1732,135 @@ int NVHip_GSProjectScaled(N_Vector z, int nvec, N_Vector* Y, int sidx, sunrealty
   return gs_project(nullptr, nullptr, z, nvec, Y, sidx, sc, dots, zz);
 }
 
+static bool is_hip(N_Vector v) {
+  return v->ops && v->ops->nvgetvectorid == N_VGetVectorID_Hip;
+}
+
+double* NVHip_ReducePartials(N_Vector v, int* capacity) {
+  if (!v || !capacity || !is_hip(v)) return nullptr;
+  RedWs* w = workspace(S(v));
+  *capacity = kMaxRedBlocks * kMaxK;
+  return w->partials;
+}
+
+int NVHip_ReduceFinish(N_Vector v, int K, int nblk, sunrealtype* out) {
+  if (!v || !out || K < 1 || K > kMaxK || nblk < 1 || nblk > kMaxRedBlocks * kMaxK / K ||
+      !is_hip(v))
+    return -1;
+  hipStream_t s = S(v);
+  RedWs* w = workspace(s);
+  const unsigned seq = ++w->seq;
+  finish_dispatch(K, s, RED_SUM, nblk, w, seq);
+  check_launch("finish");
+  wait_result(s, w, seq);
+  for (int k = 0; k < K; k++) out[k] = w->host[k];
+  return 0;
+}
+
+int NVHip_ReduceFinishAsync(N_Vector v, int K, int nblk, const int* skip_dev,
+                            unsigned* ticket) {
+  if (!v || !ticket || K < 1 || K > kMaxK || nblk < 1 || nblk > kMaxRedBlocks * kMaxK / K ||
+      !is_hip(v))
+    return -1;
+  hipStream_t s = S(v);
+  RedWs* w = workspace(s);
+  const unsigned seq = ++w->seq;
+  switch (K) {
+#define NVHIP_FIN_CASE(k) \
+    case k: launch_finish<k>(s, RED_SUM, nblk, w, seq, 0, skip_dev); break;
+    NVHIP_FIN_CASE(1) NVHIP_FIN_CASE(2) NVHIP_FIN_CASE(3) NVHIP_FIN_CASE(4)
+    NVHIP_FIN_CASE(5) NVHIP_FIN_CASE(6) NVHIP_FIN_CASE(7) NVHIP_FIN_CASE(8)
+    NVHIP_FIN_CASE(9) NVHIP_FIN_CASE(10) NVHIP_FIN_CASE(11) NVHIP_FIN_CASE(12)
+    NVHIP_FIN_CASE(13) NVHIP_FIN_CASE(14) NVHIP_FIN_CASE(15)
+    default: launch_finish<16>(s, RED_SUM, nblk, w, seq, 0, skip_dev); break;
+#undef NVHIP_FIN_CASE
+  }
+  check_launch("finish");
+  *ticket = seq;
+  return 0;
+}
+
+int NVHip_ReduceWait(N_Vector v, unsigned ticket, int K, sunrealtype* out) {
+  if (!v || !out || K < 1 || K > kMaxK || !is_hip(v)) return -1;
+  hipStream_t s = S(v);
+  RedWs* w = workspace(s);
+  if (!wait_result_checked(s, w, ticket)) return -1;
+  for (int k = 0; k < K; k++) out[k] = w->host[k];
+  return 0;
+}
+
+int NVHip_GateOpen(N_Vector v, const double** coef_dev, const int** skip_dev) {
+  if (!v || !coef_dev || !skip_dev || !is_hip(v) || g_sync_mode == 0) return -1;
+  hipStream_t s = S(v);
+  RedWs* w = workspace(s);
+  if (w->gopen) return -1;
+  if (!w->gbox) {
+    void* hb = nullptr;
+    if (hipHostMalloc(&hb, sizeof(GateBox), hipHostMallocMapped | hipHostMallocCoherent) !=
+        hipSuccess)
+      return -1;
+    std::memset(hb, 0, sizeof(GateBox));
+    if (hipMalloc(&w->gcoef, sizeof(double) * kMaxK) != hipSuccess ||
+        hipMalloc(&w->gskip, sizeof(int)) != hipSuccess) {
+      hipHostFree(hb);
+      hipFree(w->gcoef);
+      w->gcoef = nullptr;
+      return -1;
+    }
+    if (hipHostGetDevicePointer((void**)&w->gbox_dev, hb, 0) != hipSuccess) {
+      hipHostFree(hb);
+      hipFree(w->gcoef);
+      hipFree(w->gskip);
+      w->gcoef = nullptr;
+      w->gskip = nullptr;
+      return -1;
+    }
+    w->gbox = static_cast<GateBox*>(hb);
+  }
+  const GateBox* dbox = w->gbox_dev;
+  const unsigned expected = ++w->gseq;
+  // a released gate is at most microseconds late; after a second the
+  // kernels behind it are skipped and the host sees their results missing
+  const unsigned long long limit = 100000000ull;   // 1 s at 100 MHz
+  hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, s, dbox, expected, w->gcoef,
+                     w->gskip, limit);
+  check_launch("gate");
+  w->gopen = true;
+  *coef_dev = w->gcoef;
+  *skip_dev = w->gskip;
+  return 0;
+}
+
+int NVHip_GateRelease(N_Vector v, int stop, const sunrealtype* coef, int n) {
+  if (!v || !is_hip(v) || n < 0 || n > kMaxK || (n > 0 && !coef)) return -1;
+  RedWs* w = workspace(S(v));
+  if (!w->gopen) return -1;
+  GateBox* b = w->gbox;
+  for (int k = 0; k < n; k++) b->coef[k] = coef[k];
+  b->stop = stop ? 1 : 0;
+  __atomic_store_n(&b->seq, w->gseq, __ATOMIC_RELEASE);
+  w->gopen = false;
+  return 0;
+}
+
+int NVHip_ProjNorm(int nb, sunrealtype a, const sunrealtype* b, N_Vector w, N_Vector* U,
+                   N_Vector uout, sunrealtype* nn) {
+  if (!w || !nn || nb < 0 || nb > kMaxK || (nb > 0 && (!U || !b))) return -1;
+  FProjSq f{};
+  f.nb = nb;
+  f.a = a;
+  f.w = D(w);
+  for (int j = 0; j < nb; j++) {
+    if (!U[j] || LEN(U[j]) != LEN(w) || U[j] == uout) return -1;
+    f.b.c[j] = b[j];
+    f.U.p[j] = D(U[j]);
+  }
+  if (uout && (uout == w || LEN(uout) != LEN(w))) return -1;
+  f.out = uout ? D(uout) : nullptr;
+  *nn = reduce1(w, RED_SUM, f);
+  return 0;
+}
+
 int NVHip_LinCombDot(int nvec, sunrealtype* c, N_Vector* X, N_Vector z, sunrealtype* zz) {
   if (nvec < 1 || nvec > kMaxK || !z || !zz) return -1;
   FLincombSq f{};
@@ -1546,6 +1887,11 @@ void FusedNVec_FreePool(void) {
     hipFree(kv.second->partials);
     hipFree(kv.second->ticket);
     hipHostFree(kv.second->host);
+    if (kv.second->gbox) {
+      hipHostFree(kv.second->gbox);
+      hipFree(kv.second->gcoef);
+      hipFree(kv.second->gskip);
+    }
     delete kv.second;
   }
   g_ws.clear();

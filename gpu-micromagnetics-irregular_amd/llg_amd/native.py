@@ -158,6 +158,8 @@ SIGNATURES = {
     "N_VWrmsNormMaskVectorArray_Hip": (C.c_int, [C.c_int, C.POINTER(VP), C.POINTER(VP), VP, DP]),
     "NVHip_Pascal": (C.c_int, [C.c_int, C.POINTER(VP), C.c_double]),
     "NVHip_EwtSet": (C.c_int, [VP, C.c_double, C.c_double, VP, VP]),
+    "NVHip_PredictInit": (C.c_int, [C.c_int, C.POINTER(VP), VP, VP]),
+    "NVHip_WrmsNormPair": (C.c_int, [VP, C.c_double, VP, C.c_double, VP, VP, DP]),
     "NVHip_ScaleDiv": (C.c_int, [C.c_double, VP, VP, VP, VP]),
     "NVHip_ProdDotMulti": (C.c_int, [VP, VP, VP, C.c_int, C.POINTER(VP), DP]),
     "NVHip_LinCombDot": (C.c_int, [C.c_int, DP, C.POINTER(VP), VP, DP]),
@@ -166,6 +168,13 @@ SIGNATURES = {
     "NVHip_SpgmrSolve": (C.c_int, [VP, VP, VP, C.POINTER(VP), C.c_int, VP, VP, C.c_double, IP, IP,
                                    DP, DP]),
     "NVHip_LinCombDiv": (C.c_int, [C.c_int, DP, C.POINTER(VP), VP, VP]),
+    "NVHip_ReducePartials": (DP, [VP, IP]),
+    "NVHip_ReduceFinish": (C.c_int, [VP, C.c_int, C.c_int, DP]),
+    "NVHip_ReduceFinishAsync": (C.c_int, [VP, C.c_int, C.c_int, VP, C.POINTER(C.c_uint)]),
+    "NVHip_ReduceWait": (C.c_int, [VP, C.c_uint, C.c_int, DP]),
+    "NVHip_GateOpen": (C.c_int, [VP, C.POINTER(VP), C.POINTER(VP)]),
+    "NVHip_GateRelease": (C.c_int, [VP, C.c_int, DP, C.c_int]),
+    "NVHip_ProjNorm": (C.c_int, [C.c_int, C.c_double, DP, VP, C.POINTER(VP), VP, DP]),
     "NVHip_NewtonUpdate": (C.c_int, [C.c_int, VP, VP, VP, VP, VP, DP, DP]),
     "FusedNVec_Init": (None, [VP]),
     "FusedNVec_FreePool": (None, []),
@@ -214,6 +223,11 @@ SIGNATURES = {
     "LLG_Krylov": (C.c_int, [C.c_double, VP, VP, VP, C.c_double, C.c_double, VP, VP, VP]),
     "LLG_SpgmrSolve": (C.c_int, [VP, C.POINTER(VP), C.c_int, VP, VP, C.c_double, C.c_double,
                                  C.c_double, VP, VP, VP, IP, IP, DP, DP]),
+    "LLG_Arnoldi": (C.c_int, [C.c_int, C.c_int, C.c_double, DP, VP, C.POINTER(VP), VP, VP,
+                              C.c_double, C.c_double, VP, VP, VP, DP]),
+    "LLG_ArnoldiQueue": (C.c_int, [C.c_int, C.c_int, C.c_double, DP, VP, VP, VP, C.POINTER(VP),
+                                   VP, VP, C.c_double, C.c_double, VP, VP, VP,
+                                   C.POINTER(C.c_uint)]),
     "PrecondSetup": (C.c_int, [C.c_double, VP, VP, C.c_int, C.POINTER(C.c_int), C.c_double, VP]),
     "PrecondSolve": (C.c_int, [C.c_double, VP, VP, VP, VP, C.c_double, C.c_double, C.c_int, VP]),
     "Precond_Create": (VP, [C.c_int, C.c_int, C.c_int]),

@@ -107,6 +107,33 @@ typedef struct llgcv_fused_ops {
   int (*spgmr)(N_Vector t0, N_Vector* V, int maxl, N_Vector s, N_Vector x, sunrealtype delta,
                sunrealtype gamma, sunrealtype t, N_Vector y, N_Vector fy, void* user_data,
                int* status, int* iters, sunrealtype* r_norm, sunrealtype* resnorm);
+  /* SPGMR with lagged normalisation (left preconditioning, classical GS,
+   * maxl <= 8): one pass and one host wait per iteration, the operator
+   * applied to the unnormalised projection of the previous step (as
+   * LLG_Arnoldi), the last projection norm by projnorm (as NVHip_ProjNorm).
+   * Both must be set; any failure runs the standard solve instead. */
+  int (*arnoldi)(int l, int pre, sunrealtype a, const sunrealtype* b, N_Vector win,
+                 N_Vector* U, N_Vector wout, N_Vector s, sunrealtype gamma, sunrealtype t,
+                 N_Vector y, N_Vector fy, void* user_data, sunrealtype* r);
+  int (*projnorm)(int nb, sunrealtype a, const sunrealtype* b, N_Vector w, N_Vector* U,
+                  N_Vector uout, sunrealtype* nn);
+  /* the lagged-normalisation solve pipelined (all four needed): a pass
+   * queued without waiting, with its coefficients and stop flag read from
+   * device memory when given (as LLG_ArnoldiQueue), the wait for its sums
+   * (as NVHip_ReduceWait), and a gate that holds the queued pass until the
+   * host publishes its coefficients (as NVHip_GateOpen/Release) */
+  int (*arnoldi_queue)(int l, int pre, sunrealtype a, const sunrealtype* b,
+                       const double* coef_dev, const int* skip_dev, N_Vector win, N_Vector* U,
+                       N_Vector wout, N_Vector s, sunrealtype gamma, sunrealtype t, N_Vector y,
+                       N_Vector fy, void* user_data, unsigned* ticket);
+  int (*result_wait)(N_Vector v, unsigned ticket, int K, sunrealtype* r);
+  int (*gate_open)(N_Vector v, const double** coef_dev, const int** skip_dev);
+  int (*gate_release)(N_Vector v, int stop, const sunrealtype* coef, int n);
+  /* predictor + acor = 0 + y = zn[0] + acor in one pass (as NVHip_PredictInit) */
+  int (*predict_init)(int q, N_Vector* zn, N_Vector acor, N_Vector y);
+  /* nrm[0] = |x|_WRMS(w), nrm[1] = |a y + b z|_WRMS(w) (as NVHip_WrmsNormPair) */
+  int (*wrms_pair)(N_Vector x, sunrealtype a, N_Vector y, sunrealtype b, N_Vector z,
+                   N_Vector w, sunrealtype* nrm);
 } llgcv_fused_ops;
 
 void* LLGCVodeCreate(int lmm, SUNContext sunctx);

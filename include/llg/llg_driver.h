@@ -34,8 +34,18 @@ typedef struct llg_run_opts {
                          (LLG_SpgmrSolve; it halves the host waits but
                          queues every iteration up to the Krylov
                          dimension, which costs more than the waits on
-                         the 1536x512 workload); default 7.  Bits 1 to 3
-                         change no value. */
+                         the 1536x512 workload); bit 4 (with bits 1 and 2,
+                         classical GS, maxl <= 8, dense grid): SPGMR with
+                         lagged normalisation, one pass and one host wait
+                         per iteration (LLG_Arnoldi; same method, other
+                         rounding); bit 5 (with bit 4, sync mode 1 or 2):
+                         each pass of that solve queued behind a gate
+                         (NVHip_GateOpen) before the previous pass's sums
+                         are back, taking the launch off the host round
+                         trip (on the 1536x512 workload it measures level
+                         with bit 4 alone: the gate adds a kernel boundary
+                         per iteration); default 23 (bits 0, 1, 2, 4).
+                         Bits 1 to 3 change no value. */
   long   mxstep;      /* max steps per CVode call (CVODE default 500) */
   int    use_jtv;     /* analytic Jv (else difference quotient) */
   int    use_prec;    /* block-Jacobi preconditioner */

@@ -172,6 +172,36 @@ int LLG_SpgmrSolve(N_Vector t0, N_Vector* V, int maxl, N_Vector s, N_Vector x,
                    sunrealtype delta, sunrealtype gamma, sunrealtype t, N_Vector y, N_Vector fy,
                    void* user_data, int* status, int* iters, sunrealtype* r_norm,
                    sunrealtype* resnorm);
+/* One iteration of left-preconditioned SPGMR with the normalisation of the
+ * basis vector left to the caller (A = s .* P^{-1}(I - gamma J(y)) (. / s)
+ * is linear, so A(u/|u|) = (A u)/|u|), in one pass on the dense grid:
+ *   u = a win - sum_{j<l} b_j U[j]   (l > 0; the classical Gram-Schmidt
+ *                                     projection of the previous step),
+ *     = s .* P^{-1} win              (l == 0, pre != 0: the first
+ *                                     preconditioner solve and scaling),
+ *     = U[0]                         (l == 0, pre == 0);
+ *   u written to U[l] (l > 0 or pre), wout = A u, and one host wait for
+ *   r[0] = |u|^2, r[1+j] = <wout, U[j]> (j < l), r[1+l] = <wout, u>,
+ *   r[2+l] = |wout|^2, and with pre r[3] = sum_i (win_i s_i)^2 (the square
+ *   of |win|_WRMS(s) times the length, the right-hand-side norm CVLS checks).
+ * l < 8.  U[l] and wout may not alias win, s, y or each other.  Returns -1
+ * without launching on compact execution, with a learned correction
+ * active, or on aliasing.  No reference counterpart: installed as
+ * llgcv_fused_ops.arnoldi by the driver (SUNDIALS' SPGMR waits for the
+ * multi-dot and for the norm separately). */
+int LLG_Arnoldi(int l, int pre, sunrealtype a, const sunrealtype* b, N_Vector win, N_Vector* U,
+                N_Vector wout, N_Vector s, sunrealtype gamma, sunrealtype t, N_Vector y,
+                N_Vector fy, void* user_data, sunrealtype* r);
+/* LLG_Arnoldi queued without waiting: *ticket for NVHip_ReduceWait(wout,
+ * ticket, l + 3 + (pre != 0), r).  With coef_dev, a and b are read from
+ * device memory (coef_dev[0] = a, coef_dev[1 + j] = b_j) when the kernel
+ * runs; with skip_dev, the pass and its finish exit at once when
+ * *skip_dev is nonzero (the coefficients and the stop flag of an
+ * NVHip_GateOpen gate).  No reference counterpart. */
+int LLG_ArnoldiQueue(int l, int pre, sunrealtype a, const sunrealtype* b, const double* coef_dev,
+                     const int* skip_dev, N_Vector win, N_Vector* U, N_Vector wout, N_Vector s,
+                     sunrealtype gamma, sunrealtype t, N_Vector y, N_Vector fy, void* user_data,
+                     unsigned* ticket);
 int PrecondSetup(sunrealtype t, N_Vector y, N_Vector fy, sunbooleantype jok,
                  sunbooleantype* jcurPtr, sunrealtype gamma, void* user_data);
 int PrecondSolve(sunrealtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z,

@@ -113,6 +113,15 @@ SUNErrCode N_VWrmsNormMaskVectorArray_Hip(int nvec, N_Vector* X, N_Vector* W, N_
  * N_VInv sequences. */
 int NVHip_Pascal(int q, N_Vector* zn, sunrealtype sign);
 int NVHip_EwtSet(N_Vector y, sunrealtype rtol, sunrealtype atol, N_Vector abstol, N_Vector w);
+/* The predictor (NVHip_Pascal with sign +1) and the start of the Newton
+ * iteration in one pass: acor = 0 (N_VConst) and y = zn[0] + acor
+ * (N_VLinearSum(1, zn[0], 1, acor, y)).  acor and y may not alias zn. */
+int NVHip_PredictInit(int q, N_Vector* zn, N_Vector acor, N_Vector y);
+/* Two WRMS norms with one host wait: nrm[0] = |x|_WRMS(w) and nrm[1] =
+ * |a y + b z|_WRMS(w), the combination not written; each equal to
+ * N_VWrmsNorm (after N_VLinearSum for the second) bit for bit. */
+int NVHip_WrmsNormPair(N_Vector x, sunrealtype a, N_Vector y, sunrealtype b, N_Vector z,
+                       N_Vector w, sunrealtype* nrm);
 
 /* One-pass kernels for the SPGMR loop (llgcv_fused_ops), each equal
  * element for element, and reduction for reduction, to the N_Vector
@@ -183,6 +192,40 @@ typedef int (*NVHipDevOp)(const double* c_dev, const int* skip_dev, N_Vector vl,
 int NVHip_SpgmrSolve(NVHipDevOp op, void* op_data, N_Vector t0, N_Vector* V, int maxl,
                      N_Vector sv, N_Vector x, sunrealtype delta, int* status, int* iters,
                      sunrealtype* r_norm, sunrealtype* resnorm);
+
+/* Reductions for kernels outside this file (no counterpart in the
+ * reference): a kernel queued on v's stream leaves K partial sums per
+ * workgroup at partials[b*K + k] (b*K + k < *capacity, from
+ * NVHip_ReducePartials); NVHip_ReduceFinish sums them over b < nblk in
+ * workgroup order (deterministic), waits once and copies the K results to
+ * out.  K <= NVHIP_FUSED_MAX.  Returns NULL / -1 for a non-HIP vector or
+ * sizes beyond the capacity. */
+double* NVHip_ReducePartials(N_Vector v, int* capacity);
+int NVHip_ReduceFinish(N_Vector v, int K, int nblk, sunrealtype* out);
+/* The same finish queued without waiting (skipped when *skip_dev is
+ * nonzero on the device; skip_dev may be NULL); NVHip_ReduceWait waits for
+ * it and copies the K results, or returns -1 when they never arrive (the
+ * finish was skipped).  No other reduction of the stream may be waited for
+ * in between. */
+int NVHip_ReduceFinishAsync(N_Vector v, int K, int nblk, const int* skip_dev,
+                            unsigned* ticket);
+int NVHip_ReduceWait(N_Vector v, unsigned ticket, int K, sunrealtype* out);
+/* A gate on v's stream for work queued before its scalar inputs are known
+ * (no counterpart in the reference; it takes the kernel launch out of the
+ * host round trip of an iterative solve).  NVHip_GateOpen queues a
+ * one-thread kernel that waits until NVHip_GateRelease publishes up to 16
+ * coefficients (copied to *coef_dev) or stop (*skip_dev = 1, which the
+ * kernels queued behind it test to exit at once).  The wait is bounded
+ * (1 s, then stop).  One gate per stream at a time; every opened gate must
+ * be released, and nothing may synchronise the stream while one is open.
+ * Requires NVHip_SetSyncMode 1 or 2.  Return 0 or -1. */
+int NVHip_GateOpen(N_Vector v, const double** coef_dev, const int** skip_dev);
+int NVHip_GateRelease(N_Vector v, int stop, const sunrealtype* coef, int n);
+/* The projection norm of a classical Gram-Schmidt step with host
+ * coefficients: *nn = |u|^2, u = a w - sum_{j<nb} b_j U_j, u written to
+ * uout when uout is not NULL (uout may not alias w or any U_j). */
+int NVHip_ProjNorm(int nb, sunrealtype a, const sunrealtype* b, N_Vector w, N_Vector* U,
+                   N_Vector uout, sunrealtype* nn);
 
 /* Fused-ops layer with the reference's entry-point names.
  * FusedNVec_Init installs the one-launch multi-dot and linear-combination

@@ -59,9 +59,14 @@ def test_fused_integrator_kernels_are_bitwise_neutral(gstype, case):
         assert np.array_equal(ya, yb), fused
 
 
+@pytest.mark.parametrize("fused", [7, 23, 55])
 @pytest.mark.parametrize("name,kw", [("square_hole", {}), ("i6", {"demag": 4.0}),
                                      ("plain128", {})])
-def test_trajectory_matches_oracle(name, kw):
+def test_trajectory_matches_oracle(name, kw, fused):
+    """fused 7: the standard SPGMR loop with the one-pass kernels (bit-equal
+    to the plain op sequence); 23: SPGMR with lagged normalisation
+    (LLG_Arnoldi; the compact i6 grid falls back to the standard loop);
+    55: the same pipelined, each pass queued behind a gate (polling waits)"""
     ng, ny = (48, 32) if name != "plain128" else (32, 32)
     cfg = la.preset(name, ng=ng, ny=ny, **kw)
     if name == "plain128":
@@ -74,8 +79,13 @@ def test_trajectory_matches_oracle(name, kw):
         t_end = 1.0
     rtol = atol = 1e-6
     sim = la.Simulation(cfg, geom)
-    run = sim.integrator(y0, rtol=rtol, atol=atol, t_out=0.1)
-    out = run.advance(t_end)
+    L = la.lib()
+    L.NVHip_SetSyncMode(1 if fused == 55 else 0)
+    try:
+        run = sim.integrator(y0, rtol=rtol, atol=atol, t_out=0.1, fused=fused)
+        out = run.advance(t_end)
+    finally:
+        L.NVHip_SetSyncMode(0)
     y_gpu = run.state()
     y_cpu, st_cpu, _ = orc.cpu_integrate(la.lib(), cfg, geom, y0, t_end, t_out=0.1,
                                          rtol=rtol, atol=atol, mxstep=5000)
@@ -83,6 +93,7 @@ def test_trajectory_matches_oracle(name, kw):
     act = geom.active.astype(bool)
     m = y_gpu.reshape(3, -1)[:, act]
     drift = np.max(np.abs(np.sqrt((m ** 2).sum(0)) - 1.0))
-    print(f"[gpu] {name}: max|dm| {dm:.2e} (nst gpu {out['nst']} cpu {st_cpu['nst']}), "
+    print(f"[gpu] {name} fused {fused}: max|dm| {dm:.2e} (nst gpu {out['nst']} "
+          f"cpu {st_cpu['nst']}, nli gpu {out['nli']} cpu {st_cpu['nli']}), "
           f"| |m|-1 | max {drift:.2e}")
     assert dm < 1e-5

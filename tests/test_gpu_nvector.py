@@ -381,3 +381,49 @@ def test_newton_update_equals_linear_sums_and_norms(it):
     assert dn.value == dna
     if it > 0:
         assert yn.value == yna
+
+
+@pytest.mark.parametrize("q", [1, 3, 5])
+def test_predict_init_equals_pascal_const_linear_sum(q):
+    """NVHip_PredictInit == NVHip_Pascal(+1), N_VConst(0, acor) and
+    N_VLinearSum(1, zn[0], 1, acor, y), bit for bit (a -0 in zn[0] gives
+    +0 in y, as the sum does)"""
+    L = la.lib()
+    n = 30_001
+    rng = np.random.default_rng(60 + q)
+    Z = [rng.standard_normal(n) for _ in range(q + 1)]
+    Z[0][:7] = -0.0
+    for j in range(1, q + 1):
+        Z[j][:7] = 0.0
+    za, zb = [vec(a) for a in Z], [vec(a) for a in Z]
+    acor_a, y_a = vec(rng.standard_normal(n)), vec(rng.standard_normal(n))
+    acor_b, y_b = vec(rng.standard_normal(n)), vec(rng.standard_normal(n))
+    assert L.NVHip_Pascal(q, arr(za), 1.0) == 0
+    L.N_VConst_Hip(0.0, acor_a.v)
+    L.N_VLinearSum_Hip(1.0, za[0].v, 1.0, acor_a.v, y_a.v)
+    assert L.NVHip_PredictInit(q, arr(zb), acor_b.v, y_b.v) == 0
+    for j in range(q + 1):
+        assert np.array_equal(za[j].get(), zb[j].get()), j
+    assert np.array_equal(acor_a.get(), acor_b.get())
+    ya, yb = y_a.get(), y_b.get()
+    assert np.array_equal(ya, yb) and np.array_equal(np.signbit(ya), np.signbit(yb))
+    assert L.NVHip_PredictInit(q, arr(zb), zb[0].v, y_b.v) == -1   # acor aliases zn[0]
+
+
+def test_wrms_norm_pair_equals_two_norms():
+    """NVHip_WrmsNormPair == N_VWrmsNorm(x, w) and N_VLinearSum(a, y, b, z,
+    t) + N_VWrmsNorm(t, w), bit for bit, with one host wait"""
+    L = la.lib()
+    rng = np.random.default_rng(70)
+    x, yv, zv = _vecs(rng, 3)
+    w, = _vecs(rng, 1, lo=0.5)
+    t = vec(np.zeros(N_FUSED))
+    a, b = -0.734, 1.0
+    n1 = L.N_VWrmsNorm_Hip(x.v, w.v)
+    L.N_VLinearSum_Hip(a, yv.v, b, zv.v, t.v)
+    n2 = L.N_VWrmsNorm_Hip(t.v, w.v)
+    nr = (C.c_double * 2)()
+    s0 = L.NVHip_NumSyncs()
+    assert L.NVHip_WrmsNormPair(x.v, a, yv.v, b, zv.v, w.v, nr) == 0
+    assert L.NVHip_NumSyncs() - s0 == 1
+    assert nr[0] == n1 and nr[1] == n2

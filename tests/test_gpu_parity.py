@@ -229,6 +229,151 @@ def test_krylov_chain_equals_separate_calls(case):
     print(f"[gpu] {case[0]}: |vk|^2 after projection {zb.value:.6e}")
 
 
+def _dot_ok(got, a, b, tol=1e-13):
+    """|got - a.b| within tol of the Cauchy-Schwarz scale |a| |b|"""
+    ref = float(a @ b)
+    return abs(got - ref) <= tol * np.linalg.norm(a) * np.linalg.norm(b), (got, ref)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_arnoldi_step_equals_separate_calls(case):
+    """LLG_Arnoldi, one SPGMR iteration with lagged normalisation in one pass:
+    the first step's u_0 equals PrecondSolve + N_VProd and every step's u
+    equals NVHip_ProjNorm's combination, bit for bit; w = A u equals
+    LLG_Krylov(1, u) bit for bit; the sums |u|^2, <w, U_j>, <w, u>, |w|^2
+    (and the first step's sum (b s)^2) agree with numpy to 1e-13 of |a| |b|;
+    compact execution and aliasing
+    are refused without a launch"""
+    cfg, geom, y0 = make(case)
+    print(f"[gpu] case {case[0]} {cfg.ng}x{cfg.ny}", flush=True)
+    sim = la.Simulation(cfg, geom)
+    L = la.lib()
+    rng = np.random.default_rng(8)
+    n = sim.neq
+    y = perturbed(y0, geom.active, 4)
+    sim.rhs(y)
+    gamma = 0.0371
+    sim.psetup(y, gamma)
+    vy = sim.vec(y)
+    s = sim.vec(1.0 / (1e-4 * np.abs(rng.standard_normal(n)) + 1e-4))
+    vb = sim.vec(rng.standard_normal(n) * np.tile(geom.active.astype(float), 3))
+    nl = 5
+    U = [sim.vec() for _ in range(nl + 1)]
+    Up = (C.c_void_p * (nl + 1))(*[u.v for u in U])
+    w = [sim.vec(), sim.vec()]
+    r = (C.c_double * 16)()
+    rc = L.LLG_Arnoldi(0, 1, 1.0, None, vb.v, Up, w[0].v, s.v, gamma, 0.0, vy.v, None, sim.ctx,
+                       r)
+    if cfg.exec_ == la.EXEC_COMPACT:
+        assert rc == -1
+        return
+    assert rc == 0
+    t, u0, wk = sim.vec(), sim.vec(), sim.vec()
+    assert L.PrecondSolve(0.0, vy.v, None, vb.v, t.v, gamma, 0.0, la.PREC_LEFT, sim.ctx) == 0
+    L.N_VProd_Hip(s.v, t.v, u0.v)
+    assert np.array_equal(U[0].get(), u0.get())
+    assert L.LLG_Krylov(1.0, u0.v, s.v, wk.v, gamma, 0.0, vy.v, None, sim.ctx) == 0
+    assert np.array_equal(w[0].get(), wk.get())
+    u, wv = U[0].get(), w[0].get()
+    bs = vb.get() * s.get()
+    for k, (a, b) in enumerate([(u, u), (wv, u), (wv, wv), (bs, bs)]):
+        ok, vals = _dot_ok(r[k], a, b)
+        assert ok, (0, k, vals)
+    # without the preconditioner solve, step 0 applies A to U[0] as it is
+    assert L.LLG_Arnoldi(0, 0, 1.0, None, None, Up, w[1].v, s.v, gamma, 0.0, vy.v, None,
+                         sim.ctx, r) == 0
+    assert np.array_equal(w[1].get(), wk.get())
+    uref = sim.vec()
+    for l in range(1, nl + 1):
+        a = float(rng.uniform(0.5, 2.0))
+        b = (C.c_double * 8)(*rng.uniform(-1.0, 1.0, 8))
+        win, wout = w[(l - 1) % 2], w[l % 2]
+        assert L.LLG_Arnoldi(l, 0, a, b, win.v, Up, win.v, s.v, gamma, 0.0, vy.v, None,
+                             sim.ctx, r) == -1          # wout aliases win
+        assert L.LLG_Arnoldi(l, 0, a, b, win.v, Up, wout.v, s.v, gamma, 0.0, vy.v, None,
+                             sim.ctx, r) == 0
+        nn = C.c_double()
+        assert L.NVHip_ProjNorm(l, a, b, win.v, Up, uref.v, C.byref(nn)) == 0
+        assert np.array_equal(U[l].get(), uref.get()), l
+        assert L.LLG_Krylov(1.0, uref.v, s.v, wk.v, gamma, 0.0, vy.v, None, sim.ctx) == 0
+        assert np.array_equal(wout.get(), wk.get()), l
+        u, wv = U[l].get(), wout.get()
+        pairs = [(u, u)] + [(wv, U[j].get()) for j in range(l)] + [(wv, u), (wv, wv)]
+        for k, (x1, x2) in enumerate(pairs):
+            ok, vals = _dot_ok(r[k], x1, x2)
+            assert ok, (l, k, vals)
+        ok, vals = _dot_ok(nn.value, u, u)
+        assert ok, (l, "projnorm", vals)
+    print(f"[gpu] {case[0]}: step {nl} |u|^2 {r[0]:.6e} |w|^2 {r[nl + 2]:.6e}")
+
+
+def test_gated_arnoldi_step_equals_direct():
+    """LLG_ArnoldiQueue behind an NVHip_GateOpen gate, released with the
+    coefficients, gives LLG_Arnoldi's vectors and sums bit for bit; released
+    with stop, the pass and its finish are skipped (the wait reports the
+    missing result); a gate the host does not release in time (1 s) stops
+    by itself; the stream then works as before"""
+    L = la.lib()
+    L.NVHip_SetSyncMode(1)   # gates need the polling wait
+    try:
+        _gated_arnoldi_checks(L)
+    finally:
+        L.NVHip_SetSyncMode(0)
+
+
+def _gated_arnoldi_checks(L):
+    cfg, geom, y0 = make(CASES[1])
+    sim = la.Simulation(cfg, geom)
+    rng = np.random.default_rng(9)
+    n = sim.neq
+    y = perturbed(y0, geom.active, 4)
+    sim.rhs(y)
+    gamma = 0.0371
+    sim.psetup(y, gamma)
+    vy = sim.vec(y)
+    s = sim.vec(1.0 / (1e-4 * np.abs(rng.standard_normal(n)) + 1e-4))
+    U = [sim.vec(rng.standard_normal(n)) for _ in range(3)]
+    Up = (C.c_void_p * 3)(*[u.v for u in U])
+    win = sim.vec(rng.standard_normal(n))
+    wa, wb = sim.vec(), sim.vec()
+    l, a = 2, 0.8125
+    b = (C.c_double * 8)(0.25, -0.375)
+    ra, rb = (C.c_double * 16)(), (C.c_double * 16)()
+    assert L.LLG_Arnoldi(l, 0, a, b, win.v, Up, wa.v, s.v, gamma, 0.0, vy.v, None, sim.ctx,
+                         ra) == 0
+    ua = U[l].get()
+    cdev, sdev, tk = C.c_void_p(), C.c_void_p(), C.c_uint()
+    # released with the coefficients
+    assert L.NVHip_GateOpen(wb.v, C.byref(cdev), C.byref(sdev)) == 0
+    assert L.NVHip_GateOpen(wb.v, C.byref(cdev), C.byref(sdev)) == -1   # one at a time
+    assert L.LLG_ArnoldiQueue(l, 0, 0.0, None, cdev, sdev, win.v, Up, wb.v, s.v, gamma, 0.0,
+                              vy.v, None, sim.ctx, C.byref(tk)) == 0
+    cf = (C.c_double * 3)(a, b[0], b[1])
+    assert L.NVHip_GateRelease(wb.v, 0, cf, 3) == 0
+    assert L.NVHip_ReduceWait(wb.v, tk.value, l + 3, rb) == 0
+    assert np.array_equal(wb.get(), wa.get()) and np.array_equal(U[l].get(), ua)
+    assert list(rb)[:l + 3] == list(ra)[:l + 3]
+    # released with stop: nothing runs, the result never arrives
+    wc = sim.vec(np.full(n, 7.0))
+    assert L.NVHip_GateOpen(wc.v, C.byref(cdev), C.byref(sdev)) == 0
+    assert L.LLG_ArnoldiQueue(l, 0, 0.0, None, cdev, sdev, win.v, Up, wc.v, s.v, gamma, 0.0,
+                              vy.v, None, sim.ctx, C.byref(tk)) == 0
+    assert L.NVHip_GateRelease(wc.v, 1, None, 0) == 0
+    assert L.NVHip_ReduceWait(wc.v, tk.value, l + 3, rb) == -1
+    assert np.all(wc.get() == 7.0)
+    # not released in time: the gate stops by itself after 1 s
+    assert L.NVHip_GateOpen(wc.v, C.byref(cdev), C.byref(sdev)) == 0
+    assert L.LLG_ArnoldiQueue(l, 0, 0.0, None, cdev, sdev, win.v, Up, wc.v, s.v, gamma, 0.0,
+                              vy.v, None, sim.ctx, C.byref(tk)) == 0
+    assert L.NVHip_ReduceWait(wc.v, tk.value, l + 3, rb) == -1
+    assert np.all(wc.get() == 7.0)
+    assert L.NVHip_GateRelease(wc.v, 1, None, 0) == 0      # late release is harmless
+    # and the stream works as before
+    assert L.LLG_Arnoldi(l, 0, a, b, win.v, Up, wc.v, s.v, gamma, 0.0, vy.v, None, sim.ctx,
+                         rb) == 0
+    assert np.array_equal(wc.get(), wa.get())
+
+
 @pytest.mark.parametrize("ng,ny", [(96, 64), (128, 64)])
 def test_demag_autotune_keeps_the_field(ng, ny):
     """the autotuner changes launch shapes, LDS layout and the FFT kernel

@@ -129,6 +129,26 @@ struct Integrator {
   N_Vector ycur = nullptr, fcur = nullptr;   // borrowed
   N_Vector lsx = nullptr;                     // linear-solver solution
   bool newton_init = false;   // the predictor also set acor = 0, y = zn[0]
+  // The Newton residual not yet formed: dst = sum_k c_k X_k is left to the
+  // first pass of the lagged SPGMR solve, which forms it where it reads it
+  // (form_b writes it where another reader needs it).
+  struct {
+    bool on = false;
+    double c[3] = {};
+    N_Vector X[3] = {};
+    N_Vector dst = nullptr;
+  } pend_b;
+  // The SPGMR solution not yet formed: dst = (0 + sum_k c_k X_k) / s is
+  // left to the first Newton update, which forms it where it reads it
+  // (form_x writes it where another reader needs it).
+  struct {
+    bool on = false;
+    int k = 0;
+    double c[16] = {};
+    N_Vector X[16] = {};
+    N_Vector s = nullptr;
+    N_Vector dst = nullptr;
+  } pend_x;
   N_Vector V[kMaxKrylov + 1] = {};
   N_Vector xcor = nullptr, vtemp = nullptr, ytemp = nullptr;
   double Hes[kMaxKrylov + 1][kMaxKrylov] = {};
@@ -554,6 +574,22 @@ bool lagged_ok(const Integrator* m) {
          m->fused.arnoldi && m->fused.projnorm && m->maxl <= kLaggedMax;
 }
 
+void form_b(Integrator* m) {
+  if (!m->pend_b.on) return;
+  m->pend_b.on = false;
+  N_VLinearCombination(3, m->pend_b.c, m->pend_b.X, m->pend_b.dst);
+}
+
+void form_x(Integrator* m) {
+  if (!m->pend_x.on) return;
+  m->pend_x.on = false;
+  auto& p = m->pend_x;
+  if (!(m->fused.lincombdiv && m->fused.lincombdiv(p.k, p.c, p.X, p.s, p.dst) == 0)) {
+    N_VLinearCombination(p.k, p.c, p.X, m->xcor);
+    N_VDiv(m->xcor, p.s, p.dst);
+  }
+}
+
 int spgmr_lagged(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s,
                  double bnorm_floor) {
   N_Vector* V = m->V;
@@ -589,17 +625,26 @@ int spgmr_lagged(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s
                         m->gamma, m->tn, m->ycur, m->fcur, m->user_data, &gate.ticket) != 0)
       gate.stop();
   };
-  // step 0: u_0 = s .* P^-1 b (the initial residual), w_0 = A u_0
-  if (pipe) {
+  // step 0: u_0 = s .* P^-1 b (the initial residual), w_0 = A u_0; a
+  // Newton residual not yet formed is formed where the pass reads it
+  const bool lazy_b = m->pend_b.on && m->pend_b.dst == b && f.arnoldi_first;
+  if (lazy_b) {
+    if (f.arnoldi_first(3, m->pend_b.c, m->pend_b.X, V, W[0], s, m->gamma, m->tn, m->ycur,
+                        m->fcur, m->user_data, r) != 0)
+      return kLaggedFallback;
+  } else if (pipe) {
+    form_b(m);
     unsigned tk0 = 0;
     if (f.arnoldi_queue(0, 1, 1.0, nullptr, nullptr, nullptr, b, V, W[0], s, m->gamma, m->tn,
                         m->ycur, m->fcur, m->user_data, &tk0) != 0)
       return kLaggedFallback;
     queue_gated(1, 0);
     if (f.result_wait(W[0], tk0, 4, r) != 0) return kLaggedFallback;
-  } else if (f.arnoldi(0, 1, 1.0, nullptr, b, V, W[0], s, m->gamma, m->tn, m->ycur, m->fcur,
-                       m->user_data, r) != 0) {
-    return kLaggedFallback;
+  } else {
+    form_b(m);
+    if (f.arnoldi(0, 1, 1.0, nullptr, b, V, W[0], s, m->gamma, m->tn, m->ycur, m->fcur,
+                  m->user_data, r) != 0)
+      return kLaggedFallback;
   }
   m->nps += 2;
   m->njtimes++;
@@ -612,6 +657,7 @@ int spgmr_lagged(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s
   m->ls_resnorm = r_norm;
   if (r_norm <= delta) {
     gate.stop();
+    m->pend_b.on = false;   // x overwrites b
     N_VConst(0.0, x);
     return LS_SUCCESS;
   }
@@ -713,14 +759,19 @@ int spgmr_lagged(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s
   for (int i = 1; i <= krydim; i++) m->yg[i] = 0.0;
   if (qr_sol(krydim, m->Hes, m->givens, m->yg) != 0) return LS_QRSOL_FAIL;
   if (converged || rho < beta) {
+    // x = sum_k (y_k / |u_k|) u_k / s, left to the Newton update when it can
+    // form it where it reads it
+    auto& p = m->pend_x;
+    p.k = krydim;
     for (int k = 0; k < krydim; k++) {
-      m->gs_c[k] = m->yg[k] / nu[k];
-      m->gs_X[k] = V[k];
+      p.c[k] = m->yg[k] / nu[k];
+      p.X[k] = V[k];
     }
-    if (!(m->fused.lincombdiv && m->fused.lincombdiv(krydim, m->gs_c, m->gs_X, s, x) == 0)) {
-      N_VLinearCombination(krydim, m->gs_c, m->gs_X, m->xcor);
-      N_VDiv(m->xcor, s, x);
-    }
+    p.s = s;
+    p.dst = x;
+    p.on = true;
+    m->pend_b.on = false;   // x overwrites b
+    if (!f.newtonupd_comb) form_x(m);
     return converged ? LS_SUCCESS : LS_RES_REDUCED;
   }
   return LS_CONV_FAIL;
@@ -740,9 +791,15 @@ int spgmr_solve(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s,
   m->ls_iters = 0;
   if (lagged_ok(m)) {
     rv = spgmr_lagged(m, x, b, delta, s, bnorm_floor);
-    if (rv != kLaggedFallback) return rv;
+    if (rv != kLaggedFallback) {
+      // a solve that wrote (or left pending) no solution leaves b as it was
+      if (rv != LS_SUCCESS && rv != LS_RES_REDUCED && rv != kBnormSmall) form_b(m);
+      return rv;
+    }
+    form_b(m);
     m->ls_iters = 0;
   }
+  form_b(m);
   if (bnorm_floor >= 0.0 && N_VWrmsNorm(b, s) <= bnorm_floor) return kBnormSmall;
 
   // r0 = b (zero initial guess); b is read here only, so the solution may
@@ -925,6 +982,7 @@ int ls_solve(Integrator* m, N_Vector b, N_Vector weight, N_Vector ynow, N_Vector
   // in between) |b|_WRMS comes out of the solve's first pass
   const bool bnorm_in_solve = lagged_ok(m) && !m->jtsetup;
   if (!bnorm_in_solve) {
+    form_b(m);
     const double bnorm = N_VWrmsNorm(b, weight);
     if (bnorm <= deltar) {
       if (m->nls_curiter > 0) N_VConst(0.0, b);
@@ -941,7 +999,12 @@ int ls_solve(Integrator* m, N_Vector b, N_Vector weight, N_Vector ynow, N_Vector
   // the solution overwrites b (SPGMR reads b before it writes x)
   const int rv = spgmr_solve(m, b, b, delta, weight, bnorm_in_solve ? deltar : -1.0);
   if (rv == kBnormSmall) {
-    if (m->nls_curiter > 0) N_VConst(0.0, b);
+    if (m->nls_curiter > 0) {
+      m->pend_b.on = false;
+      N_VConst(0.0, b);
+    } else {
+      form_b(m);   // b itself is the solution
+    }
     return 0;
   }
   m->nli += m->ls_iters;
@@ -972,6 +1035,17 @@ int nls_residual(Integrator* m, N_Vector ycor, N_Vector negres) {
   if (rv > 0) return RHSFUNC_RECVR;
   double c[3] = {-m->rl1, -1.0, m->gamma};
   N_Vector X[3] = {m->zn[1], ycor, m->ftemp};
+  if (m->fused.arnoldi_first && lagged_ok(m) && !m->jtsetup) {
+    // formed by the linear solve's first pass (form_b for any other reader)
+    auto& p = m->pend_b;
+    for (int k = 0; k < 3; k++) {
+      p.c[k] = c[k];
+      p.X[k] = X[k];
+    }
+    p.dst = negres;
+    p.on = true;
+    return 0;
+  }
   N_VLinearCombination(3, c, X, negres);
   return 0;
 }
@@ -1052,17 +1126,33 @@ int newton_solve(Integrator* m, N_Vector ycor, double tol, bool call_lsetup, lon
     for (;;) {
       (*niters)++;
       rv = nls_lsolve(m, del);   // del holds -res: solve (I - gamma J) del = -res
-      if (rv != 0) break;
+      if (rv != 0) {
+        m->pend_b.on = m->pend_x.on = false;   // del is not read again
+        break;
+      }
       double dn = 0.0, yn = -1.0;
       const bool vec2 = m->nls_curiter > 0 && del->ops->nvwrmsnormvectorarray;
-      if ((m->nls_curiter == 0 || vec2) && m->fused.newtonupd &&
-          m->fused.newtonupd(m->nls_curiter, ycor, del, m->zn[0], m->ewt, m->y, &dn, &yn) == 0) {
-        // ycor += del, y = zn[0] + ycor and the test norms in one pass
+      auto& px = m->pend_x;
+      const bool comb = px.on && px.dst == del && m->nls_curiter == 0 &&
+                        m->fused.newtonupd_comb(ycor, px.k, px.c, px.X, px.s, m->zn[0], m->ewt,
+                                                m->y, &dn) == 0;
+      if (comb) {
+        // the solution formed where the update reads it (del not written)
+        px.on = false;
         m->y_current = true;
-        rv = nls_convtest_norms(m, ycor, dn, yn, tol);
+        rv = nls_convtest_norms(m, ycor, dn, -1.0, tol);
       } else {
-        N_VLinearSum(1.0, ycor, 1.0, del, ycor);
-        rv = nls_convtest(m, ycor, del, tol);
+        form_x(m);
+        if ((m->nls_curiter == 0 || vec2) && m->fused.newtonupd &&
+            m->fused.newtonupd(m->nls_curiter, ycor, del, m->zn[0], m->ewt, m->y, &dn, &yn) ==
+                0) {
+          // ycor += del, y = zn[0] + ycor and the test norms in one pass
+          m->y_current = true;
+          rv = nls_convtest_norms(m, ycor, dn, yn, tol);
+        } else {
+          N_VLinearSum(1.0, ycor, 1.0, del, ycor);
+          rv = nls_convtest(m, ycor, del, tol);
+        }
       }
       if (rv == 0) {
         m->nls_jcur = false;

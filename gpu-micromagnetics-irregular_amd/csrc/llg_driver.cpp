@@ -99,6 +99,8 @@ void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_hos
         fo.arnoldi = LLG_Arnoldi;
         fo.projnorm = NVHip_ProjNorm;
       }
+      if ((opts->fused & 80) == 80) fo.arnoldi_first = LLG_ArnoldiFirst;
+      if ((opts->fused & 144) == 144) fo.newtonupd_comb = NVHip_NewtonUpdateComb;
       if ((opts->fused & 48) == 48) {
         fo.arnoldi_queue = LLG_ArnoldiQueue;
         fo.result_wait = NVHip_ReduceWait;

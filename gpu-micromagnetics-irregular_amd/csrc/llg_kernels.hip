@@ -579,6 +579,11 @@ struct ArnIn {
   double* ul;                 // u_L, written for the tile's own cells (L > 0 or PRE)
   double a;                   // u = a win - sum_j b_j U_j
   double b[kArnMax];
+  // PRE with nb > 0: b = sum_{k<nb} bc_k bx_k formed where read (the Newton
+  // residual), rounded as N_VLinearCombination rounds it; win unused
+  const double* bx[3];
+  double bc[3];
+  int nb;
 };
 
 // One left-preconditioned SPGMR iteration with the normalisation of the
@@ -655,7 +660,18 @@ __global__ __launch_bounds__(BX* BY) void k_arnoldi(GeoD g, TermsD T, int pgate,
       const bool mine = sx >= 1 && sx <= BX && sy >= 1 && sy <= BY && ux < g.ng && uy < g.ny;
       double u[3];
       if (L == 0 && PRE) {
-        const double r[3] = {in.win[cidx], in.win[N + cidx], in.win[2 * N + cidx]};
+        double r[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          const int e = a * N + cidx;
+          if (in.nb > 0) {
+            double v = in.bc[0] * in.bx[0][e];
+            for (int k = 1; k < in.nb; k++) v = fma(in.bc[k], in.bx[k][e], v);
+            r[a] = v;
+          } else {
+            r[a] = in.win[e];
+          }
+        }
         double z[3];
         apply_P_cell(g, pgate, P, cidx, r, z);
 #pragma unroll
@@ -1247,36 +1263,23 @@ int LLG_SpgmrSolve(N_Vector t0, N_Vector* V, int maxl, N_Vector s, N_Vector x,
   return rc;
 }
 
-int LLG_ArnoldiQueue(int l, int pre, sunrealtype a, const sunrealtype* b, const double* coef_dev,
-                     const int* skip_dev, N_Vector win, N_Vector* U, N_Vector wout, N_Vector s,
-                     sunrealtype gamma, sunrealtype t, N_Vector y, N_Vector fy, void* user_data,
-                     unsigned* ticket) {
-  (void)t; (void)fy;
-  llg_ctx* ctx = (llg_ctx*)user_data;
-  if (!ctx || ctx->lpc || !ctx->pd || ctx->cfg.exec != LLG_EXEC_YMSK || !U || !wout || !s ||
-      !y || !ticket || l < 0 || l >= kArnMax || (pre && l != 0) ||
-      (l > 0 && (!win || (!b && !coef_dev))) || (pre && !win))
-    return -1;
-  ArnIn in{};
-  in.win = win ? N_VGetDeviceArrayPointer(win) : nullptr;
-  in.a = a;
-  for (int j = 0; j <= l; j++)
-    if (!U[j]) return -1;
-  for (int j = 0; j < l; j++) {
-    in.U[j] = N_VGetDeviceArrayPointer(U[j]);
-    in.b[j] = b ? b[j] : 0.0;
-  }
-  if (l == 0 && !pre) in.U[0] = N_VGetDeviceArrayPointer(U[0]);
-  in.ul = (l > 0 || pre) ? N_VGetDeviceArrayPointer(U[l]) : nullptr;
+}  // extern "C"
+
+namespace {
+// queue one lagged pass (its inputs in `in`) and its finish; the kernel reads
+// in.win, in.bx, s, y and U[j < l] at halo cells other workgroups own, so
+// nothing it writes may alias them
+int queue_pass(llg_ctx* ctx, int l, bool pre, ArnIn& in, const double* coef_dev,
+               const int* skip_dev, N_Vector wout, N_Vector s, sunrealtype gamma, N_Vector y,
+               unsigned* ticket) {
   double* dw = N_VGetDeviceArrayPointer(wout);
   const double* ds = N_VGetDeviceArrayPointer(s);
   const double* dy = N_VGetDeviceArrayPointer(y);
-  // the kernel reads win, s, y and U[j < l] at halo cells other workgroups
-  // own, so nothing it writes may alias them
-  const double* reads[kArnMax + 3] = {in.win, ds, dy};
+  const double* reads[kArnMax + 6] = {in.win, ds, dy};
   int nr = 3;
   for (int j = 0; j < l; j++) reads[nr++] = in.U[j];
   if (l == 0 && !pre) reads[nr++] = in.U[0];
+  for (int k = 0; k < in.nb; k++) reads[nr++] = in.bx[k];
   for (int i = 0; i < nr; i++)
     if (reads[i] && (reads[i] == dw || reads[i] == in.ul)) return -1;
   if (in.ul && in.ul == dw) return -1;
@@ -1315,6 +1318,57 @@ int LLG_ArnoldiQueue(int l, int pre, sunrealtype a, const sunrealtype* b, const 
   ctx->njv++;
   ctx->npsol += pre ? 2 : 1;
   return 0;
+}
+
+bool pass_ok(const llg_ctx* ctx) {
+  return ctx && !ctx->lpc && ctx->pd && ctx->cfg.exec == LLG_EXEC_YMSK;
+}
+}  // namespace
+
+extern "C" {
+
+int LLG_ArnoldiQueue(int l, int pre, sunrealtype a, const sunrealtype* b, const double* coef_dev,
+                     const int* skip_dev, N_Vector win, N_Vector* U, N_Vector wout, N_Vector s,
+                     sunrealtype gamma, sunrealtype t, N_Vector y, N_Vector fy, void* user_data,
+                     unsigned* ticket) {
+  (void)t; (void)fy;
+  llg_ctx* ctx = (llg_ctx*)user_data;
+  if (!pass_ok(ctx) || !U || !wout || !s || !y || !ticket || l < 0 || l >= kArnMax ||
+      (pre && l != 0) || (l > 0 && (!win || (!b && !coef_dev))) || (pre && !win))
+    return -1;
+  ArnIn in{};
+  in.win = win ? N_VGetDeviceArrayPointer(win) : nullptr;
+  in.a = a;
+  for (int j = 0; j <= l; j++)
+    if (!U[j]) return -1;
+  for (int j = 0; j < l; j++) {
+    in.U[j] = N_VGetDeviceArrayPointer(U[j]);
+    in.b[j] = b ? b[j] : 0.0;
+  }
+  if (l == 0 && !pre) in.U[0] = N_VGetDeviceArrayPointer(U[0]);
+  in.ul = (l > 0 || pre) ? N_VGetDeviceArrayPointer(U[l]) : nullptr;
+  return queue_pass(ctx, l, pre != 0, in, coef_dev, skip_dev, wout, s, gamma, y, ticket);
+}
+
+int LLG_ArnoldiFirst(int nb, const sunrealtype* c, N_Vector* X, N_Vector* U, N_Vector wout,
+                     N_Vector s, sunrealtype gamma, sunrealtype t, N_Vector y, N_Vector fy,
+                     void* user_data, sunrealtype* r) {
+  (void)t; (void)fy;
+  llg_ctx* ctx = (llg_ctx*)user_data;
+  if (!pass_ok(ctx) || nb < 1 || nb > 3 || !c || !X || !U || !U[0] || !wout || !s || !y || !r)
+    return -1;
+  ArnIn in{};
+  in.nb = nb;
+  for (int k = 0; k < nb; k++) {
+    if (!X[k]) return -1;
+    in.bx[k] = N_VGetDeviceArrayPointer(X[k]);
+    in.bc[k] = c[k];
+  }
+  in.a = 1.0;
+  in.ul = N_VGetDeviceArrayPointer(U[0]);
+  unsigned tk = 0;
+  if (queue_pass(ctx, 0, true, in, nullptr, nullptr, wout, s, gamma, y, &tk) != 0) return -1;
+  return NVHip_ReduceWait(wout, tk, 4, r);
 }
 
 int LLG_Arnoldi(int l, int pre, sunrealtype a, const sunrealtype* b, N_Vector win, N_Vector* U,

@@ -96,7 +96,7 @@ hipStream_t g_default_stream = nullptr;
 bool g_default_stream_made = false;
 std::atomic<long> g_syncs{0};
 int g_sync_mode = 0;   // 0: hipStreamSynchronize, 1: spin on the pinned sequence word
-int g_red_blocks = 512;   // partial workgroups per reduction (upper bound)
+int g_red_blocks = 1024;  // partial workgroups per reduction (upper bound)
 int g_red_mode = 1;       // 0: single launch with ticket, 1: partial + finish launches
 
 RedWs* workspace(hipStream_t s) {
@@ -397,6 +397,23 @@ struct FNewtonUpd0 {
   }
 };
 
+// first Newton iteration with the SPGMR solution formed where read:
+// del = (0 + sum_j c_j X_j) / s (as k_lincombdiv), then FNewtonUpd0 (del
+// itself is not written)
+struct FNewtonUpdComb0 {
+  int nv; CoefPack c; PtrPack X; const double *s, *w, *zn0; double *ycor, *y;
+  __device__ double operator()(sunindextype i) const {
+    double acc = 1.0 * 0.0;
+    for (int j = 0; j < nv; j++) acc = fma(c.c[j], X.p[j][i], acc);
+    const double d = acc / s[i];
+    const double cy = fma(1.0, ycor[i], 1.0 * d);
+    ycor[i] = cy;
+    y[i] = fma(1.0, zn0[i], 1.0 * cy);
+    const double p = d * w[i];
+    return p * p;
+  }
+};
+
 // Block-wide reduction of K values per thread over W waves; result valid
 // in thread 0.
 template <int K, int W = kWaves>
@@ -434,7 +451,7 @@ __device__ void finish(int kind, int nblk, const double* partials, double* out, 
   if (!SAME_KERNEL) {
     // U blocks' loads issued before any is summed (one memory round trip
     // per U blocks instead of per block)
-    constexpr int U = K <= 4 ? 4 : (K <= 8 ? 2 : 1);
+    constexpr int U = K <= 8 ? 4 : (K <= 12 ? 2 : 1);
     for (; b + (U - 1) * (int)blockDim.x < nblk; b += U * blockDim.x) {
       double p[U][K];
 #pragma unroll
@@ -1400,6 +1417,31 @@ int NVHip_NewtonUpdate(int iter, N_Vector ycor, N_Vector del, N_Vector zn0, N_Ve
   wait_result(s, wk, wk->seq);
   *dnorm = std::sqrt(wk->host[0] / (double)n);
   *ynorm = std::sqrt(wk->host[1] / (double)n);
+  return 0;
+}
+
+int NVHip_NewtonUpdateComb(N_Vector ycor, int nvec, const sunrealtype* c, N_Vector* X,
+                           N_Vector sv, N_Vector zn0, N_Vector w, N_Vector y,
+                           sunrealtype* dnorm) {
+  if (!ycor || !c || !X || !sv || !zn0 || !w || !y || !dnorm || nvec < 1 || nvec > kMaxK)
+    return -1;
+  const sunindextype n = LEN(ycor);
+  if (LEN(sv) != n || LEN(zn0) != n || LEN(w) != n || LEN(y) != n) return -1;
+  if (D(y) == D(ycor) || D(y) == D(zn0) || D(y) == D(sv) || D(y) == D(w)) return -1;
+  FNewtonUpdComb0 f{};
+  f.nv = nvec;
+  for (int j = 0; j < nvec; j++) {
+    if (!X[j] || LEN(X[j]) != n || D(X[j]) == D(y) || D(X[j]) == D(ycor)) return -1;
+    f.c.c[j] = c[j];
+    f.X.p[j] = D(X[j]);
+  }
+  f.s = D(sv);
+  f.w = D(w);
+  f.zn0 = D(zn0);
+  f.ycor = D(ycor);
+  f.y = D(y);
+  const double sq = reduce1(ycor, RED_SUM, f);
+  *dnorm = std::sqrt(sq / (double)n);
   return 0;
 }
 

@@ -176,6 +176,7 @@ SIGNATURES = {
     "NVHip_GateRelease": (C.c_int, [VP, C.c_int, DP, C.c_int]),
     "NVHip_ProjNorm": (C.c_int, [C.c_int, C.c_double, DP, VP, C.POINTER(VP), VP, DP]),
     "NVHip_NewtonUpdate": (C.c_int, [C.c_int, VP, VP, VP, VP, VP, DP, DP]),
+    "NVHip_NewtonUpdateComb": (C.c_int, [VP, C.c_int, DP, C.POINTER(VP), VP, VP, VP, VP, DP]),
     "FusedNVec_Init": (None, [VP]),
     "FusedNVec_FreePool": (None, []),
     # demag_fft.h
@@ -228,6 +229,8 @@ SIGNATURES = {
     "LLG_ArnoldiQueue": (C.c_int, [C.c_int, C.c_int, C.c_double, DP, VP, VP, VP, C.POINTER(VP),
                                    VP, VP, C.c_double, C.c_double, VP, VP, VP,
                                    C.POINTER(C.c_uint)]),
+    "LLG_ArnoldiFirst": (C.c_int, [C.c_int, DP, C.POINTER(VP), C.POINTER(VP), VP, VP,
+                                   C.c_double, C.c_double, VP, VP, VP, DP]),
     "PrecondSetup": (C.c_int, [C.c_double, VP, VP, C.c_int, C.POINTER(C.c_int), C.c_double, VP]),
     "PrecondSolve": (C.c_int, [C.c_double, VP, VP, VP, VP, C.c_double, C.c_double, C.c_int, VP]),
     "Precond_Create": (VP, [C.c_int, C.c_int, C.c_int]),

@@ -129,6 +129,16 @@ typedef struct llgcv_fused_ops {
   int (*result_wait)(N_Vector v, unsigned ticket, int K, sunrealtype* r);
   int (*gate_open)(N_Vector v, const double** coef_dev, const int** skip_dev);
   int (*gate_release)(N_Vector v, int stop, const sunrealtype* coef, int n);
+  /* with arnoldi: the first pass with the right-hand side formed where it is
+   * read, b = sum_k c_k X_k (as LLG_ArnoldiFirst; the Newton residual is then
+   * never written), and the first Newton update with the solution formed
+   * where it is read (as NVHip_NewtonUpdateComb; the solution vector is then
+   * never written) */
+  int (*arnoldi_first)(int nb, const sunrealtype* c, N_Vector* X, N_Vector* U, N_Vector wout,
+                       N_Vector s, sunrealtype gamma, sunrealtype t, N_Vector y, N_Vector fy,
+                       void* user_data, sunrealtype* r);
+  int (*newtonupd_comb)(N_Vector ycor, int nvec, const sunrealtype* c, N_Vector* X,
+                        N_Vector s, N_Vector zn0, N_Vector w, N_Vector y, sunrealtype* dnorm);
   /* predictor + acor = 0 + y = zn[0] + acor in one pass (as NVHip_PredictInit) */
   int (*predict_init)(int q, N_Vector* zn, N_Vector acor, N_Vector y);
   /* nrm[0] = |x|_WRMS(w), nrm[1] = |a y + b z|_WRMS(w) (as NVHip_WrmsNormPair) */

@@ -44,8 +44,12 @@ typedef struct llg_run_opts {
                          are back, taking the launch off the host round
                          trip (on the 1536x512 workload it measures level
                          with bit 4 alone: the gate adds a kernel boundary
-                         per iteration); default 23 (bits 0, 1, 2, 4).
-                         Bits 1 to 3 change no value. */
+                         per iteration); bits 6 and 7 (with bit 4): the
+                         first SPGMR pass forms the Newton residual, and
+                         the first Newton update the SPGMR solution, where
+                         each is read (never written; measured neutral on
+                         the 1536x512 workload); default 23 (bits 0, 1, 2,
+                         4).  Bits 1 to 3 change no value. */
   long   mxstep;      /* max steps per CVode call (CVODE default 500) */
   int    use_jtv;     /* analytic Jv (else difference quotient) */
   int    use_prec;    /* block-Jacobi preconditioner */

@@ -202,6 +202,13 @@ int LLG_ArnoldiQueue(int l, int pre, sunrealtype a, const sunrealtype* b, const 
                      const int* skip_dev, N_Vector win, N_Vector* U, N_Vector wout, N_Vector s,
                      sunrealtype gamma, sunrealtype t, N_Vector y, N_Vector fy, void* user_data,
                      unsigned* ticket);
+/* The first pass (LLG_Arnoldi with l = 0, pre) with the right-hand side
+ * formed where it is read: b = sum_{k<nb} c_k X_k (nb <= 3, rounded as
+ * N_VLinearCombination rounds it; the Newton residual), so b is never
+ * written.  r as for LLG_Arnoldi (4 sums).  No reference counterpart. */
+int LLG_ArnoldiFirst(int nb, const sunrealtype* c, N_Vector* X, N_Vector* U, N_Vector wout,
+                     N_Vector s, sunrealtype gamma, sunrealtype t, N_Vector y, N_Vector fy,
+                     void* user_data, sunrealtype* r);
 int PrecondSetup(sunrealtype t, N_Vector y, N_Vector fy, sunbooleantype jok,
                  sunbooleantype* jcurPtr, sunrealtype gamma, void* user_data);
 int PrecondSolve(sunrealtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z,

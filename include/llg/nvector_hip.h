@@ -61,7 +61,7 @@ long  NVHip_NumSyncs(void);
 int   NVHip_SetSyncMode(int mode);
 /* reduction launch shape: at most max_blocks (1..4096) partial workgroups;
  * mode 0 = one launch (last workgroup sums the partials), 1 = partial
- * launch + one-workgroup finish launch (default: 512, 1).  Results are
+ * launch + one-workgroup finish launch (default: 1024, 1).  Results are
  * deterministic for a fixed configuration and vector length. */
 int   NVHip_SetReduceConfig(int max_blocks, int mode);
 
@@ -159,6 +159,13 @@ int NVHip_LinCombDiv(int nvec, sunrealtype* c, N_Vector* X, N_Vector s, N_Vector
  * pair; iter 0 as N_VWrmsNorm).  y must not alias the inputs. */
 int NVHip_NewtonUpdate(int iter, N_Vector ycor, N_Vector del, N_Vector zn0, N_Vector w,
                        N_Vector y, sunrealtype* dnorm, sunrealtype* ynorm);
+/* The first Newton update with the SPGMR solution formed where it is read:
+ * del = (0 + sum_j c_j X_j) / s (NVHip_LinCombDiv's values, not written),
+ * then NVHip_NewtonUpdate(0, ycor, del, zn0, w, y, dnorm); bit-equal to the
+ * two calls.  y may not alias any input, ycor no X_j. */
+int NVHip_NewtonUpdateComb(N_Vector ycor, int nvec, const sunrealtype* c, N_Vector* X,
+                           N_Vector s, N_Vector zn0, N_Vector w, N_Vector y,
+                           sunrealtype* dnorm);
 
 /* Device-resident SPGMR solve: SUNLinSolSolve_SPGMR with left
  * preconditioning, zero initial guess, classical Gram-Schmidt and no

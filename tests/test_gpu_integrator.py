@@ -59,14 +59,16 @@ def test_fused_integrator_kernels_are_bitwise_neutral(gstype, case):
         assert np.array_equal(ya, yb), fused
 
 
-@pytest.mark.parametrize("fused", [7, 23, 55])
+@pytest.mark.parametrize("fused", [7, 23, 55, 215])
 @pytest.mark.parametrize("name,kw", [("square_hole", {}), ("i6", {"demag": 4.0}),
                                      ("plain128", {})])
 def test_trajectory_matches_oracle(name, kw, fused):
     """fused 7: the standard SPGMR loop with the one-pass kernels (bit-equal
     to the plain op sequence); 23: SPGMR with lagged normalisation
     (LLG_Arnoldi; the compact i6 grid falls back to the standard loop);
-    55: the same pipelined, each pass queued behind a gate (polling waits)"""
+    55: the same pipelined, each pass queued behind a gate (polling waits);
+    215: the lagged solve with the Newton residual and the SPGMR solution
+    formed where they are read (never written)"""
     ng, ny = (48, 32) if name != "plain128" else (32, 32)
     cfg = la.preset(name, ng=ng, ny=ny, **kw)
     if name == "plain128":

@@ -67,7 +67,7 @@ def dot_close(got, a, b):
 def _default_reduce_config():
     L = la.lib()
     yield
-    L.NVHip_SetReduceConfig(512, 1)
+    L.NVHip_SetReduceConfig(1024, 1)   # the library default
 
 
 @pytest.mark.parametrize("n", SIZES)
@@ -427,3 +427,30 @@ def test_wrms_norm_pair_equals_two_norms():
     assert L.NVHip_WrmsNormPair(x.v, a, yv.v, b, zv.v, w.v, nr) == 0
     assert L.NVHip_NumSyncs() - s0 == 1
     assert nr[0] == n1 and nr[1] == n2
+
+
+def test_newton_update_comb_equals_lincombdiv_then_update():
+    """NVHip_NewtonUpdateComb == NVHip_LinCombDiv then NVHip_NewtonUpdate(0),
+    bit for bit (ycor, y and the norm; the solution vector not written)"""
+    L = la.lib()
+    rng = np.random.default_rng(80)
+    X = _vecs(rng, 4)
+    sv, = _vecs(rng, 1, lo=0.5)
+    zn0, = _vecs(rng, 1)
+    w, = _vecs(rng, 1, lo=0.5)
+    ycor0 = rng.standard_normal(N_FUSED) * 1e-3
+    c = (C.c_double * 4)(0.5, -0.25, 1.5e-3, 7.0)
+    xa = vec(np.zeros(N_FUSED))
+    ya, yb = vec(ycor0), vec(ycor0)
+    outa, outb = vec(np.zeros(N_FUSED)), vec(np.zeros(N_FUSED))
+    assert L.NVHip_LinCombDiv(4, c, arr(X), sv.v, xa.v) == 0
+    dna, yna = C.c_double(), C.c_double(-1.0)
+    assert L.NVHip_NewtonUpdate(0, ya.v, xa.v, zn0.v, w.v, outa.v, C.byref(dna),
+                                C.byref(yna)) == 0
+    dnb = C.c_double()
+    assert L.NVHip_NewtonUpdateComb(yb.v, 4, c, arr(X), sv.v, zn0.v, w.v, outb.v,
+                                    C.byref(dnb)) == 0
+    assert np.array_equal(ya.get(), yb.get()) and np.array_equal(outa.get(), outb.get())
+    assert dna.value == dnb.value
+    assert L.NVHip_NewtonUpdateComb(yb.v, 4, c, arr(X), sv.v, zn0.v, w.v, X[0].v,
+                                    C.byref(dnb)) == -1   # y aliases an input

@@ -307,6 +307,37 @@ def test_arnoldi_step_equals_separate_calls(case):
     print(f"[gpu] {case[0]}: step {nl} |u|^2 {r[0]:.6e} |w|^2 {r[nl + 2]:.6e}")
 
 
+@pytest.mark.parametrize("case", CASES[:2], ids=[c[0] for c in CASES[:2]])
+def test_arnoldi_first_pass_with_residual_formed_where_read(case):
+    """LLG_ArnoldiFirst (b = sum c_k X_k formed where the first pass reads
+    it) == N_VLinearCombination then LLG_Arnoldi(0, pre), bit for bit"""
+    cfg, geom, y0 = make(case)
+    sim = la.Simulation(cfg, geom)
+    L = la.lib()
+    rng = np.random.default_rng(12)
+    n = sim.neq
+    y = perturbed(y0, geom.active, 4)
+    sim.rhs(y)
+    gamma = 0.0371
+    sim.psetup(y, gamma)
+    vy = sim.vec(y)
+    s = sim.vec(1.0 / (1e-4 * np.abs(rng.standard_normal(n)) + 1e-4))
+    X = [sim.vec(rng.standard_normal(n)) for _ in range(3)]
+    Xp = (C.c_void_p * 3)(*[x.v for x in X])
+    c = (C.c_double * 3)(-1.25, -1.0, gamma)
+    b = sim.vec()
+    L.N_VLinearCombination_Hip(3, c, Xp, b.v)
+    Ua, Ub = [sim.vec()], [sim.vec()]
+    wa, wb = sim.vec(), sim.vec()
+    ra, rb = (C.c_double * 8)(), (C.c_double * 8)()
+    assert L.LLG_Arnoldi(0, 1, 1.0, None, b.v, (C.c_void_p * 1)(Ua[0].v), wa.v, s.v, gamma,
+                         0.0, vy.v, None, sim.ctx, ra) == 0
+    assert L.LLG_ArnoldiFirst(3, c, Xp, (C.c_void_p * 1)(Ub[0].v), wb.v, s.v, gamma, 0.0,
+                              vy.v, None, sim.ctx, rb) == 0
+    assert np.array_equal(Ua[0].get(), Ub[0].get()) and np.array_equal(wa.get(), wb.get())
+    assert list(ra)[:4] == list(rb)[:4]
+
+
 def test_gated_arnoldi_step_equals_direct():
     """LLG_ArnoldiQueue behind an NVHip_GateOpen gate, released with the
     coefficients, gives LLG_Arnoldi's vectors and sums bit for bit; released

@@ -73,48 +73,68 @@ def algorithmic_bytes(kernel, ng, ny, ncell, padded=True):
     }[kernel]
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(la, cfg, geom, y0, budget_s, threads):
     """The CPU oracle (OpenMP C restatement of the callbacks + serial
     N_Vector + the same BDF integrator) on the same workload from t = 0,
-    stopped after ~budget_s seconds of integration."""
+    stopped after ~budget_s seconds of integration on `threads` threads,
+    then again on one thread for half that budget."""
     from oracle import oracle as orc
     L = orc.load()
-    if threads:
-        L.orc_set_num_threads(threads)
-    nthreads = L.orc_num_threads()
     t_init = time.perf_counter()
     sim = orc.OracleSim(cfg, geom)
     init_s = time.perf_counter() - t_init
     n = sim.neq
-    y = orc.SerialVector(n, y0)
     P = la.lib()
-    mem = P.LLGCVodeCreate(2, None)
-    P.LLGCVodeSetMaxOrd(mem, 5)
-    P.LLGCVodeInit(mem, orc.addr("orc_cv_rhs"), 0.0, y.v)
-    P.LLGCVodeSetUserData(mem, sim.ctx)
-    P.LLGCVodeSStolerances(mem, 1e-4, 1e-4)
-    P.LLGCVodeSetLinearSolverSPGMR(mem, 1, 5, 2 if n < 500000 else 1)
-    P.LLGCVodeSetJacTimes(mem, None, orc.addr("orc_cv_jtv"))
-    P.LLGCVodeSetPreconditioner(mem, orc.addr("orc_cv_psetup"), orc.addr("orc_cv_psolve"))
-    tret = C.c_double(0.0)
-    t0 = time.perf_counter()
-    tout, dt = 0.01, 0.01
-    while True:
-        rc = P.LLGCVode(mem, tout, y.v, C.byref(tret), 1)
-        if rc < 0 or time.perf_counter() - t0 > budget_s:
-            break
-        tout += dt
-    wall = time.perf_counter() - t0
-    st = la.native.llgcv_stats()
-    P.LLGCVodeGetStats(mem, C.byref(st))
-    m = C.c_void_p(mem)
-    P.LLGCVodeFree(C.byref(m))
+
+    def run(nthr, budget):
+        L.orc_set_num_threads(nthr)
+        used = L.orc_num_threads()
+        y = orc.SerialVector(n, y0)
+        mem = P.LLGCVodeCreate(2, None)
+        P.LLGCVodeSetMaxOrd(mem, 5)
+        P.LLGCVodeInit(mem, orc.addr("orc_cv_rhs"), 0.0, y.v)
+        P.LLGCVodeSetUserData(mem, sim.ctx)
+        P.LLGCVodeSStolerances(mem, 1e-4, 1e-4)
+        P.LLGCVodeSetLinearSolverSPGMR(mem, 1, 5, 2 if n < 500000 else 1)
+        P.LLGCVodeSetJacTimes(mem, None, orc.addr("orc_cv_jtv"))
+        P.LLGCVodeSetPreconditioner(mem, orc.addr("orc_cv_psetup"), orc.addr("orc_cv_psolve"))
+        tret = C.c_double(0.0)
+        t0 = time.perf_counter()
+        tout, dt = 0.01, 0.01
+        while True:
+            rc = P.LLGCVode(mem, tout, y.v, C.byref(tret), 1)
+            if rc < 0 or time.perf_counter() - t0 > budget:
+                break
+            tout += dt
+        wall = time.perf_counter() - t0
+        st = la.native.llgcv_stats()
+        P.LLGCVodeGetStats(mem, C.byref(st))
+        m = C.c_void_p(mem)
+        P.LLGCVodeFree(C.byref(m))
+        return used, tret.value, wall, st
+
+    nthreads, t_end, wall, st = run(threads or 0, budget_s)
+    one, t1, wall1, st1 = run(1, budget_s / 2)
     sim.close()
-    return {"value": tret.value / wall, "unit": "sim-time/wall-s", "cores": nthreads,
+    return {"value": t_end / wall, "unit": "sim-time/wall-s", "cores": nthreads,
             "kind": "port",
-            "sample": (f"same workload from t=0 to t={tret.value:.3f} ({st.nst} BDF steps, "
+            "sample": (f"same workload from t=0 to t={t_end:.3f} ({st.nst} BDF steps, "
                        f"{st.nfe} f evals) in {wall:.1f} s on {nthreads} OpenMP threads; "
-                       f"oracle setup (tensor + plans) {init_s:.1f} s excluded")}
+                       f"oracle setup (tensor + plans) {init_s:.1f} s excluded"),
+            "single_thread": {"value": t1 / wall1, "cores": one,
+                              "sample": f"t=0 to t={t1:.4f} ({st1.nst} BDF steps) in "
+                                        f"{wall1:.1f} s on 1 thread"},
+            "host": {"nproc": os.cpu_count(), "model": cpu_model()}}
 
 
 def ensemble_bench(la, args):

@@ -129,6 +129,7 @@ struct Integrator {
   N_Vector ycur = nullptr, fcur = nullptr;   // borrowed
   N_Vector lsx = nullptr;                     // linear-solver solution
   bool newton_init = false;   // the predictor also set acor = 0, y = zn[0]
+  bool ewt_pending = false;   // the step's error weights left to the predictor
   // The Newton residual not yet formed: dst = sum_k c_k X_k is left to the
   // first pass of the lagged SPGMR solve, which forms it where it reads it
   // (form_b writes it where another reader needs it).
@@ -280,10 +281,20 @@ void predict(Integrator* m) {
   m->tn += m->h;
   if (m->tstopset && (m->tn - m->tstop) * m->h > 0.0) m->tn = m->tstop;
   // with the Newton iteration's acor = 0 and y = zn[0] + acor in the same
-  // pass (cv_nls and the first residual then skip both)
+  // pass (cv_nls and the first residual then skip both), and the step's
+  // error weights when they were left to it
+  N_Vector w = m->ewt_pending ? m->ewt : nullptr;
   m->newton_init = m->fused.predict_init &&
-                   m->fused.predict_init(m->q, m->zn, m->acor, m->y) == 0;
-  if (m->newton_init) return;
+                   m->fused.predict_init(m->q, m->zn, m->acor, m->y, m->reltol, m->sabstol,
+                                         m->itol == 2 ? m->vabstol : nullptr, w) == 0;
+  if (m->newton_init) {
+    m->ewt_pending = false;
+    return;
+  }
+  if (m->ewt_pending) {
+    m->ewt_pending = false;
+    ewt_set(m, m->zn[0], m->ewt);
+  }
   if (m->fused.pascal && m->fused.pascal(m->q, m->zn, 1.0) == 0) return;
   for (int k = 1; k <= m->q; k++)
     for (int j = m->q; j >= k; j--) N_VLinearSum(1.0, m->zn[j - 1], 1.0, m->zn[j], m->zn[j - 1]);
@@ -1395,7 +1406,10 @@ int cv_step(Integrator* m) {
   complete_step(m);
   prepare_next_step(m, dsm);
   m->etamax = (m->nst <= kSmallNst) ? kEtaMaxEarly : kEtaMaxGrowth;
-  N_VScale(m->tq[2], m->acor, m->acor);
+  // CVODE rescales acor here into the local error estimate for
+  // CVodeGetEstLocalErrors; nothing reads acor before the next step zeroes
+  // it (the order change uses the copy in zn[qmax]) and this integrator
+  // offers no such query, so that pass is not made.
   return 0;
 }
 
@@ -1727,7 +1741,14 @@ int LLGCVode(void* mem, sunrealtype tout, N_Vector yout, sunrealtype* tret, int 
   for (;;) {
     m->next_h = m->h;
     m->next_q = m->q;
-    if (m->nst > 0) {
+    // the error weights of zn[0]: formed by the predictor's pass when it can
+    // (nothing reads them before it: the accuracy check below is skipped
+    // for rtol >= 2 uround, and adjust_params does not use them)
+    m->ewt_pending = false;
+    if (m->nst > 0 && m->fused.predict_init && m->fused.ewt && !m->atolmin0 &&
+        m->reltol >= 2.0 * m->uround) {
+      m->ewt_pending = true;
+    } else if (m->nst > 0) {
       if (ewt_set(m, m->zn[0], m->ewt) != 0) {
         istate = LLGCV_ILL_INPUT;
         m->tretlast = *tret = m->tn;

@@ -251,14 +251,22 @@ __global__ void k_pascal(OutPack Z, double sign, sunindextype n) {
 }
 
 // the predictor followed by the Newton iteration's start: acor = 0 and
-// y = zn[0] + acor, as N_VConst and N_VLinearSum would write them
+// y = zn[0] + acor, as N_VConst and N_VLinearSum would write them; with w
+// set, first the error weights of the step from the zn[0] being replaced,
+// as k_ewt forms them
 template <int Q>
 __global__ void k_pascal_init(OutPack Z, double* __restrict__ acor, double* __restrict__ y,
-                              sunindextype n) {
+                              sunindextype n, double rtol, double atol,
+                              const double* __restrict__ av, double* __restrict__ w) {
   GRID_STRIDE(i, n) {
     double z[Q + 1];
 #pragma unroll
     for (int j = 0; j <= Q; j++) z[j] = Z.p[j][i];
+    if (w) {
+      const double t = fabs(z[0]);
+      const double d = av ? fma(rtol, t, av[i]) : rtol * t + atol;
+      w[i] = 1.0 / d;
+    }
 #pragma unroll
     for (int k = 1; k <= Q; k++)
 #pragma unroll
@@ -1286,8 +1294,12 @@ int NVHip_Pascal(int q, N_Vector* zn, sunrealtype sign) {
   return 0;
 }
 
-int NVHip_PredictInit(int q, N_Vector* zn, N_Vector acor, N_Vector y) {
+int NVHip_PredictInit(int q, N_Vector* zn, N_Vector acor, N_Vector y, sunrealtype rtol,
+                      sunrealtype atol, N_Vector abstol, N_Vector ewt) {
   if (q < 1 || q > 5 || !acor || !y) return -1;
+  if (ewt && (ewt->ops->nvgetvectorid != N_VGetVectorID_Hip || LEN(ewt) != LEN(zn[0])))
+    return -1;
+  if (abstol && LEN(abstol) != LEN(zn[0])) return -1;
   for (int j = 0; j <= q; j++)
     if (!zn[j] || zn[j]->ops->nvgetvectorid != N_VGetVectorID_Hip) return -1;
   if (acor->ops->nvgetvectorid != N_VGetVectorID_Hip ||
@@ -1295,19 +1307,23 @@ int NVHip_PredictInit(int q, N_Vector* zn, N_Vector acor, N_Vector y) {
     return -1;
   const sunindextype n = LEN(zn[0]);
   if (LEN(acor) != n || LEN(y) != n) return -1;
+  double* dw = ewt ? D(ewt) : nullptr;
   for (int j = 0; j <= q; j++)
-    if (D(zn[j]) == D(acor) || D(zn[j]) == D(y)) return -1;
-  if (D(acor) == D(y)) return -1;
+    if (D(zn[j]) == D(acor) || D(zn[j]) == D(y) || D(zn[j]) == dw) return -1;
+  if (D(acor) == D(y) || D(acor) == dw || D(y) == dw) return -1;
   OutPack zp{};
   for (int j = 0; j <= q; j++) zp.p[j] = D(zn[j]);
   hipStream_t s = S(zn[0]);
   const dim3 g(ew_grid(n)), b(kBlock);
+  const double* av = abstol ? D(abstol) : nullptr;
   switch (q) {
-    case 1: hipLaunchKernelGGL(k_pascal_init<1>, g, b, 0, s, zp, D(acor), D(y), n); break;
-    case 2: hipLaunchKernelGGL(k_pascal_init<2>, g, b, 0, s, zp, D(acor), D(y), n); break;
-    case 3: hipLaunchKernelGGL(k_pascal_init<3>, g, b, 0, s, zp, D(acor), D(y), n); break;
-    case 4: hipLaunchKernelGGL(k_pascal_init<4>, g, b, 0, s, zp, D(acor), D(y), n); break;
-    default: hipLaunchKernelGGL(k_pascal_init<5>, g, b, 0, s, zp, D(acor), D(y), n); break;
+#define NVHIP_PI_CASE(k) \
+    case k: hipLaunchKernelGGL(k_pascal_init<k>, g, b, 0, s, zp, D(acor), D(y), n, rtol, atol, \
+                               av, dw); break;
+    NVHIP_PI_CASE(1) NVHIP_PI_CASE(2) NVHIP_PI_CASE(3) NVHIP_PI_CASE(4)
+#undef NVHIP_PI_CASE
+    default: hipLaunchKernelGGL(k_pascal_init<5>, g, b, 0, s, zp, D(acor), D(y), n, rtol, atol,
+                                av, dw); break;
   }
   check_launch("pascal_init");
   return 0;

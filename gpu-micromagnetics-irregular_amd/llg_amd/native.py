@@ -158,7 +158,8 @@ SIGNATURES = {
     "N_VWrmsNormMaskVectorArray_Hip": (C.c_int, [C.c_int, C.POINTER(VP), C.POINTER(VP), VP, DP]),
     "NVHip_Pascal": (C.c_int, [C.c_int, C.POINTER(VP), C.c_double]),
     "NVHip_EwtSet": (C.c_int, [VP, C.c_double, C.c_double, VP, VP]),
-    "NVHip_PredictInit": (C.c_int, [C.c_int, C.POINTER(VP), VP, VP]),
+    "NVHip_PredictInit": (C.c_int, [C.c_int, C.POINTER(VP), VP, VP, C.c_double, C.c_double,
+                                    VP, VP]),
     "NVHip_WrmsNormPair": (C.c_int, [VP, C.c_double, VP, C.c_double, VP, VP, DP]),
     "NVHip_ScaleDiv": (C.c_int, [C.c_double, VP, VP, VP, VP]),
     "NVHip_ProdDotMulti": (C.c_int, [VP, VP, VP, C.c_int, C.POINTER(VP), DP]),

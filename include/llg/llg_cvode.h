@@ -139,8 +139,10 @@ typedef struct llgcv_fused_ops {
                        void* user_data, sunrealtype* r);
   int (*newtonupd_comb)(N_Vector ycor, int nvec, const sunrealtype* c, N_Vector* X,
                         N_Vector s, N_Vector zn0, N_Vector w, N_Vector y, sunrealtype* dnorm);
-  /* predictor + acor = 0 + y = zn[0] + acor in one pass (as NVHip_PredictInit) */
-  int (*predict_init)(int q, N_Vector* zn, N_Vector acor, N_Vector y);
+  /* predictor + acor = 0 + y = zn[0] + acor in one pass, with ewt not NULL
+   * first the error weights from the old zn[0] (as NVHip_PredictInit) */
+  int (*predict_init)(int q, N_Vector* zn, N_Vector acor, N_Vector y, sunrealtype rtol,
+                      sunrealtype atol, N_Vector abstol, N_Vector ewt);
   /* nrm[0] = |x|_WRMS(w), nrm[1] = |a y + b z|_WRMS(w) (as NVHip_WrmsNormPair) */
   int (*wrms_pair)(N_Vector x, sunrealtype a, N_Vector y, sunrealtype b, N_Vector z,
                    N_Vector w, sunrealtype* nrm);

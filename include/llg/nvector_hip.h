@@ -115,8 +115,12 @@ int NVHip_Pascal(int q, N_Vector* zn, sunrealtype sign);
 int NVHip_EwtSet(N_Vector y, sunrealtype rtol, sunrealtype atol, N_Vector abstol, N_Vector w);
 /* The predictor (NVHip_Pascal with sign +1) and the start of the Newton
  * iteration in one pass: acor = 0 (N_VConst) and y = zn[0] + acor
- * (N_VLinearSum(1, zn[0], 1, acor, y)).  acor and y may not alias zn. */
-int NVHip_PredictInit(int q, N_Vector* zn, N_Vector acor, N_Vector y);
+ * (N_VLinearSum(1, zn[0], 1, acor, y)); with ewt not NULL, first the error
+ * weights of the step from the zn[0] being replaced (NVHip_EwtSet(zn[0],
+ * rtol, atol, abstol, ewt)).  Bit-equal to those calls.  acor, y and ewt
+ * may not alias zn or each other. */
+int NVHip_PredictInit(int q, N_Vector* zn, N_Vector acor, N_Vector y, sunrealtype rtol,
+                      sunrealtype atol, N_Vector abstol, N_Vector ewt);
 /* Two WRMS norms with one host wait: nrm[0] = |x|_WRMS(w) and nrm[1] =
  * |a y + b z|_WRMS(w), the combination not written; each equal to
  * N_VWrmsNorm (after N_VLinearSum for the second) bit for bit. */

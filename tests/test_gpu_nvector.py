@@ -383,11 +383,11 @@ def test_newton_update_equals_linear_sums_and_norms(it):
         assert yn.value == yna
 
 
-@pytest.mark.parametrize("q", [1, 3, 5])
-def test_predict_init_equals_pascal_const_linear_sum(q):
-    """NVHip_PredictInit == NVHip_Pascal(+1), N_VConst(0, acor) and
-    N_VLinearSum(1, zn[0], 1, acor, y), bit for bit (a -0 in zn[0] gives
-    +0 in y, as the sum does)"""
+@pytest.mark.parametrize("q,weights", [(1, None), (3, "scalar"), (5, "vector")])
+def test_predict_init_equals_pascal_const_linear_sum(q, weights):
+    """NVHip_PredictInit == NVHip_EwtSet on the old zn[0] (when asked),
+    NVHip_Pascal(+1), N_VConst(0, acor) and N_VLinearSum(1, zn[0], 1, acor,
+    y), bit for bit (a -0 in zn[0] gives +0 in y, as the sum does)"""
     L = la.lib()
     n = 30_001
     rng = np.random.default_rng(60 + q)
@@ -398,16 +398,25 @@ def test_predict_init_equals_pascal_const_linear_sum(q):
     za, zb = [vec(a) for a in Z], [vec(a) for a in Z]
     acor_a, y_a = vec(rng.standard_normal(n)), vec(rng.standard_normal(n))
     acor_b, y_b = vec(rng.standard_normal(n)), vec(rng.standard_normal(n))
+    av = vec(rng.uniform(1e-6, 1e-3, n)) if weights == "vector" else None
+    w_a, w_b = vec(np.zeros(n)), vec(np.zeros(n))
+    rtol, atol = 1e-4, (0.0 if weights == "vector" else 1e-5)
+    if weights:
+        assert L.NVHip_EwtSet(za[0].v, rtol, atol, av.v if av else None, w_a.v) == 0
     assert L.NVHip_Pascal(q, arr(za), 1.0) == 0
     L.N_VConst_Hip(0.0, acor_a.v)
     L.N_VLinearSum_Hip(1.0, za[0].v, 1.0, acor_a.v, y_a.v)
-    assert L.NVHip_PredictInit(q, arr(zb), acor_b.v, y_b.v) == 0
+    assert L.NVHip_PredictInit(q, arr(zb), acor_b.v, y_b.v, rtol, atol, av.v if av else None,
+                               w_b.v if weights else None) == 0
     for j in range(q + 1):
         assert np.array_equal(za[j].get(), zb[j].get()), j
     assert np.array_equal(acor_a.get(), acor_b.get())
     ya, yb = y_a.get(), y_b.get()
     assert np.array_equal(ya, yb) and np.array_equal(np.signbit(ya), np.signbit(yb))
-    assert L.NVHip_PredictInit(q, arr(zb), zb[0].v, y_b.v) == -1   # acor aliases zn[0]
+    if weights:
+        assert np.array_equal(w_a.get(), w_b.get())
+    assert L.NVHip_PredictInit(q, arr(zb), zb[0].v, y_b.v, rtol, atol, None,
+                               None) == -1   # acor aliases zn[0]
 
 
 def test_wrms_norm_pair_equals_two_norms():

@@ -600,7 +600,7 @@ struct ArnIn {
 // tile's own cells; w is written; each workgroup leaves its partial sums
 //   [|u|^2, <w,U_0>, .., <w,U_{L-1}>, <w,u>, |w|^2] (and with PRE also
 //   sum (win s)^2, the square of the WRMS norm of the right-hand side)
-// at partials[blockIdx.x * K + k], summed in workgroup order by the
+// at partials[k * gridDim.x + blockIdx.x], summed in workgroup order by the
 // caller's finish pass.  Workgroup b takes the tiles of the band of XCD
 // (b mod 8) in grid-stride order, so any grid size covers the grid.
 // (Keeping the own cell's U_j and P in LDS instead of re-reading them
@@ -727,17 +727,21 @@ __global__ __launch_bounds__(BX* BY) void k_arnoldi(GeoD g, TermsD T, int pgate,
     __syncthreads();   // the tiles are restaged by the next iteration
   }
   const int lane = tid & 63, wave = tid >> 6;
+  // the K cross-lane chains interleaved so their shuffle latencies overlap
 #pragma unroll
-  for (int k = 0; k < K; k++) {
-    double a = acc[k];
-    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-    if (lane == 0) sm[k][wave] = a;
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] += __shfl_xor(acc[k], off, 64);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) sm[k][wave] = acc[k];
   }
   __syncthreads();
   if (tid < K) {
     double a = sm[tid][0];
     for (int w = 1; w < NT / 64; w++) a += sm[tid][w];
-    partials[blockIdx.x * K + tid] = a;
+    partials[tid * gridDim.x + blockIdx.x] = a;
   }
 }
 

@@ -427,11 +427,16 @@ struct FNewtonUpdComb0 {
 template <int K, int W = kWaves>
 __device__ void block_reduce(int kind, double (&v)[K], double (*sm)[W]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the K cross-lane chains interleaved step by step, so their shuffle
+  // latencies overlap (each value sees the same combines in the same order)
 #pragma unroll
-  for (int k = 0; k < K; k++) {
-    double a = v[k];
-    for (int off = 32; off > 0; off >>= 1) a = combine(kind, a, __shfl_xor(a, off, 64));
-    if (lane == 0) sm[k][wave] = a;
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = combine(kind, v[k], __shfl_xor(v[k], off, 64));
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) sm[k][wave] = v[k];
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -444,7 +449,8 @@ __device__ void block_reduce(int kind, double (&v)[K], double (*sm)[W]) {
   }
 }
 
-// Final pass: partials[b*K + k] for b < nblk.  SAME_KERNEL: run by the last
+// Final pass: partials[k*nblk + b] for b < nblk (one row of nblk per
+// result, so a wave's loads are contiguous).  SAME_KERNEL: run by the last
 // workgroup of the kernel that wrote them, so each load is an agent-scope
 // atomic that no stale per-CU L1 line can serve; otherwise (a kernel of its
 // own) plain loads, which the hardware keeps in flight together.  Same
@@ -465,7 +471,7 @@ __device__ void finish(int kind, int nblk, const double* partials, double* out, 
 #pragma unroll
       for (int u = 0; u < U; u++)
 #pragma unroll
-        for (int k = 0; k < K; k++) p[u][k] = partials[(b + u * blockDim.x) * K + k];
+        for (int k = 0; k < K; k++) p[u][k] = partials[k * nblk + b + u * blockDim.x];
 #pragma unroll
       for (int u = 0; u < U; u++)
 #pragma unroll
@@ -475,9 +481,9 @@ __device__ void finish(int kind, int nblk, const double* partials, double* out, 
   for (; b < nblk; b += blockDim.x) {
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      const double p = SAME_KERNEL ? __hip_atomic_load(partials + b * K + k, __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT)
-                                   : partials[b * K + k];
+      const double p = SAME_KERNEL ? __hip_atomic_load(partials + k * nblk + b,
+                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                   : partials[k * nblk + b];
       v[k] = combine(kind, v[k], p);
     }
   }
@@ -570,8 +576,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce2(int kind, F1 f1, F2 f2, suni
                  combine(kind, combine(kind, b0, b1), combine(kind, b2, b3))};
   block_reduce<2>(kind, v, sm);
   if (threadIdx.x == 0) {
-    partials[blockIdx.x * 2] = v[0];
-    partials[blockIdx.x * 2 + 1] = v[1];
+    partials[blockIdx.x] = v[0];
+    partials[gridDim.x + blockIdx.x] = v[1];
   }
 }
 
@@ -645,13 +651,13 @@ __global__ __launch_bounds__(kBlock) void k_dotmulti(const double* x, PtrPack Y,
   if (!ticket) {
     if (threadIdx.x == 0) {
 #pragma unroll
-      for (int k = 0; k < K; k++) partials[blockIdx.x * K + k] = v[k];
+      for (int k = 0; k < K; k++) partials[k * gridDim.x + blockIdx.x] = v[k];
     }
     return;
   }
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < K; k++) partials[blockIdx.x * K + k] = v[k];
+    for (int k = 0; k < K; k++) partials[k * gridDim.x + blockIdx.x] = v[k];
     __threadfence();
     const unsigned t = atomicAdd(ticket, 1u);
     last = (t == gridDim.x - 1);
@@ -719,13 +725,13 @@ __global__ __launch_bounds__(kBlock) void k_wsqarray(PtrPack X, PtrPack W, const
   if (!ticket) {
     if (threadIdx.x == 0) {
 #pragma unroll
-      for (int k = 0; k < K; k++) partials[blockIdx.x * K + k] = v[k];
+      for (int k = 0; k < K; k++) partials[k * gridDim.x + blockIdx.x] = v[k];
     }
     return;
   }
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < K; k++) partials[blockIdx.x * K + k] = v[k];
+    for (int k = 0; k < K; k++) partials[k * gridDim.x + blockIdx.x] = v[k];
     __threadfence();
     const unsigned t = atomicAdd(ticket, 1u);
     last = (t == gridDim.x - 1);

@@ -206,7 +206,7 @@ int NVHip_SpgmrSolve(NVHipDevOp op, void* op_data, N_Vector t0, N_Vector* V, int
 
 /* Reductions for kernels outside this file (no counterpart in the
  * reference): a kernel queued on v's stream leaves K partial sums per
- * workgroup at partials[b*K + k] (b*K + k < *capacity, from
+ * workgroup at partials[k*nblk + b] (nblk*K <= *capacity, from
  * NVHip_ReducePartials); NVHip_ReduceFinish sums them over b < nblk in
  * workgroup order (deterministic), waits once and copies the K results to
  * out.  K <= NVHIP_FUSED_MAX.  Returns NULL / -1 for a non-HIP vector or

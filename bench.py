@@ -234,7 +234,9 @@ def main():
     ap.add_argument("--fused-extra", type=int, default=0,
                     help="bits added to the integrator's fused-kernel mask (64: the first "
                          "SPGMR pass forms the Newton residual where it reads it; 128: the "
-                         "first Newton update forms the SPGMR solution where it reads it)")
+                         "first Newton update forms the SPGMR solution where it reads it; "
+                         "every --spgmr mask already holds 256, the Newton residual formed "
+                         "in the RHS pass)")
     ap.add_argument("--red-blocks", type=int, default=0,
                     help="partial workgroups per reduction (NVHip_SetReduceConfig; 0: the "
                          "library default)")
@@ -283,7 +285,7 @@ def main():
     step = 1.0
     ropts = dict(rtol=1e-4, atol=1e-4, maxl=5, gstype=args.gstype, max_order=5, t_out=0.1,
                  mxstep=100000)
-    ropts["fused"] = {"pipelined": 55, "lagged": 23, "host": 7, "device": 15}[args.spgmr] | \
+    ropts["fused"] = {"pipelined": 311, "lagged": 279, "host": 263, "device": 271}[args.spgmr] | \
         args.fused_extra
     if args.red_blocks:
         la.lib().NVHip_SetReduceConfig(args.red_blocks, 1)

@@ -1040,13 +1040,25 @@ int ls_solve(Integrator* m, N_Vector b, N_Vector weight, N_Vector ynow, N_Vector
 int nls_residual(Integrator* m, N_Vector ycor, N_Vector negres) {
   if (!m->y_current) N_VLinearSum(1.0, m->zn[0], 1.0, ycor, m->y);
   m->y_current = false;
+  double c[3] = {-m->rl1, -1.0, m->gamma};
+  const bool first_pass = m->fused.arnoldi_first && lagged_ok(m) && !m->jtsetup;
+  if (m->fused.rhs_residual && m->jtimes && !m->jtsetup && !first_pass) {
+    // f and the combination in one pass; f is not stored (the difference-
+    // quotient J v and a jtimes setup would read it, so neither is in use)
+    const int rr = m->fused.rhs_residual(m->tn, m->y, c, m->zn[1], ycor, negres, m->user_data);
+    if (rr != 2) {
+      m->nfe++;
+      if (rr < 0) return LLGCV_RHSFUNC_FAIL;
+      if (rr > 0) return RHSFUNC_RECVR;
+      return 0;
+    }
+  }
   const int rv = m->f(m->tn, m->y, m->ftemp, m->user_data);
   m->nfe++;
   if (rv < 0) return LLGCV_RHSFUNC_FAIL;
   if (rv > 0) return RHSFUNC_RECVR;
-  double c[3] = {-m->rl1, -1.0, m->gamma};
   N_Vector X[3] = {m->zn[1], ycor, m->ftemp};
-  if (m->fused.arnoldi_first && lagged_ok(m) && !m->jtsetup) {
+  if (first_pass) {
     // formed by the linear solve's first pass (form_b for any other reader)
     auto& p = m->pend_b;
     for (int k = 0; k < 3; k++) {

@@ -8,11 +8,12 @@
 //                 m = u + U b straight into registers (coalesced), runs the
 //                 forward FFT of length L = Px/2, and the real-FFT split
 //                 writes bins 0..L of the row.
-//   column      : lane u of a column loads rows u + U b of the three
-//                 components, runs the three forward FFTs, multiplies by the
-//                 3x3 tensor in registers (each lane holds all three
-//                 components at the same ky), runs the inverse FFTs and
-//                 stores only the rows the field needs.
+//   column      : lane u of a column loads rows u + U b of m_z, runs its
+//                 forward FFT, multiplies by K_zz, runs the inverse and
+//                 stores; then the same for m_x and m_y as a pair, with the
+//                 xy block of the tensor applied in registers (each lane
+//                 holds both components at the same ky).  Only the rows the
+//                 field needs are stored.
 //   row inverse : the C2R pre-processing reads bins k and L-k straight from
 //                 the spectrum into the registers of the last FFT stage,
 //                 the inverse FFT ends in lane u holding z[u + U b], stored
@@ -181,17 +182,24 @@ __global__ __launch_bounds__(F::kThreads) __attribute__((amdgpu_waves_per_eu(Col
   const bool live = kx < Hx;
   if (!live) kx = Hx - 1;
   C* lds = reinterpret_cast<C*>(lds_raw) + slot * F::kSlots;
+  // z first, on its own: load, transform, multiply by K_zz (h_z = K_zz m_z),
+  // transform back, store; then x and y as a pair.  At most two components
+  // are live at once, which keeps the FP64 plans with 8 values per lane
+  // within 128 VGPRs without spilling (loading all three up front spilled).
   C r[3][V];
-  sfor<0, 3>([&](auto Cc) {
+  auto load = [&](auto Cc) {
     const C* src = spec + ((long)Cc * ROWS + rfft::opaque(lane)) * Hx + kx;
     sfor<0, NB>([&](auto B) { r[Cc][B] = ldc(src + (long)U * B * Hx); });
-  });
-  sfor<0, 3>([&](auto Cc) { F::template forward<HALF>(r[Cc], lds, lane, tw, Bar{}); });
-  // h = K m at every (kx, ky) this lane holds, a few values at a time so the
-  // tensor loads in flight stay within the register budget; the z
-  // component (h_z = K_zz m_z) is finished and stored first, so x and y
-  // are the only components live through the rest
-  // (plane bases are uniform; the per-lane offset is a 32-bit unsigned index)
+  };
+  auto store = [&](auto Cc) {
+    if (live) {
+      C* dst = spec + ((long)Cc * ROWS + rfft::opaque(lane)) * Hx + kx;
+      sfor<0, NB>([&](auto B) { stc(dst + (long)U * B * Hx, r[Cc][B]); });
+    }
+  };
+  // the tensor values this lane needs, a few at a time so the loads in
+  // flight stay within the register budget (plane bases are uniform; the
+  // per-lane offset is a 32-bit unsigned index)
   const long P = (long)Hx * L;
   const T* kxx = K + (long)kx * L;
   const T* kyy = kxx + P;
@@ -209,18 +217,20 @@ __global__ __launch_bounds__(F::kThreads) __attribute__((amdgpu_waves_per_eu(Col
       });
     });
   };
-  auto store = [&](auto Cc) {
-    if (live) {
-      C* dst = spec + ((long)Cc * ROWS + rfft::opaque(lane)) * Hx + kx;
-      sfor<0, NB>([&](auto B) { stc(dst + (long)U * B * Hx, r[Cc][B]); });
-    }
-  };
+  constexpr auto c0 = std::integral_constant<int, 0>{};
+  constexpr auto c1 = std::integral_constant<int, 1>{};
+  constexpr auto c2 = std::integral_constant<int, 2>{};
+  load(c2);
+  F::template forward<HALF>(r[2], lds, lane, tw, Bar{});
   each_k([&](int i, unsigned k) {
     const T cz = kzz[k];
     r[2][i] = mk<T>(cz * r[2][i].x, cz * r[2][i].y);
   });
   F::template inverse<HALF>(r[2], lds, lane, tw, Bar{});
-  store(std::integral_constant<int, 2>{});
+  store(c2);
+  load(c0);
+  load(c1);
+  sfor<0, 2>([&](auto Cc) { F::template forward<HALF>(r[Cc], lds, lane, tw, Bar{}); });
   each_k([&](int i, unsigned k) {
     const T a = kxx[k], b = kyy[k], xr = kxyr[k];
     const C mx = r[0][i], my = r[1][i];

@@ -233,8 +233,24 @@ __device__ inline bool map_cell(const GeoD& g, double* out, int& cell, int& gx, 
 }
 
 // ------------------------------------------------------------------ RHS
+// The Newton residual formed in the RHS pass: out = c0 zn1 + c1 ycor + c2 f,
+// combined in the order N_VLinearCombination uses (c0 x0, then fma with
+// each further term), with f the value the plain RHS pass would write.
+struct ResIn {
+  const double* zn1;
+  const double* ycor;
+  double c0, c1, c2;
+};
+__device__ inline double res_of(const ResIn& r, int i, double f) {
+  double acc = r.c0 * r.zn1[i];
+  acc = fma(r.c1, r.ycor[i], acc);
+  return fma(r.c2, f, acc);
+}
+
+template <bool RES>
 __global__ __launch_bounds__(256) void k_rhs(GeoD g, TermsD T, const double* __restrict__ y,
-                                             const double* __restrict__ hd, double* __restrict__ yd) {
+                                             const double* __restrict__ hd, double* __restrict__ yd,
+                                             ResIn rin) {
   int cell, gx, gy;
   if (!map_cell(g, yd, cell, gx, gy)) return;
   const int N = g.ncell;
@@ -250,18 +266,24 @@ __global__ __launch_bounds__(256) void k_rhs(GeoD g, TermsD T, const double* __r
     yd[cell] = f1; yd[N + cell] = f2; yd[2 * N + cell] = f3;
   } else {
     const double gt = gate_of(g, T.gate, cell);
-    yd[cell] = gt * f1; yd[N + cell] = gt * f2; yd[2 * N + cell] = gt * f3;
+    if constexpr (RES) {
+      yd[cell] = res_of(rin, cell, gt * f1);
+      yd[N + cell] = res_of(rin, N + cell, gt * f2);
+      yd[2 * N + cell] = res_of(rin, 2 * N + cell, gt * f3);
+    } else {
+      yd[cell] = gt * f1; yd[N + cell] = gt * f2; yd[2 * N + cell] = gt * f3;
+    }
   }
 }
 
 // LDS-tiled windowed RHS (the polycrystal form, 2d_i3/2d_fft.cu:451-627):
 // a (BX+2)x(BY+2) halo of w*m per component is staged in LDS once and the
 // four neighbours are read from it.  Periodic boundary only.
-template <int BX, int BY>
+template <int BX, int BY, bool RES>
 __global__ __launch_bounds__(BX* BY) void k_rhs_tiled(GeoD g, TermsD T,
                                                        const double* __restrict__ y,
                                                        const double* __restrict__ hd,
-                                                       double* __restrict__ yd) {
+                                                       double* __restrict__ yd, ResIn rin) {
   constexpr int SX = BX + 2, SY = BY + 2;
   __shared__ double tile[3][SY][SX];
   const int N = g.ncell;
@@ -307,9 +329,16 @@ __global__ __launch_bounds__(BX* BY) void k_rhs_tiled(GeoD g, TermsD T,
   const double m1 = m[0], m2 = m[1], m3 = m[2];
   const double mh = m1 * h[0] + m2 * h[1] + m3 * h[2];
   const double gt = gate_of(g, T.gate, cell);
-  yd[cell] = gt * (g.chg * (m3 * h[1] - m2 * h[2]) + g.alpha * (h[0] - mh * m1));
-  yd[N + cell] = gt * (g.chg * (m1 * h[2] - m3 * h[0]) + g.alpha * (h[1] - mh * m2));
-  yd[2 * N + cell] = gt * (g.chg * (m2 * h[0] - m1 * h[1]) + g.alpha * (h[2] - mh * m3));
+  const double f1 = gt * (g.chg * (m3 * h[1] - m2 * h[2]) + g.alpha * (h[0] - mh * m1));
+  const double f2 = gt * (g.chg * (m1 * h[2] - m3 * h[0]) + g.alpha * (h[1] - mh * m2));
+  const double f3 = gt * (g.chg * (m2 * h[0] - m1 * h[1]) + g.alpha * (h[2] - mh * m3));
+  if constexpr (RES) {
+    yd[cell] = res_of(rin, cell, f1);
+    yd[N + cell] = res_of(rin, N + cell, f2);
+    yd[2 * N + cell] = res_of(rin, 2 * N + cell, f3);
+  } else {
+    yd[cell] = f1; yd[N + cell] = f2; yd[2 * N + cell] = f3;
+  }
 }
 
 // ------------------------------------------------------------------- Jv
@@ -1061,15 +1090,13 @@ int LLG_GetCallCounts(llg_ctx* c, long* nf, long* njv, long* nps, long* npv) {
   return 0;
 }
 
-// f(): [normalize] -> demag (overwrite hd) -> RHS (compact zeroes holes in-launch)
-int LLG_Rhs(sunrealtype t, N_Vector y, N_Vector ydot, void* user_data) {
-  (void)t;
-  llg_ctx* c = (llg_ctx*)user_data;
-  if (!c) return -1;
+// f(): [normalize] -> demag (overwrite hd) -> RHS (compact zeroes holes in-launch);
+// with rin the RHS pass writes the Newton residual built from f instead of f
+static int rhs_launch(llg_ctx* c, N_Vector y, N_Vector out, const ResIn* rin) {
   c->nf++;
   double* yd = N_VGetDeviceArrayPointer(y);
-  double* fd = N_VGetDeviceArrayPointer(ydot);
-  hipStream_t s = stream_of(ydot, c);
+  double* fd = N_VGetDeviceArrayPointer(out);
+  hipStream_t s = stream_of(out, c);
   if (c->cfg.normalize != LLG_NORM_NONE) {
     hipLaunchKernelGGL(k_normalize, dim3((c->ncell + k1D - 1) / k1D), dim3(k1D), 0, s, yd,
                        c->ncell, c->cfg.normalize);
@@ -1084,19 +1111,46 @@ int LLG_Rhs(sunrealtype t, N_Vector y, N_Vector ydot, void* user_data) {
   }
   const GeoD g = geo_of(c);
   const TermsD T = to_dev(c->cfg.rhs);
+  const ResIn ri = rin ? *rin : ResIn{nullptr, nullptr, 0.0, 0.0, 0.0};
   if (c->cfg.exec == LLG_EXEC_YMSK && c->cfg.rhs.window && c->cfg.boundary == LLG_BC_PERIODIC) {
     dim3 block(32, 8), grid((c->cfg.ng + 31) / 32, (c->cfg.ny + 7) / 8);
     kprof::begin(kprof::RHS, s);
-    hipLaunchKernelGGL((k_rhs_tiled<32, 8>), grid, block, 0, s, g, T, yd, hd, fd);
+    if (rin) hipLaunchKernelGGL((k_rhs_tiled<32, 8, true>), grid, block, 0, s, g, T, yd, hd, fd, ri);
+    else hipLaunchKernelGGL((k_rhs_tiled<32, 8, false>), grid, block, 0, s, g, T, yd, hd, fd, ri);
     kprof::end(kprof::RHS, s);
   } else {
     dim3 grid, block;
     grid_for(c, grid, block);
     kprof::begin(kprof::RHS, s);
-    hipLaunchKernelGGL(k_rhs, grid, block, 0, s, g, T, yd, hd, fd);
+    if (rin) hipLaunchKernelGGL(k_rhs<true>, grid, block, 0, s, g, T, yd, hd, fd, ri);
+    else hipLaunchKernelGGL(k_rhs<false>, grid, block, 0, s, g, T, yd, hd, fd, ri);
     kprof::end(kprof::RHS, s);
   }
   return launch_status("f");
+}
+
+int LLG_Rhs(sunrealtype t, N_Vector y, N_Vector ydot, void* user_data) {
+  (void)t;
+  llg_ctx* c = (llg_ctx*)user_data;
+  if (!c) return -1;
+  return rhs_launch(c, y, ydot, nullptr);
+}
+
+int LLG_RhsResidual(sunrealtype t, N_Vector y, const sunrealtype* cf, N_Vector zn1,
+                    N_Vector ycor, N_Vector res, void* user_data) {
+  (void)t;
+  llg_ctx* c = (llg_ctx*)user_data;
+  if (!c || !cf || !y || !zn1 || !ycor || !res) return -1;
+  if (c->cfg.exec == LLG_EXEC_COMPACT) return 2;   // holes: the plain RHS and combination
+  const double* rd = N_VGetDeviceArrayPointer(res);
+  if (rd == N_VGetDeviceArrayPointer(y) || rd == N_VGetDeviceArrayPointer(zn1) ||
+      rd == N_VGetDeviceArrayPointer(ycor)) {
+    set_err("LLG_RhsResidual: res aliases an input");
+    return -1;
+  }
+  const ResIn ri{N_VGetDeviceArrayPointer(zn1), N_VGetDeviceArrayPointer(ycor), cf[0], cf[1],
+                 cf[2]};
+  return rhs_launch(c, y, res, &ri);
 }
 
 int JtvProduct(N_Vector v, N_Vector Jv, sunrealtype t, N_Vector y, N_Vector fy, void* user_data,

@@ -220,6 +220,7 @@ SIGNATURES = {
     "LLG_EventDestroy": (None, [VP]),
     "LLG_DeviceSynchronize": (C.c_int, []),
     "LLG_Rhs": (C.c_int, [C.c_double, VP, VP, VP]),
+    "LLG_RhsResidual": (C.c_int, [C.c_double, VP, DP, VP, VP, VP, VP]),
     "JtvProduct": (C.c_int, [VP, VP, C.c_double, VP, VP, VP, VP]),
     "LLG_Atimes": (C.c_int, [VP, VP, C.c_double, C.c_double, VP, VP, VP]),
     "LLG_Krylov": (C.c_int, [C.c_double, VP, VP, VP, C.c_double, C.c_double, VP, VP, VP]),

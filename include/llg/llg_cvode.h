@@ -146,6 +146,12 @@ typedef struct llgcv_fused_ops {
   /* nrm[0] = |x|_WRMS(w), nrm[1] = |a y + b z|_WRMS(w) (as NVHip_WrmsNormPair) */
   int (*wrms_pair)(N_Vector x, sunrealtype a, N_Vector y, sunrealtype b, N_Vector z,
                    N_Vector w, sunrealtype* nrm);
+  /* the Newton residual res = c0 zn1 + c1 ycor + c2 f(t, y) with f formed in
+   * the same pass and not stored (as LLG_RhsResidual: 0 done, 2 not handled,
+   * then the RHS callback and the combination run; < 0 RHS failure).  Only
+   * for RHS, jtimes and preconditioner callbacks that ignore fy. */
+  int (*rhs_residual)(sunrealtype t, N_Vector y, const sunrealtype* c, N_Vector zn1,
+                      N_Vector ycor, N_Vector res, void* user_data);
 } llgcv_fused_ops;
 
 void* LLGCVodeCreate(int lmm, SUNContext sunctx);

@@ -48,8 +48,11 @@ typedef struct llg_run_opts {
                          first SPGMR pass forms the Newton residual, and
                          the first Newton update the SPGMR solution, where
                          each is read (never written; measured neutral on
-                         the 1536x512 workload); default 23 (bits 0, 1, 2,
-                         4).  Bits 1 to 3 change no value. */
+                         the 1536x512 workload); bit 8 (with bit 1 and
+                         analytic Jv): the Newton residual formed in the
+                         RHS pass (LLG_RhsResidual; f is not stored);
+                         default 279 (bits 0, 1, 2, 4, 8).  Bits 1 to 3 and
+                         8 change no value. */
   long   mxstep;      /* max steps per CVode call (CVODE default 500) */
   int    use_jtv;     /* analytic Jv (else difference quotient) */
   int    use_prec;    /* block-Jacobi preconditioner */

@@ -142,6 +142,16 @@ int    LLG_DeviceSynchronize(void);
 
 /* CVODE plug-ins (user_data = llg_ctx*) */
 int LLG_Rhs(sunrealtype t, N_Vector y, N_Vector ydot, void* user_data);
+/* The right-hand side and the Newton residual built from it in one RHS
+ * pass: res = c[0] zn1 + c[1] ycor + c[2] f(t, y), equal bit for bit to
+ * LLG_Rhs into a vector f followed by N_VLinearCombination(3, c,
+ * {zn1, ycor, f}, res) (CVODE's cvNlsResidual); f itself is not written.
+ * res must not alias y, zn1 or ycor.  Returns 0, -1 on error, or 2 without
+ * doing anything when the grid runs in compact mode (then call LLG_Rhs and
+ * combine).  No reference counterpart: installed as the integrator's fused
+ * Newton residual. */
+int LLG_RhsResidual(sunrealtype t, N_Vector y, const sunrealtype* c, N_Vector zn1,
+                    N_Vector ycor, N_Vector res, void* user_data);
 int JtvProduct(N_Vector v, N_Vector Jv, sunrealtype t, N_Vector y, N_Vector fy,
                void* user_data, N_Vector tmp);
 /* The SPGMR operator of the Newton system in one launch:

@@ -186,6 +186,35 @@ def test_atimes_equals_jtv_then_linear_sum(case):
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_rhs_residual_equals_rhs_then_linear_combination(case):
+    """LLG_RhsResidual == LLG_Rhs into f, then N_VLinearCombination(3,
+    {-rl1, -1, gamma}, {zn1, ycor, f}), bit for bit (the Newton residual of
+    CVODE's cvNlsResidual); compact grids return 2 and leave res alone"""
+    cfg, geom, y0 = make(case)
+    sim = la.Simulation(cfg, geom)
+    L = la.lib()
+    rng = np.random.default_rng(21)
+    n = sim.neq
+    y = perturbed(y0, geom.active, 6)
+    ya, yb = sim.vec(y), sim.vec(y)   # the RHS may normalise y in place
+    zn1 = sim.vec(rng.standard_normal(n))
+    ycor = sim.vec(1e-3 * rng.standard_normal(n))
+    f, ra = sim.vec(np.zeros(n)), sim.vec(np.zeros(n))
+    rb = sim.vec(np.full(n, 7.0))
+    c = (C.c_double * 3)(-0.8125, -1.0, 0.0371)
+    assert L.LLG_Rhs(0.0, ya.v, f.v, sim.ctx) == 0
+    L.N_VLinearCombination_Hip(3, c, (C.c_void_p * 3)(zn1.v, ycor.v, f.v), ra.v)
+    rc = L.LLG_RhsResidual(0.0, yb.v, c, zn1.v, ycor.v, rb.v, sim.ctx)
+    if cfg.exec_ == la.EXEC_COMPACT:
+        assert rc == 2 and np.all(rb.get() == 7.0)
+        return
+    assert rc == 0
+    assert np.array_equal(ya.get(), yb.get())
+    assert np.array_equal(ra.get(), rb.get())
+    assert L.LLG_RhsResidual(0.0, yb.v, c, zn1.v, ycor.v, ycor.v, sim.ctx) == -1
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_krylov_chain_equals_separate_calls(case):
     """LLG_Krylov == NVHip_ScaleDiv + LLG_Atimes + PrecondSolve + N_VProd and
     NVHip_GSProjectScaled == (N_VScale of the basis vector) + NVHip_GSProject,

@@ -236,12 +236,21 @@ def main():
                          "SPGMR pass forms the Newton residual where it reads it; 128: the "
                          "first Newton update forms the SPGMR solution where it reads it; "
                          "every --spgmr mask already holds 256, the Newton residual formed "
-                         "in the RHS pass)")
+                         "in the RHS pass, and 512, each step's completion left to the next "
+                         "predictor's pass)")
     ap.add_argument("--red-blocks", type=int, default=0,
                     help="partial workgroups per reduction (NVHip_SetReduceConfig; 0: the "
                          "library default)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the launch-shape autotuner at setup (untimed)")
+    ap.add_argument("--fft-kernels", default=None,
+                    help="demag FFT kernel families ROW,COL kept after the autotuner "
+                         "(Demag_SetFftKernels: 0 / 1 register-resident variants, -1 "
+                         "Stockham), or 'auto' for the autotuner's choice.  The families "
+                         "round differently, and over 100 time units the step count the "
+                         "metric depends on follows the rounding (3,768 steps with 0,0; "
+                         "4,703 with column variant 1), so the FP64 default pins 0,0 "
+                         "(the FP32-spectra default: auto)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -281,11 +290,21 @@ def main():
             tune = {"stencil_block": [ch[0], ch[1]], "demag_row_block": ch[2],
                     "demag_col_block": ch[3], "demag_row_pad": ch[4], "demag_col_pad": ch[5],
                     "demag_row_kernel": ch[6], "demag_col_kernel": ch[7]}
+    if args.fft_kernels is None:
+        args.fft_kernels = "auto" if args.fp32_demag else "0,0"
+    if args.fft_kernels != "auto":
+        rk, ck = (int(v) for v in args.fft_kernels.split(","))
+        dm = la.lib().LLG_GetDemag(sim.ctx)
+        if dm and la.lib().Demag_SetFftKernels(dm, rk, ck) != 0:
+            raise SystemExit(f"--fft-kernels {rk},{ck}: no such kernel for this grid")
+        if tune is not None:
+            tune["demag_row_kernel"], tune["demag_col_kernel"] = rk, ck
+            tune["fft_kernels"] = "pinned"
     setup_s = time.perf_counter() - t_setup
     step = 1.0
     ropts = dict(rtol=1e-4, atol=1e-4, maxl=5, gstype=args.gstype, max_order=5, t_out=0.1,
                  mxstep=100000)
-    ropts["fused"] = {"pipelined": 311, "lagged": 279, "host": 263, "device": 271}[args.spgmr] | \
+    ropts["fused"] = {"pipelined": 823, "lagged": 791, "host": 775, "device": 783}[args.spgmr] | \
         args.fused_extra
     if args.red_blocks:
         la.lib().NVHip_SetReduceConfig(args.red_blocks, 1)

@@ -150,6 +150,16 @@ struct Integrator {
     N_Vector s = nullptr;
     N_Vector dst = nullptr;
   } pend_x;
+  // The step's completion zn[j] += l[j] acor (j = 0..q) not yet written,
+  // with the rescale zn[j] *= c[j] (j = 1..q) of a step-size change when
+  // scaled: left to the next predictor's pass (flush_completion writes it
+  // where another reader of zn needs it).
+  struct {
+    bool on = false, scaled = false;
+    int q = 0;
+    double l[kLMax] = {};
+    double c[kLMax] = {};
+  } pend_c;
   N_Vector V[kMaxKrylov + 1] = {};
   N_Vector xcor = nullptr, vtemp = nullptr, ytemp = nullptr;
   double Hes[kMaxKrylov + 1][kMaxKrylov] = {};
@@ -277,6 +287,16 @@ void set_coeffs(Integrator* m) {
   m->gamrat = (m->nst > 0) ? m->gamma / m->gammap : 1.0;
 }
 
+// write a completion (and rescale) left pending by complete_step / rescale
+void flush_completion(Integrator* m) {
+  auto& p = m->pend_c;
+  if (!p.on) return;
+  p.on = false;
+  N_VScaleAddMulti(p.q + 1, p.l, m->acor, m->zn, m->zn);
+  if (p.scaled) N_VScaleVectorArray(p.q, p.c + 1, m->zn + 1, m->zn + 1);
+  p.scaled = false;
+}
+
 void predict(Integrator* m) {
   m->tn += m->h;
   if (m->tstopset && (m->tn - m->tstop) * m->h > 0.0) m->tn = m->tstop;
@@ -284,6 +304,23 @@ void predict(Integrator* m) {
   // pass (cv_nls and the first residual then skip both), and the step's
   // error weights when they were left to it
   N_Vector w = m->ewt_pending ? m->ewt : nullptr;
+  auto& pc = m->pend_c;
+  if (pc.on) {
+    // the previous step's completion (and rescale) in the same pass
+    pc.on = false;
+    m->newton_init = pc.q == m->q && m->fused.predict_complete &&
+                     m->fused.predict_complete(m->q, m->zn, m->acor, pc.l,
+                                               pc.scaled ? pc.c : nullptr, m->y, m->reltol,
+                                               m->sabstol, m->itol == 2 ? m->vabstol : nullptr,
+                                               w) == 0;
+    if (m->newton_init) {
+      pc.scaled = false;
+      m->ewt_pending = false;
+      return;
+    }
+    pc.on = true;
+    flush_completion(m);
+  }
   m->newton_init = m->fused.predict_init &&
                    m->fused.predict_init(m->q, m->zn, m->acor, m->y, m->reltol, m->sabstol,
                                          m->itol == 2 ? m->vabstol : nullptr, w) == 0;
@@ -311,7 +348,15 @@ void rescale(Integrator* m) {
   double c[kLMax];
   c[0] = m->eta;
   for (int j = 1; j <= m->q; j++) c[j] = m->eta * c[j - 1];
-  N_VScaleVectorArray(m->q, c, m->zn + 1, m->zn + 1);
+  auto& pc = m->pend_c;
+  if (pc.on && !pc.scaled && pc.q == m->q) {
+    // zn[j] *= c[j-1], j = 1..q, after the pending completion
+    for (int j = 1; j <= m->q; j++) pc.c[j] = c[j - 1];
+    pc.scaled = true;
+  } else {
+    flush_completion(m);
+    N_VScaleVectorArray(m->q, c, m->zn + 1, m->zn + 1);
+  }
   m->h = m->hscale * m->eta;
   m->next_h = m->h;
   m->hscale = m->h;
@@ -363,6 +408,7 @@ void adjust_order(Integrator* m, int deltaq) {
 
 void adjust_params(Integrator* m) {
   if (m->qprime != m->q) {
+    flush_completion(m);
     adjust_order(m, m->qprime - m->q);
     m->q = m->qprime;
     m->L = m->q + 1;
@@ -1295,7 +1341,16 @@ void complete_step(Integrator* m) {
   for (int i = m->q; i >= 2; i--) m->tau[i] = m->tau[i - 1];
   if (m->q == 1 && m->nst > 1) m->tau[2] = m->tau[1];
   m->tau[1] = m->h;
-  N_VScaleAddMulti(m->q + 1, m->l, m->acor, m->zn, m->zn);
+  if (m->fused.predict_complete && m->fused.predict_init) {
+    // left to the next predictor's pass (flush_completion for other readers)
+    auto& pc = m->pend_c;
+    pc.on = true;
+    pc.scaled = false;
+    pc.q = m->q;
+    for (int j = 0; j <= m->q; j++) pc.l[j] = m->l[j];
+  } else {
+    N_VScaleAddMulti(m->q + 1, m->l, m->acor, m->zn, m->zn);
+  }
   m->qwait--;
   if (m->qwait == 1 && m->q != m->qmax) {
     N_VScale(1.0, m->acor, m->zn[m->qmax]);
@@ -1324,6 +1379,7 @@ void set_eta(Integrator* m) {
 double compute_etaqm1(Integrator* m) {
   m->etaqm1 = 0.0;
   if (m->q > 1) {
+    flush_completion(m);
     const double ddn = N_VWrmsNorm(m->zn[m->q], m->ewt) * m->tq[1];
     m->etaqm1 = 1.0 / (rpow(kBias1 * ddn, 1.0 / m->q) + kAddon);
   }
@@ -1383,9 +1439,20 @@ void prepare_next_step(Integrator* m, double dsm) {
   const double cquot = both ? (m->tq[5] / m->saved_tq5) *
                                   std::pow(m->h / m->tau[2], (double)m->L)
                             : 0.0;
-  if (both && m->fused.wrms_pair &&
-      m->fused.wrms_pair(m->zn[m->q], -cquot, m->zn[m->qmax], 1.0, m->acor, m->ewt, nr) == 0) {
-    // both order-change norms with one wait (the same values as below)
+  // both order-change norms with one wait (the same values as below); the
+  // completed zn[q] is read in its pending form when the completion is
+  // still left to the predictor
+  auto& pc = m->pend_c;
+  bool paired = false;
+  if (both && pc.on && !pc.scaled && pc.q == m->q && m->fused.wrms_pair_comp)
+    paired = m->fused.wrms_pair_comp(m->zn[m->q], pc.l[m->q], m->acor, -cquot, m->zn[m->qmax],
+                                     1.0, m->acor, m->ewt, nr) == 0;
+  if (both && !paired && m->fused.wrms_pair) {
+    flush_completion(m);
+    paired = m->fused.wrms_pair(m->zn[m->q], -cquot, m->zn[m->qmax], 1.0, m->acor, m->ewt,
+                                nr) == 0;
+  }
+  if (paired) {
     const double ddn = nr[0] * m->tq[1];
     m->etaqm1 = 1.0 / (rpow(kBias1 * ddn, 1.0 / m->q) + kAddon);
     const double dup = nr[1] * m->tq[3];
@@ -1510,6 +1577,7 @@ void* LLGCVodeCreate(int lmm, SUNContext sunctx) {
 int LLGCVodeSetFusedOps(void* mem, const llgcv_fused_ops* ops) {
   if (!mem) return LLGCV_MEM_NULL;
   Integrator* m = static_cast<Integrator*>(mem);
+  flush_completion(m);
   if (ops) m->fused = *ops;
   else m->fused = llgcv_fused_ops{};
   return LLGCV_SUCCESS;
@@ -1538,6 +1606,7 @@ int LLGCVodeReInit(void* mem, sunrealtype t0, N_Vector y0) {
   if (!mem) return LLGCV_MEM_NULL;
   Integrator* m = static_cast<Integrator*>(mem);
   if (!m->zn[0]) return LLGCV_ILL_INPUT;
+  m->pend_c.on = m->pend_c.scaled = false;   // zn is replaced
   m->tn = t0;
   m->tretlast = t0;
   N_VScale(1.0, y0, m->zn[0]);
@@ -1681,6 +1750,7 @@ int LLGCVodeGetDky(void* mem, sunrealtype t, int k, N_Vector dky) {
   if (!mem) return LLGCV_MEM_NULL;
   Integrator* m = static_cast<Integrator*>(mem);
   if (!dky || k < 0 || k > m->q) return LLGCV_ILL_INPUT;
+  flush_completion(m);
   double tfuzz = kFuzz * m->uround * (std::fabs(m->tn) + std::fabs(m->hu));
   if (m->hu < 0.0) tfuzz = -tfuzz;
   const double tp = m->tn - m->hu - tfuzz;
@@ -1761,9 +1831,11 @@ int LLGCVode(void* mem, sunrealtype tout, N_Vector yout, sunrealtype* tret, int 
         m->reltol >= 2.0 * m->uround) {
       m->ewt_pending = true;
     } else if (m->nst > 0) {
+      flush_completion(m);
       if (ewt_set(m, m->zn[0], m->ewt) != 0) {
         istate = LLGCV_ILL_INPUT;
         m->tretlast = *tret = m->tn;
+        flush_completion(m);
         N_VScale(1.0, m->zn[0], yout);
         break;
       }
@@ -1771,17 +1843,20 @@ int LLGCVode(void* mem, sunrealtype tout, N_Vector yout, sunrealtype* tret, int 
     if (m->mxstep > 0 && nstloc >= m->mxstep) {
       istate = LLGCV_TOO_MUCH_WORK;
       m->tretlast = *tret = m->tn;
+      flush_completion(m);
       N_VScale(1.0, m->zn[0], yout);
       break;
     }
     // ewt was just formed from zn[0] itself, so |zn0_i| ewt_i < 1/rtol and
     // uround |zn[0]|_WRMS < uround/rtol: with rtol >= 2 uround the test
     // below cannot fail and its reduction (and host wait) is skipped
+    if (m->reltol < 2.0 * m->uround) flush_completion(m);
     const double nrm = (m->reltol >= 2.0 * m->uround) ? 0.0 : N_VWrmsNorm(m->zn[0], m->ewt);
     m->tolsf = m->uround * nrm;
     if (m->tolsf > 1.0) {
       istate = LLGCV_TOO_MUCH_ACC;
       m->tretlast = *tret = m->tn;
+      flush_completion(m);
       N_VScale(1.0, m->zn[0], yout);
       m->tolsf *= 2.0;
       break;
@@ -1793,6 +1868,7 @@ int LLGCVode(void* mem, sunrealtype tout, N_Vector yout, sunrealtype* tret, int 
     if (kflag != 0) {
       istate = kflag;
       m->tretlast = *tret = m->tn;
+      flush_completion(m);
       N_VScale(1.0, m->zn[0], yout);
       break;
     }
@@ -1814,6 +1890,7 @@ int LLGCVode(void* mem, sunrealtype tout, N_Vector yout, sunrealtype* tret, int 
     if (itask == LLGCV_ONE_STEP) {
       istate = LLGCV_SUCCESS;
       m->tretlast = *tret = m->tn;
+      flush_completion(m);
       N_VScale(1.0, m->zn[0], yout);
       m->next_q = m->qprime;
       m->next_h = m->hprime;

@@ -42,7 +42,7 @@ void LLG_DefaultRunOpts(llg_run_opts* o) {
   o->gstype = 0;        // auto: classical iff neq < 500000 (2d_i4/2d_fft.cu:645-650)
   o->max_order = 5;
   o->mxstep = 500;
-  o->fused = 279;
+  o->fused = 791;
   o->use_jtv = 1;
   o->use_prec = 1;
   o->pretype = SUN_PREC_LEFT;
@@ -93,6 +93,10 @@ void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_hos
     if (opts->use_jtv) fo.atimes = LLG_Atimes;
     // the plug-in's jtimes and preconditioner callbacks never read fy
     if ((opts->fused & 256) && opts->use_jtv) fo.rhs_residual = LLG_RhsResidual;
+    if (opts->fused & 512) {
+      fo.predict_complete = NVHip_PredictComplete;
+      fo.wrms_pair_comp = NVHip_WrmsNormPairComp;
+    }
     if ((opts->fused & 4) && opts->use_jtv && opts->use_prec && opts->pretype == SUN_PREC_LEFT) {
       fo.krylov = LLG_Krylov;
       fo.gsscaled = NVHip_GSProjectScaled;

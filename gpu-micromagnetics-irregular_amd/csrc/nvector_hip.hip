@@ -254,14 +254,26 @@ __global__ void k_pascal(OutPack Z, double sign, sunindextype n) {
 // y = zn[0] + acor, as N_VConst and N_VLinearSum would write them; with w
 // set, first the error weights of the step from the zn[0] being replaced,
 // as k_ewt forms them
-template <int Q>
+// COMP: first the previous step's completion z_j = fma(l_j, acor, z_j)
+// (N_VScaleAddMulti, j = 0..Q) and the step-size rescale z_j = c_j z_j
+// (N_VScaleVectorArray, j = 1..Q; c_j = 1 leaves the value unchanged), then
+// the rest as above; zn[Q] is written too
+template <int Q, bool COMP>
 __global__ void k_pascal_init(OutPack Z, double* __restrict__ acor, double* __restrict__ y,
                               sunindextype n, double rtol, double atol,
-                              const double* __restrict__ av, double* __restrict__ w) {
+                              const double* __restrict__ av, double* __restrict__ w,
+                              CoefPack lc, CoefPack sc) {
   GRID_STRIDE(i, n) {
     double z[Q + 1];
 #pragma unroll
     for (int j = 0; j <= Q; j++) z[j] = Z.p[j][i];
+    if constexpr (COMP) {
+      const double a = acor[i];
+#pragma unroll
+      for (int j = 0; j <= Q; j++) z[j] = fma(lc.c[j], a, z[j]);
+#pragma unroll
+      for (int j = 1; j <= Q; j++) z[j] = sc.c[j] * z[j];
+    }
     if (w) {
       const double t = fabs(z[0]);
       const double d = av ? fma(rtol, t, av[i]) : rtol * t + atol;
@@ -272,7 +284,7 @@ __global__ void k_pascal_init(OutPack Z, double* __restrict__ acor, double* __re
 #pragma unroll
       for (int j = Q; j >= k; j--) z[j - 1] = fma(1.0, z[j], z[j - 1]);
 #pragma unroll
-    for (int j = 0; j < Q; j++) Z.p[j][i] = z[j];
+    for (int j = 0; j < (COMP ? Q + 1 : Q); j++) Z.p[j][i] = z[j];
     const double zero = 0.0;
     acor[i] = zero;
     y[i] = fma(1.0, z[0], 1.0 * zero);
@@ -583,6 +595,15 @@ __global__ __launch_bounds__(kBlock) void k_reduce2(int kind, F1 f1, F2 f2, suni
 
 // (a y + b z) w, squared: the FWsq term of N_VLinearSum(a, y, b, z, t)
 // followed by N_VWrmsNorm(t, w), without writing t
+// (fma(c, a, x) w)^2: the WRMS term of the completed zn[q] (N_VScaleAddMulti's
+// value) before the completion is written
+struct FWsqComp {
+  double c; const double *x, *a, *w;
+  __device__ double operator()(sunindextype i) const {
+    const double p = fma(c, a[i], x[i]) * w[i];
+    return p * p;
+  }
+};
 struct FLinWsq {
   double a, b; const double *y, *z, *w;
   __device__ double operator()(sunindextype i) const {
@@ -1300,8 +1321,9 @@ int NVHip_Pascal(int q, N_Vector* zn, sunrealtype sign) {
   return 0;
 }
 
-int NVHip_PredictInit(int q, N_Vector* zn, N_Vector acor, N_Vector y, sunrealtype rtol,
-                      sunrealtype atol, N_Vector abstol, N_Vector ewt) {
+static int predict_launch(int q, N_Vector* zn, N_Vector acor, const double* lc,
+                          const double* sc, N_Vector y, double rtol, double atol,
+                          N_Vector abstol, N_Vector ewt) {
   if (q < 1 || q > 5 || !acor || !y) return -1;
   if (ewt && (ewt->ops->nvgetvectorid != N_VGetVectorID_Hip || LEN(ewt) != LEN(zn[0])))
     return -1;
@@ -1322,23 +1344,43 @@ int NVHip_PredictInit(int q, N_Vector* zn, N_Vector acor, N_Vector y, sunrealtyp
   hipStream_t s = S(zn[0]);
   const dim3 g(ew_grid(n)), b(kBlock);
   const double* av = abstol ? D(abstol) : nullptr;
-  switch (q) {
-#define NVHIP_PI_CASE(k) \
-    case k: hipLaunchKernelGGL(k_pascal_init<k>, g, b, 0, s, zp, D(acor), D(y), n, rtol, atol, \
-                               av, dw); break;
-    NVHIP_PI_CASE(1) NVHIP_PI_CASE(2) NVHIP_PI_CASE(3) NVHIP_PI_CASE(4)
+  CoefPack lp{}, sp{};
+  for (int j = 0; j <= q; j++) {
+    lp.c[j] = lc ? lc[j] : 0.0;
+    sp.c[j] = (sc && j > 0) ? sc[j] : 1.0;
+  }
+  switch (q * 2 + (lc ? 1 : 0)) {
+#define NVHIP_PI_CASE(k)                                                                      \
+    case 2 * k: hipLaunchKernelGGL((k_pascal_init<k, false>), g, b, 0, s, zp, D(acor), D(y), n, \
+                                   rtol, atol, av, dw, lp, sp); break;                        \
+    case 2 * k + 1: hipLaunchKernelGGL((k_pascal_init<k, true>), g, b, 0, s, zp, D(acor), D(y), \
+                                       n, rtol, atol, av, dw, lp, sp); break;
+    NVHIP_PI_CASE(1) NVHIP_PI_CASE(2) NVHIP_PI_CASE(3) NVHIP_PI_CASE(4) NVHIP_PI_CASE(5)
 #undef NVHIP_PI_CASE
-    default: hipLaunchKernelGGL(k_pascal_init<5>, g, b, 0, s, zp, D(acor), D(y), n, rtol, atol,
-                                av, dw); break;
+    default: return -1;
   }
   check_launch("pascal_init");
   return 0;
 }
 
-int NVHip_WrmsNormPair(N_Vector x, sunrealtype a, N_Vector y, sunrealtype b, N_Vector z,
-                       N_Vector w, sunrealtype* nrm) {
+int NVHip_PredictInit(int q, N_Vector* zn, N_Vector acor, N_Vector y, sunrealtype rtol,
+                      sunrealtype atol, N_Vector abstol, N_Vector ewt) {
+  return predict_launch(q, zn, acor, nullptr, nullptr, y, rtol, atol, abstol, ewt);
+}
+
+int NVHip_PredictComplete(int q, N_Vector* zn, N_Vector acor, const sunrealtype* l,
+                          const sunrealtype* c, N_Vector y, sunrealtype rtol, sunrealtype atol,
+                          N_Vector abstol, N_Vector ewt) {
+  if (!l) return -1;
+  return predict_launch(q, zn, acor, l, c, y, rtol, atol, abstol, ewt);
+}
+
+// x_comp: nrm[0] is the norm of fma(cx, x_comp, x) (x + cx x_comp as
+// N_VScaleAddMulti forms it) when given
+static int wrms_pair(N_Vector x, double cx, N_Vector xc, double a, N_Vector y, double b,
+                     N_Vector z, N_Vector w, double* nrm) {
   if (!x || !y || !z || !w || !nrm) return -1;
-  N_Vector vs[4] = {x, y, z, w};
+  N_Vector vs[5] = {x, y, z, w, xc ? xc : x};
   for (N_Vector v : vs)
     if (v->ops->nvgetvectorid != N_VGetVectorID_Hip || LEN(v) != LEN(x)) return -1;
   const sunindextype n = LEN(x);
@@ -1346,14 +1388,30 @@ int NVHip_WrmsNormPair(N_Vector x, sunrealtype a, N_Vector y, sunrealtype b, N_V
   RedWs* ws = workspace(s);
   const int g = red_grid(n);
   const unsigned seq = ++ws->seq;
-  hipLaunchKernelGGL((k_reduce2<FWsq, FLinWsq>), dim3(g), dim3(kBlock), 0, s, RED_SUM,
-                     FWsq{D(x), D(w)}, FLinWsq{a, b, D(y), D(z), D(w)}, n, ws->partials);
+  if (xc)
+    hipLaunchKernelGGL((k_reduce2<FWsqComp, FLinWsq>), dim3(g), dim3(kBlock), 0, s, RED_SUM,
+                       FWsqComp{cx, D(x), D(xc), D(w)}, FLinWsq{a, b, D(y), D(z), D(w)}, n,
+                       ws->partials);
+  else
+    hipLaunchKernelGGL((k_reduce2<FWsq, FLinWsq>), dim3(g), dim3(kBlock), 0, s, RED_SUM,
+                       FWsq{D(x), D(w)}, FLinWsq{a, b, D(y), D(z), D(w)}, n, ws->partials);
   finish_dispatch(2, s, RED_SUM, g, ws, seq);
   check_launch("wrms pair");
   wait_result(s, ws, seq);
   nrm[0] = std::sqrt(ws->host[0] / (double)n);
   nrm[1] = std::sqrt(ws->host[1] / (double)n);
   return 0;
+}
+
+int NVHip_WrmsNormPair(N_Vector x, sunrealtype a, N_Vector y, sunrealtype b, N_Vector z,
+                       N_Vector w, sunrealtype* nrm) {
+  return wrms_pair(x, 0.0, nullptr, a, y, b, z, w, nrm);
+}
+
+int NVHip_WrmsNormPairComp(N_Vector x, sunrealtype cx, N_Vector xc, sunrealtype a, N_Vector y,
+                           sunrealtype b, N_Vector z, N_Vector w, sunrealtype* nrm) {
+  if (!xc) return -1;
+  return wrms_pair(x, cx, xc, a, y, b, z, w, nrm);
 }
 
 int NVHip_EwtSet(N_Vector y, sunrealtype rtol, sunrealtype atol, N_Vector abstol, N_Vector w) {

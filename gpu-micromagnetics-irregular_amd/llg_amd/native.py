@@ -161,6 +161,10 @@ SIGNATURES = {
     "NVHip_PredictInit": (C.c_int, [C.c_int, C.POINTER(VP), VP, VP, C.c_double, C.c_double,
                                     VP, VP]),
     "NVHip_WrmsNormPair": (C.c_int, [VP, C.c_double, VP, C.c_double, VP, VP, DP]),
+    "NVHip_PredictComplete": (C.c_int, [C.c_int, C.POINTER(VP), VP, DP, DP, VP, C.c_double,
+                                        C.c_double, VP, VP]),
+    "NVHip_WrmsNormPairComp": (C.c_int, [VP, C.c_double, VP, C.c_double, VP, C.c_double, VP,
+                                         VP, DP]),
     "NVHip_ScaleDiv": (C.c_int, [C.c_double, VP, VP, VP, VP]),
     "NVHip_ProdDotMulti": (C.c_int, [VP, VP, VP, C.c_int, C.POINTER(VP), DP]),
     "NVHip_LinCombDot": (C.c_int, [C.c_int, DP, C.POINTER(VP), VP, DP]),

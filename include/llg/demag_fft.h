@@ -75,9 +75,11 @@ int  Demag_SetSpecialized(DemagData* d, int on);
  * differently (both are within 1e-12 relative of the exact convolution). */
 int  Demag_SetFftKernels(DemagData* d, int row_variant, int col_variant);
 int  Demag_GetFftKernels(DemagData* d, int* row_variant, int* col_variant);
-/* Autotuner: times Demag_Apply (reps launches each) over the FFT kernel
- * variants and, for the Stockham kernels, the row/column workgroup sizes
- * and LDS paddings; keeps the fastest and returns it in choice[6] =
+/* Autotuner: times Demag_Apply (reps launches each, each after a pass
+ * over a 384 MB scratch buffer that leaves the caches cold, as the
+ * integrator's vector passes do) over the FFT kernel variants and, for the
+ * Stockham kernels, the row/column workgroup sizes and LDS paddings; keeps
+ * the fastest and returns it in choice[6] =
  * {row block, col block, row pad shift, col pad shift, row variant,
  * col variant}. */
 int  Demag_Autotune(DemagData* d, int reps, int* choice);

@@ -152,6 +152,16 @@ typedef struct llgcv_fused_ops {
    * for RHS, jtimes and preconditioner callbacks that ignore fy. */
   int (*rhs_residual)(sunrealtype t, N_Vector y, const sunrealtype* c, N_Vector zn1,
                       N_Vector ycor, N_Vector res, void* user_data);
+  /* with predict_init: the step's completion zn += l acor left to the next
+   * predictor's pass (as NVHip_PredictComplete, with the rescale of a step
+   * size change), and the order-change norm of the completed zn[q] taken
+   * from the pending form (as NVHip_WrmsNormPairComp); any other reader of
+   * zn gets the completion written first */
+  int (*predict_complete)(int q, N_Vector* zn, N_Vector acor, const sunrealtype* l,
+                          const sunrealtype* c, N_Vector y, sunrealtype rtol, sunrealtype atol,
+                          N_Vector abstol, N_Vector ewt);
+  int (*wrms_pair_comp)(N_Vector x, sunrealtype cx, N_Vector xc, sunrealtype a, N_Vector y,
+                        sunrealtype b, N_Vector z, N_Vector w, sunrealtype* nrm);
 } llgcv_fused_ops;
 
 void* LLGCVodeCreate(int lmm, SUNContext sunctx);

@@ -51,8 +51,11 @@ typedef struct llg_run_opts {
                          the 1536x512 workload); bit 8 (with bit 1 and
                          analytic Jv): the Newton residual formed in the
                          RHS pass (LLG_RhsResidual; f is not stored);
-                         default 279 (bits 0, 1, 2, 4, 8).  Bits 1 to 3 and
-                         8 change no value. */
+                         bit 9 (with bit 1): each step's completion
+                         zn += l acor left to the next predictor's pass
+                         (NVHip_PredictComplete); default 791 (bits 0, 1,
+                         2, 4, 8, 9).  Bits 1 to 3, 8 and 9 change no
+                         value. */
   long   mxstep;      /* max steps per CVode call (CVODE default 500) */
   int    use_jtv;     /* analytic Jv (else difference quotient) */
   int    use_prec;    /* block-Jacobi preconditioner */

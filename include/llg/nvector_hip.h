@@ -121,11 +121,23 @@ int NVHip_EwtSet(N_Vector y, sunrealtype rtol, sunrealtype atol, N_Vector abstol
  * may not alias zn or each other. */
 int NVHip_PredictInit(int q, N_Vector* zn, N_Vector acor, N_Vector y, sunrealtype rtol,
                       sunrealtype atol, N_Vector abstol, N_Vector ewt);
+/* NVHip_PredictInit preceded, in the same pass, by the previous step's
+ * completion zn[j] += l[j] acor, j = 0..q (N_VScaleAddMulti(q+1, l, acor,
+ * zn, zn)) and, with c not NULL, the step-size rescale zn[j] = c[j] zn[j],
+ * j = 1..q (N_VScaleVectorArray; c[0] unused); the error weights (ewt not
+ * NULL) come from the completed zn[0].  Bit-equal to those calls. */
+int NVHip_PredictComplete(int q, N_Vector* zn, N_Vector acor, const sunrealtype* l,
+                          const sunrealtype* c, N_Vector y, sunrealtype rtol, sunrealtype atol,
+                          N_Vector abstol, N_Vector ewt);
 /* Two WRMS norms with one host wait: nrm[0] = |x|_WRMS(w) and nrm[1] =
  * |a y + b z|_WRMS(w), the combination not written; each equal to
  * N_VWrmsNorm (after N_VLinearSum for the second) bit for bit. */
 int NVHip_WrmsNormPair(N_Vector x, sunrealtype a, N_Vector y, sunrealtype b, N_Vector z,
                        N_Vector w, sunrealtype* nrm);
+/* NVHip_WrmsNormPair with nrm[0] the norm of x + cx xc as N_VScaleAddMulti
+ * forms it (a pending completion of x), neither written. */
+int NVHip_WrmsNormPairComp(N_Vector x, sunrealtype cx, N_Vector xc, sunrealtype a, N_Vector y,
+                           sunrealtype b, N_Vector z, N_Vector w, sunrealtype* nrm);
 
 /* One-pass kernels for the SPGMR loop (llgcv_fused_ops), each equal
  * element for element, and reduction for reduction, to the N_Vector

@@ -39,18 +39,19 @@ def test_fused_integrator_kernels_are_bitwise_neutral(gstype, case):
     """fused = 1: standard op sequences; 3: the one-pass integrator kernels;
     7: also the one-launch Krylov operator chain; 15: also the whole SPGMR
     solve queued on the device with one host wait (classical GS only);
-    263: 7 with the Newton residual formed in the RHS pass.  All must agree
-    bit for bit, counters included."""
+    263: 7 with the Newton residual formed in the RHS pass; 775: 263 with
+    each step's completion left to the next predictor's pass.  All must
+    agree bit for bit, counters included."""
     cfg, geom, y0 = _case() if case == "square_hole" else _compact_case()
     sim = la.Simulation(cfg, geom)
     res = {}
-    for fused in (1, 3, 7, 15, 263):
+    for fused in (1, 3, 7, 15, 263, 775):
         run = sim.integrator(y0, rtol=1e-5, atol=1e-5, gstype=gstype, fused=fused)
         out = run.advance(1.0)
         res[fused] = (run.state(), out)
         run.close()
     ya, sa = res[1]
-    for fused in (3, 7, 15, 263):
+    for fused in (3, 7, 15, 263, 775):
         yb, sb = res[fused]
         print(f"[gpu] {case} gs={gstype} fused 1: nst={sa['nst']} nli={sa['nli']} "
               f"syncs={sa['nsyncs']}; fused {fused}: nst={sb['nst']} nli={sb['nli']} "
@@ -60,7 +61,7 @@ def test_fused_integrator_kernels_are_bitwise_neutral(gstype, case):
         assert np.array_equal(ya, yb), fused
 
 
-@pytest.mark.parametrize("fused", [7, 23, 55, 215, 279])
+@pytest.mark.parametrize("fused", [7, 23, 55, 215, 279, 791])
 @pytest.mark.parametrize("name,kw", [("square_hole", {}), ("i6", {"demag": 4.0}),
                                      ("plain128", {})])
 def test_trajectory_matches_oracle(name, kw, fused):
@@ -70,7 +71,8 @@ def test_trajectory_matches_oracle(name, kw, fused):
     55: the same pipelined, each pass queued behind a gate (polling waits);
     215: the lagged solve with the Newton residual and the SPGMR solution
     formed where they are read (never written); 279: 23 with the Newton
-    residual formed in the RHS pass"""
+    residual formed in the RHS pass; 791: 279 with each step's completion
+    left to the next predictor's pass"""
     ng, ny = (48, 32) if name != "plain128" else (32, 32)
     cfg = la.preset(name, ng=ng, ny=ny, **kw)
     if name == "plain128":

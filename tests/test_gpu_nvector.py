@@ -419,6 +419,66 @@ def test_predict_init_equals_pascal_const_linear_sum(q, weights):
                                None) == -1   # acor aliases zn[0]
 
 
+@pytest.mark.parametrize("q,weights,scaled", [(1, None, False), (2, "scalar", True),
+                                              (4, "vector", False), (5, "scalar", True)])
+def test_predict_complete_equals_completion_rescale_then_predict_init(q, weights, scaled):
+    """NVHip_PredictComplete == N_VScaleAddMulti(q+1, l, acor, zn, zn), then
+    (scaled) N_VScaleVectorArray(q, c[1..q], zn[1..q]), then NVHip_PredictInit,
+    bit for bit (zn[q] written too; acor read before it is zeroed)"""
+    L = la.lib()
+    n = 30_001
+    rng = np.random.default_rng(90 + q)
+    Z = [rng.standard_normal(n) for _ in range(q + 1)]
+    Z[0][:5] = -0.0
+    A = rng.standard_normal(n)
+    A[5:9] = 0.0
+    za, zb = [vec(a) for a in Z], [vec(a) for a in Z]
+    acor_a, acor_b = vec(A), vec(A)
+    y_a, y_b = vec(np.zeros(n)), vec(np.zeros(n))
+    av = vec(rng.uniform(1e-6, 1e-3, n)) if weights == "vector" else None
+    w_a, w_b = vec(np.zeros(n)), vec(np.zeros(n))
+    rtol, atol = 1e-4, (0.0 if weights == "vector" else 1e-5)
+    lv = (C.c_double * 6)(*([1.0] + list(rng.uniform(-2.0, 2.0, 5))))
+    eta = 0.8371
+    cv = (C.c_double * 6)(*[eta ** j for j in range(6)])
+    assert L.N_VScaleAddMulti_Hip(q + 1, lv, acor_a.v, arr(za), arr(za)) == 0
+    if scaled:
+        cs = (C.c_double * 5)(*[cv[j] for j in range(1, q + 1)])
+        assert L.N_VScaleVectorArray_Hip(q, cs, arr(za[1:]), arr(za[1:])) == 0
+    assert L.NVHip_PredictInit(q, arr(za), acor_a.v, y_a.v, rtol, atol, av.v if av else None,
+                               w_a.v if weights else None) == 0
+    assert L.NVHip_PredictComplete(q, arr(zb), acor_b.v, lv, cv if scaled else None, y_b.v,
+                                   rtol, atol, av.v if av else None,
+                                   w_b.v if weights else None) == 0
+    for j in range(q + 1):
+        assert np.array_equal(za[j].get(), zb[j].get()), j
+    assert np.array_equal(acor_a.get(), acor_b.get())
+    assert np.array_equal(y_a.get(), y_b.get())
+    if weights:
+        assert np.array_equal(w_a.get(), w_b.get())
+    assert L.NVHip_PredictComplete(q, arr(zb), acor_b.v, None, None, y_b.v, rtol, atol, None,
+                                   None) == -1   # no completion coefficients
+
+
+def test_wrms_norm_pair_comp_equals_completion_then_pair():
+    """NVHip_WrmsNormPairComp(x, c, xc, ...) == N_VScaleAddMulti(1, c, xc, x)
+    then NVHip_WrmsNormPair, bit for bit, x and xc left unchanged"""
+    L = la.lib()
+    rng = np.random.default_rng(71)
+    x, xc, yv, zv = _vecs(rng, 4)
+    w, = _vecs(rng, 1, lo=0.5)
+    x0, xc0 = x.get(), xc.get()
+    a, b, c = -0.734, 1.0, 0.4166
+    xt = vec(x0)
+    cc = (C.c_double * 1)(c)
+    assert L.N_VScaleAddMulti_Hip(1, cc, xc.v, arr([xt]), arr([xt])) == 0
+    na, nb = (C.c_double * 2)(), (C.c_double * 2)()
+    assert L.NVHip_WrmsNormPair(xt.v, a, yv.v, b, zv.v, w.v, na) == 0
+    assert L.NVHip_WrmsNormPairComp(x.v, c, xc.v, a, yv.v, b, zv.v, w.v, nb) == 0
+    assert list(na) == list(nb)
+    assert np.array_equal(x.get(), x0) and np.array_equal(xc.get(), xc0)
+
+
 def test_wrms_norm_pair_equals_two_norms():
     """NVHip_WrmsNormPair == N_VWrmsNorm(x, w) and N_VLinearSum(a, y, b, z,
     t) + N_VWrmsNorm(t, w), bit for bit, with one host wait"""

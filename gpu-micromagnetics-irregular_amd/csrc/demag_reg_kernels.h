@@ -8,12 +8,12 @@
 //                 m = u + U b straight into registers (coalesced), runs the
 //                 forward FFT of length L = Px/2, and the real-FFT split
 //                 writes bins 0..L of the row.
-//   column      : lane u of a column loads rows u + U b of m_z, runs its
-//                 forward FFT, multiplies by K_zz, runs the inverse and
-//                 stores; then the same for m_x and m_y as a pair, with the
-//                 xy block of the tensor applied in registers (each lane
-//                 holds both components at the same ky).  Only the rows the
-//                 field needs are stored.
+//   column      : two workgroups per column.  In one, lane u loads rows
+//                 u + U b of m_z, runs its forward FFT, multiplies by K_zz,
+//                 runs the inverse and stores; the other does the same for
+//                 m_x and m_y as a pair, with the xy block of the tensor
+//                 applied in registers (each lane holds both components at
+//                 the same ky).  Only the rows the field needs are stored.
 //   row inverse : the C2R pre-processing reads bins k and L-k straight from
 //                 the spectrum into the registers of the last FFT stage,
 //                 the inverse FFT ends in lane u holding z[u + U b], stored
@@ -178,14 +178,21 @@ __global__ __launch_bounds__(F::kThreads) __attribute__((amdgpu_waves_per_eu(Col
   extern __shared__ char lds_raw[];
   const int lane = threadIdx.x & (U - 1);
   const int slot = threadIdx.x / U;
-  int kx = reg_xcd_remap(blockIdx.x, gridDim.x) * G + slot;
+  // the grid's first half (a multiple of 8 workgroups, so that the XCD of a
+  // workgroup is the same in both halves) runs the x/y pairs, the second
+  // half the z components: twice the workgroups of one column each, and
+  // the same arithmetic per component as one workgroup doing both
+  const int half_grid = gridDim.x / 2;
+  const bool zjob = (int)blockIdx.x >= half_grid;
+  const int bid = zjob ? (int)blockIdx.x - half_grid : (int)blockIdx.x;
+  int kx = reg_xcd_remap(bid, half_grid) * G + slot;
   const bool live = kx < Hx;
   if (!live) kx = Hx - 1;
   C* lds = reinterpret_cast<C*>(lds_raw) + slot * F::kSlots;
-  // z first, on its own: load, transform, multiply by K_zz (h_z = K_zz m_z),
-  // transform back, store; then x and y as a pair.  At most two components
-  // are live at once, which keeps the FP64 plans with 8 values per lane
-  // within 128 VGPRs without spilling (loading all three up front spilled).
+  // z: load, transform, multiply by K_zz (h_z = K_zz m_z), transform back,
+  // store; x and y as a pair.  At most two components are live at once,
+  // which keeps the FP64 plans with 8 values per lane within 128 VGPRs
+  // without spilling (loading all three up front spilled).
   C r[3][V];
   auto load = [&](auto Cc) {
     const C* src = spec + ((long)Cc * ROWS + rfft::opaque(lane)) * Hx + kx;
@@ -220,14 +227,17 @@ __global__ __launch_bounds__(F::kThreads) __attribute__((amdgpu_waves_per_eu(Col
   constexpr auto c0 = std::integral_constant<int, 0>{};
   constexpr auto c1 = std::integral_constant<int, 1>{};
   constexpr auto c2 = std::integral_constant<int, 2>{};
-  load(c2);
-  F::template forward<HALF>(r[2], lds, lane, tw, Bar{});
-  each_k([&](int i, unsigned k) {
-    const T cz = kzz[k];
-    r[2][i] = mk<T>(cz * r[2][i].x, cz * r[2][i].y);
-  });
-  F::template inverse<HALF>(r[2], lds, lane, tw, Bar{});
-  store(c2);
+  if (zjob) {
+    load(c2);
+    F::template forward<HALF>(r[2], lds, lane, tw, Bar{});
+    each_k([&](int i, unsigned k) {
+      const T cz = kzz[k];
+      r[2][i] = mk<T>(cz * r[2][i].x, cz * r[2][i].y);
+    });
+    F::template inverse<HALF>(r[2], lds, lane, tw, Bar{});
+    store(c2);
+    return;
+  }
   load(c0);
   load(c1);
   sfor<0, 2>([&](auto Cc) { F::template forward<HALF>(r[Cc], lds, lane, tw, Bar{}); });
@@ -362,7 +372,8 @@ int launch_col(int L, int variant, int half, C2<T>* spec, int Hx, const C2<T>* t
   with_col<T>(L, variant, [&](auto f) {
     using F = decltype(f);
     const size_t lds = sizeof(C2<T>) * F::kSlots * F::kPerGroup;
-    const int grid = (Hx + F::kPerGroup - 1) / F::kPerGroup;
+    // x/y pairs, then z components: each half rounded up to a multiple of 8
+    const int grid = 2 * (((Hx + F::kPerGroup - 1) / F::kPerGroup + 7) / 8 * 8);
     auto kern = half ? (has_xy_im ? k_col_reg<T, F, true, true> : k_col_reg<T, F, true, false>)
                      : (has_xy_im ? k_col_reg<T, F, false, true> : k_col_reg<T, F, false, false>);
     if (prep_lds(kern, lds)) { rc = -1; return; }

@@ -276,9 +276,11 @@ __global__ __launch_bounds__(256) void k_rhs(GeoD g, TermsD T, const double* __r
   }
 }
 
-// LDS-tiled windowed RHS (the polycrystal form, 2d_i3/2d_fft.cu:451-627):
-// a (BX+2)x(BY+2) halo of w*m per component is staged in LDS once and the
-// four neighbours are read from it.  Periodic boundary only.
+// LDS-tiled RHS on the dense grid (the windowed polycrystal form is
+// 2d_i3/2d_fft.cu:451-627; without a window w = 1): a (BX+2)x(BY+2) halo of
+// w*m per component is staged in LDS once and the four neighbours are read
+// from it, the same values field() reads (periodic wrap, or the edge cell
+// itself at a Neumann edge), so the result equals k_rhs bit for bit.
 template <int BX, int BY, bool RES>
 __global__ __launch_bounds__(BX* BY) void k_rhs_tiled(GeoD g, TermsD T,
                                                        const double* __restrict__ y,
@@ -294,9 +296,16 @@ __global__ __launch_bounds__(BX* BY) void k_rhs_tiled(GeoD g, TermsD T,
   for (int s = threadIdx.y * BX + threadIdx.x; s < SX * SY; s += BX * BY) {
     const int sx = s % SX, sy = s / SX;
     int gx = x0 + sx - 1, gy = y0 + sy - 1;
-    gx = gx < 0 ? gx + g.ng : (gx >= g.ng ? gx - g.ng : gx);
-    gy = gy < 0 ? gy + g.ny : (gy >= g.ny ? gy - g.ny : gy);
-    gx = gx > g.ng - 1 ? g.ng - 1 : gx;   // never read (see k_krylov_tiled)
+    if (g.boundary == LLG_BC_NEUMANN) {
+      // a missing neighbour is the cell itself (nbrs): the halo slot holds
+      // the edge cell's value
+      gx = gx < 0 ? 0 : gx;
+      gy = gy < 0 ? 0 : gy;
+    } else {
+      gx = gx < 0 ? gx + g.ng : (gx >= g.ng ? gx - g.ng : gx);
+      gy = gy < 0 ? gy + g.ny : (gy >= g.ny ? gy - g.ny : gy);
+    }
+    gx = gx > g.ng - 1 ? g.ng - 1 : gx;   // Neumann edge, or never read (see k_krylov_tiled)
     gy = gy > g.ny - 1 ? g.ny - 1 : gy;
     const int c = gy * g.ng + gx;
     const double ww = win ? g.w[c] : 1.0;
@@ -1112,7 +1121,7 @@ static int rhs_launch(llg_ctx* c, N_Vector y, N_Vector out, const ResIn* rin) {
   const GeoD g = geo_of(c);
   const TermsD T = to_dev(c->cfg.rhs);
   const ResIn ri = rin ? *rin : ResIn{nullptr, nullptr, 0.0, 0.0, 0.0};
-  if (c->cfg.exec == LLG_EXEC_YMSK && c->cfg.rhs.window && c->cfg.boundary == LLG_BC_PERIODIC) {
+  if (c->cfg.exec == LLG_EXEC_YMSK) {
     dim3 block(32, 8), grid((c->cfg.ng + 31) / 32, (c->cfg.ny + 7) / 8);
     kprof::begin(kprof::RHS, s);
     if (rin) hipLaunchKernelGGL((k_rhs_tiled<32, 8, true>), grid, block, 0, s, g, T, yd, hd, fd, ri);

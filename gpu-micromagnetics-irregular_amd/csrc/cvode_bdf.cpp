@@ -1088,9 +1088,11 @@ int nls_residual(Integrator* m, N_Vector ycor, N_Vector negres) {
   m->y_current = false;
   double c[3] = {-m->rl1, -1.0, m->gamma};
   const bool first_pass = m->fused.arnoldi_first && lagged_ok(m) && !m->jtsetup;
-  if (m->fused.rhs_residual && m->jtimes && !m->jtsetup && !first_pass) {
+  if (m->fused.rhs_residual && m->jtimes && !m->jtsetup && !first_pass &&
+      m->jtimes == m->fused.fy_free_jtimes && m->pset == m->fused.fy_free_psetup) {
     // f and the combination in one pass; f is not stored (the difference-
-    // quotient J v and a jtimes setup would read it, so neither is in use)
+    // quotient J v and a jtimes setup would read it, so neither is in use,
+    // and the registered J v and preconditioner setup are declared fy-free)
     const int rr = m->fused.rhs_residual(m->tn, m->y, c, m->zn[1], ycor, negres, m->user_data);
     if (rr != 2) {
       m->nfe++;

@@ -498,7 +498,14 @@ __global__ void k_to_float(const double* __restrict__ a, float* __restrict__ b, 
 }
 
 // ------------------------------------------------------------ tensor
-__device__ void ctt_dev(double b, double a, double sx, double sy, double dm[10]) {
+// Closed-form field of a uniformly magnetised cell of half-width a and
+// half-thickness b at the in-plane offset (sx, sy), sz = 0
+// (2d_i4/demag_fft.cu:27-61).  dm[1..9] = xx, xy, xz, yx, yy, yz, zx, zy, zz;
+// ALL = false computes only the four the thin-film kernel uses (xx, xy,
+// yy, zz; xz, yz, zx, zy vanish at sz = 0 and yx = xy).  One source for the
+// device tensor builder and the host self-coupling / full-tensor export.
+template <bool ALL>
+__host__ __device__ void ctt(double b, double a, double sx, double sy, double dm[10]) {
   const double sz = 0.0;
   const double xn = sx - a, xp = sx + a, yn = sy - a, yp = sy + a, zn = sz - b, zp = sz + b;
   const double xn2 = xn * xn, xp2 = xp * xp, yn2 = yn * yn, yp2 = yp * yp, zn2 = zn * zn,
@@ -521,6 +528,18 @@ __device__ void ctt_dev(double b, double a, double sx, double sy, double dm[10])
           atan(xn * yp / (zn * dnpn)) + atan(xp * yp / (zn * dppn)) -
           atan(xn * yn / (zp * dnnp)) + atan(xp * yn / (zp * dpnp)) +
           atan(xn * yp / (zp * dnpp)) - atan(xp * yp / (zp * dppp));
+  if constexpr (ALL) {
+    dm[3] = log((dnnn - yn) / (dnpn - yp)) - log((dpnn - yn) / (dppn - yp)) -
+            log((dnnp - yn) / (dnpp - yp)) + log((dpnp - yn) / (dppp - yp));
+    dm[4] = log((dnnn - zn) / (dnnp - zp)) - log((dnpn - zn) / (dnpp - zp)) -
+            log((dpnn - zn) / (dpnp - zp)) + log((dppn - zn) / (dppp - zp));
+    dm[6] = log((dnnn - xn) / (dpnn - xp)) - log((dnpn - xn) / (dppn - xp)) -
+            log((dnnp - xn) / (dpnp - xp)) + log((dnpp - xn) / (dppp - xp));
+    dm[7] = log((dnnn - yn) / (dnpn - yp)) - log((dnnp - yn) / (dnpp - yp)) -
+            log((dpnn - yn) / (dppn - yp)) + log((dpnp - yn) / (dppp - yp));
+    dm[8] = log((dnnn - xn) / (dpnn - xp)) - log((dnnp - xn) / (dpnp - xp)) -
+            log((dnpn - xn) / (dppn - xp)) + log((dnpp - xn) / (dppp - xp));
+  }
 }
 
 // quadrant Q[c][dy][dx], dx in [0,qx), dy in [0,qy); c = xx, yy, zz, xy
@@ -535,7 +554,7 @@ __global__ void k_tensor_quadrant(double thick, int qx, int qy, double* __restri
     const double sy = (double)dy + 0.1 * (double)jy;
     for (int ix = -4; ix <= 4; ix++) {
       const double sx = (double)dx + 0.1 * (double)ix;
-      ctt_dev(b, a, sx, sy, dm);
+      ctt<false>(b, a, sx, sy, dm);
       sxx += dm[1]; sxy += dm[2]; syy += dm[5]; szz += dm[9];
     }
   }
@@ -577,7 +596,7 @@ __global__ void k_tensor_fill(const double* __restrict__ Q, int qx, int qy, int 
   K[k] = Q[q];
   K[P + k] = Q[PQ + q];
   K[2 * P + k] = Q[2 * PQ + q];
-  K[3 * P + k] = (dx == 0 || dy == 0) ? sgn * Q[3 * PQ + q] : sgn * Q[3 * PQ + q];
+  K[3 * P + k] = sgn * Q[3 * PQ + q];
 }
 
 // spectra [ky][kx] (one plane) -> stored plane [kx][ky] (transposed), scaled
@@ -699,44 +718,6 @@ void free_plan(HostPlan<T>& hp) {
   hp.d_tw = nullptr;
 }
 
-// host restatement of ctt (used only for the self-coupling N(0))
-void ctt_host(double b, double a, double sx, double sy, double dm[10]) {
-  const double sz = 0.0;
-  const double xn = sx - a, xp = sx + a, yn = sy - a, yp = sy + a, zn = sz - b, zp = sz + b;
-  const double xn2 = xn * xn, xp2 = xp * xp, yn2 = yn * yn, yp2 = yp * yp, zn2 = zn * zn,
-               zp2 = zp * zp;
-  const double dnnn = std::sqrt(xn2 + yn2 + zn2), dpnn = std::sqrt(xp2 + yn2 + zn2);
-  const double dnpn = std::sqrt(xn2 + yp2 + zn2), dnnp = std::sqrt(xn2 + yn2 + zp2);
-  const double dppn = std::sqrt(xp2 + yp2 + zn2), dnpp = std::sqrt(xn2 + yp2 + zp2);
-  const double dpnp = std::sqrt(xp2 + yn2 + zp2), dppp = std::sqrt(xp2 + yp2 + zp2);
-  using std::atan;
-  using std::log;
-  dm[1] = atan(zn * yn / (xn * dnnn)) - atan(zp * yn / (xn * dnnp)) -
-          atan(zn * yp / (xn * dnpn)) + atan(zp * yp / (xn * dnpp)) -
-          atan(zn * yn / (xp * dpnn)) + atan(zp * yn / (xp * dpnp)) +
-          atan(zn * yp / (xp * dppn)) - atan(zp * yp / (xp * dppp));
-  dm[2] = log((dnnn - zn) / (dnnp - zp)) - log((dpnn - zn) / (dpnp - zp)) -
-          log((dnpn - zn) / (dnpp - zp)) + log((dppn - zn) / (dppp - zp));
-  dm[3] = log((dnnn - yn) / (dnpn - yp)) - log((dpnn - yn) / (dppn - yp)) -
-          log((dnnp - yn) / (dnpp - yp)) + log((dpnp - yn) / (dppp - yp));
-  dm[4] = log((dnnn - zn) / (dnnp - zp)) - log((dnpn - zn) / (dnpp - zp)) -
-          log((dpnn - zn) / (dpnp - zp)) + log((dppn - zn) / (dppp - zp));
-  dm[5] = atan(zn * xn / (yn * dnnn)) - atan(zp * xn / (yn * dnnp)) -
-          atan(zn * xp / (yn * dpnn)) + atan(zp * xp / (yn * dpnp)) -
-          atan(zn * xn / (yp * dnpn)) + atan(zp * xn / (yp * dnpp)) +
-          atan(zn * xp / (yp * dppn)) - atan(zp * xp / (yp * dppp));
-  dm[6] = log((dnnn - xn) / (dpnn - xp)) - log((dnpn - xn) / (dppn - xp)) -
-          log((dnnp - xn) / (dpnp - xp)) + log((dnpp - xn) / (dppp - xp));
-  dm[7] = log((dnnn - yn) / (dnpn - yp)) - log((dnnp - yn) / (dnpp - yp)) -
-          log((dpnn - yn) / (dppn - yp)) + log((dpnp - yn) / (dppp - yp));
-  dm[8] = log((dnnn - xn) / (dpnn - xp)) - log((dnnp - xn) / (dpnp - xp)) -
-          log((dnpn - xn) / (dppn - xp)) + log((dnpp - xn) / (dppp - xp));
-  dm[9] = atan(xn * yn / (zn * dnnn)) - atan(xp * yn / (zn * dpnn)) -
-          atan(xn * yp / (zn * dnpn)) + atan(xp * yp / (zn * dppn)) -
-          atan(xn * yn / (zp * dnnp)) + atan(xp * yn / (zp * dpnp)) +
-          atan(xn * yp / (zp * dnpp)) - atan(xp * yp / (zp * dppp));
-}
-
 void tensor_at_host(double thick, int dx, int dy, double out9[9]) {
   const double a = 0.49999, b = 0.5 * thick;
   double dm[10];
@@ -745,7 +726,7 @@ void tensor_at_host(double thick, int dx, int dy, double out9[9]) {
     const double sy = (double)dy + 0.1 * (double)jy;
     for (int ix = -4; ix <= 4; ix++) {
       const double sx = (double)dx + 0.1 * (double)ix;
-      ctt_host(b, a, sx, sy, dm);
+      ctt<true>(b, a, sx, sy, dm);
       for (int c = 0; c < 9; c++) out9[c] += dm[c + 1];
     }
   }

@@ -92,7 +92,11 @@ void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_hos
     fo.wrms_pair = NVHip_WrmsNormPair;
     if (opts->use_jtv) fo.atimes = LLG_Atimes;
     // the plug-in's jtimes and preconditioner callbacks never read fy
-    if ((opts->fused & 256) && opts->use_jtv) fo.rhs_residual = LLG_RhsResidual;
+    if ((opts->fused & 256) && opts->use_jtv) {
+      fo.rhs_residual = LLG_RhsResidual;
+      fo.fy_free_jtimes = JtvProduct;
+      fo.fy_free_psetup = opts->use_prec ? PrecondSetup : nullptr;
+    }
     if (opts->fused & 512) {
       fo.predict_complete = NVHip_PredictComplete;
       fo.wrms_pair_comp = NVHip_WrmsNormPairComp;
@@ -167,6 +171,46 @@ int LLG_RunGetState(void* run, double* y_host) {
   Run* r = static_cast<Run*>(run);
   if (!r || !y_host) return -1;
   return N_VGetToHost_Hip(r->y, y_host);
+}
+
+int LLG_RunStage(void* run, int which, const double* happ, double t_stage,
+                 llg_run_result* res, double* m_avg) {
+  Run* r = static_cast<Run*>(run);
+  if (!r || !happ || !(t_stage > 0.0)) return -1;
+  if (LLG_SetAppliedField(r->ctx, which, happ) != 0) return -1;
+  if (LLGCVodeReInit(r->cv, r->t, r->y) != LLGCV_SUCCESS) return -1;
+  const int rc = LLG_RunAdvance(run, r->t + t_stage, res);
+  if (m_avg && LLG_MeanMagnetization(r->ctx, r->y, m_avg) != 0) return -1;
+  return rc;
+}
+
+int LLG_RunRamp(void* run, int which, int nstages, const double* happ3, double t_stage,
+                double* m_avg, llg_run_result* res) {
+  Run* r = static_cast<Run*>(run);
+  if (!r || !happ3 || nstages < 1) return -1;
+  llg_run_result tot{};
+  int rc = 0;
+  for (int k = 0; k < nstages && rc == 0; k++) {
+    llg_run_result one{};
+    rc = LLG_RunStage(run, which, happ3 + 3 * k, t_stage, &one, m_avg ? m_avg + 3 * k : nullptr);
+    llgcv_stats& a = tot.st;
+    const llgcv_stats& b = one.st;
+    a.nst += b.nst; a.nfe += b.nfe; a.nsetups += b.nsetups; a.nni += b.nni;
+    a.ncfn += b.ncfn; a.netf += b.netf; a.nli += b.nli; a.nlcf += b.nlcf;
+    a.njvevals += b.njvevals; a.npe += b.npe; a.nps += b.nps; a.nge += b.nge;
+    a.nfeLS += b.nfeLS;
+    a.qlast = b.qlast; a.qcur = b.qcur; a.hlast = b.hlast; a.hcur = b.hcur;
+    a.tcur = b.tcur; a.h0u = b.h0u; a.nsyncs = b.nsyncs;
+    tot.wall_s += one.wall_s;
+    tot.gpu_ms += one.gpu_ms;
+    tot.t_reached = one.t_reached;
+    tot.nout += one.nout;
+    tot.retval = one.retval;
+  }
+  const double span = (double)nstages * t_stage;
+  tot.sim_per_wall = tot.wall_s > 0 ? span / tot.wall_s : 0.0;
+  if (res) *res = tot;
+  return rc;
 }
 
 void LLG_RunDestroy(void* run) {

@@ -826,6 +826,46 @@ __global__ void k_normalize(double* y, int ncell, int mode) {
   }
 }
 
+
+// Mean magnetisation of the active cells: kMeanBlocks workgroups sum
+// three components over a fixed cell partition (grid-stride), a second
+// launch sums the partials in slot order -- the same bits on every call.
+constexpr int kMeanBlocks = 512;
+__global__ __launch_bounds__(256) void k_mean_partial(const double* __restrict__ y,
+                                                      const unsigned char* __restrict__ act,
+                                                      int ncell, double* __restrict__ part) {
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < ncell; i += kMeanBlocks * 256)
+    if (act[i]) {
+      s[0] += y[i];
+      s[1] += y[ncell + i];
+      s[2] += y[2 * (long)ncell + i];
+    }
+  __shared__ double red[3][256];
+  for (int a = 0; a < 3; a++) red[a][threadIdx.x] = s[a];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int a = 0; a < 3; a++) red[a][threadIdx.x] += red[a][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[threadIdx.x * kMeanBlocks + blockIdx.x] = red[threadIdx.x][0];
+}
+__global__ __launch_bounds__(256) void k_mean_final(double* __restrict__ part) {
+  __shared__ double red[3][256];
+  for (int a = 0; a < 3; a++)
+    red[a][threadIdx.x] = part[a * kMeanBlocks + threadIdx.x] +
+                          part[a * kMeanBlocks + 256 + threadIdx.x];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int a = 0; a < 3; a++) red[a][threadIdx.x] += red[a][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[3 * kMeanBlocks + threadIdx.x] = red[threadIdx.x][0];
+}
+static_assert(kMeanBlocks == 512, "k_mean_final folds exactly 2 x 256 partials");
+
 }  // namespace
 
 // ============================================================ context
@@ -856,6 +896,7 @@ struct llg_ctx {
   long nf, njv, npset, npsol;
   llg_lpc* lpc = nullptr;   // learned correction applied by PrecondSolve
   int bx = kBX, by = kBY;   // 2D stencil workgroup shape (LLG_SetStencilBlock)
+  double* d_mean = nullptr; // partial sums of LLG_MeanMagnetization
 };
 
 namespace {
@@ -1038,6 +1079,7 @@ llg_ctx* LLG_Create(const llg_config* cfg, const unsigned char* active, const do
 }
 
 void LLG_Destroy(llg_ctx* c) {
+  if (c) hipFree(c->d_mean);
   if (!c) return;
   hipFree(c->d_active); hipFree(c->d_w); hipFree(c->d_cmsk); hipFree(c->d_cnsk);
   hipFree(c->d_ids); hipFree(c->d_hole_ids); hipFree(c->d_hd);
@@ -1524,5 +1566,49 @@ int LLG_GetGrid(llg_ctx* ctx, int* ng, int* ny) {
 }
 
 const unsigned char* LLG_GetActiveDevice(llg_ctx* ctx) { return ctx ? ctx->d_active : nullptr; }
+
+int LLG_SetAppliedField(llg_ctx* ctx, int which, const double* happ) {
+  if (!ctx || !happ || which < 1 || which > 7) {
+    set_err("LLG_SetAppliedField: NULL argument or term-set mask outside 1..7");
+    return -1;
+  }
+  for (int a = 0; a < 3; a++) {
+    if (which & LLG_TERMS_RHS) ctx->cfg.rhs.happ[a] = happ[a];
+    if (which & LLG_TERMS_JV) ctx->cfg.jv.happ[a] = happ[a];
+    if (which & LLG_TERMS_PC) ctx->cfg.pc.happ[a] = happ[a];
+  }
+  return 0;
+}
+
+int LLG_GetAppliedField(llg_ctx* ctx, int which, double* happ) {
+  if (!ctx || !happ) return -1;
+  const llg_terms* t = which == LLG_TERMS_RHS ? &ctx->cfg.rhs
+                     : which == LLG_TERMS_JV  ? &ctx->cfg.jv
+                     : which == LLG_TERMS_PC  ? &ctx->cfg.pc : nullptr;
+  if (!t) return -1;
+  for (int a = 0; a < 3; a++) happ[a] = t->happ[a];
+  return 0;
+}
+
+int LLG_MeanMagnetization(llg_ctx* ctx, N_Vector y, double* m) {
+  if (!ctx || !y || !m || N_VGetLength_Hip(y) != (sunindextype)ctx->neq) return -1;
+  if (!ctx->d_mean && hipMalloc(&ctx->d_mean, sizeof(double) * (3 * kMeanBlocks + 3)) != hipSuccess) {
+    set_err("LLG_MeanMagnetization: hipMalloc failed");
+    return -1;
+  }
+  hipStream_t s = stream_of(y, ctx);
+  hipLaunchKernelGGL(k_mean_partial, dim3(kMeanBlocks), dim3(256), 0, s,
+                     (const double*)N_VGetDeviceArrayPointer_Hip(y), ctx->d_active, ctx->ncell,
+                     ctx->d_mean);
+  hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(256), 0, s, ctx->d_mean);
+  if (launch_status("LLG_MeanMagnetization")) return -1;
+  double sum[3];
+  if (hipMemcpyAsync(sum, ctx->d_mean + 3 * kMeanBlocks, sizeof(sum), hipMemcpyDeviceToHost, s) !=
+          hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    return -1;
+  const double n = ctx->n_active > 0 ? (double)ctx->n_active : 1.0;
+  for (int a = 0; a < 3; a++) m[a] = sum[a] / n;
+  return 0;
+}
 
 }  // extern "C"

@@ -20,7 +20,8 @@ from .native import (llg_config, llg_terms, llg_run_opts, llg_run_result, llg_gr
                      EXEC_YMSK, EXEC_COMPACT, ANISO_LINEAR, ANISO_CUBIC, NORM_NONE,
                      NORM_EXACT, NORM_REGULARIZED, BC_PERIODIC, BC_NEUMANN, DEMAG_OFF,
                      DEMAG_PERIODIC, DEMAG_PADDED, GATE_NONE, GATE_MASK, GATE_WINDOW,
-                     PC_ALWAYS, PC_JOK, PREC_LEFT, PREC_RIGHT, MODIFIED_GS, CLASSICAL_GS)
+                     PC_ALWAYS, PC_JOK, PREC_LEFT, PREC_RIGHT, MODIFIED_GS, CLASSICAL_GS,
+                     TERMS_RHS, TERMS_JV, TERMS_PC)
 
 __all__ = [
     "native", "lib", "terms", "make_config", "preset", "PRESETS", "Geometry", "Simulation",
@@ -372,6 +373,25 @@ class Simulation:
         lib().LLG_GetActiveIds(self.ctx, _ip(a), _ip(h))
         return a, h
 
+    def set_applied_field(self, happ, which=TERMS_RHS | TERMS_PC):
+        """LLG_SetAppliedField: the Zeeman field of the term sets in `which`"""
+        h = (C.c_double * 3)(*happ)
+        if lib().LLG_SetAppliedField(self.ctx, which, h) != 0:
+            raise ValueError("LLG_SetAppliedField: " + lib().LLG_LastError().decode())
+
+    def applied_field(self, which=TERMS_RHS):
+        h = (C.c_double * 3)()
+        if lib().LLG_GetAppliedField(self.ctx, which, h) != 0:
+            raise ValueError("LLG_GetAppliedField: one term set per call")
+        return tuple(h)
+
+    def mean_magnetization(self, y):
+        v = self.vec(y)
+        m = (C.c_double * 3)()
+        if lib().LLG_MeanMagnetization(self.ctx, v.v, m) != 0:
+            raise RuntimeError("LLG_MeanMagnetization failed")
+        return np.array(m)
+
     def self_coupling(self):
         d = lib().LLG_GetDemag(self.ctx)
         v = (C.c_double * 3)()
@@ -418,6 +438,20 @@ class Run:
         y = np.empty(self.sim.neq)
         lib().LLG_RunGetState(self.h, _dp(y))
         return y
+
+    def ramp(self, fields, t_stage, which=TERMS_RHS | TERMS_PC):
+        """LLG_RunRamp: one quasi-static stage per row of `fields` (n x 3),
+        t_stage each; returns the per-stage mean magnetisation (n x 3) and
+        the summed statistics"""
+        f = np.ascontiguousarray(fields, dtype=np.float64).reshape(-1, 3)
+        m = np.zeros_like(f)
+        res = llg_run_result()
+        rc = lib().LLG_RunRamp(self.h, which, f.shape[0], _dp(f), t_stage, _dp(m), C.byref(res))
+        out = {"retval": res.retval, "wall_s": res.wall_s, "t": res.t_reached,
+               "sim_per_wall": res.sim_per_wall, **res.st.as_dict()}
+        if rc != 0:
+            raise RuntimeError(f"ramp failed (rc {rc}) at t={res.t_reached}")
+        return m, out
 
     def close(self):
         if self.h:

@@ -18,6 +18,7 @@ BC_PERIODIC, BC_NEUMANN = 0, 1
 DEMAG_OFF, DEMAG_PERIODIC, DEMAG_PADDED = 0, 1, 2
 GATE_NONE, GATE_MASK, GATE_WINDOW = 0, 1, 2
 PC_ALWAYS, PC_JOK = 0, 1
+TERMS_RHS, TERMS_JV, TERMS_PC = 1, 2, 4
 PREC_NONE, PREC_LEFT, PREC_RIGHT, PREC_BOTH = 0, 1, 2, 3
 MODIFIED_GS, CLASSICAL_GS = 1, 2
 
@@ -281,6 +282,9 @@ SIGNATURES = {
     "LLG_LpcCorrect": (C.c_int, [VP, DP, C.c_double, DP]),
     "LLG_SetLearnedPrecond": (C.c_int, [VP, VP]),
     "LLG_GetGrid": (C.c_int, [VP, IP, IP]),
+    "LLG_SetAppliedField": (C.c_int, [VP, C.c_int, DP]),
+    "LLG_GetAppliedField": (C.c_int, [VP, C.c_int, DP]),
+    "LLG_MeanMagnetization": (C.c_int, [VP, VP, DP]),
     "LLG_SetStencilBlock": (C.c_int, [VP, C.c_int, C.c_int]),
     "LLG_Autotune": (C.c_int, [VP, C.c_int, IP]),
     "LLG_GetActiveDevice": (VP, [VP]),
@@ -316,6 +320,9 @@ SIGNATURES = {
     "LLG_RunCreate": (VP, [VP, C.POINTER(llg_run_opts), DP]),
     "LLG_RunAdvance": (C.c_int, [VP, C.c_double, C.POINTER(llg_run_result)]),
     "LLG_RunGetState": (C.c_int, [VP, DP]),
+    "LLG_RunStage": (C.c_int, [VP, C.c_int, DP, C.c_double, C.POINTER(llg_run_result), DP]),
+    "LLG_RunRamp": (C.c_int, [VP, C.c_int, C.c_int, DP, C.c_double, DP,
+                              C.POINTER(llg_run_result)]),
     "LLG_RunDestroy": (None, [VP]),
 }
 

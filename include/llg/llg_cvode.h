@@ -148,10 +148,16 @@ typedef struct llgcv_fused_ops {
                    N_Vector w, sunrealtype* nrm);
   /* the Newton residual res = c0 zn1 + c1 ycor + c2 f(t, y) with f formed in
    * the same pass and not stored (as LLG_RhsResidual: 0 done, 2 not handled,
-   * then the RHS callback and the combination run; < 0 RHS failure).  Only
-   * for RHS, jtimes and preconditioner callbacks that ignore fy. */
+   * then the RHS callback and the combination run; < 0 RHS failure).  f
+   * is never stored, so the fy a later jtimes or preconditioner setup call
+   * receives would be stale: the integrator takes this path only while the
+   * registered jtimes and preconditioner setup are the two callbacks named
+   * in fy_free_jtimes / fy_free_psetup (ones that never read fy; a NULL
+   * psetup means none may be registered), and never with a jtimes setup. */
   int (*rhs_residual)(sunrealtype t, N_Vector y, const sunrealtype* c, N_Vector zn1,
                       N_Vector ycor, N_Vector res, void* user_data);
+  CVLsJacTimesVecFn fy_free_jtimes;
+  CVLsPrecSetupFn fy_free_psetup;
   /* with predict_init: the step's completion zn += l acor left to the next
    * predictor's pass (as NVHip_PredictComplete, with the rescale of a step
    * size change), and the order-change norm of the completed zn[q] taken

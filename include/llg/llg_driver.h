@@ -78,6 +78,23 @@ void  LLG_DefaultRunOpts(llg_run_opts* o);
 void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_host);
 int   LLG_RunAdvance(void* run, double t_end, llg_run_result* res);
 int   LLG_RunGetState(void* run, double* y_host);
+/* One quasi-static stage of a field sweep: set the applied field of the
+ * term sets in `which` (LLG_SetAppliedField), restart the integrator at
+ * the run's current time and state (a field step is a discontinuity of f,
+ * so the BDF history is dropped, as CVodeReInit does), advance by t_stage
+ * on the output cadence, and return the stage's statistics in res and the
+ * mean magnetisation of the active cells at its end in m_avg[3].  No
+ * reference counterpart: the reference's field is a compile-time constant
+ * (src/irregular/2d_fft/2d_i4/2d_fft.cu:178-180). */
+int   LLG_RunStage(void* run, int which, const double* happ, double t_stage,
+                   llg_run_result* res, double* m_avg);
+/* A field ramp of nstages stages (LLG_RunStage each): stage s applies
+ * happ3[3s .. 3s+2] for t_stage; m_avg[3s .. 3s+2] receives its final mean
+ * magnetisation.  res sums the counters and wall time of all stages
+ * (t_reached is the last stage's).  Returns 0, or the first failing stage's
+ * negative code (stages after it are not run). */
+int   LLG_RunRamp(void* run, int which, int nstages, const double* happ3, double t_stage,
+                  double* m_avg, llg_run_result* res);
 void  LLG_RunDestroy(void* run);
 
 #ifdef __cplusplus

@@ -140,6 +140,25 @@ double LLG_EventElapsedMs(void* e0, void* e1);
 void   LLG_EventDestroy(void* ev);
 int    LLG_DeviceSynchronize(void);
 
+/* Term sets of a context (bit mask for LLG_SetAppliedField) */
+#define LLG_TERMS_RHS 1
+#define LLG_TERMS_JV  2
+#define LLG_TERMS_PC  4
+/* Set the applied (Zeeman) field of the term sets in `which` (a mask of
+ * LLG_TERMS_*), e.g. one stage of a quasi-static hysteresis sweep; it takes
+ * effect at the next callback.  The reference's field is a compile-time
+ * constant in each of its three mirrors (2d_i4/2d_fft.cu:178-180 happ1..3,
+ * precond.cu:68-70 pc_happ1..3; its Jv has none, jtv.cu:36-39), so a field
+ * change there means a rebuild.  Returns 0, or -1 on a NULL argument or a
+ * mask outside 1..7.  LLG_GetAppliedField reads one set (which = one bit). */
+int LLG_SetAppliedField(llg_ctx* ctx, int which, const double* happ);
+int LLG_GetAppliedField(llg_ctx* ctx, int which, double* happ);
+/* Mean magnetisation over the active cells of y (3 values), reduced in a
+ * fixed order on the device (the same bits on every call); one host wait.
+ * No reference counterpart (the reference writes whole frames,
+ * 2d_i4/2d_fft.cu:393-411, and averages them in its plotting scripts). */
+int LLG_MeanMagnetization(llg_ctx* ctx, N_Vector y, double* m);
+
 /* CVODE plug-ins (user_data = llg_ctx*) */
 int LLG_Rhs(sunrealtype t, N_Vector y, N_Vector ydot, void* user_data);
 /* The right-hand side and the Newton residual built from it in one RHS

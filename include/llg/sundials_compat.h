@@ -11,7 +11,8 @@
  * written from its public API so that the plug-in signatures below are the
  * ones the reference registers with CVodeInit / CVodeSetJacTimes /
  * CVodeSetPreconditioner (2d_i4/2d_fft.cu:625,642,643).  The ops-table
- * field order follows the public `struct _generic_N_Vector_Ops`.
+ * field order follows the public SUNDIALS 7 `struct _generic_N_Vector_Ops`
+ * (every slot, including the XBraid buffer ops), checked below.
  */
 #ifndef LLG_SUNDIALS_COMPAT_H
 #define LLG_SUNDIALS_COMPAT_H
@@ -133,6 +134,10 @@ struct _generic_N_Vector_Ops {
   /* single-buffer reductions */
   SUNErrCode (*nvdotprodmultilocal)(int, N_Vector, N_Vector*, sunrealtype*);
   SUNErrCode (*nvdotprodmultiallreduce)(int, N_Vector, sunrealtype*);
+  /* XBraid buffer interface (left NULL by the HIP vector) */
+  SUNErrCode (*nvbufsize)(N_Vector, sunindextype*);
+  SUNErrCode (*nvbufpack)(N_Vector, void*);
+  SUNErrCode (*nvbufunpack)(N_Vector, void*);
   /* debugging */
   void (*nvprint)(N_Vector);
   void (*nvprintfile)(N_Vector, FILE*);
@@ -143,6 +148,29 @@ struct _generic_N_Vector {
   N_Vector_Ops ops;
   SUNContext   sunctx;
 };
+
+/* SUNDIALS 7 layout of the ops table: 56 function pointers, the XBraid
+ * buffer ops at slots 51-53 and the two print ops last.  A real SUNDIALS
+ * build copies and indexes the table with these offsets (N_VCopyOps,
+ * N_VPrint, N_VBufSize), so they are checked at compile time. */
+#define LLG_NV_OPS_SLOTS 56
+#ifdef __cplusplus
+#define LLG_STATIC_ASSERT(c, m) static_assert(c, m)
+#else
+#define LLG_STATIC_ASSERT(c, m) _Static_assert(c, m)
+#endif
+LLG_STATIC_ASSERT(sizeof(struct _generic_N_Vector_Ops) == LLG_NV_OPS_SLOTS * sizeof(void*),
+                  "N_Vector ops table must hold the 56 SUNDIALS 7 slots");
+LLG_STATIC_ASSERT(__builtin_offsetof(struct _generic_N_Vector_Ops, nvlinearcombination) ==
+                      30 * sizeof(void*), "fused ops start at slot 30");
+LLG_STATIC_ASSERT(__builtin_offsetof(struct _generic_N_Vector_Ops, nvdotprodlocal) ==
+                      40 * sizeof(void*), "local reductions start at slot 40");
+LLG_STATIC_ASSERT(__builtin_offsetof(struct _generic_N_Vector_Ops, nvbufsize) ==
+                      51 * sizeof(void*), "XBraid buffer ops start at slot 51");
+LLG_STATIC_ASSERT(__builtin_offsetof(struct _generic_N_Vector_Ops, nvprint) ==
+                      54 * sizeof(void*), "nvprint is slot 54");
+LLG_STATIC_ASSERT(sizeof(struct _generic_N_Vector) == 3 * sizeof(void*),
+                  "N_Vector object is {content, ops, sunctx}");
 
 /* CVODE / CVLS user-supplied callback types (public SUNDIALS signatures) */
 typedef int (*CVRhsFn)(sunrealtype t, N_Vector y, N_Vector ydot, void* user_data);

@@ -8,6 +8,9 @@ separately from oracle/llg_oracle.c -- evaluated on small seeded grids:
   * f / Jv / block-Jacobi P^-1 / P-solve for the i4 (cubic, ymsk, Zeeman),
     i1 (linear anisotropy + DMI, exact normalisation), i6 (compact) and the
     plain 128x128-form (Neumann Laplacian, LL prefactor) physics
+  * the i3 windowed polycrystal (per-cell axes, DMI on w*m, w output gate,
+    windowed demag gather, fixed-axis Jv and preconditioner), written term
+    by term from the i3 kernels rather than through the shared helpers
   * a scipy Radau (rtol 1e-11) trajectory of the plain form on a 6x5 grid
 Usage: python tests/golden/gen_fixtures.py   (writes tests/golden/*.npz)
 """
@@ -220,6 +223,116 @@ def case(name, ng, ny, P_rhs, P_jv, P_pc, alpha, chg, active, y, neumann=False, 
     print("wrote", name)
 
 
+# ------------------------------------------------------- i3 polycrystal
+# Written straight from the i3 kernels (src/irregular/2d_fft/2d_i3/), term by
+# term, without the shared `field` helper above: the tiled RHS's LDS tile of
+# w*m, its per-cell axes and DMI and the w_self output gate
+# (2d_fft.cu:451-627), the regularised normalisation (2d_fft.cu:418-428),
+# the windowed demag gather (demag_fft.cu:125-139), the fixed-axis Jv
+# (jtv.cu:43-96) and the block-Jacobi build and apply (precond.cu:108-241,
+# 244-268; anisotropy axis and DMI direction fixed at x, no gate).
+def i3_fixture(name, ng, ny, seed):
+    rng = np.random.default_rng(seed)
+    N = ng * ny
+    c_chk, c_che, c_alpha, c_chg, c_cha, c_chb = 1.0, 4.0, 0.2, 1.0, 0.0, 0.6   # 2d_fft.cu:176-181
+    j_chk, j_che, j_alpha, j_chg, j_cha, j_chb = 4.0, 4.0, 0.2, 1.0, 0.0, 0.3   # jtv.cu:30-35
+    p_chk, p_che, p_alpha, p_chg, p_cha, p_chb = 4.0, 4.0, 0.2, 1.0, 0.0, 0.3   # precond.cu:58-63
+    strength = 1.0
+    w = rng.uniform(1e-3, 1.0, N)
+    w[rng.random(N) < 0.3] = 1.0
+    w[rng.random(N) < 0.1] = 1e-3
+    def unit(k):
+        a = rng.standard_normal((3, N))
+        return (a / np.linalg.norm(a, axis=0)).reshape(-1)
+    msk, nsk = unit(0), unit(1)
+    y = rng.standard_normal(3 * N)
+
+    # normalize_m_kernel: m / (|m| + 1e-3), in place on y
+    m = y.reshape(3, N)
+    ymp = np.sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2])
+    inv = 1.0 / (ymp + 1.0e-3)
+    yn = np.concatenate([m[0] * inv, m[1] * inv, m[2] * inv])
+
+    # Demag_ApplyWindowed: gather w*y, then the periodic pipeline
+    wy = np.concatenate([w * yn[:N], w * yn[N:2 * N], w * yn[2 * N:]])
+    hd = demag_periodic(wy, ng, ny, 1.0, strength)
+    t0 = tensor_grid(1.0, np.array(0.0), np.array(0.0))
+    nxx0, nyy0, nzz0 = t0[0] * strength, t0[1] * strength, t0[2] * strength
+
+    def wrapped(a):   # a (ny, ng) -> left, right, up, down with periodic wrap
+        return np.roll(a, 1, 1), np.roll(a, -1, 1), np.roll(a, 1, 0), np.roll(a, -1, 0)
+
+    M = [yn[k * N:(k + 1) * N].reshape(ny, ng) for k in range(3)]
+    W = w.reshape(ny, ng)
+    MS = [msk[k * N:(k + 1) * N].reshape(ny, ng) for k in range(3)]
+    NS = [nsk[k * N:(k + 1) * N].reshape(ny, ng) for k in range(3)]
+    HD = [hd[k * N:(k + 1) * N].reshape(ny, ng) for k in range(3)]
+    # tile of w*m and its neighbours
+    T = [W * M[k] for k in range(3)]
+    lx1, rx1, ux1, dx1 = wrapped(T[0])
+    lx2, rx2, ux2, dx2 = wrapped(T[1])
+    lx3, rx3, ux3, dx3 = wrapped(T[2])
+    m1, m2, m3 = M
+    mdotmsk = m1 * MS[0] + m2 * MS[1] + m3 * MS[2]
+    aniso_factor = c_chk * mdotmsk + c_cha
+    h1 = c_che * (lx1 + rx1 + ux1 + dx1) + MS[0] * aniso_factor + c_chb * NS[0] * (lx1 + rx1) + HD[0]
+    h2 = c_che * (lx2 + rx2 + ux2 + dx2) + MS[1] * aniso_factor + c_chb * NS[1] * (lx2 + rx2) + HD[1]
+    h3 = c_che * (lx3 + rx3 + ux3 + dx3) + MS[2] * aniso_factor + c_chb * NS[2] * (lx3 + rx3) + HD[2]
+    mh = m1 * h1 + m2 * h2 + m3 * h3
+    f = np.concatenate([
+        (W * (c_chg * (m3 * h2 - m2 * h3) + c_alpha * (h1 - mh * m1))).ravel(),
+        (W * (c_chg * (m1 * h3 - m3 * h1) + c_alpha * (h2 - mh * m2))).ravel(),
+        (W * (c_chg * (m2 * h1 - m1 * h2) + c_alpha * (h3 - mh * m3))).ravel()])
+
+    # jtv_kernel (unwindowed neighbours, anisotropy and DMI along x)
+    v = rng.standard_normal(3 * N)
+    V = [v[k * N:(k + 1) * N].reshape(ny, ng) for k in range(3)]
+    yL1, yR1, yU1, yD1 = wrapped(m1)
+    yL2, yR2, yU2, yD2 = wrapped(m2)
+    yL3, yR3, yU3, yD3 = wrapped(m3)
+    v1L, v1R, v1U, v1D = wrapped(V[0])
+    v2L, v2R, v2U, v2D = wrapped(V[1])
+    v3L, v3R, v3U, v3D = wrapped(V[2])
+    v1, v2, v3 = V
+    che_chb = j_che + j_chb
+    jh1 = che_chb * (yL1 + yR1) + j_che * (yU1 + yD1) + j_chk * m1 + j_cha
+    jh2 = j_che * (yL2 + yR2 + yU2 + yD2)
+    jh3 = j_che * (yL3 + yR3 + yU3 + yD3)
+    jmh = m1 * jh1 + m2 * jh2 + m3 * jh3
+    dh1 = che_chb * (v1L + v1R) + j_che * (v1U + v1D) + j_chk * v1
+    dh2 = j_che * (v2L + v2R + v2U + v2D)
+    dh3 = j_che * (v3L + v3R + v3U + v3D)
+    dmh = (v1 * jh1 + v2 * jh2 + v3 * jh3) + (m1 * dh1 + m2 * dh2 + m3 * dh3)
+    Jv = np.concatenate([
+        (j_chg * (v3 * jh2 + m3 * dh2 - v2 * jh3 - m2 * dh3) + j_alpha * (dh1 - dmh * m1 - jmh * v1)).ravel(),
+        (j_chg * (v1 * jh3 + m1 * dh3 - v3 * jh1 - m3 * dh1) + j_alpha * (dh2 - dmh * m2 - jmh * v2)).ravel(),
+        (j_chg * (v2 * jh1 + m2 * dh1 - v1 * jh2 - m1 * dh2) + j_alpha * (dh3 - dmh * m3 - jmh * v3)).ravel()])
+
+    # build_J_kernel: h with the last f()'s demag field, k = (chk + N_xx(0), N_yy(0), N_zz(0))
+    gamma = 0.0413
+    ph1 = p_che * (yL1 + yR1 + yU1 + yD1) + (p_chk * m1 + p_cha) + p_chb * (yL1 + yR1) + HD[0]
+    ph2 = p_che * (yL2 + yR2 + yU2 + yD2) + HD[1]
+    ph3 = p_che * (yL3 + yR3 + yU3 + yD3) + HD[2]
+    k1, k2, k3 = p_chk + nxx0, nyy0, nzz0
+    e1, e2, e3 = ph1 + m1 * k1, ph2 + m2 * k2, ph3 + m3 * k3
+    pmh = m1 * ph1 + m2 * ph2 + m3 * ph3
+    J = np.array([
+        [p_alpha * (k1 - e1 * m1 - pmh), p_chg * (m3 * k2 - ph3) - p_alpha * e2 * m1,
+         p_chg * (ph2 - m2 * k3) - p_alpha * e3 * m1],
+        [p_chg * (ph3 - m3 * k1) - p_alpha * e1 * m2, p_alpha * (k2 - e2 * m2 - pmh),
+         p_chg * (m1 * k3 - ph1) - p_alpha * e3 * m2],
+        [p_chg * (m2 * k1 - ph2) - p_alpha * e1 * m3, p_chg * (ph1 - m1 * k2) - p_alpha * e2 * m3,
+         p_alpha * (k3 - e3 * m3 - pmh)]])
+    A = np.eye(3)[:, :, None, None] - gamma * J
+    Pi = np.moveaxis(np.linalg.inv(np.moveaxis(A, (0, 1), (2, 3))), (2, 3), (0, 1))
+    r = rng.standard_normal(3 * N)
+    z = np.einsum("abij,bij->aij", Pi, r.reshape(3, ny, ng)).reshape(-1)
+    np.savez_compressed(OUT / f"{name}.npz", y=y, y_norm=yn, active=np.ones(N, np.uint8), w=w,
+                        msk=msk, nsk=nsk, f=f, hd=hd, v=v, Jv=Jv, gamma=gamma,
+                        Pinv=Pi.reshape(9, -1).reshape(-1), r=r, z=z, ng=ng, ny=ny)
+    print("wrote", name)
+
+
 def square_hole(ng, ny, cxf, cyf, hf):
     cx, cy, hs = cxf * (ng - 1), cyf * (ny - 1), hf * ny
     I, J = np.meshgrid(np.arange(ng), np.arange(ny))
@@ -269,6 +382,9 @@ def main():
     case("plain_9x7", ng, ny, pl, pl, pl, a / (1 + a * a), 1 / (1 + a * a),
          np.ones(ng * ny, np.uint8), rng.standard_normal(3 * ng * ny), neumann=True,
          exch_self=True, gate=False)
+
+    # i3: windowed polycrystal (per-cell axes, DMI, w gate, windowed demag)
+    i3_fixture("i3_14x10", 14, 10, seed=31)
 
     # Radau trajectory of the plain form on 6x5 (tight tolerance)
     from scipy.integrate import solve_ivp

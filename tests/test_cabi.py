@@ -50,3 +50,41 @@ def test_python_binding_covers_exports():
 def test_load_fails_loudly_when_missing(tmp_path):
     with pytest.raises(RuntimeError):
         la.native.load(tmp_path / "nope.so")
+
+
+OPS_CHECK = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "llg/sundials_compat.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(struct _generic_N_Vector_Ops),
+         offsetof(struct _generic_N_Vector_Ops, nvdotprodmulti),
+         offsetof(struct _generic_N_Vector_Ops, nvdotprodmultiallreduce),
+         offsetof(struct _generic_N_Vector_Ops, nvbufsize),
+         offsetof(struct _generic_N_Vector_Ops, nvbufunpack),
+         offsetof(struct _generic_N_Vector_Ops, nvprint),
+         offsetof(struct _generic_N_Vector_Ops, nvprintfile));
+  return 0;
+}
+"""
+
+
+@pytest.mark.parametrize("compiler,lang", [("gcc", "c"), ("g++", "c++")])
+def test_ops_table_has_sundials7_layout(tmp_path, compiler, lang):
+    """The N_Vector ops table has SUNDIALS 7's 56 slots in order: an
+    unmodified CVODE copies it with sizeof and calls N_VPrint / N_VBufSize
+    through these offsets.  The header's static asserts must hold in C and
+    C++, and the offsets are checked here against the public field order
+    (11 utility, 19 standard, 3 fused, 7 array, 9 local-reduction, 2
+    single-buffer, 3 XBraid buffer and 2 print ops)."""
+    import subprocess
+    src = tmp_path / ("ops." + ("c" if lang == "c" else "cpp"))
+    src.write_text(OPS_CHECK)
+    exe = tmp_path / "ops"
+    r = subprocess.run([compiler, "-x", lang, "-I", str(ROOT / "include"), str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    vals = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
+                                           check=True).stdout.split()]
+    p = C.sizeof(C.c_void_p)
+    assert vals == [56 * p, 32 * p, 50 * p, 51 * p, 53 * p, 54 * p, 55 * p]

@@ -523,3 +523,59 @@ def test_newton_update_comb_equals_lincombdiv_then_update():
     assert dna.value == dnb.value
     assert L.NVHip_NewtonUpdateComb(yb.v, 4, c, arr(X), sv.v, zn0.v, w.v, X[0].v,
                                     C.byref(dnb)) == -1   # y aliases an input
+
+
+class _NVecObj(C.Structure):
+    _fields_ = [("content", C.c_void_p), ("ops", C.POINTER(C.c_void_p * 56)),
+                ("sunctx", C.c_void_p)]
+
+
+def _ops(v):
+    return list(C.cast(v, C.POINTER(_NVecObj)).contents.ops.contents)
+
+
+def test_fused_nvec_init_propagates_to_clones():
+    """FusedNVec_Init (2d_i4/deferred_nvector.cu:182-235) installs the
+    one-launch multi-dot and linear combination on a vector, and CVODE's
+    internal clones (N_VClone through the ops table, then clones of clones)
+    carry the same 56 slots; the XBraid buffer slots stay NULL.  The fused
+    results equal the standard ops' results, and K > 16 (the reference's
+    fallback point, deferred_nvector.cu:151-155) is still exact."""
+    L = la.lib()
+    n = 200_003
+    rng = np.random.default_rng(77)
+    v = vec(rng.standard_normal(n))
+    base = _ops(v.v)
+    assert base[32] is None and base[30] is None      # created without fused ops
+    assert base[54] is not None and base[55] is not None
+    L.FusedNVec_Init(v.v)
+    ops = _ops(v.v)
+    assert ops[30] is not None and ops[32] is not None   # lincomb, dotprodmulti
+    assert ops[51] is None and ops[52] is None and ops[53] is None
+    clone_fn = C.CFUNCTYPE(C.c_void_p, C.c_void_p)(ops[1])
+    c1 = clone_fn(v.v)
+    c2 = C.CFUNCTYPE(C.c_void_p, C.c_void_p)(_ops(c1)[1])(c1)
+    try:
+        assert _ops(c1) == ops and _ops(c2) == ops
+        assert L.N_VGetLength_Hip(c2) == n
+        x = rng.standard_normal(n)
+        vx = vec(x)
+        for k in (3, 16, 19):
+            Y = [rng.standard_normal(n) for _ in range(k)]
+            vY = [vec(a) for a in Y]
+            dotmulti = C.CFUNCTYPE(C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p)(ops[32])
+            d = (C.c_double * k)()
+            assert dotmulti(k, vx.v, C.cast(arr(vY), C.c_void_p), C.cast(d, C.c_void_p)) == 0
+            for j in range(k):
+                assert dot_close(d[j], x, Y[j])
+            c = rng.standard_normal(k)
+            lincomb = C.CFUNCTYPE(C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p)(ops[30])
+            assert lincomb(k, C.cast((C.c_double * k)(*c), C.c_void_p),
+                           C.cast(arr(vY), C.c_void_p), c2) == 0
+            got = np.empty(n)
+            assert L.N_VGetToHost_Hip(c2, got.ctypes.data_as(C.POINTER(C.c_double))) == 0
+            ref = sum(c[j] * Y[j] for j in range(k))
+            assert np.max(np.abs(got - ref)) <= 1e-14 * k * np.max(np.abs(ref))
+    finally:
+        L.N_VDestroy_Hip(c2)
+        L.N_VDestroy_Hip(c1)

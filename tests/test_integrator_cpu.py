@@ -159,3 +159,56 @@ def test_rhs_failures_and_mxstep():
     assert la.lib().LLGCVode(mem, 10.0, y.v, C.byref(tret), 1) == -1     # TOO_MUCH_WORK
     assert 0.0 < tret.value < 10.0
     _free(mem)
+
+
+def _fused_ops_type():
+    """ctypes mirror of llgcv_fused_ops: every member is a function pointer,
+    in the header's order."""
+    import re
+    text = (Path(__file__).resolve().parent.parent / "include" / "llg" / "llg_cvode.h").read_text()
+    body = text[text.index("typedef struct llgcv_fused_ops {"):text.index("} llgcv_fused_ops;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = re.findall(r"\(\*(\w+)\)\s*\(|^\s*CVLs\w+\s+(\w+);", body, flags=re.M)
+    names = [a or b for a, b in names]
+    return type("llgcv_fused_ops", (C.Structure,), {"_fields_": [(n, C.c_void_p) for n in names]})
+
+
+RESFN = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                    C.c_void_p, C.c_void_p)
+
+
+@pytest.mark.parametrize("declared", [True, False])
+def test_rhs_residual_only_with_fy_free_callbacks(declared):
+    """The fused Newton residual never stores f, so the fy a preconditioner
+    setup receives would be stale: the integrator must take that path only
+    while the registered jtimes / psetup are the ones declared fy-free.  The
+    hook here declines every call (returns 2), so the trajectory is the
+    plain one either way; only whether it is consulted differs."""
+    cfg = la.preset("plain128", ng=6, ny=5)
+    sim = orc.OracleSim(cfg, la.Geometry.full(6, 5))
+    rng = np.random.default_rng(5)
+    th, ph = 0.1 * rng.random(30), 2 * np.pi * rng.random(30)
+    y = orc.SerialVector(90, np.concatenate([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph),
+                                             np.cos(th)]))
+    jtv, pset = orc.addr("orc_cv_jtv"), orc.addr("orc_cv_psetup")
+    mem = _integrator(orc.addr("orc_cv_rhs"), sim.ctx, y, 1e-6, 1e-8, jtv=jtv, pset=pset,
+                      psol=orc.addr("orc_cv_psolve"))
+    calls = [0]
+
+    def hook(t, yv, c, zn1, ycor, res, ud):
+        calls[0] += 1
+        return 2
+
+    cb = RESFN(hook)
+    fo = _fused_ops_type()()
+    fo.rhs_residual = C.cast(cb, C.c_void_p).value
+    fo.fy_free_jtimes = jtv
+    fo.fy_free_psetup = pset if declared else None
+    try:
+        assert la.lib().LLGCVodeSetFusedOps(mem, C.byref(fo)) == 0
+        tret = C.c_double()
+        assert la.lib().LLGCVode(mem, 1.0, y.v, C.byref(tret), 1) == 0
+    finally:
+        _free(mem)
+        sim.close()
+    assert (calls[0] > 0) == declared

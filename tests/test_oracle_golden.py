@@ -5,7 +5,11 @@ restatement of the reference formulas (tests/golden/gen_fixtures.py):
 numpy.fft for the periodic demag, a direct linear sum for the padded one,
 numpy.linalg.inv for the 3x3 blocks.  Agreement to ~1e-12 relative pins the
 C oracle's RHS / Jv / P^-1 / P-solve / demag for the i4, i4-padded, i1,
-i6-compact and plain variants.
+i6-compact and plain variants, and the i3 windowed polycrystal: window-
+weighted exchange, per-cell anisotropy axes and DMI directions, DMI on w*m,
+the w output gate and the windowed demag gather (i3_14x10 is written term
+by term from the i3 kernels, not through the fixture generator's shared
+helpers, so a misreading shared by the oracle and the HIP kernels fails it).
 """
 import numpy as np
 import pytest
@@ -28,6 +32,8 @@ def load(name):
 
 def geom_of(d):
     ng, ny = int(d["ng"]), int(d["ny"])
+    if "w" in d:
+        return la.Geometry(ng, ny, d["active"].astype(np.uint8), d["w"], d["msk"], d["nsk"])
     return la.Geometry(ng, ny, d["active"].astype(np.uint8))
 
 
@@ -37,6 +43,7 @@ CASES = {
     "i1_14x10": lambda: la.preset("i1", ng=14, ny=10, demag=2.0),
     "i6_16x16": lambda: la.preset("i6", ng=16, ny=16, demag=4.0),
     "plain_9x7": lambda: la.preset("plain128", ng=9, ny=7),
+    "i3_14x10": lambda: la.preset("i3", ng=14, ny=10, demag=1.0),
 }
 
 

@@ -205,3 +205,66 @@ def test_singular_block_falls_back_to_identity():
     interior = [j * ng + i for j in range(1, ny - 1) for i in range(1, ng - 1)]
     for c in interior:
         assert np.array_equal(P[:, :, c], np.eye(3))
+
+
+def _fd_block(sim, y, N, cell, eps=1e-6):
+    """local 3x3 Jacobian of f at y with respect to the cell's own m"""
+    J = np.zeros((3, 3))
+    for b in range(3):
+        e = np.zeros(3 * N)
+        e[b * N + cell] = eps
+        fp, _ = sim.rhs(y + e)
+        fm, _ = sim.rhs(y - e)
+        J[:, b] = ((fp - fm) / (2 * eps)).reshape(3, N)[:, cell]
+    return J
+
+
+@pytest.mark.parametrize("demag", [0.0, 4.0])
+def test_jok_two_phase_reuses_the_stored_jacobian(demag):
+    """Two-phase protocol (src/2d/2d_heter/precond.cu:154-192,265-314):
+    psetup(y1, g1, jok=0) builds J(y1) and reports jcur = TRUE; a later
+    psetup(y2, g2, jok=1) keeps J(y1), refactors with g2 and reports
+    jcur = FALSE, so P^-1 = inv(I - g2 J_FD(y1)) -- not J(y2) -- while
+    jok = 0 at y2 gives inv(I - g2 J_FD(y2)).  J_FD is the central
+    difference of f in the cell's own m (the preconditioner's field set
+    equal to the RHS's; with demag the self term strength*N(0) enters both)."""
+    ng, ny = 8, 6
+    N = ng * ny
+    cfg = la.preset("i4", ng=ng, ny=ny, demag=demag, pc_mode=la.PC_JOK)
+    cfg.pc.happ[0] = cfg.rhs.happ[0]
+    sim = orc.OracleSim(cfg, la.Geometry.full(ng, ny))
+    rng = np.random.default_rng(21)
+    y1 = _unit(rng.standard_normal(3 * N), N)
+    y2 = _unit(y1 + 0.3 * rng.standard_normal(3 * N), N)
+    g1, g2 = 0.05, 0.02
+    sim.rhs(y1)
+    assert sim.psetup(y1, g1, jok=0) == 1
+    sim.rhs(y2)
+    assert sim.psetup(y2, g2, jok=1) == 0
+    P_reuse = sim.pinv().reshape(3, 3, N).copy()
+    assert sim.psetup(y2, g2, jok=0) == 1
+    P_fresh = sim.pinv().reshape(3, 3, N).copy()
+    for cell in (0, 7, N // 2, N - 1):
+        J1 = _fd_block(sim, y1, N, cell)
+        J2 = _fd_block(sim, y2, N, cell)
+        A1, A2 = np.eye(3) - g2 * J1, np.eye(3) - g2 * J2
+        assert np.max(np.abs(A1 @ P_reuse[:, :, cell] - np.eye(3))) < 1e-7
+        assert np.max(np.abs(A2 @ P_fresh[:, :, cell] - np.eye(3))) < 1e-7
+        # the two differ far beyond the tolerance: reusing J(y2) would fail
+        assert np.max(np.abs(A1 @ P_fresh[:, :, cell] - np.eye(3))) > 1e-3
+
+
+def test_i3_torque_invariant_on_unit_sphere():
+    """The windowed polycrystal RHS (per-cell axes, DMI on w*m, windowed
+    demag, w output gate) is a torque: m . f = w alpha (m.h)(1 - |m|^2) = 0
+    on the unit sphere (normalisation off so |m| = 1 exactly)."""
+    ng, ny = 16, 12
+    geom = la.Geometry.polycrystal(ng, ny, num_grains=6, dead_frac=0.34)
+    cfg = la.preset("i3", ng=ng, ny=ny, demag=1.0)
+    cfg.normalize = la.NORM_NONE
+    sim = orc.OracleSim(cfg, geom)
+    y = _unit(np.random.default_rng(8).standard_normal(3 * ng * ny), ng * ny)
+    f, yn = sim.rhs(y)
+    dot = (yn.reshape(3, -1) * f.reshape(3, -1)).sum(0)
+    assert np.max(np.abs(f)) > 1.0
+    assert np.max(np.abs(dot)) < 1e-12 * np.max(np.abs(f))

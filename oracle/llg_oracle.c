@@ -23,6 +23,8 @@
  *                      padded: same sum without wrap (no reference counterpart)
  *   orc_demag_fft      i4/demag_fft.cu:184-363 (9 complex tensor spectra,
  *                      forward FFT of m, 3x3 product, inverse, shift, scale)
+ *   orc_set_happ       field-sweep stage (the applied field changed between
+ *                      integrator restarts; no reference counterpart)
  *   orc_nv_*           SUNDIALS serial N_Vector semantics; the multi-dot and
  *                      linear combination of i4/deferred_nvector.cu:95-180 and
  *                      heter/deferred_nvector.cu:135-151
@@ -397,6 +399,16 @@ const int* orc_active_ids(orc_ctx* c) { return c->active_ids; }
 const int* orc_inactive_ids(orc_ctx* c) { return c->inactive_ids; }
 const double* orc_hd(orc_ctx* c) { return c->hd; }
 const double* orc_pinv(orc_ctx* c) { return c->P; }
+/* applied field of the term sets in `which` (1 RHS, 2 Jv, 4 PC), the
+ * oracle side of a field-sweep stage (no reference counterpart: the
+ * reference's field is a compile-time constant, i4/2d_fft.cu:178-180) */
+void orc_set_happ(orc_ctx* c, int which, const double* happ) {
+  for (int a = 0; a < 3; a++) {
+    if (which & 1) c->cfg.rhs.happ[a] = happ[a];
+    if (which & 2) c->cfg.jv.happ[a] = happ[a];
+    if (which & 4) c->cfg.pc.happ[a] = happ[a];
+  }
+}
 void orc_self_coupling(orc_ctx* c, double* n3) { n3[0] = c->n0[0]; n3[1] = c->n0[1]; n3[2] = c->n0[2]; }
 
 /* neighbour indices with the configured boundary; ghost flags for Neumann */

@@ -54,6 +54,7 @@ def load():
         "orc_hd": (DP, [VP]),
         "orc_pinv": (DP, [VP]),
         "orc_self_coupling": (None, [VP, DP]),
+        "orc_set_happ": (None, [VP, C.c_int, DP]),
         "orc_normalize": (None, [VP, DP]),
         "orc_rhs_arrays": (None, [VP, DP, DP]),
         "orc_jtv_arrays": (None, [VP, DP, DP, DP]),
@@ -203,6 +204,10 @@ class OracleSim:
         load().orc_self_coupling(self.ctx, _dp(v))
         return v
 
+    def set_applied_field(self, happ, which=5):
+        h = np.ascontiguousarray(happ, dtype=np.float64)
+        load().orc_set_happ(self.ctx, which, _dp(h))
+
 
 class SerialVector:
     def __init__(self, n, values=None):
@@ -262,3 +267,41 @@ def cpu_integrate(product_lib, cfg, geom, y0, t_end, t_out=0.1, rtol=1e-4, atol=
     out = y.data().copy()
     sim.close()
     return out, dict(st.as_dict(), retval=rc, t=tret.value), wall
+
+
+def cpu_ramp(product_lib, cfg, geom, y0, fields, t_stage, which=5, t_out=0.1, rtol=1e-4,
+             atol=1e-4, maxl=5, gstype=None, max_order=5, mxstep=5000):
+    """A field ramp on the oracle's callbacks with the product's integrator:
+    per stage set the applied field, restart the integrator at the current
+    (t, y) and advance t_stage on the t_out cadence (the CPU counterpart of
+    LLG_RunRamp).  Returns (final y, per-stage states)."""
+    sim = OracleSim(cfg, geom)
+    n = sim.neq
+    y = SerialVector(n, y0)
+    P = product_lib
+    mem = P.LLGCVodeCreate(2, None)
+    gs = gstype if gstype else (2 if n < 500000 else 1)
+    assert P.LLGCVodeSetMaxOrd(mem, max_order) == 0
+    assert P.LLGCVodeInit(mem, addr("orc_cv_rhs"), 0.0, y.v) == 0
+    assert P.LLGCVodeSetUserData(mem, sim.ctx) == 0
+    assert P.LLGCVodeSStolerances(mem, rtol, atol) == 0
+    assert P.LLGCVodeSetLinearSolverSPGMR(mem, 1, maxl, gs) == 0
+    assert P.LLGCVodeSetJacTimes(mem, None, addr("orc_cv_jtv")) == 0
+    assert P.LLGCVodeSetPreconditioner(mem, addr("orc_cv_psetup"), addr("orc_cv_psolve")) == 0
+    assert P.LLGCVodeSetMaxNumSteps(mem, mxstep) == 0
+    tret = C.c_double(0.0)
+    t = 0.0
+    states = []
+    for h in np.asarray(fields, dtype=np.float64).reshape(-1, 3):
+        sim.set_applied_field(h, which)
+        assert P.LLGCVodeReInit(mem, t, y.v) == 0
+        tout, t_end = t + t_out, t + t_stage
+        while tout <= t_end + 1e-9 * max(1.0, abs(t_end)):
+            assert P.LLGCVode(mem, tout, y.v, C.byref(tret), 1) >= 0
+            t = tret.value
+            tout += t_out
+        states.append(y.data().copy())
+    m = C.c_void_p(mem)
+    P.LLGCVodeFree(C.byref(m))
+    sim.close()
+    return states[-1], states

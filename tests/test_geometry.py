@@ -131,6 +131,31 @@ def test_polycrystal_matches_reference_restatement():
     assert np.array_equal(msk[2], np.array([grains[k][2] for k in gid]))
 
 
+@pytest.mark.parametrize("ng,ny,G", [(200, 160, 120), (257, 97, 400), (96, 96, 1)])
+def test_polycrystal_block_pruned_voronoi_equals_full_scan(ng, ny, G):
+    """The builder scans, per 32x32 block, only the seed images that can be
+    nearest; the ids must equal the full scan over every grain and all 3x3
+    periodic images with the reference's d2 arithmetic and first-in-scan-
+    order ties (2d_i3/2d_fft.cu:300-315).  Grids that are not multiples of
+    the block and 400 grains (the 4096^2 workload's count) included."""
+    g = la.Geometry.polycrystal(ng, ny, num_grains=G, dead_frac=0.1, seed=7)
+    S = np.array([[g.grains[k].sx, g.grains[k].sy] for k in range(G)])
+    I, J = np.meshgrid(np.arange(ng) + 0.5, np.arange(ny) + 0.5)
+    x, y = I.ravel(), J.ravel()
+    best = np.full(x.size, 1e30)
+    gid = np.zeros(x.size, np.int64)
+    for k in range(G):                      # the reference's loop order
+        for dyp in (-1, 0, 1):
+            for dxp in (-1, 0, 1):
+                dx = x - (S[k, 0] + dxp * float(ng))
+                dy = y - (S[k, 1] + dyp * float(ny))
+                d2 = dx * dx + dy * dy
+                take = d2 < best
+                best = np.where(take, d2, best)
+                gid = np.where(take, k, gid)
+    assert np.array_equal(g.grain_id, gid)
+
+
 def test_two_domain_ic():
     ng, ny = 40, 32
     g = la.Geometry.square_hole(ng, ny, halfside_frac=0.2)

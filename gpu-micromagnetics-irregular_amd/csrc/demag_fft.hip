@@ -609,6 +609,20 @@ __global__ void k_extract(const double2* __restrict__ S, int Py, int Hx, double 
   outp[(long)kx * Py + ky] = scale * (take_imag ? v.y : v.x);
 }
 
+// Zero-padded grids: the kernel planes are even in dy (xx, yy, zz) or odd
+// (xy), and the row dy = -Py/2 is zero, so their column spectra satisfy
+// K(Py - ky) = +-K(ky).  The FFT computes the two halves with different
+// roundings; store the upper half as the exact mirror of the lower so a
+// column kernel may read ky <= Py/2 only and lose no bit.
+__global__ void k_mirror_ky(double* __restrict__ K, int Hx, int Py) {
+  const int kx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ky = Py / 2 + 1 + blockIdx.y;
+  const int c = blockIdx.z;
+  if (kx >= Hx || ky >= Py) return;
+  double* plane = K + (long)c * Hx * Py + (long)kx * Py;
+  plane[ky] = (c == 3 ? -1.0 : 1.0) * plane[Py - ky];
+}
+
 // ---------------------------------------------------------- host helpers
 bool factorize(int L, std::vector<int>& rad) {
   rad.clear();
@@ -874,13 +888,19 @@ DemagData* init_common(int nx, int ny, double thick, double strength, int padded
       hipLaunchKernelGGL(k_extract, dim3((Hx + 255) / 256, Py), dim3(256), 0, d->stream,
                          (const double2*)d->d_spec, Py, Hx, scale, 1, d->d_K + 4 * P);
   }
+  if (padded && Py / 2 - 1 > 0)
+    hipLaunchKernelGGL(k_mirror_ky, dim3((Hx + 255) / 256, Py / 2 - 1, 4), dim3(256), 0, d->stream,
+                       d->d_K, Hx, Py);
   hipStreamSynchronize(d->stream);
   hipFree(dQ);
   hipFree(dKr);
   if (!launch_ok("Demag_Init")) { destroy(d); return nullptr; }
   // register-resident FFT kernels wherever a plan covers the lengths
   d->reg_row = demag_reg::has_row_plan<double>(d->px / 2, 0) ? 0 : -1;
-  d->reg_col = demag_reg::has_col_plan<double>(d->py, 0) ? 0 : -1;
+  // columns: the blocked form (4 columns per workgroup, same bits as the
+  // one-column form) where it exists
+  d->reg_col = demag_reg::has_col_plan<double>(d->py, 3) ? 3
+             : demag_reg::has_col_plan<double>(d->py, 0) ? 0 : -1;
   return d;
 }
 
@@ -1145,7 +1165,7 @@ int Demag_Autotune(DemagData* d, int reps, int* choice) {
     if (v < 0 ? d->row.block > 0 && (!d->d_K32 || d->row32.block > 0)
               : demag_reg::has_row_plan<double>(d->px / 2, v))
       row_vars.push_back(v);
-  for (int v : cands(cv0, {0, 1, -1}))
+  for (int v : cands(cv0, {0, 1, 2, 3, 4, -1}))
     if (v < 0 ? d->col.block > 0 && (!d->d_K32 || d->col32.block > 0)
               : demag_reg::has_col_plan<double>(d->py, v))
       col_vars.push_back(v);

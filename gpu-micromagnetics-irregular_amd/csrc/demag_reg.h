@@ -21,7 +21,10 @@ using C2 = rfft::Cplx<T>;
 // true when register plan `variant` exists for a row (complex length
 // L = Px/2) or a column (length L = Py) transform.  Rows have variant 0;
 // columns have variant 0 (8 values per component and lane) for every
-// length and variant 1 (16 values per lane) for lengths 1024 to 8192.
+// length, variant 1 (16 values per lane) for lengths 1024 to 8192, and
+// variants 2, 3, 4 (variant 0's transforms with 2, 4 or 8 adjacent columns
+// per workgroup and the spectrum moved in runs of that many values) for
+// lengths 512 to 2048 where the block fits one workgroup.
 template <class T> bool has_row_plan(int L, int variant);
 template <class T> bool has_col_plan(int L, int variant);
 

@@ -25,6 +25,9 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
+#include <cstdint>
+#include <type_traits>
+
 #include "demag_reg.h"
 #include "fft_regs.h"
 
@@ -262,6 +265,150 @@ __global__ __launch_bounds__(F::kThreads) __attribute__((amdgpu_waves_per_eu(Col
   });
 }
 
+// ------------------------------------------------- column, blocked by CB
+// CB adjacent columns per workgroup.  The one-column kernel above loads and
+// stores one complex value per lane at a stride of a spectrum row (Hx * 16 B
+// in FP64), so every wave instruction touches 64 cache lines and fills
+// 16 B of each; partly written lines go back to memory before the
+// neighbouring columns' workgroups complete them.  Here the spectrum rows
+// move as runs of CB consecutive values (CB * 16 B): thread t reads row
+// t / CB + U i of column t % CB and stages it in LDS, and each column's
+// transforms then run on threads [c U, (c + 1) U) exactly as in the
+// one-column kernel -- the same plan, the same values in the same lanes,
+// hence the same bits.  The column stride in LDS is the plan's slot count
+// padded so that the CB columns of one row land P / CB apart modulo the
+// bank period of P complex values (P = 256 B / value size): each group of
+// P lanes of a staging access hits distinct banks.
+// HALF (zero-padded grids): the tensor planes are read for ky <= Py / 2
+// only; the builder stores K(Py - ky) = K(ky) for xx, yy, zz and
+// K(Py - ky) = -K(ky) for xy exactly, so the mirrored read changes no bit.
+template <class T, class F, int CB>
+struct ColCb {
+  static constexpr int P = 256 / (int)sizeof(C2<T>);
+  static constexpr int want = P / CB;
+  static constexpr int CS = F::kSlots + (((want - F::kSlots % P) % P) + P) % P;
+  static constexpr int kThreads = F::U * CB;
+  static constexpr size_t kLds = sizeof(C2<T>) * (size_t)CB * CS;
+  static constexpr bool ok = F::kPerGroup == 1 && kThreads <= 1024 && kLds <= 160 * 1024;
+};
+
+template <class T, class F, bool HALF, bool XYI, int CB>
+__global__ __launch_bounds__(F::U * CB) __attribute__((amdgpu_waves_per_eu(ColWaves<T, F>::value)))
+void k_col_cb(C2<T>* __restrict__ spec, int Hx, const C2<T>* __restrict__ tw,
+              const T* __restrict__ K) {
+  using C = C2<T>;
+  constexpr int U = F::U, V = F::V, L = F::L;
+  constexpr int NB = HALF ? V / 2 : V;
+  constexpr int ROWS = HALF ? L / 2 : L;
+  constexpr int CS = ColCb<T, F, CB>::CS;
+  using Last = typename F::Last;
+  constexpr int Q = Last::Q, CH = Q < 4 ? Q : 4;
+  static_assert(ColCb<T, F, CB>::ok, "plan does not fit a blocked column workgroup");
+  extern __shared__ char lds_raw[];
+  C* const lds_all = reinterpret_cast<C*>(lds_raw);
+  const int tid = threadIdx.x;
+  const int c = tid / U, lane = tid - c * U;      // transform view
+  const int cio = tid % CB, rio = tid / CB;       // staging view
+  // first half of the grid: x/y pairs; second half: z (as the one-column
+  // kernel; each half a multiple of 8 workgroups)
+  const int half_grid = gridDim.x / 2;
+  const bool zjob = (int)blockIdx.x >= half_grid;
+  const int bid = zjob ? (int)blockIdx.x - half_grid : (int)blockIdx.x;
+  const int kx0 = reg_xcd_remap(bid, half_grid) * CB;
+  if (kx0 >= Hx) return;                          // whole workgroup
+  int kx = kx0 + c;
+  if (kx >= Hx) kx = Hx - 1;                      // computed, never stored
+  const bool live_io = kx0 + cio < Hx;
+  const int kx_io = live_io ? kx0 + cio : Hx - 1;
+  C* const lds = lds_all + c * CS;
+  C r[3][V];
+  // global loads of one component: runs of CB consecutive columns
+  auto fetch = [&](auto Cc, C* v) {
+    const C* src = spec + (long)Cc * ROWS * Hx + kx_io;
+    sfor<0, NB>([&](auto I) { v[I] = ldc(src + (long)(rio + U * I) * Hx); });
+  };
+  // staged values -> this lane's registers (natural order n = lane + U b)
+  auto stage = [&](auto Cc, const C* v) {
+    sfor<0, NB>([&](auto I) { lds_all[cio * CS + rio + U * I] = v[I]; });
+    __syncthreads();
+    sfor<0, NB>([&](auto B) { r[Cc][B] = lds[lane + U * B]; });
+    __syncthreads();
+  };
+  auto store = [&](auto Cc) {
+    __syncthreads();                              // the inverse's last LDS reads
+    sfor<0, NB>([&](auto B) { lds[lane + U * B] = r[Cc][B]; });
+    __syncthreads();
+    if (live_io) {
+      C* dst = spec + (long)Cc * ROWS * Hx + kx_io;
+      sfor<0, NB>([&](auto I) { stc(dst + (long)(rio + U * I) * Hx, lds_all[cio * CS + rio + U * I]); });
+    }
+  };
+  const long P = (long)Hx * L;
+  const T* kxx = K + (long)kx * L;
+  const T* kyy = kxx + P;
+  const T* kzz = kyy + P;
+  const T* kxyr = kzz + P;
+  const T* kxyi = kxyr + P;
+  // index of ky in the stored half and the sign of the odd (xy) plane there
+  auto mirror = [](unsigned k) { return (HALF && k > (unsigned)(L / 2)) ? (unsigned)L - k : k; };
+  auto odd_sign = [](unsigned k) { return (HALF && k > (unsigned)(L / 2)) ? T(-1) : T(1); };
+  auto each_k = [&](auto&& fn) {
+    sfor<0, Last::NG>([&](auto J) {
+      sfor<0, Q / CH>([&](auto Ch) {
+        const int ln = rfft::opaque(lane);
+        sfor<0, CH>([&](auto Tq) {
+          constexpr int t = Ch * CH + Tq;
+          fn(J * Q + t, (unsigned)Last::k_of(ln, J, t));
+        });
+      });
+    });
+  };
+  constexpr auto c0 = std::integral_constant<int, 0>{};
+  constexpr auto c1 = std::integral_constant<int, 1>{};
+  constexpr auto c2 = std::integral_constant<int, 2>{};
+  if (zjob) {
+    C v[NB];
+    fetch(c2, v);
+    stage(c2, v);
+    F::template forward<HALF>(r[2], lds, lane, tw, Bar{});
+    each_k([&](int i, unsigned k) {
+      const T cz = kzz[mirror(k)];
+      r[2][i] = mk<T>(cz * r[2][i].x, cz * r[2][i].y);
+    });
+    F::template inverse<HALF>(r[2], lds, lane, tw, Bar{});
+    store(c2);
+    return;
+  }
+  {
+    C v0[NB], v1[NB];
+    fetch(c0, v0);
+    fetch(c1, v1);
+    stage(c0, v0);
+    stage(c1, v1);
+  }
+  sfor<0, 2>([&](auto Cc) { F::template forward<HALF>(r[Cc], lds, lane, tw, Bar{}); });
+  each_k([&](int i, unsigned k) {
+    const unsigned km = mirror(k);
+    const T a = kxx[km], b = kyy[km], xr = odd_sign(k) * kxyr[km];
+    const C mx = r[0][i], my = r[1][i];
+    C cxy, cyx;
+    if constexpr (XYI) {
+      const C xy = mk<T>(xr, kxyi[km]);
+      cxy = cmul(xy, my);
+      cyx = cmul(xy, mx);
+    } else {
+      cxy = mk<T>(xr * my.x, xr * my.y);
+      cyx = mk<T>(xr * mx.x, xr * mx.y);
+    }
+    r[0][i] = mk<T>(fmad(a, mx.x, cxy.x), fmad(a, mx.y, cxy.y));
+    r[1][i] = mk<T>(fmad(b, my.x, cyx.x), fmad(b, my.y, cyx.y));
+  });
+  sfor<0, 2>([&](auto Cc) {
+    F::template inverse<HALF>(r[Cc], lds, lane, tw, Bar{});
+    store(Cc);
+  });
+}
+
 // ---------------------------------------------------------------- plans
 // Row transforms (complex length L = Px/2, one component per transform).
 template <class T> using RowP128 = rfft::RegFft<T, 128, 16, 4, 4>;
@@ -343,8 +490,40 @@ template <class T>
 bool has_row_plan(int L, int variant) {
   return with_row<T>(L, variant, [](auto) {});
 }
+// column variants 2, 3, 4: the blocked kernel with CB = 2, 4, 8 columns per
+// workgroup over the variant-0 plan, for the column lengths of the
+// benchmarked grids, where the plan fits a workgroup (threads, LDS)
+constexpr int col_cb_of(int variant) { return variant == 2 ? 2 : variant == 3 ? 4 : variant == 4 ? 8 : 0; }
+
+template <class T, class Fn>
+bool with_col_cb(int L, int variant, Fn&& fn) {
+  const int cb = col_cb_of(variant);
+  bool done = false;
+  auto with_plan = [&](auto f) {
+    using F = decltype(f);
+    auto go = [&](auto CBc) {
+      constexpr int CBv = decltype(CBc)::value;
+      if constexpr (ColCb<T, F, CBv>::ok) {
+        fn(f, CBc);
+        done = true;
+      }
+    };
+    if (cb == 2) go(std::integral_constant<int, 2>{});
+    else if (cb == 4) go(std::integral_constant<int, 4>{});
+    else if (cb == 8) go(std::integral_constant<int, 8>{});
+  };
+  switch (L) {
+    case 512: with_plan(ColP512<T>{}); break;
+    case 1024: with_plan(ColP1024<T>{}); break;
+    case 2048: with_plan(ColP2048<T>{}); break;
+    default: break;
+  }
+  return done;
+}
+
 template <class T>
 bool has_col_plan(int L, int variant) {
+  if (col_cb_of(variant)) return with_col_cb<T>(L, variant, [](auto, auto) {});
   return with_col<T>(L, variant, [](auto) {});
 }
 
@@ -369,6 +548,23 @@ template <class T>
 int launch_col(int L, int variant, int half, C2<T>* spec, int Hx, const C2<T>* tw, const T* K, int has_xy_im,
                hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   int rc = 0;
+  if (col_cb_of(variant)) {
+    // zero-padded grids (real xy spectrum) or periodic grids (complex xy)
+    if ((half != 0) == (has_xy_im != 0)) return 0;
+    with_col_cb<T>(L, variant, [&](auto f, auto CBc) {
+      using F = decltype(f);
+      constexpr int CBv = decltype(CBc)::value;
+      using G = ColCb<T, F, CBv>;
+      const int nblk = (Hx + CBv - 1) / CBv;
+      const int grid = 2 * ((nblk + 7) / 8 * 8);
+      auto kern = half ? k_col_cb<T, F, true, false, CBv> : k_col_cb<T, F, false, true, CBv>;
+      if (prep_lds(kern, G::kLds)) { rc = -1; return; }
+      hipExtLaunchKernelGGL(kern, dim3(grid), dim3(G::kThreads), (std::uint32_t)G::kLds, s, e0, e1,
+                            0u, spec, Hx, tw, K);
+      rc = hipPeekAtLastError() == hipSuccess ? 1 : -1;
+    });
+    return rc;
+  }
   with_col<T>(L, variant, [&](auto f) {
     using F = decltype(f);
     const size_t lds = sizeof(C2<T>) * F::kSlots * F::kPerGroup;

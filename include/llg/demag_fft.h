@@ -68,9 +68,12 @@ int  Demag_SetSpecialized(DemagData* d, int on);
  * the tensor product happens in registers): rows have variant 0 for
  * Px/2 in {128, 256, 384, 512, 1024, 1536, 2048, 4096}; columns have
  * variant 0 (8 values per component and lane) for Py in {128, 256, 512,
- * 1024, 2048, 4096, 8192} and variant 1 (16 per lane) for Py >= 1024.
+ * 1024, 2048, 4096, 8192}, variant 1 (16 per lane) for Py >= 1024, and
+ * variants 2, 3, 4 (variant 0's transforms on 2, 4 or 8 adjacent columns
+ * per workgroup, the spectrum read and written in runs of that many
+ * values; the same bits as variant 0) for Py in {512, 1024, 2048}.
  * -1 selects the LDS Stockham kernels (any 2^a 3^b 5^c length).  Default:
- * variant 0 where it exists.  Returns -1 (nothing changed) when a
+ * rows variant 0, columns variant 3 where it exists, else 0.  Returns -1 (nothing changed) when a
  * requested variant does not exist for the grid.  The two families round
  * differently (both are within 1e-12 relative of the exact convolution). */
 int  Demag_SetFftKernels(DemagData* d, int row_variant, int col_variant);

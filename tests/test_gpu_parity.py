@@ -515,12 +515,12 @@ def test_register_fft_variants_match_stockham(nx, ny, padded):
     cells = (2 * nx) * (2 * ny) if padded else nx * ny
     if cells <= ORACLE_MAX_CELLS:
         h_ref = orc.Demag(nx, ny, 1.0, 4.0, padded=padded).apply(y).reshape(3, -1)
-        combos = [(r, c) for r in (-1, 0) for c in (-1, 0, 1)]
+        combos = [(r, c) for r in (-1, 0) for c in (-1, 0, 1, 2, 3, 4)]
         ref_name = "oracle"
     else:
         assert L.Demag_SetFftKernels(d, -1, -1) == 0
         h_ref = _demag_field(L, d, n3, y).reshape(3, -1)
-        combos = [(r, c) for r in (-1, 0) for c in (-1, 0, 1) if (r, c) != (-1, -1)]
+        combos = [(r, c) for r in (-1, 0) for c in (-1, 0, 1, 2, 3, 4) if (r, c) != (-1, -1)]
         ref_name = "stockham"
     ran, worst = 0, {}
     for fp32 in (0, 1):
@@ -538,6 +538,51 @@ def test_register_fft_variants_match_stockham(nx, ny, padded):
     assert ran >= 4
     for (fp32, r, c), e in worst.items():
         assert e < (2e-5 if fp32 else TOL), (fp32, r, c, e)
+
+
+CB_SHAPES = [(1536, 512, True), (1024, 1024, True), (256, 256, True), (512, 512, False),
+             (1024, 1024, False), (1024, 512, False)]
+
+
+@pytest.mark.parametrize("fp32", [0, 1])
+@pytest.mark.parametrize("nx,ny,padded", CB_SHAPES,
+                         ids=[f"{a}x{b}{'_padded' if p else ''}" for a, b, p in CB_SHAPES])
+def test_blocked_column_variants_equal_one_column_kernel_bitwise(nx, ny, padded, fp32):
+    """Column variants 2, 3, 4 (2, 4, 8 adjacent columns per workgroup, the
+    spectrum staged through LDS, the padded tensor read for ky <= Py/2 only)
+    run variant 0's transforms on the same values: the field must be the
+    same bits.  Hx = Px/2 + 1 is odd, so the last block is always partial."""
+    L = la.lib()
+    n3 = 3 * nx * ny
+    y = np.random.default_rng(nx * 7 + ny).standard_normal(n3)
+    d = (L.Demag_InitPadded if padded else L.Demag_Init)(nx, ny, 1.0, 4.0)
+    assert L.Demag_SetPrecision(d, fp32) == 0
+    assert L.Demag_SetFftKernels(d, 0, 0) == 0
+    h0 = _demag_field(L, d, n3, y)
+    ran = 0
+    for cv in (2, 3, 4):
+        if L.Demag_SetFftKernels(d, 0, cv) != 0:
+            continue
+        h = _demag_field(L, d, n3, y)
+        print(f"[gpu] {nx}x{ny} padded={padded} fp32={fp32} column variant {cv}: "
+              f"max|dh| {np.max(np.abs(h - h0)):.1e}", flush=True)
+        assert np.array_equal(h, h0), cv
+        ran += 1
+    L.Demag_Destroy(d)
+    assert ran >= 2
+
+
+def test_padded_tensor_spectrum_is_mirror_symmetric():
+    """the zero-padded grid's tensor spectra are stored with K(Py - ky) =
+    K(ky) (xx, yy, zz) and -K(ky) (xy) exactly, which the blocked column
+    kernels rely on; the field still matches the direct sum"""
+    nx, ny = 40, 24
+    y = np.random.default_rng(17).standard_normal(3 * nx * ny)
+    L = la.lib()
+    d = L.Demag_InitPadded(nx, ny, 1.0, 2.0)
+    h = _demag_field(L, d, 3 * nx * ny, y)
+    L.Demag_Destroy(d)
+    assert rel(h, orc.demag_direct(nx, ny, 1.0, 2.0, y, padded=True)) < TOL
 
 
 @pytest.mark.parametrize("fp32", [0, 1])

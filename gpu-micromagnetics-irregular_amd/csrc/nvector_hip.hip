@@ -979,6 +979,54 @@ N_Vector make_vector(sunindextype n, hipStream_t s, bool alloc, SUNContext sunct
 
 }  // namespace
 
+// ----------------------------------------------- allocation-free multi-dot
+// Device scratch of N_VDotProdMultiNoAlloc_Hip: the per-workgroup partial
+// sums of at most kNoAllocBlocks workgroups for up to kMaxK products, then
+// one result slot (kMaxK results and the sequence word).  A file-scope
+// device array: every GPU the library runs on holds its own copy until the
+// process ends.  In host code `scratch` is only a symbol; its device address
+// comes from hipGetSymbolAddress, once per device (noalloc_scratch).
+constexpr int kNoAllocBlocks = 1024;
+constexpr int kNoAllocPartials = kNoAllocBlocks * kMaxK;
+constexpr int kNoAllocDoubles = kNoAllocPartials + kSlot;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_na_mu;
+double* g_na_ptr[kMaxDevices] = {};
+std::atomic<unsigned> g_na_seq{0};
+
+// the current device's address of `scratch` (looked up on its first use)
+double* noalloc_scratch() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "[nvector_hip] hipGetSymbolAddress(scratch) on device %d: %s\n", dev,
+              hipGetErrorString(e));
+      return nullptr;
+    }
+    g_na_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_na_ptr[dev];
+}
+
+template <int K>
+void launch_dotmulti_noalloc(hipStream_t s, const double* x, const PtrPack& Y, sunindextype n,
+                             int g, double* ws, unsigned seq) {
+  double* out = ws + kNoAllocPartials;
+  hipLaunchKernelGGL(k_dotmulti<K>, dim3(g), dim3(kBlock), 0, s, x, Y, n, ws, (unsigned*)nullptr,
+                     out, (double*)nullptr, seq, (const double*)nullptr, (const double*)nullptr,
+                     (double*)nullptr, 0u, -1, 1.0, (const double*)nullptr, (const int*)nullptr);
+  hipLaunchKernelGGL(k_finish<K>, dim3(1), dim3(kFinishBlock), 0, s, (int)RED_SUM, g,
+                     (const double*)ws, out, (double*)nullptr, seq, (const int*)nullptr);
+}
+}  // namespace
+
 // =================================================================== C API
 extern "C" {
 
@@ -1201,6 +1249,63 @@ SUNErrCode N_VDotProdMulti_Hip(int nvec, N_Vector x, N_Vector* Y, sunrealtype* d
     wait_result(s, w, w->seq);
     for (int j = 0; j < k; j++) dotprods[base + j] = w->host[j];
   }
+  return SUN_SUCCESS;
+}
+
+SUNErrCode N_VDotProdMultiNoAlloc_Hip(int nvec, N_Vector x, N_Vector* Y, sunrealtype* dotprods) {
+  if (nvec < 1) return SUN_SUCCESS;
+  if (nvec > kMaxK || !x || !Y || !dotprods) {
+    fprintf(stderr, "[nvector_hip] N_VDotProdMultiNoAlloc_Hip: nvec = %d outside 1..%d or a "
+                    "NULL argument (the device scratch holds %d products)\n",
+            nvec, kMaxK, kMaxK);
+    return -1;
+  }
+  const sunindextype n = LEN(x);
+  PtrPack yp{};
+  for (int j = 0; j < nvec; j++) {
+    if (!Y[j] || LEN(Y[j]) != n) {
+      fprintf(stderr, "[nvector_hip] N_VDotProdMultiNoAlloc_Hip: vector %d missing or of "
+                      "another length\n", j);
+      return -1;
+    }
+    yp.p[j] = D(Y[j]);
+  }
+  double* ws = noalloc_scratch();
+  if (!ws) return -1;
+  // the same launch shape as N_VDotProdMulti_Hip up to kNoAllocBlocks
+  // workgroups (the library default), hence the same bits; every partial
+  // the finish reads is written by this call, so nothing is cleared
+  int g = red_grid(n);
+  if (g > kNoAllocBlocks) g = kNoAllocBlocks;
+  const unsigned seq = ++g_na_seq;
+  hipStream_t s = S(x);
+  switch (nvec) {
+    case 1: launch_dotmulti_noalloc<1>(s, D(x), yp, n, g, ws, seq); break;
+    case 2: launch_dotmulti_noalloc<2>(s, D(x), yp, n, g, ws, seq); break;
+    case 3: launch_dotmulti_noalloc<3>(s, D(x), yp, n, g, ws, seq); break;
+    case 4: launch_dotmulti_noalloc<4>(s, D(x), yp, n, g, ws, seq); break;
+    case 5: launch_dotmulti_noalloc<5>(s, D(x), yp, n, g, ws, seq); break;
+    case 6: launch_dotmulti_noalloc<6>(s, D(x), yp, n, g, ws, seq); break;
+    case 7: launch_dotmulti_noalloc<7>(s, D(x), yp, n, g, ws, seq); break;
+    case 8: launch_dotmulti_noalloc<8>(s, D(x), yp, n, g, ws, seq); break;
+    case 9: launch_dotmulti_noalloc<9>(s, D(x), yp, n, g, ws, seq); break;
+    case 10: launch_dotmulti_noalloc<10>(s, D(x), yp, n, g, ws, seq); break;
+    case 11: launch_dotmulti_noalloc<11>(s, D(x), yp, n, g, ws, seq); break;
+    case 12: launch_dotmulti_noalloc<12>(s, D(x), yp, n, g, ws, seq); break;
+    case 13: launch_dotmulti_noalloc<13>(s, D(x), yp, n, g, ws, seq); break;
+    case 14: launch_dotmulti_noalloc<14>(s, D(x), yp, n, g, ws, seq); break;
+    case 15: launch_dotmulti_noalloc<15>(s, D(x), yp, n, g, ws, seq); break;
+    default: launch_dotmulti_noalloc<16>(s, D(x), yp, n, g, ws, seq); break;
+  }
+  if (hipPeekAtLastError() != hipSuccess) {
+    check_launch("dotmulti (no-alloc)");
+    return -1;
+  }
+  g_syncs++;
+  if (hipMemcpyAsync(dotprods, ws + kNoAllocPartials, sizeof(double) * nvec,
+                     hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return -1;
   return SUN_SUCCESS;
 }
 

@@ -151,6 +151,7 @@ SIGNATURES = {
     "N_VLinearCombination_Hip": (C.c_int, [C.c_int, DP, C.POINTER(VP), VP]),
     "N_VScaleAddMulti_Hip": (C.c_int, [C.c_int, DP, VP, C.POINTER(VP), C.POINTER(VP)]),
     "N_VDotProdMulti_Hip": (C.c_int, [C.c_int, VP, C.POINTER(VP), DP]),
+    "N_VDotProdMultiNoAlloc_Hip": (C.c_int, [C.c_int, VP, C.POINTER(VP), DP]),
     "N_VLinearSumVectorArray_Hip": (C.c_int, [C.c_int, C.c_double, C.POINTER(VP), C.c_double,
                                               C.POINTER(VP), C.POINTER(VP)]),
     "N_VScaleVectorArray_Hip": (C.c_int, [C.c_int, DP, C.POINTER(VP), C.POINTER(VP)]),

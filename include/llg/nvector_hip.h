@@ -97,6 +97,20 @@ sunrealtype  N_VWSqrSumMaskLocal_Hip(N_Vector x, N_Vector w, N_Vector id);
 SUNErrCode N_VLinearCombination_Hip(int nvec, sunrealtype* c, N_Vector* X, N_Vector z);
 SUNErrCode N_VScaleAddMulti_Hip(int nvec, sunrealtype* a, N_Vector x, N_Vector* Y, N_Vector* Z);
 SUNErrCode N_VDotProdMulti_Hip(int nvec, N_Vector x, N_Vector* Y, sunrealtype* dotprods);
+/* N_VDotProdMulti_Hip without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device or pinned memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The partial sums live in a file-scope device array of
+ * 1024 x 16 + 17 doubles (131 KB) that each GPU holds until the process
+ * ends; the results come back by a device-to-host copy into `dotprods` and
+ * one stream synchronisation.  nvec must be 1..16 (NVHIP_FUSED_MAX; -1 and
+ * nothing launched otherwise, or on vectors of unequal length).  Every call
+ * on one GPU shares that array, so calls that overlap on one GPU (two host
+ * threads, two streams) race.  With the library's default reduction shape
+ * (at most 1024 partial workgroups) the results are bitwise those of
+ * N_VDotProdMulti_Hip for nvec >= 2 (nvec = 1 takes the multi-dot kernel,
+ * where N_VDotProdMulti_Hip takes the single-dot one). */
+SUNErrCode N_VDotProdMultiNoAlloc_Hip(int nvec, N_Vector x, N_Vector* Y, sunrealtype* dotprods);
 
 /* vector-array ops */
 SUNErrCode N_VLinearSumVectorArray_Hip(int nvec, sunrealtype a, N_Vector* X, sunrealtype b,

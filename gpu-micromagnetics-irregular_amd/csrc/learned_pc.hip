@@ -307,6 +307,226 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
+// ---------------------------------------------------- fused inference pass
+// features -> C1 -> C2 -> C3 -> z += active * out in ONE kernel: a
+// workgroup owns a 16 x 16 output tile; the features of the tile plus a
+// 3-cell halo, the 32-channel layer-1 activations (2-cell halo) and the
+// layer-2 activations (1-cell halo) stay in LDS (BF16), so only u, y, z and
+// the active flags cross HBM.  The products run on gfx950's
+// v_mfma_f32_16x16x32_bf16: 16 cells x 32 reduction values x 16 output
+// channels per instruction; the reduction index is tap * C_in + channel
+// (layer 1: 9 taps x 16 channels padded to 5 x 32, layers 2 and 3: one tap
+// of 32 channels per instruction).  Operand layout (lane l, q = l / 16):
+// A[l % 16][8q + j], B[8q + j][l % 16], D[4q + j][l % 16], j = 0..7 / 0..3;
+// the weights are the A operand and the activations the B operand, so D
+// is [output channel][cell].
+// The weights' B fragments (FP32 master weights rounded to BF16) are built
+// by k_make_frag32 and held in registers, one 16-channel output block per
+// wave.  Activations outside the grid are zero (zero padding) or wrapped
+// (periodic), as in the layer-by-layer kernels; the FP32 accumulation order
+// differs from theirs (K = 32 per instruction), so results agree to BF16
+// rounding of the hidden layers, not bit for bit.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+constexpr int kT = 16;                        // output tile edge
+constexpr int kWF = kT + 6, kW1 = kT + 4, kW2 = kT + 2;
+constexpr int kNF = kWF * kWF, kN1 = kW1 * kW1, kN2 = kW2 * kW2, kNO = kT * kT;
+// LDS layout of an activation box: [chunk][cell][8 channels] (BF16), one
+// plane per 8 channels.  The 16 lanes of an operand read take 16
+// consecutive cells of one plane: 256 contiguous bytes.  The plane stride
+// (in 16-bit units) is padded to 64 bytes modulo 256, so lanes of the four
+// planes that the hardware may serve in one pass start on distinct banks.
+constexpr int plane_stride(int ncell) { return ncell * 8 + ((32 - (ncell * 8) % 128) % 128 + 128) % 128; }
+constexpr int kPF = plane_stride(kNF), kP1 = plane_stride(kN1), kP2 = plane_stride(kN2);
+constexpr int kFusedLds = 2 * (4 * kP1 + (2 * kPF > 4 * kP2 ? 2 * kPF : 4 * kP2));
+// fragment steps per layer and 16-channel output blocks
+constexpr int kS1steps = 5, kS2steps = 9, kS3steps = 9;
+static_assert((kPF * 2) % 256 == 64 && (kP1 * 2) % 256 == 64 && (kP2 * 2) % 256 == 64,
+              "plane strides 64 B modulo 256 B");
+
+__device__ inline bf16x8 lds8(const uint16_t* p) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(p));
+}
+
+// B fragments for the 16x16x32 instruction from W[9][cin][cout] (FP32):
+// layer with cin == 16: step s covers taps 2s, 2s+1; cin == 32: step = tap
+__global__ void k_make_frag32(const float* __restrict__ W, int cin, int cout,
+                              u16x8* __restrict__ frag) {
+  const int NB = cout / 16, steps = cin == 16 ? 5 : 9;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= steps * NB * 64) return;
+  const int l = idx & 63, blk = idx >> 6;
+  const int nb = blk % NB, st = blk / NB;
+  const int q = l >> 4, co = nb * 16 + (l & 15);
+  u16x8 v;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    int tap, ci;
+    if (cin == 16) { tap = 2 * st + (q >> 1); ci = 8 * (q & 1) + j; }
+    else { tap = st; ci = 8 * q + j; }
+    v[j] = tap < 9 ? f2bf(W[((long)tap * cin + ci) * cout + co]) : (uint16_t)0;
+  }
+  frag[idx] = v;
+}
+
+// the grid cell at tile-box coordinates, -1 outside a non-periodic grid
+__device__ inline int box_cell(int gx, int gy, int ng, int ny, int periodic) {
+  if (periodic) {
+    gx = ((gx % ng) + ng) % ng;
+    gy = ((gy % ny) + ny) % ny;
+  } else if (gx < 0 || gx >= ng || gy < 0 || gy >= ny) {
+    return -1;
+  }
+  return gy * ng + gx;
+}
+
+// One layer on the tile boxes: output box WO x WO (halo HO around the tile)
+// from the input box (WO + 2) x (WO + 2), input plane stride PIN, output
+// plane stride POUT.  The product is formed transposed, D[channel][cell] =
+// W^T[channel][k] x A^T[k][cell], so a lane ends with 4 consecutive
+// channels of ONE cell (D[4q + j][l % 16]) and stores them as one 8-byte
+// word; the weight fragments in A position have the lane layout they have
+// in B position (lane l: W[k = 8q + j][channel l % 16]).
+template <int CIN, int STEPS, int NB, int WO, int HO, int PIN, int POUT>
+__device__ inline void conv_layer(const uint16_t* __restrict__ in,
+                                  const u16x8* __restrict__ frag, int x0, int y0, int ng, int ny,
+                                  int periodic, uint16_t* __restrict__ out) {
+  constexpr int WIN = WO + 2, NO = WO * WO;
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+  // every wave holds the weights of all NB output blocks, so one operand
+  // read from LDS feeds NB instructions
+  bf16x8 w[NB][STEPS];
+#pragma unroll
+  for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+    for (int st = 0; st < STEPS; st++)
+      w[nb][st] = __builtin_bit_cast(bf16x8, frag[(st * NB + nb) * 64 + l]);
+  // this lane's operand plane and, for 16 input channels, its tap parity
+  const uint16_t* inq = in + (CIN == 16 ? (q & 1) : q) * PIN;
+  const int todd = CIN == 16 ? (q >> 1) : 0;
+  constexpr int ngrp = (NO + 15) / 16;
+  for (int g = wave; g < ngrp; g += 4) {
+    const int c = g * 16 + r;                  // this lane's cell (B column)
+    const int cc = c < NO ? c : NO - 1;        // computed, not stored
+    const int bx = cc % WO, by = cc / WO;
+    const uint16_t* base = inq + (by * WIN + bx) * 8;
+    f32x4 acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int st = 0; st < STEPS; st++) {
+      // tap of this step (layer 1: 2 st + parity, the tenth tap's weights
+      // are zero and it reads tap 8 again)
+      const int t0 = CIN == 16 ? 2 * st : st;
+      const int off0 = ((t0 / 3) * WIN + t0 % 3) * 8;
+      const int t1 = t0 + 1 > 8 ? 8 : t0 + 1;
+      const int off1 = ((t1 / 3) * WIN + t1 % 3) * 8;
+      const bf16x8 b = lds8(base + (todd ? off1 : off0));
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++)
+        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[nb][st], b, acc[nb], 0, 0, 0);
+    }
+    if (c < NO) {
+      const int gx = x0 - HO + bx, gy = y0 - HO + by;
+      const bool inside = periodic || (gx >= 0 && gx < ng && gy >= 0 && gy < ny);
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++) {
+        bf16x4v v;
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = (__bf16)(inside ? acc[nb][j] : 0.f);
+        // channels nb*16 + 4q .. +3: plane 2 nb + q / 2, offset 4 (q % 2)
+        *reinterpret_cast<bf16x4v*>(out + (2 * nb + (q >> 1)) * POUT + c * 8 + 4 * (q & 1)) = v;
+      }
+    }
+  }
+}
+
+// layer 3 and z = u + active * o (channels 0..2 of 16) on every cell of the
+// tile: the lanes with q = 0 hold channels 0..3 of cell l % 16
+__device__ inline void conv_out(const uint16_t* __restrict__ in, const u16x8* __restrict__ frag,
+                                int x0, int y0, int ng, int ny, long n,
+                                const unsigned char* __restrict__ active,
+                                const double* __restrict__ u, double* __restrict__ z) {
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+  bf16x8 w[kS3steps];
+#pragma unroll
+  for (int st = 0; st < kS3steps; st++) w[st] = __builtin_bit_cast(bf16x8, frag[st * 64 + l]);
+  const uint16_t* inq = in + q * kP2;
+  constexpr int ngrp = kNO / 16;
+  for (int g = wave; g < ngrp; g += 4) {
+    const int c = g * 16 + r;
+    const int bx = c % kT, by = c / kT;
+    const uint16_t* base = inq + (by * kW2 + bx) * 8;
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int st = 0; st < kS3steps; st++)
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+          w[st], lds8(base + ((st / 3) * kW2 + st % 3) * 8), acc, 0, 0, 0);
+    const int gx = x0 + bx, gy = y0 + by;
+    if (q == 0 && gx < ng && gy < ny) {
+      const long cell = (long)gy * ng + gx;
+      const bool on = !active || active[cell];
+#pragma unroll
+      for (int k = 0; k < kNout; k++) {
+        const double uc = u[k * n + cell];
+        z[k * n + cell] = on ? uc + (double)acc[k] : uc;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kWg) void k_lpc_fused(const double* __restrict__ y,
+                                                   const unsigned char* __restrict__ active,
+                                                   double sg, int ng, int ny, int periodic,
+                                                   const u16x8* __restrict__ f1,
+                                                   const u16x8* __restrict__ f2,
+                                                   const u16x8* __restrict__ f3,
+                                                   const double* __restrict__ u,
+                                                   double* __restrict__ z) {
+  extern __shared__ uint16_t lds[];
+  uint16_t* H1 = lds;                          // 4 planes of kP1
+  uint16_t* FA = lds + 4 * kP1;                // features (2 planes of kPF), later layer 2
+  const long n = (long)ng * ny;
+  const int tiles_x = ng / kT;
+  const int x0 = (blockIdx.x % tiles_x) * kT, y0 = (blockIdx.x / tiles_x) * kT;
+  // features of the tile plus a 3-cell halo: gamma * [u, m x u, (m.u) m],
+  // zero off the mask and outside a non-periodic grid
+  for (int i = threadIdx.x; i < kNF; i += kWg) {
+    const int bx = i % kWF, by = i / kWF;
+    const int c = box_cell(x0 - 3 + bx, y0 - 3 + by, ng, ny, periodic);
+    float v[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = 0.f;
+    if (c >= 0 && (!active || active[c])) {
+      // FP32 is ample ahead of the BF16 rounding of the features
+      const float s = (float)sg;
+      const float u1 = s * (float)u[c], u2 = s * (float)u[n + c], u3 = s * (float)u[2 * n + c];
+      const float m1 = (float)y[c], m2 = (float)y[n + c], m3 = (float)y[2 * n + c];
+      const float mu = m1 * u1 + m2 * u2 + m3 * u3;
+      v[0] = u1; v[1] = u2; v[2] = u3;
+      v[3] = m2 * u3 - m3 * u2; v[4] = m3 * u1 - m1 * u3; v[5] = m1 * u2 - m2 * u1;
+      v[6] = mu * m1; v[7] = mu * m2; v[8] = mu * m3;
+    }
+    bf16x8 lo, hi;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      lo[k] = (__bf16)v[k];
+      hi[k] = (__bf16)v[8 + k];
+    }
+    *reinterpret_cast<bf16x8*>(FA + i * 8) = lo;
+    *reinterpret_cast<bf16x8*>(FA + kPF + i * 8) = hi;
+  }
+  __syncthreads();
+  conv_layer<16, kS1steps, 2, kW1, 2, kPF, kP1>(FA, f1, x0, y0, ng, ny, periodic, H1);
+  __syncthreads();
+  // every wave has read the features (one barrier), then H1 -> layer 2 in FA
+  conv_layer<32, kS2steps, 2, kW2, 1, kP1, kP2>(H1, f2, x0, y0, ng, ny, periodic, FA);
+  __syncthreads();
+  // each tile writes z on its own cells only; the halos read u
+  conv_out(FA, f3, x0, y0, ng, ny, n, active, u, z);
+}
+
 }  // namespace
 
 struct llg_lpc {
@@ -323,6 +543,9 @@ struct llg_lpc {
   uint16_t *f, *h1, *h2, *g3, *g2, *g1;
   float* o;
   int enabled;
+  u16x8* frag32[3];   // B fragments of the 16x16x32 instruction (fused pass)
+  double* ubuf;       // 3n: the block-Jacobi output the correction reads
+  int fused;          // 1: the fused inference kernel; 0: layer by layer
 };
 
 namespace {
@@ -362,10 +585,18 @@ bool alloc_all(llg_lpc* p) {
   A((void**)&p->g2, sizeof(uint16_t) * n * kC2);
   A((void**)&p->g1, sizeof(uint16_t) * n * kC1);
   A((void**)&p->o, sizeof(float) * n * 4);
+  A((void**)&p->ubuf, sizeof(double) * 3 * n);
+  for (int k = 0; k < 3; k++)
+    A((void**)&p->frag32[k], sizeof(u16x8) * (k == 0 ? 5 : 9) * (kCout[k] / 16) * 64);
   return ok;
 }
 
 void make_frags(llg_lpc* p) {
+  for (int k = 0; k < 3; k++) {
+    const int nf = (k == 0 ? 5 : 9) * (kCout[k] / 16) * 64;
+    hipLaunchKernelGGL(k_make_frag32, dim3(cdiv(nf, 256)), dim3(256), 0, p->s, p->W + p->off[k],
+                       kCin[k], kCout[k], p->frag32[k]);
+  }
   for (int k = 0; k < 3; k++) {
     const int tot = p->nw[k] / 4;
     hipLaunchKernelGGL(k_make_frag, dim3(cdiv(tot, 256)), dim3(256), 0, p->s, p->W + p->off[k],
@@ -401,6 +632,21 @@ void forward(llg_lpc* p, const double* u, const double* y, double sg) {
   conv<kC2, kC3, true>(p, p->h2, p->frag[2], p->o, kNout, 4);
 }
 
+// z = u + active * N(u) at state y (u and z distinct)
+void apply_correction(llg_lpc* p, const double* y, double gamma, const double* u, double* z) {
+  if (p->fused) {
+    const int tiles = (p->ng / kT) * cdiv(p->ny, kT);
+    hipLaunchKernelGGL(k_lpc_fused, dim3(tiles), dim3(kWg), kFusedLds, p->s, y, p->d_active,
+                       gamma, p->ng, p->ny, p->periodic, (const u16x8*)p->frag32[0],
+                       (const u16x8*)p->frag32[1], (const u16x8*)p->frag32[2], u, z);
+    return;
+  }
+  forward(p, u, y, gamma);
+  hipMemcpyAsync(z, u, sizeof(double) * 3 * p->n, hipMemcpyDeviceToDevice, p->s);
+  hipLaunchKernelGGL(k_apply_out, dim3(cdiv(p->n, 256)), dim3(256), 0, p->s, p->o, p->d_active,
+                     p->n, z);
+}
+
 bool ok_launch(const char* what) {
   const hipError_t e = hipPeekAtLastError();
   if (e != hipSuccess) {
@@ -433,6 +679,7 @@ llg_lpc* LLG_LpcCreate(llg_ctx* ctx, int periodic) {
   p->periodic = periodic ? 1 : 0;
   p->s = (hipStream_t)LLG_GetStream(ctx);
   p->d_active = LLG_GetActiveDevice(ctx);
+  p->fused = 1;
   if (!alloc_all(p)) {
     g_lerr = "learned preconditioner: device allocation failed";
     LLG_LpcDestroy(p);
@@ -448,7 +695,8 @@ void LLG_LpcDestroy(llg_lpc* p) {
   if (!p) return;
   void* bufs[] = {p->W, p->G, p->M, p->V, p->part, p->d_lpart, p->frag[0], p->frag[1],
                   p->frag[2], p->fragT[0], p->fragT[1], p->fragT[2], p->f, p->h1, p->h2,
-                  p->g3, p->g2, p->g1, p->o};
+                  p->g3, p->g2, p->g1, p->o, p->ubuf, p->frag32[0], p->frag32[1],
+                  p->frag32[2]};
   for (void* b : bufs)
     if (b) hipFree(b);
   delete p;
@@ -537,10 +785,24 @@ int LLG_LpcTrainStep(llg_lpc* p, const double* u_dev, const double* y_dev,
 
 int LLG_LpcCorrect(llg_lpc* p, const double* y_dev, double gamma, double* z_dev) {
   if (!p || !y_dev || !z_dev) return -1;
-  forward(p, z_dev, y_dev, gamma);
-  hipLaunchKernelGGL(k_apply_out, dim3(cdiv(p->n, 256)), dim3(256), 0, p->s, p->o, p->d_active,
-                     p->n, z_dev);
+  hipMemcpyAsync(p->ubuf, z_dev, sizeof(double) * 3 * p->n, hipMemcpyDeviceToDevice, p->s);
+  apply_correction(p, y_dev, gamma, p->ubuf, z_dev);
   return ok_launch("LLG_LpcCorrect") ? 0 : -1;
+}
+
+int LLG_LpcApply(llg_lpc* p, const double* y_dev, double gamma, const double* u_dev,
+                 double* z_dev) {
+  if (!p || !y_dev || !u_dev || !z_dev || u_dev == z_dev) return -1;
+  apply_correction(p, y_dev, gamma, u_dev, z_dev);
+  return ok_launch("LLG_LpcApply") ? 0 : -1;
+}
+
+double* LLG_LpcInputBuffer(llg_lpc* p) { return p ? p->ubuf : nullptr; }
+
+int LLG_LpcSetFused(llg_lpc* p, int on) {
+  if (!p) return -1;
+  p->fused = on ? 1 : 0;
+  return 0;
 }
 
 }  // extern "C"

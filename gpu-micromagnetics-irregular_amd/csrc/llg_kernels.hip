@@ -1276,13 +1276,16 @@ int PrecondSolve(sunrealtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z,
   c->npsol++;
   hipStream_t s = stream_of(z, c);
   const int n = (c->cfg.exec == LLG_EXEC_COMPACT) ? c->n_active + c->n_hole : c->ncell;
+  // with a learned correction the block-Jacobi output goes to the
+  // network's input buffer and z = u + active N(u) is formed from it
+  double* u = c->lpc ? LLG_LpcInputBuffer(c->lpc) : N_VGetDeviceArrayPointer(z);
   kprof::begin(kprof::APPLY_P, s);
   hipLaunchKernelGGL(k_apply_P, dim3((n + k1D - 1) / k1D), dim3(k1D), 0, s, geo_of(c),
                      c->cfg.pc.gate, (const double*)c->pd->d_P,
-                     (const double*)N_VGetDeviceArrayPointer(r), N_VGetDeviceArrayPointer(z));
+                     (const double*)N_VGetDeviceArrayPointer(r), u);
   kprof::end(kprof::APPLY_P, s);
   if (c->lpc &&
-      LLG_LpcCorrect(c->lpc, N_VGetDeviceArrayPointer(y), gamma, N_VGetDeviceArrayPointer(z)) != 0)
+      LLG_LpcApply(c->lpc, N_VGetDeviceArrayPointer(y), gamma, u, N_VGetDeviceArrayPointer(z)) != 0)
     return -1;
   return launch_status("PrecondSolve");
 }

@@ -672,6 +672,18 @@ class LearnedPrecond:
                 print(f"[lpc] step {it} loss {loss.value:.4e}", flush=True)
         return hist
 
+    def set_fused(self, on=True):
+        """inference form: the one-kernel MFMA pass (default) or the
+        layer-by-layer kernels"""
+        lib().LLG_LpcSetFused(self.h, 1 if on else 0)
+
+    def apply(self, y, gamma, u):
+        """z = u + active * N(u) at state y (LLG_LpcApply), host arrays"""
+        vy, vu, vz = self.sim.vec(y), self.sim.vec(u), self.sim.vec()
+        if lib().LLG_LpcApply(self.h, self.dptr(vy), gamma, self.dptr(vu), self.dptr(vz)) != 0:
+            raise RuntimeError("LLG_LpcApply failed: " + lib().LLG_LpcLastError().decode())
+        return vz.get()
+
     def enable(self, on=True):
         lib().LLG_SetLearnedPrecond(self.sim.ctx, self.h if on else None)
 

@@ -281,6 +281,9 @@ SIGNATURES = {
     "LLG_LpcAdamStep": (C.c_int, [VP, C.c_float]),
     "LLG_LpcTrainStep": (C.c_int, [VP, DP, DP, DP, C.c_double, C.c_double, C.c_float, DP]),
     "LLG_LpcCorrect": (C.c_int, [VP, DP, C.c_double, DP]),
+    "LLG_LpcApply": (C.c_int, [VP, DP, C.c_double, DP, DP]),
+    "LLG_LpcInputBuffer": (DP, [VP]),
+    "LLG_LpcSetFused": (C.c_int, [VP, C.c_int]),
     "LLG_SetLearnedPrecond": (C.c_int, [VP, VP]),
     "LLG_GetGrid": (C.c_int, [VP, IP, IP]),
     "LLG_SetAppliedField": (C.c_int, [VP, C.c_int, DP]),

@@ -48,6 +48,17 @@ int      LLG_LpcTrainStep(llg_lpc* p, const double* u_dev, const double* y_dev,
                           double* loss);
 /* z (holding u = P_bj^-1 r) += active * N(u) at state y */
 int      LLG_LpcCorrect(llg_lpc* p, const double* y_dev, double gamma, double* z_dev);
+/* z = u + active * N(u) at state y, u and z distinct (3n doubles each) */
+int      LLG_LpcApply(llg_lpc* p, const double* y_dev, double gamma, const double* u_dev,
+                      double* z_dev);
+/* a 3n-double device buffer owned by the network, where PrecondSolve puts
+ * the block-Jacobi output before LLG_LpcApply */
+double*  LLG_LpcInputBuffer(llg_lpc* p);
+/* inference form: 1 (default) one fused kernel per application -- features,
+ * the three convolutions on v_mfma_f32_16x16x32_bf16 with the hidden layers
+ * in LDS, and the update; 0 the layer-by-layer kernels the training uses
+ * (hidden layers through HBM) */
+int      LLG_LpcSetFused(llg_lpc* p, int on);
 
 /* PrecondSolve applies the correction after block-Jacobi while set (NULL:
  * plain block-Jacobi, the fallback) */

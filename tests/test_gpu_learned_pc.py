@@ -81,6 +81,72 @@ def test_zero_weights_is_block_jacobi():
     lpc.close()
 
 
+FUSED_CASES = [("square_hole", 64, 48, False), ("square_hole", 96, 40, False),
+               ("i1", 64, 64, True), ("polycrystal_compact", 128, 96, True)]
+
+
+@pytest.mark.parametrize("name,ng,ny,periodic", FUSED_CASES,
+                         ids=[f"{c[0]}_{c[1]}x{c[2]}" for c in FUSED_CASES])
+def test_fused_inference_matches_oracle_and_layer_kernels(name, ng, ny, periodic):
+    """The one-kernel inference pass (16x16 tiles, hidden layers in LDS,
+    v_mfma_f32_16x16x32_bf16) against the numpy oracle and against the
+    layer-by-layer kernels: z - u within the BF16 tolerance of max|o|
+    (grids whose ny is not a multiple of the tile, zero and periodic
+    padding, the compact polycrystal whose holes must come out as u)."""
+    if name == "polycrystal_compact":
+        geom = la.Geometry.polycrystal(ng, ny, num_grains=10, dead_frac=0.3, seed=3)
+        rhs = la.terms(che=4.0, chk=1.0, chb=0.6, aniso=la.ANISO_LINEAR, cell_axes=1, window=1,
+                       gate=la.GATE_MASK)
+        fixed = la.terms(che=4.0, chk=4.0, chb=0.3, msk=(1, 0, 0), nsk=(1, 0, 0),
+                         aniso=la.ANISO_LINEAR, gate=la.GATE_MASK)
+        cfg = la.make_config(ng, ny, rhs, fixed, fixed, exec_=la.EXEC_COMPACT,
+                             normalize=la.NORM_REGULARIZED, demag=la.DEMAG_PERIODIC,
+                             demag_strength=1.0)
+        y0 = geom.ic_random_sphere(seed=5)
+    elif name == "i1":
+        geom = la.Geometry.full(ng, ny)
+        cfg = la.preset("i1", ng=ng, ny=ny, demag=2.0)
+        y0 = geom.ic_stripes()
+    else:
+        geom = la.Geometry.square_hole(ng, ny, halfside_frac=0.2)
+        cfg = la.preset("square_hole", ng=ng, ny=ny)
+        y0 = geom.ic_split_x()
+    sim = la.Simulation(cfg, geom)
+    lpc = la.LearnedPrecond(sim, periodic=periodic)
+    lpc.init_random(seed=11, std=0.08)
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(sim.neq) * np.tile(geom.active.astype(float), 3)
+    gamma = 0.031
+    z = lpc.apply(y0, gamma, u)
+    lpc.set_fused(False)
+    z_layers = lpc.apply(y0, gamma, u)
+    lpc.set_fused(True)
+    w = lpc.weights().astype(np.float64)
+    st = lo.forward(w, u, y0, geom.active, gamma, 1.0, ng, ny, periodic)
+    o_ref = np.moveaxis(st["o"][..., :3], -1, 0).reshape(3, -1) * geom.active.astype(float)
+    d = (z - u).reshape(3, -1)
+    d_layers = (z_layers - u).reshape(3, -1)
+    scale = np.max(np.abs(o_ref))
+    e_ref = np.max(np.abs(d - o_ref)) / scale
+    e_layers = np.max(np.abs(d - d_layers)) / scale
+    print(f"[gpu] fused lpc {name} {ng}x{ny}: rel err vs oracle {e_ref:.2e}, vs layer kernels "
+          f"{e_layers:.2e}", flush=True)
+    assert scale > 0
+    assert e_ref < 2e-2 and e_layers < 2e-2
+    inactive = geom.active == 0
+    assert np.array_equal(z.reshape(3, -1)[:, inactive], u.reshape(3, -1)[:, inactive])
+    lpc.close()
+    sim.close()
+
+
+def test_fused_zero_weights_is_block_jacobi():
+    sim, geom, y0 = _sim(96, 40)
+    lpc = la.LearnedPrecond(sim)
+    u = np.random.default_rng(2).standard_normal(sim.neq)
+    assert np.array_equal(lpc.apply(y0, 0.05, u), u)
+    lpc.close()
+
+
 def _heldout_errors(lpc, states, rng, n=6, gamma_range=(2e-3, 5e-2)):
     """relative error of block-Jacobi alone and of the corrected
     preconditioner against the exact solution of fresh pairs"""

@@ -132,7 +132,7 @@ def cpu_baseline(la, cfg, geom, y0, budget_s, threads):
     one, t1, wall1, st1 = run(1, budget_s / 2)
     sim.close()
     return {"value": t_end / wall, "unit": "sim-time/wall-s", "cores": nthreads,
-            "kind": "port",
+            "kind": "port", "t_end": t_end,
             "sample": (f"same workload from t=0 to t={t_end:.3f} ({st.nst} BDF steps, "
                        f"{st.nfe} f evals) in {wall:.1f} s on {nthreads} OpenMP threads; "
                        f"oracle setup (tensor + plans) {init_s:.1f} s excluded"),
@@ -199,6 +199,148 @@ def ensemble_bench(la, args):
     print(json.dumps(rec), flush=True)
 
 
+def training_states(geom, make_ic, integrator, seeds=(2,), times=(1.0, 2.0, 3.0, 4.0, 5.0)):
+    """states of trajectories from other seeds than the timed one (the
+    learned correction never sees the benchmarked trajectory)"""
+    states = []
+    for sd in seeds:
+        y = make_ic(sd)
+        states.append(y)
+        tr = integrator(y)
+        for t in times:
+            tr.advance(t)
+            states.append(tr.state())
+        tr.close()
+    return states
+
+
+def train_lpc(la, sim, states, steps):
+    """train the learned correction on exact Newton-system pairs of the
+    FP64 operator at the given states; returns (weights, info)"""
+    t0 = time.perf_counter()
+    lpc = la.LearnedPrecond(sim)
+    lpc.init_random(seed=0, std=1e-3)
+    hist = lpc.train(states, steps=steps // 2, lr=2e-3, gamma_range=(1e-3, 2e-1), seed=1)
+    hist += lpc.train(states, steps=steps - steps // 2, lr=5e-4, gamma_range=(1e-3, 2e-1), seed=2)
+    w = lpc.weights()
+    rng = np.random.default_rng(123)
+    e_bj, e_l = [], []
+    for i in range(4):            # held-out pairs: relative error of the preconditioned solution
+        y = states[(2 * i + 1) % len(states)]
+        g = float(np.exp(rng.uniform(np.log(1e-3), np.log(2e-1))))
+        sc = lpc.make_pair(y, g, rng)
+        o = lpc.forward(g, sc)[:, :3].astype(np.float64)
+        u = lpc._u.get().reshape(3, -1).T * sc
+        x = lpc._x.get().reshape(3, -1).T * sc
+        e_bj.append(np.linalg.norm(u - x) / np.linalg.norm(x))
+        e_l.append(np.linalg.norm(u + o - x) / np.linalg.norm(x))
+    lpc.close()
+    info = {"train_steps": steps, "train_s": round(time.perf_counter() - t0, 1),
+            "train_states": len(states), "loss_first": float(np.mean(hist[:10])),
+            "loss_last": float(np.mean(hist[-10:])),
+            "heldout_rel_err_block_jacobi": round(float(np.mean(e_bj)), 4),
+            "heldout_rel_err_learned": round(float(np.mean(e_l)), 4)}
+    return w, info
+
+
+def lpc_apply_us(la, sim, y0, reps=30):
+    """one learned-correction application (fused kernel), HIP events"""
+    L = la.lib()
+    lpc = la.LearnedPrecond(sim)
+    lpc.init_random(seed=1, std=1e-3)
+    vy, vu, vz = sim.vec(y0), sim.vec(np.random.default_rng(0).standard_normal(sim.neq)), sim.vec()
+    e0, e1 = L.LLG_EventCreate(), L.LLG_EventCreate()
+    for _ in range(3):
+        L.LLG_LpcApply(lpc.h, lpc.dptr(vy), 0.03, lpc.dptr(vu), lpc.dptr(vz))
+    L.LLG_EventRecord(e0, sim.stream)
+    for _ in range(reps):
+        L.LLG_LpcApply(lpc.h, lpc.dptr(vy), 0.03, lpc.dptr(vu), lpc.dptr(vz))
+    L.LLG_EventRecord(e1, sim.stream)
+    L.LLG_DeviceSynchronize()
+    us = 1e3 * L.LLG_EventElapsedMs(e0, e1) / reps
+    L.LLG_EventDestroy(e0)
+    L.LLG_EventDestroy(e1)
+    lpc.close()
+    return round(us, 2)
+
+
+def north_star_ab(la, cfg, geom, y0, y_plain, out_plain, value_plain, t_end, ropts, args,
+                  make_ic, kernels):
+    """The forms beside the plain FP64 block-Jacobi run, on the same workload
+    and window: FP32 spectral demag, the learned correction, and both; each
+    with its wall rate and ratio to the plain run, Krylov counts, and the
+    max |dm| of its final state -- and of the plain run's -- against an FP64
+    reference of the same trajectory at rtol = atol = 1e-6 (the accuracy the
+    plain method reaches at rtol = 1e-4 is the bar)."""
+    L = la.lib()
+
+    def make_sim(fp32):
+        c = type(cfg).from_buffer_copy(cfg)
+        c.demag_fp32 = fp32
+        sim = la.Simulation(c, geom)
+        dm = L.LLG_GetDemag(sim.ctx)
+        if dm and kernels:
+            L.Demag_SetFftKernels(dm, *kernels)
+        return sim
+
+    sim64 = make_sim(0)
+    t0 = time.perf_counter()
+    states = training_states(geom, make_ic, lambda y: sim64.integrator(y, **ropts))
+    w, info = train_lpc(la, sim64, states, args.lpc_steps)
+    info["state_s"] = round(time.perf_counter() - t0 - info["train_s"], 1)
+    print(f"[bench] learned correction trained: {info}", file=sys.stderr, flush=True)
+    ref = sim64.integrator(y0, **dict(ropts, rtol=1e-6, atol=1e-6))
+    ref.advance(t_end)
+    y_ref = ref.state()
+    ref.close()
+    act = np.tile(geom.active.astype(bool), 3)
+    dm_plain = float(np.max(np.abs(y_plain - y_ref)[act]))
+    forms = {}
+    for name, fp32, learned in (("fp32_demag", 1, False), ("learned_pc", 0, True),
+                                ("learned_pc_fp32_demag", 1, True)):
+        sim = sim64 if fp32 == 0 else make_sim(1)
+        lpc = None
+        if learned:
+            lpc = la.LearnedPrecond(sim)
+            lpc.set_weights(w)
+            lpc.enable(True)
+        if args.warmup > 0:
+            wr = sim.integrator(y0, **ropts)
+            wr.advance(1.0)
+            wr.close()
+        run = sim.integrator(y0, **ropts)
+        o = run.advance(t_end)
+        y = run.state()
+        run.close()
+        if lpc is not None:
+            lpc.close()
+        if sim is not sim64:
+            sim.close()
+        v = t_end / o["wall_s"]
+        forms[name] = {"value": round(v, 4), "ratio_to_plain_fp64": round(v / value_plain, 3),
+                       "nst": o["nst"], "nli": o["nli"], "nni": o["nni"], "ncfn": o["ncfn"],
+                       "max_dm_vs_rtol1e-6_reference": float(f"{np.max(np.abs(y - y_ref)[act]):.3e}"),
+                       "max_dm_vs_plain": float(f"{np.max(np.abs(y - y_plain)[act]):.3e}")}
+        print(f"[bench] form {name}: {forms[name]}", file=sys.stderr, flush=True)
+    apply_us = lpc_apply_us(la, sim64, y0)
+    sim64.close()
+    return {"window": f"t in [0, {t_end:g}], same initial state, rtol=atol=1e-4",
+            "plain_fp64": {"value": round(value_plain, 4), "nst": out_plain["nst"],
+                           "nli": out_plain["nli"], "nni": out_plain["nni"],
+                           "ncfn": out_plain["ncfn"],
+                           "max_dm_vs_rtol1e-6_reference": float(f"{dm_plain:.3e}")},
+            "forms": forms, "learned_pc": info, "learned_pc_apply_us": apply_us}
+
+
+def gpu_same_window(la, sim, y0, t_end, ropts, dt=0.01):
+    """the GPU over the CPU baseline's window, on its output cadence"""
+    run = sim.integrator(y0, **dict(ropts, t_out=dt))
+    o = run.advance(t_end)
+    run.close()
+    return {"t_end": round(t_end, 4), "t_out": dt, "value": round(o["t"] / o["wall_s"], 4),
+            "nst": o["nst"], "unit": "sim-time/wall-s"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,10 +360,15 @@ def main():
                     help="reduction wait: 1 poll the pinned result word (default), 0 stream sync")
     ap.add_argument("--fp32-demag", action="store_true", help="FP32 spectral demag form")
     ap.add_argument("--workload", default="square_hole",
-                    choices=["square_hole", "ref512", "ensemble"],
+                    choices=["square_hole", "ref512", "polycrystal", "ensemble"],
                     help="square_hole: 1536x512 cells, padded demag (default); ref512: the "
                          "reference's published 512x512 periodic i1 run (use --steps 1000); "
-                         "ensemble: 64 independent instances run concurrently")
+                         "polycrystal: 1024x1024 Voronoi polycrystal (72 grains, axes uniform "
+                         "on the sphere, 16 %% dead), compact execution, windowed demag 1.0, "
+                         "random initial m; ensemble: 64 independent instances run concurrently")
+    ap.add_argument("--no-ab", action="store_true",
+                    help="skip the FP32-demag / learned-preconditioner forms measured beside "
+                         "the plain run (square_hole and polycrystal workloads)")
     ap.add_argument("--instances", type=int, default=64, help="ensemble size")
     ap.add_argument("--learned-pc", action="store_true",
                     help="train the learned block-Jacobi correction (on states of a "
@@ -271,18 +418,30 @@ def main():
         return ensemble_bench(la, args)
 
     ref = args.workload == "ref512"
-    if ref:
+    poly = args.workload == "polycrystal"
+    if poly:
+        ng = ny = 1024
+        padded = False
+        geom = la.Geometry.polycrystal(ng, ny, num_grains=72, dead_frac=0.16, seed=20251104,
+                                       z_bias=1.0)
+        cfg = la.preset("polycrystal", ng=ng, ny=ny, demag=1.0, compact=True)
+        make_ic = lambda sd: geom.ic_random_sphere(seed=sd)   # noqa: E731
+        y0 = make_ic(1)
+    elif ref:
         ng = ny = 512
         padded = False
         geom = la.Geometry.full(ng, ny)
         cfg = la.preset("i1", ng=ng, ny=ny, demag=2.0)
-        y0 = geom.ic_stripes(left=0.25, right=0.75, eps=0.01)
+        make_ic = lambda sd: geom.ic_stripes(left=0.25, right=0.75, eps=0.01,   # noqa: E731
+                                             seed=12345 if sd == 1 else 777 + sd)
+        y0 = make_ic(1)
     else:
         ng, ny = args.ng, args.ny
         padded = not args.periodic
         geom = la.Geometry.square_hole(ng, ny, halfside_frac=222.0 / 512.0)
         cfg = la.preset("square_hole", ng=ng, ny=ny, padded=padded)
-        y0 = geom.ic_split_x(max_angle=0.05, seed=1)
+        make_ic = lambda sd: geom.ic_split_x(max_angle=0.05, seed=sd)   # noqa: E731
+        y0 = make_ic(1)
     cfg.demag_fp32 = 1 if args.fp32_demag else 0
     print(f"[bench] workload {args.workload} {ng}x{ny} cells, hole {int((geom.active == 0).sum())} "
           f"cells, {'padded' if padded else 'periodic'} demag; building context",
@@ -316,25 +475,13 @@ def main():
         la.lib().NVHip_SetReduceConfig(args.red_blocks, 1)
 
     lpc_info = None
+    lpc = None
     if args.learned_pc:
-        t_tr = time.perf_counter()
-        ytr = (geom.ic_split_x(max_angle=0.05, seed=2) if not ref
-               else geom.ic_stripes(left=0.25, right=0.75, eps=0.01, seed=777))
-        tr = sim.integrator(ytr, **ropts)
-        states = [ytr]
-        for t in (1.0, 2.0, 3.0, 4.0, 5.0):
-            tr.advance(t)
-            states.append(tr.state())
-        tr.close()
+        states = training_states(geom, make_ic, lambda y: sim.integrator(y, **ropts))
+        wts, lpc_info = train_lpc(la, sim, states, args.lpc_steps)
         lpc = la.LearnedPrecond(sim)
-        lpc.init_random(seed=0, std=1e-3)
-        hist = lpc.train(states, steps=args.lpc_steps // 2, lr=2e-3, gamma_range=(1e-3, 2e-1),
-                         seed=1)
-        hist += lpc.train(states, steps=args.lpc_steps - args.lpc_steps // 2, lr=5e-4,
-                          gamma_range=(1e-3, 2e-1), seed=2)
+        lpc.set_weights(wts)
         lpc.enable(True)
-        lpc_info = {"train_steps": args.lpc_steps, "train_s": round(time.perf_counter() - t_tr, 1),
-                    "loss_first": float(np.mean(hist[:10])), "loss_last": float(np.mean(hist[-10:]))}
         print(f"[bench] learned correction trained: {lpc_info}", file=sys.stderr, flush=True)
 
     if args.warmup > 0:
@@ -349,6 +496,8 @@ def main():
     out = run.advance(args.steps * step)
     tot_ms, nl, dropped = C.c_double(), C.c_long(), C.c_long()
     L.LLG_ProfileStop(C.byref(tot_ms), C.byref(nl), C.byref(dropped))
+    y_final = run.state()
+    run.close()
     wall = out["wall_s"]
     value = args.steps * step / wall
     print(f"[bench] timed {args.steps} steps in {wall:.3f} s: nst={out['nst']} nfe={out['nfe']} "
@@ -372,13 +521,30 @@ def main():
             "bytes_per_launch": nbytes, "avg_launch_us": round(avg_s * 1e6, 2),
             "launches": nl.value}
 
+    ab = None
+    if not args.no_ab and not ref and not args.learned_pc and not args.fp32_demag:
+        print("[bench] forms beside the plain run", file=sys.stderr, flush=True)
+        kern = None if args.fft_kernels == "auto" else tuple(int(v) for v in
+                                                             args.fft_kernels.split(","))
+        ab = north_star_ab(la, cfg, geom, y0, y_final, out, value, args.steps * step, ropts,
+                           args, make_ic, kern)
+
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(la, cfg, geom, y0, args.cpu_budget,
                            args.cpu_threads or min(16, os.cpu_count() or 1))
+        # the GPU over the same window and output cadence as the CPU sample
+        cpu["gpu_same_window"] = gpu_same_window(la, sim, y0, cpu["t_end"], ropts)
+        print(f"[bench] gpu over the cpu window: {cpu['gpu_same_window']}", file=sys.stderr,
+              flush=True)
 
-    if ref:
+    if poly:
+        workload = ("polycrystal 1024x1024 cells, 72 grains, axes uniform on the sphere, 16% "
+                    "dead grains as holes, compact execution, windowed periodic FFT demag 1.0, "
+                    "rtol=atol=1e-4, Krylov 5, block-Jacobi")
+        data = "synthetic (seeded Voronoi geometry and random initial magnetisation)"
+    elif ref:
         workload = ("reference reading of 1536x512: 512x512 cells, periodic 512x512 FFT demag "
                     "2.0, i1 physics, head-on stripes, rtol=atol=1e-4, Krylov 5, block-Jacobi")
         data = "synthetic (seeded head-on stripes, the reference's i1 initial state)"
@@ -388,7 +554,8 @@ def main():
                     "FFT demag 4.0, rtol=atol=1e-4, Krylov 5, block-Jacobi")
         data = "synthetic (seeded two-domain state on the square-hole geometry)"
     rec = {
-        "metric": "sim-time/wall-s at rtol=1e-4 (1536x512+demag)",
+        "metric": ("sim-time/wall-s at rtol=1e-4 (1024x1024 polycrystal, compact)" if poly
+                   else "sim-time/wall-s at rtol=1e-4 (1536x512+demag)"),
         "value": round(value, 4),
         "unit": "sim-time/wall-s",
         "n_gpus": 1,
@@ -416,6 +583,7 @@ def main():
                              "device": "device-resident"}[args.spgmr]},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "forms": ab,
     }
     print(json.dumps(rec), flush=True)
 

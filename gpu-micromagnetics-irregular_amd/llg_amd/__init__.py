@@ -72,8 +72,8 @@ def make_config(ng, ny, rhs, jv=None, pc=None, exec_=EXEC_YMSK, boundary=BC_PERI
 
 def _preset_i4(ng=512, ny=512, demag=4.0, demag_mode=DEMAG_PERIODIC, **kw):
     """2d_i4: square hole, ymsk, Zeeman (-0.2,0,0); Jv without Zeeman/demag;
-    preconditioner field with pc_happ1 = -1.0 (2d_fft.cu:169-180, jtv.cu:36-39,
-    precond.cu:65-71)."""
+    preconditioner field with pc_happ1 = -1.0 (2d_fft.cu:175-180, jtv.cu:36-39,
+    precond.cu:64-70)."""
     cub = dict(che=20.0, chk=1.0, msk=(1, 1, 1), nsk=(1, 0, 0), aniso=ANISO_CUBIC, gate=GATE_MASK)
     return make_config(ng, ny, terms(happ=(-0.2, 0, 0), **cub), terms(**cub),
                        terms(happ=(-1.0, 0, 0), **cub), demag=demag_mode if demag > 0 else DEMAG_OFF,
@@ -131,10 +131,30 @@ def _preset_square_hole(ng=1536, ny=512, demag=4.0, padded=True, **kw):
                       **kw)
 
 
+def _preset_polycrystal(ng=1024, ny=1024, demag=1.0, compact=True, padded=False, happ=(0, 0, 0),
+                        **kw):
+    """The polycrystal workloads (1024^2, 72 grains, 16 % dead; 4096^2, 400
+    grains, 10 % dead): the i3 constants (window-weighted exchange 4, per-
+    grain uniaxial anisotropy 1, DMI 0.6 on the x neighbours; the Jv and
+    preconditioner mirrors' fixed-axis set: anisotropy 4, DMI 0.3,
+    2d_i3/2d_fft.cu:176-181, jtv.cu:30-35, precond.cu:58-63), dead grains as
+    hard holes (compact active-cell lists, or the byte mask), windowed FFT
+    demag, regularised normalisation (2d_i3/2d_fft.cu:418-428); an applied
+    field in all three term sets for field sweeps."""
+    rhs = terms(che=4.0, chk=1.0, cha=0.0, chb=0.6, happ=happ, aniso=ANISO_LINEAR, cell_axes=1,
+                window=1, gate=GATE_MASK)
+    fixed = terms(che=4.0, chk=4.0, cha=0.0, chb=0.3, msk=(1, 0, 0), nsk=(1, 0, 0), happ=happ,
+                  aniso=ANISO_LINEAR, gate=GATE_MASK)
+    mode = DEMAG_PADDED if padded else DEMAG_PERIODIC
+    return make_config(ng, ny, rhs, fixed, fixed, exec_=EXEC_COMPACT if compact else EXEC_YMSK,
+                       normalize=NORM_REGULARIZED, demag=mode if demag > 0 else DEMAG_OFF,
+                       demag_strength=demag, **kw)
+
+
 PRESETS = {
     "i1": _preset_i1, "i2": _preset_i2, "i3": _preset_i3, "i4": _preset_i4,
     "i5": _preset_i5, "i6": _preset_i6, "plain128": _preset_plain128,
-    "square_hole": _preset_square_hole,
+    "square_hole": _preset_square_hole, "polycrystal": _preset_polycrystal,
 }
 
 
@@ -609,13 +629,17 @@ class LearnedPrecond:
         zero off the active cells"""
         g = self.sim.geom
         ng, ny = g.ng, g.ny
-        X, Y = np.meshgrid(np.arange(ng) / ng, np.arange(ny) / ny)
+        x, yy = np.arange(ng) / ng, np.arange(ny) / ny
         out = np.zeros((3, ny, ng))
         for c in range(3):
             for _ in range(nmodes):
                 kx, ky = rng.integers(0, kmax + 1, 2)
                 ph = rng.uniform(0, 2 * math.pi)
-                out[c] += rng.normal() * np.cos(2 * math.pi * (kx * X + ky * Y) + ph)
+                amp = rng.normal()
+                # cos(a + b) with a along x (phase included) and b along y
+                a, b = 2 * math.pi * kx * x + ph, 2 * math.pi * ky * yy
+                out[c] += np.outer(amp * np.cos(b), np.cos(a))
+                out[c] -= np.outer(amp * np.sin(b), np.sin(a))
         if noise is None:
             noise = float(rng.uniform(0.0, 1.0))
         out += noise * math.sqrt(nmodes) * rng.standard_normal(out.shape)

@@ -194,3 +194,54 @@ def test_training_lowers_loss_and_preconditioner_error():
     assert ol["nli"] <= 1.05 * ob["nli"]
     assert np.max(np.abs(yb - yl)) < 1e-3
     lpc.close()
+
+
+def test_training_on_compact_polycrystal():
+    """The correction trained on the polycrystal workload's physics with
+    compact execution (pairs from the FP64 operator at states of
+    trajectories from other seeds than the evaluated one): the loss falls,
+    the held-out preconditioned-solution error falls below block-Jacobi's,
+    and an integration with the correction (compact PrecondSolve + fused
+    inference) converges to the tolerance of the block-Jacobi run."""
+    ng = ny = 256
+    geom = la.Geometry.polycrystal(ng, ny, num_grains=12, dead_frac=0.16, seed=20251104,
+                                   z_bias=1.0)
+    cfg = la.preset("polycrystal", ng=ng, ny=ny, demag=1.0, compact=True)
+    assert cfg.exec_ == la.EXEC_COMPACT
+    sim = la.Simulation(cfg, geom)
+    states = []
+    for sd in (2, 3):
+        y = geom.ic_random_sphere(seed=sd)
+        states.append(y)
+        run = sim.integrator(y, rtol=1e-4, atol=1e-4)
+        for t in (0.05, 0.2):
+            run.advance(t)
+            states.append(run.state())
+        run.close()
+    lpc = la.LearnedPrecond(sim)
+    lpc.init_random(seed=0, std=1e-3)
+    hist = lpc.train(states, steps=300, lr=2e-3, gamma_range=(2e-3, 5e-2), seed=3)
+    first, last = np.mean(hist[:20]), np.mean(hist[-20:])
+    e_bj, e_lpc = _heldout_errors(lpc, states, np.random.default_rng(98))
+    print(f"[gpu] compact polycrystal training loss {first:.4e} -> {last:.4e}; held-out error "
+          f"block-Jacobi {e_bj:.3f}, learned {e_lpc:.3f}", flush=True)
+    # block-Jacobi is already close on this physics (exchange 4, not 20):
+    # the gain is a few per cent of the error
+    assert last < 0.9 * first
+    assert e_lpc < e_bj
+    y0 = geom.ic_random_sphere(seed=1)
+    res = {}
+    for on in (False, True):
+        lpc.enable(on)
+        run = sim.integrator(y0, rtol=1e-5, atol=1e-5)
+        out = run.advance(0.2)
+        res[on] = (out, run.state())
+        run.close()
+    lpc.enable(False)
+    (ob, yb), (ol, yl) = res[False], res[True]
+    print(f"[gpu] compact polycrystal: block-Jacobi nst {ob['nst']} nli {ob['nli']} | learned "
+          f"nst {ol['nst']} nli {ol['nli']}  max|dm| {np.max(np.abs(yb - yl)):.2e}", flush=True)
+    assert np.max(np.abs(yb - yl)) < 1e-3
+    assert np.all(yl.reshape(3, -1)[:, geom.active == 0] == 0.0)
+    lpc.close()
+    sim.close()

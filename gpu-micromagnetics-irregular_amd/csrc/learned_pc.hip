@@ -320,9 +320,9 @@ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 // A[l % 16][8q + j], B[8q + j][l % 16], D[4q + j][l % 16], j = 0..7 / 0..3;
 // the weights are the A operand and the activations the B operand, so D
 // is [output channel][cell].
-// The weights' B fragments (FP32 master weights rounded to BF16) are built
-// by k_make_frag32 and held in registers, one 16-channel output block per
-// wave.  Activations outside the grid are zero (zero padding) or wrapped
+// The weight fragments (FP32 master weights rounded to BF16) are built by
+// k_make_frag32 and held in registers; every wave holds all output blocks
+// of a layer.  Activations outside the grid are zero (zero padding) or wrapped
 // (periodic), as in the layer-by-layer kernels; the FP32 accumulation order
 // differs from theirs (K = 32 per instruction), so results agree to BF16
 // rounding of the hidden layers, not bit for bit.
@@ -349,7 +349,8 @@ __device__ inline bf16x8 lds8(const uint16_t* p) {
   return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(p));
 }
 
-// B fragments for the 16x16x32 instruction from W[9][cin][cout] (FP32):
+// Weight fragments for the 16x16x32 instruction from W[9][cin][cout] (FP32),
+// lane l holding W[k = 8q + j][channel l % 16] (used in A position):
 // layer with cin == 16: step s covers taps 2s, 2s+1; cin == 32: step = tap
 __global__ void k_make_frag32(const float* __restrict__ W, int cin, int cout,
                               u16x8* __restrict__ frag) {
@@ -543,7 +544,7 @@ struct llg_lpc {
   uint16_t *f, *h1, *h2, *g3, *g2, *g1;
   float* o;
   int enabled;
-  u16x8* frag32[3];   // B fragments of the 16x16x32 instruction (fused pass)
+  u16x8* frag32[3];   // weight fragments of the 16x16x32 instruction (fused pass)
   double* ubuf;       // 3n: the block-Jacobi output the correction reads
   int fused;          // 1: the fused inference kernel; 0: layer by layer
 };

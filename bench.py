@@ -56,12 +56,13 @@ KERNELS = {"rhs": 1, "jtv": 2, "build_p": 3, "apply_p": 4, "demag_row_fwd": 5,
 
 def algorithmic_bytes(kernel, ng, ny, ncell, padded=True, col_variant=0):
     """Unique bytes read + written by one launch (FP64, byte mask).  The
-    blocked column kernels (variants 2-4) read the zero-padded grid's tensor
-    planes for ky <= Py/2 only (the planes are mirror-symmetric in ky)."""
+    blocked and wave-per-transform column kernels (variants 2-6) read the
+    zero-padded grid's tensor planes for ky <= Py/2 only (the planes are
+    mirror-symmetric in ky)."""
     px, py = (2 * ng, 2 * ny) if padded else (ng, ny)
     hx = px // 2 + 1
     spec = 3 * ny * hx * 16                     # intermediate spectrum (rows stored)
-    if padded and col_variant in (2, 3, 4):
+    if padded and col_variant in (2, 3, 4, 5, 6):
         tensor = 4 * hx * (py // 2 + 1) * 8      # xx, yy, zz, xy; half of each column
     else:
         tensor = (4 if padded else 5) * hx * py * 8  # real tensor planes
@@ -398,12 +399,13 @@ def main():
     ap.add_argument("--fft-kernels", default=None,
                     help="demag FFT kernel families ROW,COL kept after the autotuner "
                          "(Demag_SetFftKernels: 0 / 1 register-resident variants, 2 / 3 / 4 "
-                         "variant 0 on 2 / 4 / 8 columns per workgroup (same bits as 0), -1 "
+                         "variant 0 on 2 / 4 / 8 columns per workgroup (same bits as 0), "
+                         "5 / 6 variant 1 with one transform per wave (same bits as 1), -1 "
                          "Stockham), or 'auto' for the autotuner's choice.  The families "
                          "round differently, and over 100 time units the step count the "
-                         "metric depends on follows the rounding (3,768 steps with 0,0; "
-                         "4,703 with column variant 1), so the FP64 default pins 0,3 "
-                         "(the FP32-spectra default: auto)")
+                         "metric depends on follows the rounding (4,109 steps with 0,5; "
+                         "4,202 with 0,3), so the FP64 default pins 0,5, the fastest "
+                         "column kernel (the FP32-spectra default: auto)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -456,7 +458,7 @@ def main():
                     "demag_col_block": ch[3], "demag_row_pad": ch[4], "demag_col_pad": ch[5],
                     "demag_row_kernel": ch[6], "demag_col_kernel": ch[7]}
     if args.fft_kernels is None:
-        args.fft_kernels = "auto" if args.fp32_demag else "0,3"
+        args.fft_kernels = "auto" if args.fp32_demag else "0,5"
     if args.fft_kernels != "auto":
         rk, ck = (int(v) for v in args.fft_kernels.split(","))
         dm = la.lib().LLG_GetDemag(sim.ctx)

@@ -897,9 +897,10 @@ DemagData* init_common(int nx, int ny, double thick, double strength, int padded
   if (!launch_ok("Demag_Init")) { destroy(d); return nullptr; }
   // register-resident FFT kernels wherever a plan covers the lengths
   d->reg_row = demag_reg::has_row_plan<double>(d->px / 2, 0) ? 0 : -1;
-  // columns: the blocked form (4 columns per workgroup, same bits as the
-  // one-column form) where it exists
-  d->reg_col = demag_reg::has_col_plan<double>(d->py, 3) ? 3
+  // columns: one transform per wave (variant 5) where it exists, else the
+  // blocked form (4 columns per workgroup, variant 0's bits), else variant 0
+  d->reg_col = demag_reg::has_col_plan<double>(d->py, 5) ? 5
+             : demag_reg::has_col_plan<double>(d->py, 3) ? 3
              : demag_reg::has_col_plan<double>(d->py, 0) ? 0 : -1;
   return d;
 }
@@ -1165,7 +1166,7 @@ int Demag_Autotune(DemagData* d, int reps, int* choice) {
     if (v < 0 ? d->row.block > 0 && (!d->d_K32 || d->row32.block > 0)
               : demag_reg::has_row_plan<double>(d->px / 2, v))
       row_vars.push_back(v);
-  for (int v : cands(cv0, {0, 1, 2, 3, 4, -1}))
+  for (int v : cands(cv0, {0, 1, 2, 3, 4, 5, 6, -1}))
     if (v < 0 ? d->col.block > 0 && (!d->d_K32 || d->col32.block > 0)
               : demag_reg::has_col_plan<double>(d->py, v))
       col_vars.push_back(v);

@@ -24,7 +24,9 @@ using C2 = rfft::Cplx<T>;
 // length, variant 1 (16 values per lane) for lengths 1024 to 8192, and
 // variants 2, 3, 4 (variant 0's transforms with 2, 4 or 8 adjacent columns
 // per workgroup and the spectrum moved in runs of that many values) for
-// lengths 512 to 2048 where the block fits one workgroup.
+// lengths 512 to 2048 where the block fits one workgroup, and variants 5, 6
+// (variant 1's transforms, one per wave, in workgroups of 2 or 4 columns'
+// x/y or 4 or 8 columns' z) for length 1024.
 template <class T> bool has_row_plan(int L, int variant);
 template <class T> bool has_col_plan(int L, int variant);
 

@@ -14,6 +14,10 @@
 //                 m_x and m_y as a pair, with the xy block of the tensor
 //                 applied in registers (each lane holds both components at
 //                 the same ky).  Only the rows the field needs are stored.
+//                 The blocked form (k_col_cb) moves several adjacent
+//                 columns per workgroup through LDS; the wave form
+//                 (k_col_wave, the default for 1024-point columns) runs
+//                 each component's transform on one wave.
 //   row inverse : the C2R pre-processing reads bins k and L-k straight from
 //                 the spectrum into the registers of the last FFT stage,
 //                 the inverse FFT ends in lane u holding z[u + U b], stored
@@ -409,6 +413,162 @@ void k_col_cb(C2<T>* __restrict__ spec, int Hx, const C2<T>* __restrict__ tw,
   });
 }
 
+// ---------------------------------------- column, one transform per wave
+// The kernels above run a transform on 128 or more lanes, so every LDS
+// exchange is a workgroup barrier and all waves of a workgroup step through
+// load, FFT stages, tensor product and store together; with two workgroups
+// per CU the CU then alternates between vector-memory, LDS and VALU phases
+// instead of overlapping them.  Here a transform runs on ONE wave (64 lanes
+// of V values: the 1024-point plan with 16 values per lane), so its
+// exchanges only wait for the wave's own LDS accesses and the waves of a CU
+// drift apart.  The exchanges move real and imaginary parts separately
+// (RegFft::forward_split), so a wave's LDS buffer is kSlots values of T and
+// 16 waves of 1024-point transforms fit one CU.
+//
+// A workgroup of W = 2 CB waves holds either CB adjacent columns' x and y
+// components (wave 2c: x of column c, wave 2c + 1: its y) or W adjacent
+// columns' z.  The spectrum moves in runs of CB (x/y) or W (z) consecutive
+// values per row, staged through the waves' buffers (one barrier each way).
+// The xy block of the tensor needs both components at each ky: the x and y
+// waves of a column swap their forward spectra through LDS, half of the
+// registers at a time (workgroup barriers), and each forms its own output
+// with the one-column kernel's arithmetic -- same plan, same values, same
+// operations: variant 1's bits for 1024-point columns, variant 0's for 512.
+struct WaveBar {
+  // a wave's own LDS writes complete before its later LDS reads (and the
+  // compiler keeps the accesses on their side)
+  __device__ __forceinline__ void operator()() const { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+};
+
+template <class T, class F, bool HALF, int CB>
+struct ColWave {
+  static constexpr int W = 2 * CB;                 // waves per workgroup
+  static constexpr int kThreads = 64 * W;
+  static constexpr int ROWS = HALF ? F::L / 2 : F::L;
+  // buffer stride in T: the exchange slots or the staged column (ROWS
+  // complex values), whichever is larger, padded to 32 B modulo 256 B so
+  // that adjacent buffers' staging accesses fall on different banks
+  static constexpr int PB = 256 / (int)sizeof(T), PAD = 32 / (int)sizeof(T);
+  static constexpr int base = F::kSlots > 2 * ROWS ? F::kSlots : 2 * ROWS;
+  static constexpr int kBuf = base + ((PAD - base % PB) % PB + PB) % PB;
+  static constexpr size_t kLds = sizeof(T) * (size_t)W * kBuf;
+  static constexpr bool ok = F::U == 64 && F::kPerGroup == 1 && kThreads <= 1024 &&
+                             kLds <= 160 * 1024 && ROWS % (kThreads / CB) == 0;
+};
+
+template <class T, class F, bool HALF, bool XYI, int CB>
+__global__ __launch_bounds__(128 * CB) __attribute__((amdgpu_waves_per_eu(4)))
+void k_col_wave(C2<T>* __restrict__ spec, int Hx, int nxy, const C2<T>* __restrict__ tw,
+                const T* __restrict__ K) {
+  using C = C2<T>;
+  using G = ColWave<T, F, HALF, CB>;
+  constexpr int V = F::V, L = F::L, W = G::W, NT = G::kThreads, kBuf = G::kBuf;
+  constexpr int NB = HALF ? V / 2 : V;
+  constexpr int ROWS = G::ROWS;
+  using Last = typename F::Last;
+  constexpr int Q = Last::Q;
+  static_assert(G::ok, "plan does not fit a wave-per-transform column workgroup");
+  extern __shared__ char lds_raw[];
+  T* const lds_all = reinterpret_cast<T*>(lds_raw);
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  T* const mybuf = lds_all + wave * kBuf;
+  // grid: [0, nxy) x/y workgroups, [nxy, gridDim) z workgroups (nxy a
+  // multiple of 8, so both parts keep the XCD mapping)
+  const bool zjob = (int)blockIdx.x >= nxy;
+  const int bid = zjob ? (int)blockIdx.x - nxy : (int)blockIdx.x;
+  const int blk = reg_xcd_remap(bid, zjob ? (int)gridDim.x - nxy : nxy);
+  const long P = (long)Hx * L;
+  auto mirror = [](unsigned k) { return (HALF && k > (unsigned)(L / 2)) ? (unsigned)L - k : k; };
+  auto odd_sign = [](unsigned k) { return (HALF && k > (unsigned)(L / 2)) ? T(-1) : T(1); };
+  auto job = [&](auto Zc) {
+    constexpr bool Z = decltype(Zc)::value;
+    constexpr int NCOL = Z ? W : CB, NCOMP = Z ? 1 : 2;
+    constexpr int RPR = NT / NCOL, RND = ROWS / RPR;   // staging rows per round, rounds
+    const int kx0 = blk * NCOL;
+    if (kx0 >= Hx) return;                           // whole workgroup
+    const int wcol = Z ? wave : wave >> 1;
+    int kx = kx0 + wcol;
+    if (kx >= Hx) kx = Hx - 1;                       // computed, never stored
+    const int cio = tid % NCOL, rio = tid / NCOL;    // staging view
+    const bool live_io = kx0 + cio < Hx;
+    const int kx_io = live_io ? kx0 + cio : Hx - 1;
+    auto buf_of = [&](int col, int comp) { return lds_all + (Z ? col : 2 * col + comp) * kBuf; };
+    // ---- load the workgroup's columns in runs of NCOL values per row
+    {
+      C v[NCOMP][RND];
+      sfor<0, NCOMP>([&](auto Cc) {
+        const C* src = spec + (long)(Z ? 2 : (int)Cc) * ROWS * Hx + kx_io;
+        sfor<0, RND>([&](auto I) { v[Cc][I] = ldc(src + (long)(rio + RPR * I) * Hx); });
+      });
+      sfor<0, NCOMP>([&](auto Cc) {
+        C* b = reinterpret_cast<C*>(buf_of(cio, Cc));
+        sfor<0, RND>([&](auto I) { b[rio + RPR * I] = v[Cc][I]; });
+      });
+    }
+    __syncthreads();
+    C r[V];
+    sfor<0, V>([&](auto B) {
+      if constexpr (B < NB) r[B] = reinterpret_cast<const C*>(mybuf)[lane + 64 * B];
+      else r[B] = mk<T>(T(0), T(0));                 // not read by the half-input stage
+    });
+    F::template forward_split<HALF>(r, mybuf, lane, tw, WaveBar{});
+    // ---- tensor product
+    if constexpr (Z) {
+      const T* kzz = K + 2 * P + (long)kx * L;
+      sfor<0, V>([&](auto I) {
+        constexpr int i = I;
+        const unsigned k = (unsigned)Last::k_of(rfft::opaque(lane), i / Q, i % Q);
+        const T cz = kzz[mirror(k)];
+        r[i] = mk<T>(cz * r[i].x, cz * r[i].y);
+      });
+    } else {
+      const int comp = wave & 1;
+      const T* kd = K + (long)comp * P + (long)kx * L;        // K_xx or K_yy
+      const T* kxyr = K + 3 * P + (long)kx * L;
+      const T* kxyi = kxyr + P;
+      const C* pbuf = reinterpret_cast<const C*>(lds_all + (wave ^ 1) * kBuf);
+      // two rounds over halves of the registers: publish own values,
+      // read the partner's at the same positions (same lane -> k map)
+      sfor<0, 2>([&](auto H) {
+        constexpr int HV = V / 2;
+        __syncthreads();                             // the partner read the previous round
+        sfor<0, HV>([&](auto I) { reinterpret_cast<C*>(mybuf)[lane + 64 * I] = r[H * HV + I]; });
+        __syncthreads();
+        sfor<0, HV>([&](auto I) {
+          constexpr int i = H * HV + I;
+          const unsigned k = (unsigned)Last::k_of(rfft::opaque(lane), i / Q, i % Q);
+          const unsigned km = mirror(k);
+          const T d = kd[km], xr = odd_sign(k) * kxyr[km];
+          const C o = pbuf[lane + 64 * I];          // the other component at this k
+          C c;
+          if constexpr (XYI) {
+            c = cmul(mk<T>(xr, kxyi[km]), o);
+          } else {
+            c = mk<T>(xr * o.x, xr * o.y);
+          }
+          r[i] = mk<T>(fmad(d, r[i].x, c.x), fmad(d, r[i].y, c.y));
+        });
+      });
+      __syncthreads();                               // the partner's last reads of this buffer
+    }
+    F::template inverse_split<HALF>(r, mybuf, lane, tw, WaveBar{});
+    // ---- store in runs of NCOL values per row
+    WaveBar{}();
+    sfor<0, NB>([&](auto B) { reinterpret_cast<C*>(mybuf)[lane + 64 * B] = r[B]; });
+    __syncthreads();
+    if (live_io) {
+      sfor<0, NCOMP>([&](auto Cc) {
+        const C* b = reinterpret_cast<const C*>(buf_of(cio, Cc));
+        C* dst = spec + (long)(Z ? 2 : (int)Cc) * ROWS * Hx + kx_io;
+        sfor<0, RND>([&](auto I) { stc(dst + (long)(rio + RPR * I) * Hx, b[rio + RPR * I]); });
+      });
+    }
+  };
+  if (zjob) job(std::true_type{});
+  else job(std::false_type{});
+}
+
 // ---------------------------------------------------------------- plans
 // Row transforms (complex length L = Px/2, one component per transform).
 template <class T> using RowP128 = rfft::RegFft<T, 128, 16, 4, 4>;
@@ -521,9 +681,36 @@ bool with_col_cb(int L, int variant, Fn&& fn) {
   return done;
 }
 
+// column variants 5, 6: the wave-per-transform kernel with CB = 2, 4
+// (x/y workgroups of CB columns, z workgroups of 2 CB columns) on the
+// 1024-point plan with 16 values per lane, where both the zero-padded and
+// the periodic form fit a workgroup
+constexpr int col_wave_of(int variant) { return variant == 5 ? 2 : variant == 6 ? 4 : 0; }
+
+template <class T, class Fn>
+bool with_col_wave(int L, int variant, Fn&& fn) {
+  const int cb = col_wave_of(variant);
+  bool done = false;
+  auto with_plan = [&](auto f) {
+    using F = decltype(f);
+    auto go = [&](auto CBc) {
+      constexpr int CBv = decltype(CBc)::value;
+      if constexpr (ColWave<T, F, true, CBv>::ok && ColWave<T, F, false, CBv>::ok) {
+        fn(f, CBc);
+        done = true;
+      }
+    };
+    if (cb == 2) go(std::integral_constant<int, 2>{});
+    else if (cb == 4) go(std::integral_constant<int, 4>{});
+  };
+  if (L == 1024) with_plan(ColP1024w<T>{});
+  return done;
+}
+
 template <class T>
 bool has_col_plan(int L, int variant) {
   if (col_cb_of(variant)) return with_col_cb<T>(L, variant, [](auto, auto) {});
+  if (col_wave_of(variant)) return with_col_wave<T>(L, variant, [](auto, auto) {});
   return with_col<T>(L, variant, [](auto) {});
 }
 
@@ -561,6 +748,23 @@ int launch_col(int L, int variant, int half, C2<T>* spec, int Hx, const C2<T>* t
       if (prep_lds(kern, G::kLds)) { rc = -1; return; }
       hipExtLaunchKernelGGL(kern, dim3(grid), dim3(G::kThreads), (std::uint32_t)G::kLds, s, e0, e1,
                             0u, spec, Hx, tw, K);
+      rc = hipPeekAtLastError() == hipSuccess ? 1 : -1;
+    });
+    return rc;
+  }
+  if (col_wave_of(variant)) {
+    if ((half != 0) == (has_xy_im != 0)) return 0;
+    with_col_wave<T>(L, variant, [&](auto f, auto CBc) {
+      using F = decltype(f);
+      constexpr int CBv = decltype(CBc)::value;
+      // x/y workgroups, then z workgroups, each part a multiple of 8
+      const int nxy = ((Hx + CBv - 1) / CBv + 7) / 8 * 8;
+      const int nz = ((Hx + 2 * CBv - 1) / (2 * CBv) + 7) / 8 * 8;
+      auto kern = half ? k_col_wave<T, F, true, false, CBv> : k_col_wave<T, F, false, true, CBv>;
+      const size_t lds = half ? ColWave<T, F, true, CBv>::kLds : ColWave<T, F, false, CBv>::kLds;
+      if (prep_lds(kern, lds)) { rc = -1; return; }
+      hipExtLaunchKernelGGL(kern, dim3(nxy + nz), dim3(ColWave<T, F, true, CBv>::kThreads),
+                            (std::uint32_t)lds, s, e0, e1, 0u, spec, Hx, nxy, tw, K);
       rc = hipPeekAtLastError() == hipSuccess ? 1 : -1;
     });
     return rc;

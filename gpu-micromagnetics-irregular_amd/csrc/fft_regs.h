@@ -352,30 +352,83 @@ struct RegFft {
   static_assert((U_ & (U_ - 1)) == 0, "U must be a power of two");
   static_assert(qprod(M) == U_, "stage radices must multiply to U");
 
+  // ---------------- twiddle runs
+  // w_t = tw[base t] = W_L^(base t) for t in [1, N) (base t < L), handed to
+  // fn(t, w_t) in increasing t.  Each run of four consecutive t reads its
+  // first value from the table and forms the other three as products with
+  // w_1, w_2 = w_1 w_1 and w_3 = w_2 w_1 (at most three roundings deep, a
+  // few ulp): one table read per four twiddles.  The table reads go through
+  // the vector-memory address unit, one per CU, which every lane-varying
+  // read occupies for 16 cycles per wave (16-byte values); the products run
+  // on the four SIMDs.
+  template <int N, class Fn>
+  RF_HD RF_INL static void twiddle_run(const C* tw, unsigned base, Fn&& fn) {
+    if constexpr (N > 1) {
+      const C w1 = ldc(tw + base);
+      fn(std::integral_constant<int, 1>{}, w1);
+      if constexpr (N > 2) {
+        const C w2 = cmul(w1, w1);
+        fn(std::integral_constant<int, 2>{}, w2);
+        if constexpr (N > 3) {
+          const C w3 = cmul(w2, w1);
+          fn(std::integral_constant<int, 3>{}, w3);
+          sfor<1, (N + 3) / 4>([&](auto Cq) {
+            constexpr int c = 4 * Cq;
+            const C wc = ldc(tw + base * (unsigned)c);
+            fn(std::integral_constant<int, c>{}, wc);
+            if constexpr (c + 1 < N) fn(std::integral_constant<int, c + 1>{}, cmul(wc, w1));
+            if constexpr (c + 2 < N) fn(std::integral_constant<int, c + 2>{}, cmul(wc, w2));
+            if constexpr (c + 3 < N) fn(std::integral_constant<int, c + 3>{}, cmul(wc, w3));
+          });
+        }
+      }
+    }
+  }
+
+  // w_t = W_L^(u t) for t in [1, N) from the one table read w_1 = tw[u]:
+  // w_2 = w_1 w_1, w_3 = w_2 w_1, w_4 = w_2 w_2, and each further run of
+  // four from its first value w_c = w_(c-4) w_4 times w_1, w_2, w_3 (at
+  // most N/4 + 3 roundings deep).  Stage F1's table index u t strides the
+  // lanes by t values, so every read past t = 1 would touch one cache
+  // sector per lane; the read of tw[u] is contiguous across the lanes.
+  template <int N, class Fn>
+  RF_HD RF_INL static void twiddle_pow(const C* tw, unsigned u, Fn&& fn) {
+    if constexpr (N > 1) {
+      const C w1 = ldc(tw + u);
+      fn(std::integral_constant<int, 1>{}, w1);
+      if constexpr (N > 2) {
+        const C w2 = cmul(w1, w1);
+        fn(std::integral_constant<int, 2>{}, w2);
+        if constexpr (N > 3) {
+          const C w3 = cmul(w2, w1);
+          fn(std::integral_constant<int, 3>{}, w3);
+          if constexpr (N > 4) {
+            const C w4 = cmul(w2, w2);
+            C wc = w4;
+            sfor<1, (N + 3) / 4>([&](auto Cq) {
+              constexpr int c = 4 * Cq;
+              if constexpr (c > 4) wc = cmul(wc, w4);
+              fn(std::integral_constant<int, c>{}, wc);
+              if constexpr (c + 1 < N) fn(std::integral_constant<int, c + 1>{}, cmul(wc, w1));
+              if constexpr (c + 2 < N) fn(std::integral_constant<int, c + 2>{}, cmul(wc, w2));
+              if constexpr (c + 3 < N) fn(std::integral_constant<int, c + 3>{}, cmul(wc, w3));
+            });
+          }
+        }
+      }
+    }
+  }
+
   // ---------------- stage F1 (lane u holds x[u + U b], b in [0, V))
-  // the twiddles W_L^(u kb) are fetched kTwChunk at a time (register budget)
-  static constexpr int kTwChunk = 4;
   template <bool HALF_IN>
   RF_HD RF_INL static void f1_fwd(C* r, int u, const C* tw) {
     if constexpr (HALF_IN) rdft_half_in<V, -1, T>(r);
     else rdft<V, -1, T>(r);
-    sfor<0, (V + kTwChunk - 1) / kTwChunk>([&](auto Ch) {
-      const int uu = opaque(u);
-      sfor<0, kTwChunk>([&](auto I) {
-        constexpr int kb = Ch * kTwChunk + I;
-        if constexpr (kb > 0 && kb < V) r[kb] = cmul(r[kb], ldc(tw + (unsigned)(uu * kb)));
-      });
-    });
+    twiddle_pow<V>(tw, (unsigned)opaque(u), [&](auto KB, const C& w) { r[KB] = cmul(r[KB], w); });
   }
   template <bool HALF_OUT>
   RF_HD RF_INL static void f1_inv(C* r, int u, const C* tw) {
-    sfor<0, (V + kTwChunk - 1) / kTwChunk>([&](auto Ch) {
-      const int uu = opaque(u);
-      sfor<0, kTwChunk>([&](auto I) {
-        constexpr int kb = Ch * kTwChunk + I;
-        if constexpr (kb > 0 && kb < V) r[kb] = cmulc(r[kb], ldc(tw + (unsigned)(uu * kb)));
-      });
-    });
+    twiddle_pow<V>(tw, (unsigned)opaque(u), [&](auto KB, const C& w) { r[KB] = cmulc(r[KB], w); });
     if constexpr (HALF_OUT) rdft_half_out<V, 1, T>(r);
     else rdft<V, 1, T>(r);
   }
@@ -441,8 +494,8 @@ struct RegFft {
         if constexpr (P > 1) {
           int kb, kp, a1;
           group(opaque(lane), J, kb, kp, a1);
-          const int base = a1 * V * QP;
-          sfor<1, Q>([&](auto Tt) { r[J * Q + Tt] = cmul(r[J * Q + Tt], ldc(tw + (unsigned)(base * Tt))); });
+          twiddle_run<Q>(tw, (unsigned)(a1 * V * QP),
+                         [&](auto Tt, const C& w) { r[J * Q + Tt] = cmul(r[J * Q + Tt], w); });
         }
       });
     }
@@ -451,10 +504,52 @@ struct RegFft {
         if constexpr (P > 1) {
           int kb, kp, a1;
           group(opaque(lane), J, kb, kp, a1);
-          const int base = a1 * V * QP;
-          sfor<1, Q>([&](auto Tt) { r[J * Q + Tt] = cmulc(r[J * Q + Tt], ldc(tw + (unsigned)(base * Tt))); });
+          twiddle_run<Q>(tw, (unsigned)(a1 * V * QP),
+                         [&](auto Tt, const C& w) { r[J * Q + Tt] = cmulc(r[J * Q + Tt], w); });
         }
         rdft<Q, 1, T>(r + J * Q);
+      });
+    }
+    // one part (0: real, 1: imaginary) of every value through a buffer of
+    // T, at the same slots as the whole-value forms above
+    template <int PART>
+    RF_HD RF_INL static void put_in_part(T* buf, const C* r, int lane) {
+      sfor<0, NG>([&](auto J) {
+        int kb, kp, a1;
+        group(lane, J, kb, kp, a1);
+        sfor<0, Q>([&](auto Tt) { buf[slot(kb, x_in(kp, a1, Tt))] = PART ? r[J * Q + Tt].y : r[J * Q + Tt].x; });
+      });
+    }
+    template <int PART>
+    RF_HD RF_INL static void get_in_part(const T* buf, C* r, int lane) {
+      sfor<0, NG>([&](auto J) {
+        int kb, kp, a1;
+        group(lane, J, kb, kp, a1);
+        sfor<0, Q>([&](auto Tt) {
+          const T v = buf[slot(kb, x_in(kp, a1, Tt))];
+          if constexpr (PART) r[J * Q + Tt].y = v;
+          else r[J * Q + Tt].x = v;
+        });
+      });
+    }
+    template <int PART>
+    RF_HD RF_INL static void put_out_part(T* buf, const C* r, int lane) {
+      sfor<0, NG>([&](auto J) {
+        int kb, kp, a1;
+        group(lane, J, kb, kp, a1);
+        sfor<0, Q>([&](auto Tt) { buf[slot(kb, x_out(kp, a1, Tt))] = PART ? r[J * Q + Tt].y : r[J * Q + Tt].x; });
+      });
+    }
+    template <int PART>
+    RF_HD RF_INL static void get_out_part(const T* buf, C* r, int lane) {
+      sfor<0, NG>([&](auto J) {
+        int kb, kp, a1;
+        group(lane, J, kb, kp, a1);
+        sfor<0, Q>([&](auto Tt) {
+          const T v = buf[slot(kb, x_out(kp, a1, Tt))];
+          if constexpr (PART) r[J * Q + Tt].y = v;
+          else r[J * Q + Tt].x = v;
+        });
       });
     }
     // output index k of register J*Q + t after the last forward stage
@@ -506,6 +601,71 @@ struct RegFft {
     bar();
     ln = opaque(lane);
     get_f1(lds, r, ln);
+    f1_inv<HALF_OUT>(r, ln, tw);
+  }
+
+  // ---------------- the same transforms with split exchanges
+  // Each exchange moves the real parts of all values, then the imaginary
+  // parts, through a buffer of kSlots values of T: half the LDS of a
+  // whole-value exchange, twice the accesses at half the width.  Same
+  // arithmetic and slots, so the results equal forward / inverse bit for bit.
+  template <int PART>
+  RF_HD RF_INL static void put_f1_part(T* buf, const C* r, int u) {
+    sfor<0, V>([&](auto KB) { buf[KB + (V + 1) * u] = PART ? r[KB].y : r[KB].x; });
+  }
+  template <int PART>
+  RF_HD RF_INL static void get_f1_part(const T* buf, C* r, int u) {
+    sfor<0, V>([&](auto KB) {
+      const T v = buf[KB + (V + 1) * u];
+      if constexpr (PART) r[KB].y = v;
+      else r[KB].x = v;
+    });
+  }
+  template <bool HALF_IN, class Bar>
+  RF_HD RF_INL static void forward_split(C* r, T* buf, int lane, const C* tw, Bar&& bar) {
+    int ln = opaque(lane);
+    f1_fwd<HALF_IN>(r, ln, tw);
+    sfor<0, 2>([&](auto Pt) {
+      bar();
+      put_f1_part<Pt>(buf, r, ln);
+      bar();
+      Stage<0>::template get_in_part<Pt>(buf, r, ln);
+    });
+    ln = opaque(lane);
+    Stage<0>::fwd(r, ln, tw);
+    sfor<1, M>([&](auto D) {
+      sfor<0, 2>([&](auto Pt) {
+        bar();
+        Stage<D - 1>::template put_out_part<Pt>(buf, r, ln);
+        bar();
+        Stage<D>::template get_in_part<Pt>(buf, r, ln);
+      });
+      ln = opaque(lane);
+      Stage<D>::fwd(r, ln, tw);
+    });
+  }
+  template <bool HALF_OUT, class Bar>
+  RF_HD RF_INL static void inverse_split(C* r, T* buf, int lane, const C* tw, Bar&& bar) {
+    int ln = opaque(lane);
+    sfor<0, M - 1>([&](auto I) {
+      constexpr int D = M - 1 - I;
+      Stage<D>::inv(r, ln, tw);
+      sfor<0, 2>([&](auto Pt) {
+        bar();
+        Stage<D>::template put_in_part<Pt>(buf, r, ln);
+        bar();
+        Stage<D - 1>::template get_out_part<Pt>(buf, r, ln);
+      });
+      ln = opaque(lane);
+    });
+    Stage<0>::inv(r, ln, tw);
+    sfor<0, 2>([&](auto Pt) {
+      bar();
+      Stage<0>::template put_in_part<Pt>(buf, r, ln);
+      bar();
+      get_f1_part<Pt>(buf, r, ln);
+    });
+    ln = opaque(lane);
     f1_inv<HALF_OUT>(r, ln, tw);
   }
 };

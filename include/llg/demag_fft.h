@@ -71,11 +71,15 @@ int  Demag_SetSpecialized(DemagData* d, int on);
  * 1024, 2048, 4096, 8192}, variant 1 (16 per lane) for Py >= 1024, and
  * variants 2, 3, 4 (variant 0's transforms on 2, 4 or 8 adjacent columns
  * per workgroup, the spectrum read and written in runs of that many
- * values; the same bits as variant 0) for Py in {512, 1024, 2048}.
+ * values; the same bits as variant 0) for Py in {512, 1024, 2048}, and
+ * variants 5, 6 (variant 1's transforms, each on one wave, in workgroups
+ * of 2 or 4 adjacent columns' x/y or 4 or 8 columns' z; the same bits as
+ * variant 1) for Py = 1024.
  * -1 selects the LDS Stockham kernels (any 2^a 3^b 5^c length).  Default:
- * rows variant 0, columns variant 3 where it exists, else 0.  Returns -1 (nothing changed) when a
- * requested variant does not exist for the grid.  The two families round
- * differently (both are within 1e-12 relative of the exact convolution). */
+ * rows variant 0, columns variant 5 where it exists, else 3, else 0.
+ * Returns -1 (nothing changed) when a requested variant does not exist for
+ * the grid.  The families round differently (all are within 1e-12
+ * relative of the exact convolution). */
 int  Demag_SetFftKernels(DemagData* d, int row_variant, int col_variant);
 int  Demag_GetFftKernels(DemagData* d, int* row_variant, int* col_variant);
 /* Autotuner: times Demag_Apply (reps launches each, each after a pass

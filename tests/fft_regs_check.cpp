@@ -88,6 +88,7 @@ struct Cases {
     const bool ok_f = cover && err <= tol * mag;
     std::printf("%-28s %s forward: rel err %.3e  cover %d  %s\n", name, half ? "half" : "full",
                 err / mag, (int)cover, ok_f ? "ok" : "FAIL");
+    const std::vector<std::vector<Cplx<T>>> rf = r;   // forward result
     // inverse back to x (times L); half: only the first half of the outputs
     rfft::sfor<0, F::M - 1>([&](auto I) {
       constexpr int D = F::M - 1 - I;
@@ -112,7 +113,58 @@ struct Cases {
     const bool ok_i = ierr <= tol * imag;
     std::printf("%-28s %s inverse: rel err %.3e  %s\n", name, half ? "half" : "full", ierr / imag,
                 ok_i ? "ok" : "FAIL");
-    if (!ok_f || !ok_i) fails++;
+    // split exchanges (real parts, then imaginary parts, through a buffer of
+    // T, as forward_split / inverse_split do) give the whole-value results
+    // bit for bit, forward and inverse
+    std::vector<T> sbuf(F::kSlots);
+    std::vector<std::vector<Cplx<T>>> s(U, std::vector<Cplx<T>>(V));
+    for (int u = 0; u < U; u++) {
+      for (int b = 0; b < V; b++) s[u][b] = x[u + U * b];
+      if (half) F::template f1_fwd<true>(s[u].data(), u, tw.data());
+      else F::template f1_fwd<false>(s[u].data(), u, tw.data());
+    }
+    rfft::sfor<0, 2>([&](auto Pt) {
+      for (int u = 0; u < U; u++) F::template put_f1_part<Pt>(sbuf.data(), s[u].data(), u);
+      for (int u = 0; u < U; u++) F::template Stage<0>::template get_in_part<Pt>(sbuf.data(), s[u].data(), u);
+    });
+    for (int u = 0; u < U; u++) F::template Stage<0>::fwd(s[u].data(), u, tw.data());
+    rfft::sfor<1, F::M>([&](auto D) {
+      rfft::sfor<0, 2>([&](auto Pt) {
+        for (int u = 0; u < U; u++)
+          F::template Stage<D - 1>::template put_out_part<Pt>(sbuf.data(), s[u].data(), u);
+        for (int u = 0; u < U; u++)
+          F::template Stage<D>::template get_in_part<Pt>(sbuf.data(), s[u].data(), u);
+      });
+      for (int u = 0; u < U; u++) F::template Stage<D>::fwd(s[u].data(), u, tw.data());
+    });
+    bool ok_s = true;
+    for (int u = 0; u < U; u++)
+      for (int b = 0; b < V; b++)
+        ok_s = ok_s && s[u][b].x == rf[u][b].x && s[u][b].y == rf[u][b].y;
+    rfft::sfor<0, F::M - 1>([&](auto I) {
+      constexpr int D = F::M - 1 - I;
+      for (int u = 0; u < U; u++) F::template Stage<D>::inv(s[u].data(), u, tw.data());
+      rfft::sfor<0, 2>([&](auto Pt) {
+        for (int u = 0; u < U; u++)
+          F::template Stage<D>::template put_in_part<Pt>(sbuf.data(), s[u].data(), u);
+        for (int u = 0; u < U; u++)
+          F::template Stage<D - 1>::template get_out_part<Pt>(sbuf.data(), s[u].data(), u);
+      });
+    });
+    for (int u = 0; u < U; u++) F::template Stage<0>::inv(s[u].data(), u, tw.data());
+    rfft::sfor<0, 2>([&](auto Pt) {
+      for (int u = 0; u < U; u++) F::template Stage<0>::template put_in_part<Pt>(sbuf.data(), s[u].data(), u);
+      for (int u = 0; u < U; u++) F::template get_f1_part<Pt>(sbuf.data(), s[u].data(), u);
+    });
+    for (int u = 0; u < U; u++) {
+      if (half) F::template f1_inv<true>(s[u].data(), u, tw.data());
+      else F::template f1_inv<false>(s[u].data(), u, tw.data());
+      for (int b = 0; b < (half ? V / 2 : V); b++)
+        ok_s = ok_s && s[u][b].x == r[u][b].x && s[u][b].y == r[u][b].y;
+    }
+    std::printf("%-28s %s split exchange == whole: %s\n", name, half ? "half" : "full",
+                ok_s ? "ok" : "FAIL");
+    if (!ok_f || !ok_i || !ok_s) fails++;
   }
 };
 

@@ -572,6 +572,37 @@ def test_blocked_column_variants_equal_one_column_kernel_bitwise(nx, ny, padded,
     assert ran >= 2
 
 
+WAVE_SHAPES = [(1536, 512, True), (512, 512, True), (40, 512, True), (512, 1024, False),
+               (96, 1024, False)]
+
+
+@pytest.mark.parametrize("fp32", [0, 1])
+@pytest.mark.parametrize("nx,ny,padded", WAVE_SHAPES,
+                         ids=[f"{a}x{b}{'_padded' if p else ''}" for a, b, p in WAVE_SHAPES])
+def test_wave_column_variants_equal_one_column_kernel_bitwise(nx, ny, padded, fp32):
+    """Column variants 5 and 6 run each 1024-point column transform on one
+    wave (16 values per lane, exchanges split into real and imaginary
+    parts, x and y waves swapping their spectra through LDS) with the
+    arithmetic of variant 1's one-column kernel: the field must be the same
+    bits.  The shapes leave partial x/y and z workgroups (Hx odd, and 21 or
+    49 columns for the narrow grids)."""
+    L = la.lib()
+    n3 = 3 * nx * ny
+    y = np.random.default_rng(nx * 11 + ny).standard_normal(n3)
+    d = (L.Demag_InitPadded if padded else L.Demag_Init)(nx, ny, 1.0, 4.0)
+    assert L.Demag_SetPrecision(d, fp32) == 0
+    rv = 0 if L.Demag_SetFftKernels(d, 0, 1) == 0 else -1   # rows without a register plan
+    assert L.Demag_SetFftKernels(d, rv, 1) == 0
+    h1 = _demag_field(L, d, n3, y)
+    for cv in (5, 6):
+        assert L.Demag_SetFftKernels(d, rv, cv) == 0, cv
+        h = _demag_field(L, d, n3, y)
+        print(f"[gpu] {nx}x{ny} padded={padded} fp32={fp32} column variant {cv}: "
+              f"max|dh| {np.max(np.abs(h - h1)):.1e}", flush=True)
+        assert np.array_equal(h, h1), cv
+    L.Demag_Destroy(d)
+
+
 def test_padded_tensor_spectrum_is_mirror_symmetric():
     """the zero-padded grid's tensor spectra are stored with K(Py - ky) =
     K(ky) (xx, yy, zz) and -K(ky) (xy) exactly, which the blocked column

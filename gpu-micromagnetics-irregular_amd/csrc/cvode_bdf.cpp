@@ -647,6 +647,74 @@ void form_x(Integrator* m) {
   }
 }
 
+// spgmr_lagged queued whole on the device (llgcv_fused_ops.lag_*): the
+// first pass, passes 1..maxl-1 (coefficients from device memory, skipped
+// once the solve has ended) and the last column's projection norm, each
+// followed by the device's scalar step; ONE host wait, then the same exit
+// as the host loop from the state it would have reached (the device steps
+// repeat its operations), counters included.
+int spgmr_lagged_dev(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s,
+                     double bnorm_floor) {
+  const llgcv_fused_ops& f = m->fused;
+  N_Vector* V = m->V;
+  const int l_max = m->maxl;
+  N_Vector W[2] = {m->vtemp, m->xcor};
+  form_b(m);
+  const double* coef = nullptr;
+  const int *skip = nullptr, *pskip = nullptr;
+  if (f.lag_begin(b, l_max, delta, bnorm_floor, (double)N_VGetLength(b), &coef, &skip, &pskip) != 0)
+    return kLaggedFallback;
+  if (f.arnoldi_lag(0, 1, nullptr, nullptr, b, V, W[0], s, m->gamma, m->tn, m->ycur, m->fcur,
+                    m->user_data) != 0)
+    return kLaggedFallback;
+  for (int l = 1; l < l_max; l++)
+    if (f.arnoldi_lag(l, 0, coef, skip, W[(l - 1) & 1], V, W[l & 1], s, m->gamma, m->tn,
+                      m->ycur, m->fcur, m->user_data) != 0)
+      return kLaggedFallback;
+  if (f.lag_projnorm(W[(l_max - 1) & 1], l_max, V) != 0) return kLaggedFallback;
+  llgcv_lag_result res;
+  if (f.lag_wait(b, &res) != 0) return kLaggedFallback;
+  m->nps += 2 + res.passes;
+  m->njtimes += 1 + res.passes;
+  m->ls_iters += res.iters;
+  if (res.status == LLGCV_LAG_BNORM) return kBnormSmall;
+  m->ls_resnorm = res.rho;
+  switch (res.status) {
+    case LLGCV_LAG_ZERO:
+      m->pend_b.on = false;   // x overwrites b
+      N_VConst(0.0, x);
+      return LS_SUCCESS;
+    case LLGCV_LAG_FALLBACK: return kLaggedFallback;
+    case LLGCV_LAG_QRFACT: return LS_QRFACT_FAIL;
+    case LLGCV_LAG_CONVERGED:
+    case LLGCV_LAG_MAXL: break;
+    default: return kLaggedFallback;
+  }
+  const int krydim = res.krydim;
+  for (int i = 0; i <= l_max; i++)
+    for (int j = 0; j < l_max; j++) m->Hes[i][j] = res.H[i][j];
+  for (int k = 0; k < 2 * l_max; k++) m->givens[k] = res.q[k];
+  const bool converged = res.status == LLGCV_LAG_CONVERGED;
+  m->yg[0] = res.r_norm;
+  for (int i = 1; i <= krydim; i++) m->yg[i] = 0.0;
+  if (qr_sol(krydim, m->Hes, m->givens, m->yg) != 0) return LS_QRSOL_FAIL;
+  if (converged || res.rho < res.r_norm) {
+    auto& p = m->pend_x;
+    p.k = krydim;
+    for (int k = 0; k < krydim; k++) {
+      p.c[k] = m->yg[k] / res.nu[k];
+      p.X[k] = V[k];
+    }
+    p.s = s;
+    p.dst = x;
+    p.on = true;
+    m->pend_b.on = false;   // x overwrites b
+    if (!f.newtonupd_comb) form_x(m);
+    return converged ? LS_SUCCESS : LS_RES_REDUCED;
+  }
+  return LS_CONV_FAIL;
+}
+
 int spgmr_lagged(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s,
                  double bnorm_floor) {
   N_Vector* V = m->V;
@@ -685,6 +753,8 @@ int spgmr_lagged(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s
   // step 0: u_0 = s .* P^-1 b (the initial residual), w_0 = A u_0; a
   // Newton residual not yet formed is formed where the pass reads it
   const bool lazy_b = m->pend_b.on && m->pend_b.dst == b && f.arnoldi_first;
+  if (!lazy_b && !pipe && f.lag_begin && f.arnoldi_lag && f.lag_projnorm && f.lag_wait)
+    return spgmr_lagged_dev(m, x, b, delta, s, bnorm_floor);
   if (lazy_b) {
     if (f.arnoldi_first(3, m->pend_b.c, m->pend_b.X, V, W[0], s, m->gamma, m->tn, m->ycur,
                         m->fcur, m->user_data, r) != 0)

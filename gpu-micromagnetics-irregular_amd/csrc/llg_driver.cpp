@@ -30,6 +30,30 @@ struct Run {
 
 }  // namespace
 
+namespace {
+// llgcv_lag_result and NVHipLagResult hold the same fields
+int lag_wait(N_Vector v, llgcv_lag_result* out) {
+  NVHipLagResult r;
+  if (NVHip_LagWait(v, &r) != 0) return -1;
+  static_assert(LLGCV_LAG_DIM == NVHIP_FUSED_MAX, "lagged SPGMR state dimensions");
+  static_assert(LLGCV_LAG_CONVERGED == NVHIP_LAG_CONVERGED && LLGCV_LAG_MAXL == NVHIP_LAG_MAXL &&
+                    LLGCV_LAG_ZERO == NVHIP_LAG_ZERO && LLGCV_LAG_BNORM == NVHIP_LAG_BNORM &&
+                    LLGCV_LAG_FALLBACK == NVHIP_LAG_FALLBACK && LLGCV_LAG_QRFACT == NVHIP_LAG_QRFACT,
+                "lagged SPGMR status codes");
+  out->status = r.status;
+  out->krydim = r.krydim;
+  out->iters = r.iters;
+  out->passes = r.passes;
+  out->r_norm = r.r_norm;
+  out->rho = r.rho;
+  for (int i = 0; i < LLGCV_LAG_DIM; i++) out->nu[i] = r.nu[i];
+  for (int i = 0; i < LLGCV_LAG_DIM; i++)
+    for (int j = 0; j < LLGCV_LAG_DIM; j++) out->H[i][j] = r.H[i][j];
+  for (int k = 0; k < 2 * LLGCV_LAG_DIM; k++) out->q[k] = r.q[k];
+  return 0;
+}
+}  // namespace
+
 extern "C" {
 
 void LLG_DefaultRunOpts(llg_run_opts* o) {
@@ -42,7 +66,7 @@ void LLG_DefaultRunOpts(llg_run_opts* o) {
   o->gstype = 0;        // auto: classical iff neq < 500000 (2d_i4/2d_fft.cu:645-650)
   o->max_order = 5;
   o->mxstep = 500;
-  o->fused = 791;
+  o->fused = 1815;
   o->use_jtv = 1;
   o->use_prec = 1;
   o->pretype = SUN_PREC_LEFT;
@@ -116,6 +140,13 @@ void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_hos
         fo.result_wait = NVHip_ReduceWait;
         fo.gate_open = NVHip_GateOpen;
         fo.gate_release = NVHip_GateRelease;
+      }
+      // bit 10: the lagged SPGMR queued whole, its scalar steps on the device
+      if ((opts->fused & 1040) == 1040) {
+        fo.lag_begin = NVHip_LagBegin;
+        fo.arnoldi_lag = LLG_ArnoldiLag;
+        fo.lag_projnorm = NVHip_LagProjNorm;
+        fo.lag_wait = lag_wait;
       }
     }
     LLGCVodeSetFusedOps(r->cv, &fo);

@@ -81,6 +81,14 @@ struct RedWs {
   int* gskip = nullptr;
   unsigned gseq = 0;
   bool gopen = false;
+  // device-driven lagged SPGMR (NVHip_Lag*): device state, pinned published
+  // result and its device alias, the current solve's number and parameters
+  struct LagDev* lag = nullptr;
+  struct LagPub* lagpub = nullptr;
+  struct LagPub* lagpub_dev = nullptr;
+  unsigned lagseq = 0;
+  int lagmaxl = 0;
+  double lagdelta = 0.0, lagbfloor = -1.0, lagnlen = 1.0;
 };
 
 // Mailbox of a gate in pinned host memory, written by the host only.
@@ -467,10 +475,10 @@ __device__ void block_reduce(int kind, double (&v)[K], double (*sm)[W]) {
 // atomic that no stale per-CU L1 line can serve; otherwise (a kernel of its
 // own) plain loads, which the hardware keeps in flight together.  Same
 // order of summation either way.
+// sum K rows of nblk partials in slot order; thread 0 ends with the sums
 template <int K, bool SAME_KERNEL, int W = kWaves>
-__device__ void finish(int kind, int nblk, const double* partials, double* out, double* dout,
-                       double (*sm)[W], unsigned seq) {
-  double v[K];
+__device__ void sum_partials(int kind, int nblk, const double* partials, double (*sm)[W],
+                             double (&v)[K]) {
 #pragma unroll
   for (int k = 0; k < K; k++) v[k] = identity(kind);
   int b = threadIdx.x;
@@ -501,6 +509,13 @@ __device__ void finish(int kind, int nblk, const double* partials, double* out, 
   }
   __syncthreads();
   block_reduce<K, W>(kind, v, sm);
+}
+
+template <int K, bool SAME_KERNEL, int W = kWaves>
+__device__ void finish(int kind, int nblk, const double* partials, double* out, double* dout,
+                       double (*sm)[W], unsigned seq) {
+  double v[K];
+  sum_partials<K, SAME_KERNEL, W>(kind, nblk, partials, sm, v);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; k++) out[k] = v[k];
@@ -1951,6 +1966,343 @@ int NVHip_SpgmrSolve(NVHipDevOp op, void* op_data, N_Vector t0, N_Vector* V, int
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// -------------------------------------- device-driven lagged SPGMR
+// The scalar step of the integrator's SPGMR with lagged normalisation (its
+// host loop in cvode_bdf.cpp, spgmr_lagged), on thread 0 of the kernel
+// that sums a pass's partials, with the same operations in the same order
+// (no contraction in this file; FP64 division and square root correctly
+// rounded), so the Hessenberg matrix, rotations and decisions are the host
+// loop's bit for bit.  After pass l the sums are r[0] = |u_l|^2,
+// r[1 + j] = <w_l, u_j> (j <= l), r[2 + l] = |w_l|^2 (and for pass 0 the
+// right-hand side's r[3] = |b s|^2).
+constexpr double kLagGsFactor = 1000.0;   // classical GS re-orthogonalisation test
+constexpr double kLagEstFloor = 1e-4;     // the Pythagorean estimate's floor
+constexpr int kLagMax = 8;
+enum LagRun { LAG_RUNNING = -1 };
+
+struct LagDev {
+  double H[kMaxK][kMaxK];
+  double q[2 * kMaxK];
+  double nu[kMaxK];
+  double coef[kMaxK + 1];   // the next pass's a, b_0 .. b_l
+  double r_norm, rotprod, rho, vk;
+  int done, pskip, status, krydim, iters, passes;
+};
+struct LagPub {
+  NVHipLagResult r;
+  unsigned seq;
+};
+
+__device__ inline double lag_givens_sine(double t1, double t2) {
+  if (t2 == 0.0) return 0.0;
+  if (fabs(t2) >= fabs(t1)) {
+    const double t3 = t1 / t2;
+    return -1.0 / sqrt(1.0 + t3 * t3);
+  }
+  const double t3 = t2 / t1;
+  return -(1.0 / sqrt(1.0 + t3 * t3)) * t3;
+}
+
+__device__ inline void lag_end(LagDev* st, int status) {
+  st->status = status;
+  st->done = 1;
+}
+
+// column l complete with h_{l+1,l} = nu_next: re-orthogonalisation test,
+// its Givens rotation (qr_fact of the new column), residual estimate and
+// convergence; true when the solve has ended
+__device__ bool lag_finalize(LagDev* st, int l, double nu_next, int maxl, double delta) {
+  if ((kLagGsFactor * nu_next) < st->vk) {
+    lag_end(st, NVHIP_LAG_FALLBACK);
+    return true;
+  }
+  st->nu[l + 1] = nu_next;
+  st->H[l + 1][l] = nu_next;
+  for (int k = 0; k < l; k++) {
+    const double t1 = st->H[k][l], t2 = st->H[k + 1][l], c = st->q[2 * k], s = st->q[2 * k + 1];
+    st->H[k][l] = c * t1 - s * t2;
+    st->H[k + 1][l] = s * t1 + c * t2;
+  }
+  double c, s;
+  const double t1 = st->H[l][l], t2 = st->H[l + 1][l];
+  sp_rotation(t1, t2, c, s);
+  st->q[2 * l] = c;
+  st->q[2 * l + 1] = s;
+  if ((st->H[l][l] = c * t1 - s * t2) == 0.0) {
+    lag_end(st, NVHIP_LAG_QRFACT);
+    return true;
+  }
+  st->rotprod *= st->q[2 * l + 1];
+  st->rho = fabs(st->rotprod * st->r_norm);
+  if (st->rho <= delta) {
+    lag_end(st, NVHIP_LAG_CONVERGED);
+    return true;
+  }
+  if (l + 1 == maxl) {
+    lag_end(st, NVHIP_LAG_MAXL);
+    return true;
+  }
+  return false;
+}
+
+// column l from pass l's sums r: Hessenberg entries, the next pass's
+// coefficients, |A V_l| and the Pythagorean estimate of h_{l+1,l}; a column
+// the estimate shows converged (or the last one, when the estimate is well
+// conditioned) is completed here; true when the solve has ended
+__device__ bool lag_column(LagDev* st, int l, const double* r, int maxl, double delta) {
+  st->iters++;
+  st->krydim = l + 1;
+  const double inv = 1.0 / st->nu[l];
+  double hh = 0.0;
+  for (int j = 0; j <= l; j++) {
+    st->H[j][l] = (r[1 + j] * inv) / st->nu[j];
+    hh += st->H[j][l] * st->H[j][l];
+    st->coef[1 + j] = st->H[j][l] / st->nu[j];
+  }
+  st->coef[0] = inv;
+  const double vk_norm = sqrt(r[2 + l]) * inv;
+  st->vk = vk_norm;
+  const double est = vk_norm * vk_norm - hh;
+  bool use_est = false;
+  if (est > kLagEstFloor * vk_norm * vk_norm) {
+    if (l + 1 == maxl) {
+      use_est = true;
+    } else {
+      double t[kLagMax + 1];
+      for (int j = 0; j <= l; j++) t[j] = st->H[j][l];
+      for (int k = 0; k < l; k++) {
+        const double c = st->q[2 * k], sn = st->q[2 * k + 1];
+        const double t1 = t[k], t2 = t[k + 1];
+        t[k] = c * t1 - sn * t2;
+        t[k + 1] = sn * t1 + c * t2;
+      }
+      const double sn = lag_givens_sine(t[l], sqrt(est));
+      use_est = fabs(st->rotprod * sn * st->r_norm) <= delta;
+    }
+  }
+  if (use_est) {
+    // the trial rotation is the one the completion forms, so an estimate
+    // that shows convergence converges; anything else goes to the host
+    if (!lag_finalize(st, l, sqrt(est), maxl, delta)) lag_end(st, NVHIP_LAG_FALLBACK);
+    return true;
+  }
+  if (l + 1 == maxl) st->pskip = 0;   // the last column's projection norm
+  return false;
+}
+
+__device__ void lag_publish(const LagDev* st, LagPub* pub, unsigned seq, int maxl) {
+  NVHipLagResult& o = pub->r;
+  o.status = st->status;
+  o.krydim = st->krydim;
+  o.iters = st->iters;
+  o.passes = st->passes;
+  o.r_norm = st->r_norm;
+  o.rho = st->rho;
+  for (int i = 0; i <= maxl; i++) o.nu[i] = st->nu[i];
+  for (int i = 0; i <= maxl; i++)
+    for (int j = 0; j < maxl; j++) o.H[i][j] = st->H[i][j];
+  for (int k = 0; k < 2 * maxl; k++) o.q[k] = st->q[k];
+  __threadfence_system();
+  __hip_atomic_store(&pub->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// one step of the solve on the state st with the sums r; true when the
+// solve has ended (the state is then to be published)
+__device__ bool lag_run(LagDev* st, const double* r, int p, int mode, int maxl, double delta,
+                        double bfloor, double nlen) {
+  if (mode == 1) {
+    st->pskip = 1;
+    lag_finalize(st, maxl - 1, sqrt(r[0]), maxl, delta);
+    return true;
+  }
+  if (p == 0) {
+    st->done = 0;
+    st->pskip = 1;
+    st->status = LAG_RUNNING;
+    st->iters = 0;
+    st->passes = 0;
+    st->krydim = 0;
+    st->rotprod = 1.0;
+    st->r_norm = 0.0;
+    st->rho = 0.0;
+    if (bfloor >= 0.0 && sqrt(r[3] / nlen) <= bfloor) {
+      lag_end(st, NVHIP_LAG_BNORM);
+      return true;
+    }
+    const double r_norm = sqrt(r[0]);
+    st->r_norm = r_norm;
+    st->rho = r_norm;
+    if (r_norm <= delta) {
+      lag_end(st, NVHIP_LAG_ZERO);
+      return true;
+    }
+    for (int i = 0; i <= maxl; i++)
+      for (int j = 0; j < maxl; j++) st->H[i][j] = 0.0;
+    st->nu[0] = r_norm;
+    return lag_column(st, 0, r, maxl, delta);
+  }
+  st->passes++;
+  return lag_finalize(st, p - 1, sqrt(r[0]), maxl, delta) || lag_column(st, p, r, maxl, delta);
+}
+
+
+// mode 0: the sums of pass p (p = 0 starts the solve); mode 1: the last
+// column's projection norm |u|^2 (one sum).  The state is copied to LDS by
+// all threads while the partials are summed, stepped there by thread 0
+// (its dependent accesses then cost LDS latency, not memory round trips)
+// and copied back.
+template <int K>
+__global__ __launch_bounds__(kFinishBlock) void k_lag_step(int nblk, const double* partials, int p,
+                                                           int mode, LagDev* gst, LagPub* pub,
+                                                           unsigned seq, int maxl, double delta,
+                                                           double bfloor, double nlen) {
+  if (mode == 1 ? gst->pskip != 0 : (p > 0 && gst->done != 0)) return;   // uniform
+  static_assert(sizeof(LagDev) % sizeof(double) == 0, "LagDev copied as doubles");
+  constexpr int NW = (int)(sizeof(LagDev) / sizeof(double));
+  __shared__ double lst[NW];
+  __shared__ double sm[K][kFinishBlock / 64];
+  for (int i = threadIdx.x; i < NW; i += blockDim.x) lst[i] = reinterpret_cast<const double*>(gst)[i];
+  double r[K];
+  sum_partials<K, false, kFinishBlock / 64>(RED_SUM, nblk, partials, sm, r);   // barriers inside
+  LagDev* st = reinterpret_cast<LagDev*>(lst);
+  bool ended = false;
+  if (threadIdx.x == 0) ended = lag_run(st, r, p, mode, maxl, delta, bfloor, nlen);
+  __syncthreads();
+  for (int i = threadIdx.x; i < NW; i += blockDim.x) reinterpret_cast<double*>(gst)[i] = lst[i];
+  if (threadIdx.x == 0 && ended) lag_publish(st, pub, seq, maxl);
+}
+
+// u = a w - sum_j b_j U_j with (a, b) read from device memory, |u|^2
+// (rounded as FProjSq rounds it)
+struct FProjSqDev {
+  int nb; const double* coef; const double* w; PtrPack U;
+  __device__ double operator()(sunindextype i) const {
+    double u = coef[0] * w[i];
+    for (int j = 0; j < nb; j++) u = fma(-coef[1 + j], U.p[j][i], u);
+    return u * u;
+  }
+};
+
+void lag_step_dispatch(int K, hipStream_t s, int nblk, const RedWs* w, int p, int mode) {
+  switch (K) {
+#define NVHIP_LAG_CASE(k)                                                                     \
+    case k:                                                                                   \
+      hipLaunchKernelGGL(k_lag_step<k>, dim3(1), dim3(kFinishBlock), 0, s, nblk, w->partials, \
+                         p, mode, w->lag, w->lagpub_dev, w->lagseq, w->lagmaxl, w->lagdelta,  \
+                         w->lagbfloor, w->lagnlen);                                           \
+      break;
+    NVHIP_LAG_CASE(1) NVHIP_LAG_CASE(3) NVHIP_LAG_CASE(4) NVHIP_LAG_CASE(5)
+    NVHIP_LAG_CASE(6) NVHIP_LAG_CASE(7) NVHIP_LAG_CASE(8) NVHIP_LAG_CASE(9)
+    default: NVHIP_LAG_CASE(10)
+#undef NVHIP_LAG_CASE
+  }
+}
+
+}  // namespace
+
+static bool is_hip(N_Vector v);
+
+extern "C" {
+
+int NVHip_LagBegin(N_Vector v, int maxl, sunrealtype delta, sunrealtype bnorm_floor,
+                   sunrealtype nlen, const double** coef_dev, const int** skip_dev,
+                   const int** pskip_dev) {
+  if (!v || !coef_dev || !skip_dev || !pskip_dev || !is_hip(v) || maxl < 1 ||
+      maxl > kLagMax || g_red_mode == 0 || !(nlen > 0.0))
+    return -1;
+  RedWs* w = workspace(S(v));
+  if (!w->lag) {
+    void* hb = nullptr;
+    if (hipMalloc(&w->lag, sizeof(LagDev)) != hipSuccess) {
+      w->lag = nullptr;
+      return -1;
+    }
+    if (hipHostMalloc(&hb, sizeof(LagPub), hipHostMallocMapped | hipHostMallocCoherent) !=
+            hipSuccess ||
+        hipHostGetDevicePointer((void**)&w->lagpub_dev, hb, 0) != hipSuccess) {
+      if (hb) hipHostFree(hb);
+      hipFree(w->lag);
+      w->lag = nullptr;
+      return -1;
+    }
+    w->lagpub = static_cast<LagPub*>(hb);
+    std::memset(hb, 0, sizeof(LagPub));
+  }
+  w->lagseq++;
+  w->lagmaxl = maxl;
+  w->lagdelta = delta;
+  w->lagbfloor = bnorm_floor;
+  w->lagnlen = nlen;
+  *coef_dev = w->lag->coef;
+  *skip_dev = &w->lag->done;
+  *pskip_dev = &w->lag->pskip;
+  return 0;
+}
+
+int NVHip_LagStep(N_Vector v, int K, int nblk, int p) {
+  if (!v || !is_hip(v) || K < 3 || K > 10 || nblk < 1 || nblk * K > kMaxRedBlocks * kMaxK ||
+      p < 0)
+    return -1;
+  RedWs* w = workspace(S(v));
+  if (!w->lag || p >= w->lagmaxl || K != (p == 0 ? 4 : p + 3)) return -1;
+  lag_step_dispatch(K, S(v), nblk, w, p, 0);
+  check_launch("lag step");
+  return 0;
+}
+
+int NVHip_LagProjNorm(N_Vector wv, int nb, N_Vector* U) {
+  if (!wv || !U || !is_hip(wv)) return -1;
+  hipStream_t s = S(wv);
+  RedWs* w = workspace(s);
+  if (!w->lag || nb != w->lagmaxl) return -1;
+  FProjSqDev f{};
+  f.nb = nb;
+  f.coef = w->lag->coef;
+  f.w = D(wv);
+  for (int j = 0; j < nb; j++) {
+    if (!U[j] || LEN(U[j]) != LEN(wv)) return -1;
+    f.U.p[j] = D(U[j]);
+  }
+  const sunindextype n = LEN(wv);
+  const int g = red_grid(n);
+  hipLaunchKernelGGL(k_reduce1<FProjSqDev>, dim3(g), dim3(kBlock), 0, s, RED_SUM, f, n,
+                     w->partials, (unsigned*)nullptr, w->host_dev, (double*)nullptr, 0u,
+                     (const int*)&w->lag->pskip);
+  lag_step_dispatch(1, s, g, w, 0, 1);
+  check_launch("lag projection norm");
+  return 0;
+}
+
+int NVHip_LagWait(N_Vector v, NVHipLagResult* res) {
+  if (!v || !res || !is_hip(v)) return -1;
+  hipStream_t s = S(v);
+  RedWs* w = workspace(s);
+  if (!w->lag) return -1;
+  g_syncs++;
+  const unsigned* word = &w->lagpub->seq;
+  bool got = false;
+  if (g_sync_mode == 1 || g_sync_mode == 2) {
+    auto t0 = std::chrono::steady_clock::now();
+    for (unsigned it = 0;; it++) {
+      if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == w->lagseq) { got = true; break; }
+      if (g_sync_mode == 2 && (it & 15u) == 15u) std::this_thread::yield();
+      if ((it & 4095u) == 4095u &&
+          std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2))
+        break;
+    }
+  }
+  if (!got) {
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    if (__atomic_load_n(word, __ATOMIC_ACQUIRE) != w->lagseq) return -1;
+  }
+  std::memcpy(res, &w->lagpub->r, sizeof(NVHipLagResult));
+  return 0;
+}
+
 int NVHip_GSProjectScaled(N_Vector z, int nvec, N_Vector* Y, int sidx, sunrealtype sc,
                           sunrealtype* dots, sunrealtype* zz) {
   if (!z || nvec < 2 || nvec > kMaxK || !dots || !zz || Y[nvec - 1] != z || sidx < 0 ||
@@ -2118,6 +2470,10 @@ void FusedNVec_FreePool(void) {
       hipHostFree(kv.second->gbox);
       hipFree(kv.second->gcoef);
       hipFree(kv.second->gskip);
+    }
+    if (kv.second->lag) {
+      hipFree(kv.second->lag);
+      hipHostFree(kv.second->lagpub);
     }
     delete kv.second;
   }

@@ -181,6 +181,11 @@ SIGNATURES = {
     "NVHip_ReduceWait": (C.c_int, [VP, C.c_uint, C.c_int, DP]),
     "NVHip_GateOpen": (C.c_int, [VP, C.POINTER(VP), C.POINTER(VP)]),
     "NVHip_GateRelease": (C.c_int, [VP, C.c_int, DP, C.c_int]),
+    "NVHip_LagBegin": (C.c_int, [VP, C.c_int, C.c_double, C.c_double, C.c_double,
+                                 C.POINTER(VP), C.POINTER(VP), C.POINTER(VP)]),
+    "NVHip_LagStep": (C.c_int, [VP, C.c_int, C.c_int, C.c_int]),
+    "NVHip_LagProjNorm": (C.c_int, [VP, C.c_int, C.POINTER(VP)]),
+    "NVHip_LagWait": (C.c_int, [VP, VP]),
     "NVHip_ProjNorm": (C.c_int, [C.c_int, C.c_double, DP, VP, C.POINTER(VP), VP, DP]),
     "NVHip_NewtonUpdate": (C.c_int, [C.c_int, VP, VP, VP, VP, VP, DP, DP]),
     "NVHip_NewtonUpdateComb": (C.c_int, [VP, C.c_int, DP, C.POINTER(VP), VP, VP, VP, VP, DP]),
@@ -237,6 +242,8 @@ SIGNATURES = {
     "LLG_ArnoldiQueue": (C.c_int, [C.c_int, C.c_int, C.c_double, DP, VP, VP, VP, C.POINTER(VP),
                                    VP, VP, C.c_double, C.c_double, VP, VP, VP,
                                    C.POINTER(C.c_uint)]),
+    "LLG_ArnoldiLag": (C.c_int, [C.c_int, C.c_int, VP, VP, VP, C.POINTER(VP), VP, VP,
+                                 C.c_double, C.c_double, VP, VP, VP]),
     "LLG_ArnoldiFirst": (C.c_int, [C.c_int, DP, C.POINTER(VP), C.POINTER(VP), VP, VP,
                                    C.c_double, C.c_double, VP, VP, VP, DP]),
     "PrecondSetup": (C.c_int, [C.c_double, VP, VP, C.c_int, C.POINTER(C.c_int), C.c_double, VP]),

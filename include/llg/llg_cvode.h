@@ -84,6 +84,26 @@ typedef struct llgcv_stats {
  *                        (the registered jtimes and psolve callbacks, fused)
  *   gsscaled(z, k, Y, i, c, d, zz): Y_i = c Y_i, then the gsproject
  *                        projection of z (already formed) against Y      */
+/* State a device-driven lagged SPGMR solve ends in (llgcv_fused_ops.lag_*):
+ * status LLGCV_LAG_*, the Hessenberg columns formed, Krylov iterations
+ * started, passes run after the first, |u_0|, the final residual
+ * estimate, |u_l|, the Hessenberg matrix (rows 0..maxl) and the Givens
+ * rotations (c, s) -- the values the host loop would hold. */
+#define LLGCV_LAG_CONVERGED 0
+#define LLGCV_LAG_MAXL 1
+#define LLGCV_LAG_ZERO 2
+#define LLGCV_LAG_BNORM 3
+#define LLGCV_LAG_FALLBACK 4
+#define LLGCV_LAG_QRFACT 5
+#define LLGCV_LAG_DIM 16
+typedef struct llgcv_lag_result {
+  int status, krydim, iters, passes;
+  sunrealtype r_norm, rho;
+  sunrealtype nu[LLGCV_LAG_DIM];
+  sunrealtype H[LLGCV_LAG_DIM][LLGCV_LAG_DIM];
+  sunrealtype q[2 * LLGCV_LAG_DIM];
+} llgcv_lag_result;
+
 typedef struct llgcv_fused_ops {
   int (*pascal)(int q, N_Vector* zn, sunrealtype sign);
   int (*ewt)(N_Vector y, sunrealtype rtol, sunrealtype atol, N_Vector abstol, N_Vector w);
@@ -168,6 +188,21 @@ typedef struct llgcv_fused_ops {
                           N_Vector abstol, N_Vector ewt);
   int (*wrms_pair_comp)(N_Vector x, sunrealtype cx, N_Vector xc, sunrealtype a, N_Vector y,
                         sunrealtype b, N_Vector z, N_Vector w, sunrealtype* nrm);
+  /* SPGMR with lagged normalisation driven by the device: with all four
+   * set (and the lagged form in use, its first pass not formed in place),
+   * the integrator queues a solve whole -- lag_begin, the first pass and
+   * passes 1..maxl-1 by arnoldi_lag (as LLG_ArnoldiLag), the last column's
+   * projection norm by lag_projnorm -- and waits once (lag_wait) for the
+   * state the host loop would reach; the scalar steps run on the device
+   * (as NVHip_Lag*), so the solve is the host loop's bit for bit. */
+  int (*lag_begin)(N_Vector v, int maxl, sunrealtype delta, sunrealtype bnorm_floor,
+                   sunrealtype nlen, const double** coef_dev, const int** skip_dev,
+                   const int** pskip_dev);
+  int (*arnoldi_lag)(int l, int pre, const double* coef_dev, const int* skip_dev, N_Vector win,
+                     N_Vector* U, N_Vector wout, N_Vector s, sunrealtype gamma, sunrealtype t,
+                     N_Vector y, N_Vector fy, void* user_data);
+  int (*lag_projnorm)(N_Vector w, int nb, N_Vector* U);
+  int (*lag_wait)(N_Vector v, llgcv_lag_result* res);
 } llgcv_fused_ops;
 
 void* LLGCVodeCreate(int lmm, SUNContext sunctx);

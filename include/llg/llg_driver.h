@@ -53,9 +53,13 @@ typedef struct llg_run_opts {
                          RHS pass (LLG_RhsResidual; f is not stored);
                          bit 9 (with bit 1): each step's completion
                          zn += l acor left to the next predictor's pass
-                         (NVHip_PredictComplete); default 791 (bits 0, 1,
-                         2, 4, 8, 9).  Bits 1 to 3, 8 and 9 change no
-                         value. */
+                         (NVHip_PredictComplete); bit 10 (with bit 4, its
+                         first pass not formed in place): each lagged
+                         SPGMR solve queued whole, its scalar steps on
+                         the device (NVHip_Lag*), one host wait per
+                         solve; default 1815 (bits 0, 1, 2, 4, 8, 9, 10).
+                         Bits 1 to 3, 8, 9 and 10 change no value (bit 10
+                         none of bit 4's). */
   long   mxstep;      /* max steps per CVode call (CVODE default 500) */
   int    use_jtv;     /* analytic Jv (else difference quotient) */
   int    use_prec;    /* block-Jacobi preconditioner */

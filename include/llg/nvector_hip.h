@@ -230,6 +230,51 @@ int NVHip_SpgmrSolve(NVHipDevOp op, void* op_data, N_Vector t0, N_Vector* V, int
                      N_Vector sv, N_Vector x, sunrealtype delta, int* status, int* iters,
                      sunrealtype* r_norm, sunrealtype* resnorm);
 
+/* Device-driven SPGMR with lagged normalisation (no counterpart in the
+ * reference): the scalar step of each Krylov iteration of the integrator's
+ * lagged SPGMR -- the Hessenberg column from a pass's sums, the Pythagorean
+ * estimate of h_{l+1,l} and its trial rotation, the re-orthogonalisation
+ * test, the Givens rotation, the residual estimate, the convergence test
+ * and the next pass's projection coefficients -- runs on the device, on
+ * thread 0 of the kernel that sums the pass's partials, with the host
+ * loop's operations in its order.  A solve is queued whole: pass 0 (the
+ * preconditioned right-hand side) with NVHip_LagStep(p = 0), passes
+ * p = 1 .. maxl-1 reading their coefficients from *coef_dev and skipped
+ * when *skip_dev is set, each with NVHip_LagStep(p), and the projection
+ * norm of the last column (NVHip_LagProjNorm, skipped unless needed); the
+ * host waits once (NVHip_LagWait) for the state the solve ends in.
+ *   NVHip_LagBegin: v's stream; maxl <= 8; delta the residual tolerance;
+ *     bnorm_floor (< 0: none) compared with sqrt(|b s|^2 / nlen); returns
+ *     the device coefficient array (a, b_0 .. b_l) and the two skip flags.
+ *   NVHip_LagStep: sums the K partials of pass p over nblk workgroups
+ *     (NVHip_ReducePartials) in workgroup order and steps the state.
+ *   NVHip_LagProjNorm: u = a w - sum_{j<nb} b_j U_j from the device
+ *     coefficients, |u|^2 (u is not stored), then the last column's step.
+ * One solve per stream at a time; returns 0 or -1. */
+#define NVHIP_LAG_CONVERGED 0
+#define NVHIP_LAG_MAXL 1
+#define NVHIP_LAG_ZERO 2
+#define NVHIP_LAG_BNORM 3
+#define NVHIP_LAG_FALLBACK 4
+#define NVHIP_LAG_QRFACT 5
+typedef struct {
+  int status;     /* NVHIP_LAG_* */
+  int krydim;     /* Hessenberg columns formed */
+  int iters;      /* Krylov iterations started */
+  int passes;     /* passes run after pass 0 */
+  double r_norm;  /* |u_0|, the preconditioned initial residual */
+  double rho;     /* the final residual estimate */
+  double nu[NVHIP_FUSED_MAX];                    /* |u_l| */
+  double H[NVHIP_FUSED_MAX][NVHIP_FUSED_MAX];    /* Hessenberg, rows 0..maxl */
+  double q[2 * NVHIP_FUSED_MAX];                 /* Givens rotations (c, s) */
+} NVHipLagResult;
+int NVHip_LagBegin(N_Vector v, int maxl, sunrealtype delta, sunrealtype bnorm_floor,
+                   sunrealtype nlen, const double** coef_dev, const int** skip_dev,
+                   const int** pskip_dev);
+int NVHip_LagStep(N_Vector v, int K, int nblk, int p);
+int NVHip_LagProjNorm(N_Vector w, int nb, N_Vector* U);
+int NVHip_LagWait(N_Vector v, NVHipLagResult* res);
+
 /* Reductions for kernels outside this file (no counterpart in the
  * reference): a kernel queued on v's stream leaves K partial sums per
  * workgroup at partials[k*nblk + b] (nblk*K <= *capacity, from

@@ -61,7 +61,43 @@ def test_fused_integrator_kernels_are_bitwise_neutral(gstype, case):
         assert np.array_equal(ya, yb), fused
 
 
-@pytest.mark.parametrize("fused", [7, 23, 55, 215, 279, 791])
+LAG_CASES = [("square_hole_48x32", lambda: _case()),
+             ("square_hole_padded_128x64", lambda: _case(ng=128, ny=64)),
+             ("i1_periodic_64x64", lambda: (la.preset("i1", ng=64, ny=64, demag=2.0),
+                                            la.Geometry.full(64, 64),
+                                            la.Geometry.full(64, 64).ic_stripes()))]
+
+
+@pytest.mark.parametrize("maxl", [5, 1])
+@pytest.mark.parametrize("name,make", LAG_CASES, ids=[c[0] for c in LAG_CASES])
+def test_device_driven_lagged_spgmr_is_bitwise_neutral(name, make, maxl):
+    """fused 1815 (791 plus bit 10): every lagged SPGMR solve queued whole,
+    the Hessenberg column, Pythagorean estimate, Givens rotation,
+    convergence test and next coefficients computed on the device, one
+    host wait per solve.  The device repeats the host loop's operations,
+    so trajectory and counters equal the host-stepped lagged solve's (791)
+    bit for bit; the host waits drop.  maxl 1 makes every column the last:
+    its Pythagorean estimate or projection norm, and the Krylov cap."""
+    cfg, geom, y0 = make()
+    sim = la.Simulation(cfg, geom)
+    res = {}
+    for fused in (791, 1815):
+        run = sim.integrator(y0, rtol=1e-5, atol=1e-5, gstype=2, fused=fused, maxl=maxl)
+        s0 = la.lib().NVHip_NumSyncs()      # a process-wide count
+        out = run.advance(1.0)
+        res[fused] = (run.state(), out, la.lib().NVHip_NumSyncs() - s0)
+        run.close()
+    (ya, sa, wa), (yb, sb, wb) = res[791], res[1815]
+    print(f"[gpu] {name} maxl={maxl}: host-stepped nst={sa['nst']} nli={sa['nli']} "
+          f"waits={wa}; device-stepped nst={sb['nst']} nli={sb['nli']} waits={wb}", flush=True)
+    for k in ("nst", "nfe", "nni", "nli", "netf", "ncfn", "nsetups"):
+        assert sa[k] == sb[k], k
+    assert np.array_equal(ya, yb)
+    assert wb < wa
+    sim.close()
+
+
+@pytest.mark.parametrize("fused", [7, 23, 55, 215, 279, 791, 1815])
 @pytest.mark.parametrize("name,kw", [("square_hole", {}), ("i6", {"demag": 4.0}),
                                      ("plain128", {})])
 def test_trajectory_matches_oracle(name, kw, fused):
@@ -72,7 +108,8 @@ def test_trajectory_matches_oracle(name, kw, fused):
     215: the lagged solve with the Newton residual and the SPGMR solution
     formed where they are read (never written); 279: 23 with the Newton
     residual formed in the RHS pass; 791: 279 with each step's completion
-    left to the next predictor's pass"""
+    left to the next predictor's pass; 1815: 791 with each SPGMR solve
+    queued whole and stepped on the device"""
     ng, ny = (48, 32) if name != "plain128" else (32, 32)
     cfg = la.preset(name, ng=ng, ny=ny, **kw)
     if name == "plain128":

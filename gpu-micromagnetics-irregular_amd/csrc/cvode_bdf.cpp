@@ -165,6 +165,7 @@ struct Integrator {
   double Hes[kMaxKrylov + 1][kMaxKrylov] = {};
   double givens[2 * kMaxKrylov] = {};
   double yg[kMaxKrylov + 1] = {};
+  int lag_pred = 64;   // passes the device-stepped SPGMR queues at first (clamped to maxl - 1)
   double gs_c[kMaxKrylov + 2] = {};
   double gs_d[kMaxKrylov + 2] = {};
   bool y_current = false;   // y == zn[0] + ycor already (fused Newton update)
@@ -664,16 +665,28 @@ int spgmr_lagged_dev(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vect
   const int *skip = nullptr, *pskip = nullptr;
   if (f.lag_begin(b, l_max, delta, bnorm_floor, (double)N_VGetLength(b), &coef, &skip, &pskip) != 0)
     return kLaggedFallback;
-  if (f.arnoldi_lag(0, 1, nullptr, nullptr, b, V, W[0], s, m->gamma, m->tn, m->ycur, m->fcur,
-                    m->user_data) != 0)
-    return kLaggedFallback;
-  for (int l = 1; l < l_max; l++)
-    if (f.arnoldi_lag(l, 0, coef, skip, W[(l - 1) & 1], V, W[l & 1], s, m->gamma, m->tn,
-                      m->ycur, m->fcur, m->user_data) != 0)
-      return kLaggedFallback;
-  if (f.lag_projnorm(W[(l_max - 1) & 1], l_max, V) != 0) return kLaggedFallback;
+  // first part: pass 0 and as many further passes as the previous solve
+  // needed plus one (a pass queued past convergence still costs its
+  // dispatch); a solve still running after them comes back with status
+  // RUNNING and gets the remaining passes and the projection norm
+  const int kq = std::min(std::max(m->lag_pred, 0), l_max - 1);
+  auto pass = [&](int l, int last) {
+    return f.arnoldi_lag(l, l == 0, last, l == 0 ? nullptr : coef, l == 0 ? nullptr : skip,
+                         l == 0 ? b : W[(l - 1) & 1], V, W[l & 1], s, m->gamma, m->tn, m->ycur,
+                         m->fcur, m->user_data);
+  };
+  for (int l = 0; l <= kq; l++)
+    if (pass(l, l == kq) != 0) return kLaggedFallback;
   llgcv_lag_result res;
   if (f.lag_wait(b, &res) != 0) return kLaggedFallback;
+  if (res.status == LLGCV_LAG_RUNNING) {
+    if (f.lag_next(b) != 0) return kLaggedFallback;
+    for (int l = kq + 1; l < l_max; l++)
+      if (pass(l, 0) != 0) return kLaggedFallback;
+    if (f.lag_projnorm(W[(l_max - 1) & 1], l_max, V) != 0) return kLaggedFallback;
+    if (f.lag_wait(b, &res) != 0) return kLaggedFallback;
+  }
+  m->lag_pred = std::min(res.passes + 1, l_max - 1);
   m->nps += 2 + res.passes;
   m->njtimes += 1 + res.passes;
   m->ls_iters += res.iters;
@@ -753,7 +766,8 @@ int spgmr_lagged(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vector s
   // step 0: u_0 = s .* P^-1 b (the initial residual), w_0 = A u_0; a
   // Newton residual not yet formed is formed where the pass reads it
   const bool lazy_b = m->pend_b.on && m->pend_b.dst == b && f.arnoldi_first;
-  if (!lazy_b && !pipe && f.lag_begin && f.arnoldi_lag && f.lag_projnorm && f.lag_wait)
+  if (!lazy_b && !pipe && f.lag_begin && f.arnoldi_lag && f.lag_projnorm && f.lag_wait &&
+      f.lag_next)
     return spgmr_lagged_dev(m, x, b, delta, s, bnorm_floor);
   if (lazy_b) {
     if (f.arnoldi_first(3, m->pend_b.c, m->pend_b.X, V, W[0], s, m->gamma, m->tn, m->ycur,

@@ -40,6 +40,7 @@ int lag_wait(N_Vector v, llgcv_lag_result* out) {
                     LLGCV_LAG_ZERO == NVHIP_LAG_ZERO && LLGCV_LAG_BNORM == NVHIP_LAG_BNORM &&
                     LLGCV_LAG_FALLBACK == NVHIP_LAG_FALLBACK && LLGCV_LAG_QRFACT == NVHIP_LAG_QRFACT,
                 "lagged SPGMR status codes");
+  static_assert(LLGCV_LAG_RUNNING == NVHIP_LAG_RUNNING, "lagged SPGMR status codes");
   out->status = r.status;
   out->krydim = r.krydim;
   out->iters = r.iters;
@@ -147,6 +148,7 @@ void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_hos
         fo.arnoldi_lag = LLG_ArnoldiLag;
         fo.lag_projnorm = NVHip_LagProjNorm;
         fo.lag_wait = lag_wait;
+        fo.lag_next = NVHip_LagNext;
       }
     }
     LLGCVodeSetFusedOps(r->cv, &fo);

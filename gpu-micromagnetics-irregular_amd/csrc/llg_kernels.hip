@@ -1381,11 +1381,12 @@ namespace {
 // queue one lagged pass (its inputs in `in`) and its finish; the kernel reads
 // in.win, in.bx, s, y and U[j < l] at halo cells other workgroups own, so
 // nothing it writes may alias them
-// lag_step: the pass's sums go to the device-driven SPGMR step
-// (NVHip_LagStep) instead of a finish the host waits for (ticket unused)
+// lag_step >= 0: the pass's sums go to the device-driven SPGMR step
+// (NVHip_LagStep, last = lag_step) instead of a finish the host waits for
+// (ticket unused)
 int queue_pass(llg_ctx* ctx, int l, bool pre, ArnIn& in, const double* coef_dev,
                const int* skip_dev, N_Vector wout, N_Vector s, sunrealtype gamma, N_Vector y,
-               unsigned* ticket, bool lag_step = false) {
+               unsigned* ticket, int lag_step = -1) {
   double* dw = N_VGetDeviceArrayPointer(wout);
   const double* ds = N_VGetDeviceArrayPointer(s);
   const double* dy = N_VGetDeviceArrayPointer(y);
@@ -1428,8 +1429,8 @@ int queue_pass(llg_ctx* ctx, int l, bool pre, ArnIn& in, const double* coef_dev,
   }
   kprof::end(kprof::JTV, st);
   if (launch_status("arnoldi") != 0) return -1;
-  if (lag_step) {
-    if (NVHip_LagStep(wout, K, nwg, l) != 0) return -1;
+  if (lag_step >= 0) {
+    if (NVHip_LagStep(wout, K, nwg, l, lag_step) != 0) return -1;
   } else if (NVHip_ReduceFinishAsync(wout, K, nwg, skip_dev, ticket) != 0) {
     return -1;
   }
@@ -1468,9 +1469,9 @@ int LLG_ArnoldiQueue(int l, int pre, sunrealtype a, const sunrealtype* b, const 
   return queue_pass(ctx, l, pre != 0, in, coef_dev, skip_dev, wout, s, gamma, y, ticket);
 }
 
-int LLG_ArnoldiLag(int l, int pre, const double* coef_dev, const int* skip_dev, N_Vector win,
-                   N_Vector* U, N_Vector wout, N_Vector s, sunrealtype gamma, sunrealtype t,
-                   N_Vector y, N_Vector fy, void* user_data) {
+int LLG_ArnoldiLag(int l, int pre, int last, const double* coef_dev, const int* skip_dev,
+                   N_Vector win, N_Vector* U, N_Vector wout, N_Vector s, sunrealtype gamma,
+                   sunrealtype t, N_Vector y, N_Vector fy, void* user_data) {
   (void)t; (void)fy;
   llg_ctx* ctx = (llg_ctx*)user_data;
   if (!pass_ok(ctx) || !U || !wout || !s || !y || !win || l < 0 || l >= kArnMax ||
@@ -1484,7 +1485,7 @@ int LLG_ArnoldiLag(int l, int pre, const double* coef_dev, const int* skip_dev, 
   for (int j = 0; j < l; j++) in.U[j] = N_VGetDeviceArrayPointer(U[j]);
   in.ul = N_VGetDeviceArrayPointer(U[l]);
   return queue_pass(ctx, l, pre != 0, in, l > 0 ? coef_dev : nullptr, l > 0 ? skip_dev : nullptr,
-                    wout, s, gamma, y, nullptr, true);
+                    wout, s, gamma, y, nullptr, last != 0 ? 1 : 0);
 }
 
 int LLG_ArnoldiFirst(int nb, const sunrealtype* c, N_Vector* X, N_Vector* U, N_Vector wout,

@@ -2151,15 +2151,18 @@ __device__ bool lag_run(LagDev* st, const double* r, int p, int mode, int maxl, 
 
 
 // mode 0: the sums of pass p (p = 0 starts the solve); mode 1: the last
-// column's projection norm |u|^2 (one sum).  The state is copied to LDS by
+// column's projection norm |u|^2 (one sum).  last: no further step of the
+// solve is queued yet (the state is published also when the solve goes
+// on, with status LAG_RUNNING).  The state is copied to LDS by
 // all threads while the partials are summed, stepped there by thread 0
 // (its dependent accesses then cost LDS latency, not memory round trips)
 // and copied back.
 template <int K>
 __global__ __launch_bounds__(kFinishBlock) void k_lag_step(int nblk, const double* partials, int p,
-                                                           int mode, LagDev* gst, LagPub* pub,
-                                                           unsigned seq, int maxl, double delta,
-                                                           double bfloor, double nlen) {
+                                                           int mode, int last, LagDev* gst,
+                                                           LagPub* pub, unsigned seq, int maxl,
+                                                           double delta, double bfloor,
+                                                           double nlen) {
   if (mode == 1 ? gst->pskip != 0 : (p > 0 && gst->done != 0)) return;   // uniform
   static_assert(sizeof(LagDev) % sizeof(double) == 0, "LagDev copied as doubles");
   constexpr int NW = (int)(sizeof(LagDev) / sizeof(double));
@@ -2173,7 +2176,8 @@ __global__ __launch_bounds__(kFinishBlock) void k_lag_step(int nblk, const doubl
   if (threadIdx.x == 0) ended = lag_run(st, r, p, mode, maxl, delta, bfloor, nlen);
   __syncthreads();
   for (int i = threadIdx.x; i < NW; i += blockDim.x) reinterpret_cast<double*>(gst)[i] = lst[i];
-  if (threadIdx.x == 0 && ended) lag_publish(st, pub, seq, maxl);
+  // a solve still running after the last step queued so far publishes too
+  if (threadIdx.x == 0 && (ended || last)) lag_publish(st, pub, seq, maxl);
 }
 
 // u = a w - sum_j b_j U_j with (a, b) read from device memory, |u|^2
@@ -2187,13 +2191,14 @@ struct FProjSqDev {
   }
 };
 
-void lag_step_dispatch(int K, hipStream_t s, int nblk, const RedWs* w, int p, int mode) {
+void lag_step_dispatch(int K, hipStream_t s, int nblk, const RedWs* w, int p, int mode,
+                       int last) {
   switch (K) {
 #define NVHIP_LAG_CASE(k)                                                                     \
     case k:                                                                                   \
       hipLaunchKernelGGL(k_lag_step<k>, dim3(1), dim3(kFinishBlock), 0, s, nblk, w->partials, \
-                         p, mode, w->lag, w->lagpub_dev, w->lagseq, w->lagmaxl, w->lagdelta,  \
-                         w->lagbfloor, w->lagnlen);                                           \
+                         p, mode, last, w->lag, w->lagpub_dev, w->lagseq, w->lagmaxl,         \
+                         w->lagdelta, w->lagbfloor, w->lagnlen);                              \
       break;
     NVHIP_LAG_CASE(1) NVHIP_LAG_CASE(3) NVHIP_LAG_CASE(4) NVHIP_LAG_CASE(5)
     NVHIP_LAG_CASE(6) NVHIP_LAG_CASE(7) NVHIP_LAG_CASE(8) NVHIP_LAG_CASE(9)
@@ -2243,13 +2248,13 @@ int NVHip_LagBegin(N_Vector v, int maxl, sunrealtype delta, sunrealtype bnorm_fl
   return 0;
 }
 
-int NVHip_LagStep(N_Vector v, int K, int nblk, int p) {
+int NVHip_LagStep(N_Vector v, int K, int nblk, int p, int last) {
   if (!v || !is_hip(v) || K < 3 || K > 10 || nblk < 1 || nblk * K > kMaxRedBlocks * kMaxK ||
       p < 0)
     return -1;
   RedWs* w = workspace(S(v));
   if (!w->lag || p >= w->lagmaxl || K != (p == 0 ? 4 : p + 3)) return -1;
-  lag_step_dispatch(K, S(v), nblk, w, p, 0);
+  lag_step_dispatch(K, S(v), nblk, w, p, 0, last != 0);
   check_launch("lag step");
   return 0;
 }
@@ -2272,8 +2277,16 @@ int NVHip_LagProjNorm(N_Vector wv, int nb, N_Vector* U) {
   hipLaunchKernelGGL(k_reduce1<FProjSqDev>, dim3(g), dim3(kBlock), 0, s, RED_SUM, f, n,
                      w->partials, (unsigned*)nullptr, w->host_dev, (double*)nullptr, 0u,
                      (const int*)&w->lag->pskip);
-  lag_step_dispatch(1, s, g, w, 0, 1);
+  lag_step_dispatch(1, s, g, w, 0, 1, 0);
   check_launch("lag projection norm");
+  return 0;
+}
+
+int NVHip_LagNext(N_Vector v) {
+  if (!v || !is_hip(v)) return -1;
+  RedWs* w = workspace(S(v));
+  if (!w->lag) return -1;
+  w->lagseq++;
   return 0;
 }
 

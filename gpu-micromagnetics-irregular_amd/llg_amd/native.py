@@ -183,7 +183,8 @@ SIGNATURES = {
     "NVHip_GateRelease": (C.c_int, [VP, C.c_int, DP, C.c_int]),
     "NVHip_LagBegin": (C.c_int, [VP, C.c_int, C.c_double, C.c_double, C.c_double,
                                  C.POINTER(VP), C.POINTER(VP), C.POINTER(VP)]),
-    "NVHip_LagStep": (C.c_int, [VP, C.c_int, C.c_int, C.c_int]),
+    "NVHip_LagStep": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "NVHip_LagNext": (C.c_int, [VP]),
     "NVHip_LagProjNorm": (C.c_int, [VP, C.c_int, C.POINTER(VP)]),
     "NVHip_LagWait": (C.c_int, [VP, VP]),
     "NVHip_ProjNorm": (C.c_int, [C.c_int, C.c_double, DP, VP, C.POINTER(VP), VP, DP]),
@@ -242,8 +243,8 @@ SIGNATURES = {
     "LLG_ArnoldiQueue": (C.c_int, [C.c_int, C.c_int, C.c_double, DP, VP, VP, VP, C.POINTER(VP),
                                    VP, VP, C.c_double, C.c_double, VP, VP, VP,
                                    C.POINTER(C.c_uint)]),
-    "LLG_ArnoldiLag": (C.c_int, [C.c_int, C.c_int, VP, VP, VP, C.POINTER(VP), VP, VP,
-                                 C.c_double, C.c_double, VP, VP, VP]),
+    "LLG_ArnoldiLag": (C.c_int, [C.c_int, C.c_int, C.c_int, VP, VP, VP, C.POINTER(VP), VP,
+                                 VP, C.c_double, C.c_double, VP, VP, VP]),
     "LLG_ArnoldiFirst": (C.c_int, [C.c_int, DP, C.POINTER(VP), C.POINTER(VP), VP, VP,
                                    C.c_double, C.c_double, VP, VP, VP, DP]),
     "PrecondSetup": (C.c_int, [C.c_double, VP, VP, C.c_int, C.POINTER(C.c_int), C.c_double, VP]),

@@ -89,6 +89,7 @@ typedef struct llgcv_stats {
  * started, passes run after the first, |u_0|, the final residual
  * estimate, |u_l|, the Hessenberg matrix (rows 0..maxl) and the Givens
  * rotations (c, s) -- the values the host loop would hold. */
+#define LLGCV_LAG_RUNNING (-1)
 #define LLGCV_LAG_CONVERGED 0
 #define LLGCV_LAG_MAXL 1
 #define LLGCV_LAG_ZERO 2
@@ -188,21 +189,25 @@ typedef struct llgcv_fused_ops {
                           N_Vector abstol, N_Vector ewt);
   int (*wrms_pair_comp)(N_Vector x, sunrealtype cx, N_Vector xc, sunrealtype a, N_Vector y,
                         sunrealtype b, N_Vector z, N_Vector w, sunrealtype* nrm);
-  /* SPGMR with lagged normalisation driven by the device: with all four
+  /* SPGMR with lagged normalisation driven by the device: with all five
    * set (and the lagged form in use, its first pass not formed in place),
-   * the integrator queues a solve whole -- lag_begin, the first pass and
-   * passes 1..maxl-1 by arnoldi_lag (as LLG_ArnoldiLag), the last column's
-   * projection norm by lag_projnorm -- and waits once (lag_wait) for the
-   * state the host loop would reach; the scalar steps run on the device
-   * (as NVHip_Lag*), so the solve is the host loop's bit for bit. */
+   * the integrator queues a solve in at most two parts -- lag_begin, the
+   * first pass and as many further passes as the previous solve needed
+   * plus one by arnoldi_lag (as LLG_ArnoldiLag), the last with last = 1;
+   * lag_wait; and when the state comes back running, lag_next, the
+   * remaining passes and the last column's projection norm (lag_projnorm),
+   * lag_wait -- for the state the host loop would reach; the scalar steps
+   * run on the device (as NVHip_Lag*), so the solve is the host loop's bit
+   * for bit. */
   int (*lag_begin)(N_Vector v, int maxl, sunrealtype delta, sunrealtype bnorm_floor,
                    sunrealtype nlen, const double** coef_dev, const int** skip_dev,
                    const int** pskip_dev);
-  int (*arnoldi_lag)(int l, int pre, const double* coef_dev, const int* skip_dev, N_Vector win,
-                     N_Vector* U, N_Vector wout, N_Vector s, sunrealtype gamma, sunrealtype t,
-                     N_Vector y, N_Vector fy, void* user_data);
+  int (*arnoldi_lag)(int l, int pre, int last, const double* coef_dev, const int* skip_dev,
+                     N_Vector win, N_Vector* U, N_Vector wout, N_Vector s, sunrealtype gamma,
+                     sunrealtype t, N_Vector y, N_Vector fy, void* user_data);
   int (*lag_projnorm)(N_Vector w, int nb, N_Vector* U);
   int (*lag_wait)(N_Vector v, llgcv_lag_result* res);
+  int (*lag_next)(N_Vector v);
 } llgcv_fused_ops;
 
 void* LLGCVodeCreate(int lmm, SUNContext sunctx);

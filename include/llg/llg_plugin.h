@@ -235,11 +235,11 @@ int LLG_ArnoldiQueue(int l, int pre, sunrealtype a, const sunrealtype* b, const 
  * pass l (l = 0 with pre: the preconditioned right-hand side win = b;
  * l > 0: u_l = a win - sum_{j<l} b_j U_j with a, b read from coef_dev,
  * the pass and its step exiting at once when *skip_dev is nonzero), its
- * sums summed by NVHip_LagStep(wout, ..., l) on the device.  Nothing is
- * waited for.  No reference counterpart. */
-int LLG_ArnoldiLag(int l, int pre, const double* coef_dev, const int* skip_dev, N_Vector win,
-                   N_Vector* U, N_Vector wout, N_Vector s, sunrealtype gamma, sunrealtype t,
-                   N_Vector y, N_Vector fy, void* user_data);
+ * sums summed by NVHip_LagStep(wout, ..., l, last) on the device.  Nothing
+ * is waited for.  No reference counterpart. */
+int LLG_ArnoldiLag(int l, int pre, int last, const double* coef_dev, const int* skip_dev,
+                   N_Vector win, N_Vector* U, N_Vector wout, N_Vector s, sunrealtype gamma,
+                   sunrealtype t, N_Vector y, N_Vector fy, void* user_data);
 /* The first pass (LLG_Arnoldi with l = 0, pre) with the right-hand side
  * formed where it is read: b = sum_{k<nb} c_k X_k (nb <= 3, rounded as
  * N_VLinearCombination rounds it; the Newton residual), so b is never

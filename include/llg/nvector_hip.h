@@ -247,10 +247,12 @@ int NVHip_SpgmrSolve(NVHipDevOp op, void* op_data, N_Vector t0, N_Vector* V, int
  *     bnorm_floor (< 0: none) compared with sqrt(|b s|^2 / nlen); returns
  *     the device coefficient array (a, b_0 .. b_l) and the two skip flags.
  *   NVHip_LagStep: sums the K partials of pass p over nblk workgroups
- *     (NVHip_ReducePartials) in workgroup order and steps the state.
+ *     (NVHip_ReducePartials) in workgroup order and steps the state; last
+ *     (see NVHip_LagNext) publishes a solve that goes on.
  *   NVHip_LagProjNorm: u = a w - sum_{j<nb} b_j U_j from the device
  *     coefficients, |u|^2 (u is not stored), then the last column's step.
  * One solve per stream at a time; returns 0 or -1. */
+#define NVHIP_LAG_RUNNING (-1)
 #define NVHIP_LAG_CONVERGED 0
 #define NVHIP_LAG_MAXL 1
 #define NVHIP_LAG_ZERO 2
@@ -271,9 +273,14 @@ typedef struct {
 int NVHip_LagBegin(N_Vector v, int maxl, sunrealtype delta, sunrealtype bnorm_floor,
                    sunrealtype nlen, const double** coef_dev, const int** skip_dev,
                    const int** pskip_dev);
-int NVHip_LagStep(N_Vector v, int K, int nblk, int p);
+int NVHip_LagStep(N_Vector v, int K, int nblk, int p, int last);
 int NVHip_LagProjNorm(N_Vector w, int nb, N_Vector* U);
 int NVHip_LagWait(N_Vector v, NVHipLagResult* res);
+/* a solve may be queued in parts: a step queued with last = 1 publishes
+ * the state also when the solve goes on (status NVHIP_LAG_RUNNING); after
+ * that wait NVHip_LagNext opens the next part (further passes, the
+ * projection norm) with its own wait. */
+int NVHip_LagNext(N_Vector v);
 
 /* Reductions for kernels outside this file (no counterpart in the
  * reference): a kernel queued on v's stream leaves K partial sums per

@@ -93,7 +93,8 @@ def test_device_driven_lagged_spgmr_is_bitwise_neutral(name, make, maxl):
     for k in ("nst", "nfe", "nni", "nli", "netf", "ncfn", "nsetups"):
         assert sa[k] == sb[k], k
     assert np.array_equal(ya, yb)
-    assert wb < wa
+    # maxl 1: both forms wait for pass 0 and, when needed, the projection norm
+    assert wb < wa if maxl > 1 else wb <= wa
     sim.close()
 
 

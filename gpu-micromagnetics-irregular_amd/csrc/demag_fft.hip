@@ -1022,6 +1022,78 @@ void Demag_ApplyWindowed(DemagData* d, const double* y_dev, const double* w_dev,
   apply(d, y_dev, w_dev, h_out_dev);
 }
 
+int Demag_ApplyBatch(int n, DemagData* const* d, const double* const* y,
+                     const double* const* w, double* const* h, void* stream) {
+  if (n < 0 || (n > 0 && (!d || !y || !h))) {
+    derr("Demag_ApplyBatch: NULL argument");
+    return -1;
+  }
+  g_derr.clear();
+  hipStream_t s = (hipStream_t)stream;
+  // instances share a launch when everything a pass kernel's shape and
+  // arithmetic depend on is equal
+  auto same_plan = [](const DemagData* a, const DemagData* b) {
+    return a->nx == b->nx && a->ny == b->ny && a->px == b->px && a->py == b->py &&
+           a->hx == b->hx && a->padded == b->padded && a->has_xy_im == b->has_xy_im &&
+           a->reg_row == b->reg_row && a->reg_col == b->reg_col && !a->fp32 && !b->fp32;
+  };
+  std::vector<char> done(n, 0);
+  for (int i = 0; i < n; i++) {
+    if (done[i]) continue;
+    DemagData* a = d[i];
+    if (!a || !y[i] || !h[i]) {
+      derr("Demag_ApplyBatch: NULL instance");
+      return -1;
+    }
+    done[i] = 1;
+    const bool reg = !a->fp32 && a->reg_row >= 0 && a->reg_col >= 0 &&
+                     demag_reg::has_row_plan<double>(a->px / 2, a->reg_row) &&
+                     demag_reg::has_col_plan<double>(a->py, a->reg_col);
+    if (!reg) {   // its own passes, on the batch's stream
+      a->stream = s;
+      apply(a, y[i], w ? w[i] : nullptr, h[i]);
+      if (!g_derr.empty()) return -1;
+      continue;
+    }
+    std::vector<int> grp{i};
+    for (int j = i + 1; j < n; j++)
+      if (!done[j] && d[j] && same_plan(a, d[j])) {
+        grp.push_back(j);
+        done[j] = 1;
+      }
+    const long N = (long)a->nx * a->ny;
+    const auto* tw_row = a->row.p.tw;
+    const auto* tw_col = a->col.p.tw;
+    const auto* post = (const V2<double>*)a->d_post;
+    for (size_t c0 = 0; c0 < grp.size(); c0 += demag_reg::kBatchMax) {
+      const int m = (int)std::min<size_t>(demag_reg::kBatchMax, grp.size() - c0);
+      demag_reg::RegBatch<double> b{};
+      for (int k = 0; k < m; k++) {
+        const int j = grp[c0 + k];
+        if (!y[j] || !h[j]) {
+          derr("Demag_ApplyBatch: NULL instance");
+          return -1;
+        }
+        b.in[k] = y[j];
+        b.w[k] = w ? w[j] : nullptr;
+        b.spec[k] = reinterpret_cast<V2<double>*>(d[j]->d_spec);
+        b.K[k] = d[j]->d_K;
+        b.out[k] = h[j];
+      }
+      if (demag_reg::launch_row_fwd_batch<double>(a->row.p.L, a->reg_row, a->padded, m, b, N,
+                                                   a->nx, a->ny, tw_row, post, s) != 1 ||
+          demag_reg::launch_col_batch<double>(a->py, a->reg_col, a->padded, m, b, a->hx, tw_col,
+                                               a->has_xy_im, s) != 1 ||
+          demag_reg::launch_row_inv_batch<double>(a->row.p.L, a->reg_row, a->padded, m, b,
+                                                   a->ny, tw_row, post, N, a->nx, s) != 1) {
+        if (g_derr.empty()) derr("Demag_ApplyBatch: a batched pass failed to launch");
+        return -1;
+      }
+    }
+  }
+  return launch_ok("Demag_ApplyBatch") ? 0 : -1;
+}
+
 void Demag_SetStream(DemagData* d, void* stream) {
   if (!d) return;
   d->stream = stream ? (hipStream_t)stream : (hipStream_t)NVHip_DefaultStream();

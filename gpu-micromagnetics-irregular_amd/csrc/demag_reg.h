@@ -18,6 +18,24 @@ namespace demag_reg {
 template <class T>
 using C2 = rfft::Cplx<T>;
 
+// A batch runs one pass for up to kBatchMax instances that share a plan
+// (same lengths, precision and variant) in one launch: blockIdx.y selects
+// the instance, whose buffers come from these arrays (kernel arguments,
+// copied at launch, so the caller may reuse them at once).  Each
+// instance's workgroups do exactly the work of its own launch -- same
+// plan, same lanes, same operations -- so every instance gets the bits of
+// its single launch.  The twiddle and post-processing tables depend only
+// on the plan and are taken once for all.
+constexpr int kBatchMax = 32;
+template <class T>
+struct RegBatch {
+  const double* in[kBatchMax];   // row forward: SoA state
+  const double* w[kBatchMax];    // row forward: window, or null
+  C2<T>* spec[kBatchMax];        // intermediate spectrum
+  const T* K[kBatchMax];         // column: tensor planes
+  double* out[kBatchMax];        // row inverse: SoA field (overwrite)
+};
+
 // true when register plan `variant` exists for a row (complex length
 // L = Px/2) or a column (length L = Py) transform.  Rows have variant 0;
 // columns have variant 0 (8 values per component and lane) for every
@@ -51,6 +69,29 @@ int launch_col(int L, int variant, int half, C2<T>* spec, int Hx, const C2<T>* t
 template <class T>
 int launch_row_inv(int L, int variant, int half, const C2<T>* spec, int rows, const C2<T>* tw,
                    const C2<T>* post, double* out, long comp_stride, int nx_out, hipStream_t s);
+
+// batched forms of the three passes (FP64 pipeline): n <= kBatchMax
+// instances of one plan, buffers from b; same return convention
+template <class T>
+int launch_row_fwd_batch(int L, int variant, int half, int n, const RegBatch<T>& b,
+                         long comp_stride, int nx_in, int rows, const C2<T>* tw,
+                         const C2<T>* post, hipStream_t s);
+template <class T>
+int launch_col_batch(int L, int variant, int half, int n, const RegBatch<T>& b, int Hx,
+                     const C2<T>* tw, int has_xy_im, hipStream_t s);
+template <class T>
+int launch_row_inv_batch(int L, int variant, int half, int n, const RegBatch<T>& b, int rows,
+                         const C2<T>* tw, const C2<T>* post, long comp_stride, int nx_out,
+                         hipStream_t s);
+
+extern template int launch_row_fwd_batch<double>(int, int, int, int, const RegBatch<double>&,
+                                                 long, int, int, const C2<double>*,
+                                                 const C2<double>*, hipStream_t);
+extern template int launch_col_batch<double>(int, int, int, int, const RegBatch<double>&, int,
+                                             const C2<double>*, int, hipStream_t);
+extern template int launch_row_inv_batch<double>(int, int, int, int, const RegBatch<double>&, int,
+                                                 const C2<double>*, const C2<double>*, long, int,
+                                                 hipStream_t);
 
 extern template bool has_row_plan<double>(int, int);
 extern template bool has_col_plan<double>(int, int);

@@ -63,12 +63,23 @@ struct Bar {
   __device__ __forceinline__ void operator()() const { __syncthreads(); }
 };
 
+// the batch argument of a pass kernel: the instances' buffers (RegBatch,
+// blockIdx.y selects the instance) or nothing for a single launch
+struct NoBatch {};
+template <class T, bool B>
+using BatchArg = std::conditional_t<B, RegBatch<T>, NoBatch>;
+
 // ------------------------------------------------------------ row forward
-template <class T, class F, bool HALF>
+template <class T, class F, bool HALF, bool BATCH = false>
 __global__ __launch_bounds__(F::kThreads) void k_row_fwd_reg(
     const double* __restrict__ in, const double* __restrict__ w, long comp_stride, int nx_in,
     int rows, const C2<T>* __restrict__ tw, const C2<T>* __restrict__ post,
-    C2<T>* __restrict__ spec) {
+    C2<T>* __restrict__ spec, BatchArg<T, BATCH> bt) {
+  if constexpr (BATCH) {
+    in = bt.in[blockIdx.y];
+    w = bt.w[blockIdx.y];
+    spec = bt.spec[blockIdx.y];
+  }
   using C = C2<T>;
   constexpr int U = F::U, V = F::V, L = F::L, G = F::kPerGroup;
   constexpr int NB = HALF ? V / 2 : V;
@@ -121,10 +132,15 @@ __global__ __launch_bounds__(F::kThreads) void k_row_fwd_reg(
 }
 
 // ------------------------------------------------------------ row inverse
-template <class T, class F, bool HALF>
+template <class T, class F, bool HALF, bool BATCH = false>
 __global__ __launch_bounds__(F::kThreads) void k_row_inv_reg(
     const C2<T>* __restrict__ spec, int rows, const C2<T>* __restrict__ tw,
-    const C2<T>* __restrict__ post, double* __restrict__ out, long comp_stride, int nx_out) {
+    const C2<T>* __restrict__ post, double* __restrict__ out, long comp_stride, int nx_out,
+    BatchArg<T, BATCH> bt) {
+  if constexpr (BATCH) {
+    spec = bt.spec[blockIdx.y];
+    out = bt.out[blockIdx.y];
+  }
   using C = C2<T>;
   constexpr int U = F::U, V = F::V, L = F::L, G = F::kPerGroup;
   constexpr int NB = HALF ? V / 2 : V;
@@ -172,10 +188,15 @@ struct ColWaves {
 };
 // XYI: the xy tensor plane has an imaginary part (periodic grids); the
 // zero-padded grid's xy spectrum is real (odd-odd kernel)
-template <class T, class F, bool HALF, bool XYI>
+template <class T, class F, bool HALF, bool XYI, bool BATCH = false>
 __global__ __launch_bounds__(F::kThreads) __attribute__((amdgpu_waves_per_eu(ColWaves<T, F>::value))) void k_col_reg(C2<T>* __restrict__ spec, int Hx,
                                                          const C2<T>* __restrict__ tw,
-                                                         const T* __restrict__ K) {
+                                                         const T* __restrict__ K,
+                                                         BatchArg<T, BATCH> bt) {
+  if constexpr (BATCH) {
+    spec = bt.spec[blockIdx.y];
+    K = bt.K[blockIdx.y];
+  }
   using C = C2<T>;
   constexpr int U = F::U, V = F::V, L = F::L, G = F::kPerGroup;
   constexpr int NB = HALF ? V / 2 : V;
@@ -296,10 +317,14 @@ struct ColCb {
   static constexpr bool ok = F::kPerGroup == 1 && kThreads <= 1024 && kLds <= 160 * 1024;
 };
 
-template <class T, class F, bool HALF, bool XYI, int CB>
+template <class T, class F, bool HALF, bool XYI, int CB, bool BATCH = false>
 __global__ __launch_bounds__(F::U * CB) __attribute__((amdgpu_waves_per_eu(ColWaves<T, F>::value)))
 void k_col_cb(C2<T>* __restrict__ spec, int Hx, const C2<T>* __restrict__ tw,
-              const T* __restrict__ K) {
+              const T* __restrict__ K, BatchArg<T, BATCH> bt) {
+  if constexpr (BATCH) {
+    spec = bt.spec[blockIdx.y];
+    K = bt.K[blockIdx.y];
+  }
   using C = C2<T>;
   constexpr int U = F::U, V = F::V, L = F::L;
   constexpr int NB = HALF ? V / 2 : V;
@@ -456,10 +481,14 @@ struct ColWave {
                              kLds <= 160 * 1024 && ROWS % (kThreads / CB) == 0;
 };
 
-template <class T, class F, bool HALF, bool XYI, int CB>
+template <class T, class F, bool HALF, bool XYI, int CB, bool BATCH = false>
 __global__ __launch_bounds__(128 * CB) __attribute__((amdgpu_waves_per_eu(4)))
 void k_col_wave(C2<T>* __restrict__ spec, int Hx, int nxy, const C2<T>* __restrict__ tw,
-                const T* __restrict__ K) {
+                const T* __restrict__ K, BatchArg<T, BATCH> bt) {
+  if constexpr (BATCH) {
+    spec = bt.spec[blockIdx.y];
+    K = bt.K[blockIdx.y];
+  }
   using C = C2<T>;
   using G = ColWave<T, F, HALF, CB>;
   constexpr int V = F::V, L = F::L, W = G::W, NT = G::kThreads, kBuf = G::kBuf;
@@ -714,26 +743,30 @@ bool has_col_plan(int L, int variant) {
   return with_col<T>(L, variant, [](auto) {});
 }
 
-template <class T>
-int launch_row_fwd(int L, int variant, int half, const double* in, const double* w, long comp_stride, int nx_in,
-                   int rows, const C2<T>* tw, const C2<T>* post, C2<T>* spec, hipStream_t s) {
+// The launchers: BATCH launches n instances (gridDim.y = n) with buffers
+// from bt; otherwise one instance (n = 1) with the buffers passed directly.
+template <class T, bool BATCH>
+int launch_row_fwd_t(int L, int variant, int half, int n, const BatchArg<T, BATCH>& bt,
+                     const double* in, const double* w, long comp_stride, int nx_in, int rows,
+                     const C2<T>* tw, const C2<T>* post, C2<T>* spec, hipStream_t s) {
   int rc = 0;
   with_row<T>(L, variant, [&](auto f) {
     using F = decltype(f);
     const size_t lds = sizeof(C2<T>) * F::kSlots * F::kPerGroup;
     const int grid = (3 * rows + F::kPerGroup - 1) / F::kPerGroup;
-    auto kern = half ? k_row_fwd_reg<T, F, true> : k_row_fwd_reg<T, F, false>;
+    auto kern = half ? k_row_fwd_reg<T, F, true, BATCH> : k_row_fwd_reg<T, F, false, BATCH>;
     if (prep_lds(kern, lds)) { rc = -1; return; }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(F::kThreads), lds, s, in, w, comp_stride, nx_in,
-                       rows, tw, post, spec);
+    hipLaunchKernelGGL(kern, dim3(grid, n), dim3(F::kThreads), lds, s, in, w, comp_stride, nx_in,
+                       rows, tw, post, spec, bt);
     rc = hipPeekAtLastError() == hipSuccess ? 1 : -1;
   });
   return rc;
 }
 
-template <class T>
-int launch_col(int L, int variant, int half, C2<T>* spec, int Hx, const C2<T>* tw, const T* K, int has_xy_im,
-               hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+template <class T, bool BATCH>
+int launch_col_t(int L, int variant, int half, int n, const BatchArg<T, BATCH>& bt, C2<T>* spec,
+                 int Hx, const C2<T>* tw, const T* K, int has_xy_im, hipStream_t s,
+                 hipEvent_t e0, hipEvent_t e1) {
   int rc = 0;
   if (col_cb_of(variant)) {
     // zero-padded grids (real xy spectrum) or periodic grids (complex xy)
@@ -744,10 +777,11 @@ int launch_col(int L, int variant, int half, C2<T>* spec, int Hx, const C2<T>* t
       using G = ColCb<T, F, CBv>;
       const int nblk = (Hx + CBv - 1) / CBv;
       const int grid = 2 * ((nblk + 7) / 8 * 8);
-      auto kern = half ? k_col_cb<T, F, true, false, CBv> : k_col_cb<T, F, false, true, CBv>;
+      auto kern = half ? k_col_cb<T, F, true, false, CBv, BATCH>
+                       : k_col_cb<T, F, false, true, CBv, BATCH>;
       if (prep_lds(kern, G::kLds)) { rc = -1; return; }
-      hipExtLaunchKernelGGL(kern, dim3(grid), dim3(G::kThreads), (std::uint32_t)G::kLds, s, e0, e1,
-                            0u, spec, Hx, tw, K);
+      hipExtLaunchKernelGGL(kern, dim3(grid, n), dim3(G::kThreads), (std::uint32_t)G::kLds, s,
+                            e0, e1, 0u, spec, Hx, tw, K, bt);
       rc = hipPeekAtLastError() == hipSuccess ? 1 : -1;
     });
     return rc;
@@ -760,11 +794,12 @@ int launch_col(int L, int variant, int half, C2<T>* spec, int Hx, const C2<T>* t
       // x/y workgroups, then z workgroups, each part a multiple of 8
       const int nxy = ((Hx + CBv - 1) / CBv + 7) / 8 * 8;
       const int nz = ((Hx + 2 * CBv - 1) / (2 * CBv) + 7) / 8 * 8;
-      auto kern = half ? k_col_wave<T, F, true, false, CBv> : k_col_wave<T, F, false, true, CBv>;
+      auto kern = half ? k_col_wave<T, F, true, false, CBv, BATCH>
+                       : k_col_wave<T, F, false, true, CBv, BATCH>;
       const size_t lds = half ? ColWave<T, F, true, CBv>::kLds : ColWave<T, F, false, CBv>::kLds;
       if (prep_lds(kern, lds)) { rc = -1; return; }
-      hipExtLaunchKernelGGL(kern, dim3(nxy + nz), dim3(ColWave<T, F, true, CBv>::kThreads),
-                            (std::uint32_t)lds, s, e0, e1, 0u, spec, Hx, nxy, tw, K);
+      hipExtLaunchKernelGGL(kern, dim3(nxy + nz, n), dim3(ColWave<T, F, true, CBv>::kThreads),
+                            (std::uint32_t)lds, s, e0, e1, 0u, spec, Hx, nxy, tw, K, bt);
       rc = hipPeekAtLastError() == hipSuccess ? 1 : -1;
     });
     return rc;
@@ -774,31 +809,81 @@ int launch_col(int L, int variant, int half, C2<T>* spec, int Hx, const C2<T>* t
     const size_t lds = sizeof(C2<T>) * F::kSlots * F::kPerGroup;
     // x/y pairs, then z components: each half rounded up to a multiple of 8
     const int grid = 2 * (((Hx + F::kPerGroup - 1) / F::kPerGroup + 7) / 8 * 8);
-    auto kern = half ? (has_xy_im ? k_col_reg<T, F, true, true> : k_col_reg<T, F, true, false>)
-                     : (has_xy_im ? k_col_reg<T, F, false, true> : k_col_reg<T, F, false, false>);
+    auto kern = half ? (has_xy_im ? k_col_reg<T, F, true, true, BATCH>
+                                  : k_col_reg<T, F, true, false, BATCH>)
+                     : (has_xy_im ? k_col_reg<T, F, false, true, BATCH>
+                                  : k_col_reg<T, F, false, false, BATCH>);
     if (prep_lds(kern, lds)) { rc = -1; return; }
-    hipExtLaunchKernelGGL(kern, dim3(grid), dim3(F::kThreads), (std::uint32_t)lds, s, e0, e1, 0u,
-                          spec, Hx, tw, K);
+    hipExtLaunchKernelGGL(kern, dim3(grid, n), dim3(F::kThreads), (std::uint32_t)lds, s, e0, e1,
+                          0u, spec, Hx, tw, K, bt);
+    rc = hipPeekAtLastError() == hipSuccess ? 1 : -1;
+  });
+  return rc;
+}
+
+template <class T, bool BATCH>
+int launch_row_inv_t(int L, int variant, int half, int n, const BatchArg<T, BATCH>& bt,
+                     const C2<T>* spec, int rows, const C2<T>* tw, const C2<T>* post,
+                     double* out, long comp_stride, int nx_out, hipStream_t s) {
+  int rc = 0;
+  with_row<T>(L, variant, [&](auto f) {
+    using F = decltype(f);
+    const size_t lds = sizeof(C2<T>) * F::kSlots * F::kPerGroup;
+    const int grid = (3 * rows + F::kPerGroup - 1) / F::kPerGroup;
+    auto kern = half ? k_row_inv_reg<T, F, true, BATCH> : k_row_inv_reg<T, F, false, BATCH>;
+    if (prep_lds(kern, lds)) { rc = -1; return; }
+    hipLaunchKernelGGL(kern, dim3(grid, n), dim3(F::kThreads), lds, s, spec, rows, tw, post, out,
+                       comp_stride, nx_out, bt);
     rc = hipPeekAtLastError() == hipSuccess ? 1 : -1;
   });
   return rc;
 }
 
 template <class T>
+int launch_row_fwd(int L, int variant, int half, const double* in, const double* w, long comp_stride, int nx_in,
+                   int rows, const C2<T>* tw, const C2<T>* post, C2<T>* spec, hipStream_t s) {
+  return launch_row_fwd_t<T, false>(L, variant, half, 1, NoBatch{}, in, w, comp_stride, nx_in,
+                                    rows, tw, post, spec, s);
+}
+
+template <class T>
+int launch_col(int L, int variant, int half, C2<T>* spec, int Hx, const C2<T>* tw, const T* K, int has_xy_im,
+               hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  return launch_col_t<T, false>(L, variant, half, 1, NoBatch{}, spec, Hx, tw, K, has_xy_im, s,
+                                e0, e1);
+}
+
+template <class T>
 int launch_row_inv(int L, int variant, int half, const C2<T>* spec, int rows, const C2<T>* tw,
                    const C2<T>* post, double* out, long comp_stride, int nx_out, hipStream_t s) {
-  int rc = 0;
-  with_row<T>(L, variant, [&](auto f) {
-    using F = decltype(f);
-    const size_t lds = sizeof(C2<T>) * F::kSlots * F::kPerGroup;
-    const int grid = (3 * rows + F::kPerGroup - 1) / F::kPerGroup;
-    auto kern = half ? k_row_inv_reg<T, F, true> : k_row_inv_reg<T, F, false>;
-    if (prep_lds(kern, lds)) { rc = -1; return; }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(F::kThreads), lds, s, spec, rows, tw, post, out,
-                       comp_stride, nx_out);
-    rc = hipPeekAtLastError() == hipSuccess ? 1 : -1;
-  });
-  return rc;
+  return launch_row_inv_t<T, false>(L, variant, half, 1, NoBatch{}, spec, rows, tw, post, out,
+                                    comp_stride, nx_out, s);
+}
+
+template <class T>
+int launch_row_fwd_batch(int L, int variant, int half, int n, const RegBatch<T>& b,
+                         long comp_stride, int nx_in, int rows, const C2<T>* tw,
+                         const C2<T>* post, hipStream_t s) {
+  if (n < 1 || n > kBatchMax) return -1;
+  return launch_row_fwd_t<T, true>(L, variant, half, n, b, nullptr, nullptr, comp_stride, nx_in,
+                                   rows, tw, post, nullptr, s);
+}
+
+template <class T>
+int launch_col_batch(int L, int variant, int half, int n, const RegBatch<T>& b, int Hx,
+                     const C2<T>* tw, int has_xy_im, hipStream_t s) {
+  if (n < 1 || n > kBatchMax) return -1;
+  return launch_col_t<T, true>(L, variant, half, n, b, nullptr, Hx, tw, nullptr, has_xy_im, s,
+                               nullptr, nullptr);
+}
+
+template <class T>
+int launch_row_inv_batch(int L, int variant, int half, int n, const RegBatch<T>& b, int rows,
+                         const C2<T>* tw, const C2<T>* post, long comp_stride, int nx_out,
+                         hipStream_t s) {
+  if (n < 1 || n > kBatchMax) return -1;
+  return launch_row_inv_t<T, true>(L, variant, half, n, b, nullptr, rows, tw, post, nullptr,
+                                   comp_stride, nx_out, s);
 }
 
 }  // namespace demag_reg

@@ -21,6 +21,7 @@
 // (512 B coalesced per plane); the neighbour rows come through L1/L2.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -29,7 +30,9 @@
 #include <vector>
 
 #include "llg/llg_plugin.h"
+#include "llg/llg_batch.h"
 #include "llg/learned_pc.h"
+#include "llg_hook.h"
 #include "llg/nvector_hip.h"
 #include "kprof.h"
 
@@ -281,16 +284,15 @@ __global__ __launch_bounds__(256) void k_rhs(GeoD g, TermsD T, const double* __r
 // w*m per component is staged in LDS once and the four neighbours are read
 // from it, the same values field() reads (periodic wrap, or the edge cell
 // itself at a Neumann edge), so the result equals k_rhs bit for bit.
+// One tile (tbx, tby) of the RHS; `tile` is the workgroup's LDS halo.
 template <int BX, int BY, bool RES>
-__global__ __launch_bounds__(BX* BY) void k_rhs_tiled(GeoD g, TermsD T,
-                                                       const double* __restrict__ y,
-                                                       const double* __restrict__ hd,
-                                                       double* __restrict__ yd, ResIn rin) {
+__device__ __forceinline__ void rhs_tile(double (&tile)[3][BY + 2][BX + 2], const GeoD& g,
+                                         const TermsD& T, const double* __restrict__ y,
+                                         const double* __restrict__ hd,
+                                         double* __restrict__ yd, const ResIn& rin, int tbx,
+                                         int tby) {
   constexpr int SX = BX + 2, SY = BY + 2;
-  __shared__ double tile[3][SY][SX];
   const int N = g.ncell;
-  int tbx, tby;
-  tile_of(tbx, tby);
   const int x0 = tbx * BX, y0 = tby * BY;
   const bool win = T.window && g.w;
   for (int s = threadIdx.y * BX + threadIdx.x; s < SX * SY; s += BX * BY) {
@@ -348,6 +350,17 @@ __global__ __launch_bounds__(BX* BY) void k_rhs_tiled(GeoD g, TermsD T,
   } else {
     yd[cell] = f1; yd[N + cell] = f2; yd[2 * N + cell] = f3;
   }
+}
+
+template <int BX, int BY, bool RES>
+__global__ __launch_bounds__(BX* BY) void k_rhs_tiled(GeoD g, TermsD T,
+                                                       const double* __restrict__ y,
+                                                       const double* __restrict__ hd,
+                                                       double* __restrict__ yd, ResIn rin) {
+  __shared__ double tile[3][BY + 2][BX + 2];
+  int tbx, tby;
+  tile_of(tbx, tby);
+  rhs_tile<BX, BY, RES>(tile, g, T, y, hd, yd, rin, tbx, tby);
 }
 
 // ------------------------------------------------------------------- Jv
@@ -810,9 +823,7 @@ __global__ __launch_bounds__(256) void k_apply_P(GeoD g, int gate, const double*
 }
 
 // --------------------------------------------------------------- normalise
-__global__ void k_normalize(double* y, int ncell, int mode) {
-  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
-  if (cell >= ncell) return;
+__device__ __forceinline__ void normalize_cell(double* y, int ncell, int mode, int cell) {
   const double m1 = y[cell], m2 = y[ncell + cell], m3 = y[2 * ncell + cell];
   const double ymp = sqrt(m1 * m1 + m2 * m2 + m3 * m3);
   if (mode == LLG_NORM_EXACT) {
@@ -824,6 +835,101 @@ __global__ void k_normalize(double* y, int ncell, int mode) {
     const double inv = 1.0 / (ymp + 1.0e-3);
     y[cell] = m1 * inv; y[ncell + cell] = m2 * inv; y[2 * ncell + cell] = m3 * inv;
   }
+}
+
+__global__ void k_normalize(double* y, int ncell, int mode) {
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= ncell) return;
+  normalize_cell(y, ncell, mode, cell);
+}
+
+// ------------------------------------------------------------- batches
+// One launch per pass over many instances (ensembles): a workgroup finds
+// its instance and its tile there, then does exactly the work of that
+// instance's own launch on that tile -- the same per-cell arithmetic --
+// so each instance gets the bits of its single-instance call.  What stays
+// fixed per instance (geometry, term sets, demag field buffer) sits in a
+// device table; the per-call vectors come by value in the arguments.
+constexpr int kBatchMax = 32;
+
+struct BatchStatic {
+  GeoD g;
+  TermsD rhs, jv;
+  const double* hd;   // demag field of the instance's context, or null
+  int ntx, nty;       // tiles of kBatchBX x kBatchBY cells
+  int norm;           // normalisation mode of f()
+  int pad;
+};
+constexpr int kBatchBX = 32, kBatchBY = 8;
+
+struct StencilBatch {
+  int n;                     // instances in this launch
+  int t0[kBatchMax + 1];     // tile prefix: instance k owns tiles [t0[k], t0[k+1])
+  int idx[kBatchMax];        // row of instance k in the BatchStatic table
+  const double* y[kBatchMax];
+  const double* v[kBatchMax];   // Jv: the direction
+  double* out[kBatchMax];
+};
+struct ResBatch {
+  ResIn r[kBatchMax];
+};
+
+// XCD-aware tile over the whole batch (as tile_of), then the instance that
+// owns it (binary search over the prefix) and the tile inside it
+__device__ inline void batch_tile(const StencilBatch& b, int& k, int& lt) {
+  const int G = gridDim.x;
+  const int per = G / 8, rem = G % 8;
+  const int x = blockIdx.x % 8, i = blockIdx.x / 8;
+  const int t = x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+  int lo = 0, hi = b.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t >= b.t0[mid]) lo = mid;
+    else hi = mid - 1;
+  }
+  k = lo;
+  lt = t - b.t0[lo];
+}
+
+template <bool RES>
+__global__ __launch_bounds__(kBatchBX* kBatchBY) void k_rhs_batch(
+    const BatchStatic* __restrict__ st, StencilBatch b, ResBatch rb) {
+  __shared__ double tile[3][kBatchBY + 2][kBatchBX + 2];
+  int k, lt;
+  batch_tile(b, k, lt);
+  const BatchStatic& S = st[b.idx[k]];
+  const GeoD g = S.g;
+  const TermsD T = S.rhs;
+  ResIn rin{nullptr, nullptr, 0.0, 0.0, 0.0};
+  if constexpr (RES) rin = rb.r[k];
+  rhs_tile<kBatchBX, kBatchBY, RES>(tile, g, T, b.y[k], S.hd, b.out[k], rin, lt % S.ntx,
+                                    lt / S.ntx);
+}
+
+__global__ __launch_bounds__(kBatchBX* kBatchBY) void k_jtv_batch(
+    const BatchStatic* __restrict__ st, StencilBatch b) {
+  int k, lt;
+  batch_tile(b, k, lt);
+  const BatchStatic& S = st[b.idx[k]];
+  const GeoD g = S.g;
+  const TermsD T = S.jv;
+  const int gx = (lt % S.ntx) * kBatchBX + threadIdx.x;
+  const int gy = (lt / S.ntx) * kBatchBY + threadIdx.y;
+  if (gx >= g.ng || gy >= g.ny) return;
+  const int cell = gy * g.ng + gx, N = g.ncell;
+  double o[3];
+  jtv_cell<false>(g, T, b.y[k], VPlain{b.v[k]}, cell, gx, gy, 0.0, o);
+  double* Jv = b.out[k];
+  Jv[cell] = o[0]; Jv[N + cell] = o[1]; Jv[2 * N + cell] = o[2];
+}
+
+// blockIdx.y = instance k; cells beyond its grid return
+__global__ void k_normalize_batch(const BatchStatic* __restrict__ st, StencilBatch b) {
+  const int k = blockIdx.y;
+  const BatchStatic& S = st[b.idx[k]];
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (S.norm == LLG_NORM_NONE || cell >= S.g.ncell) return;
+  normalize_cell(const_cast<double*>(b.y[k]), S.g.ncell, S.norm, cell);
 }
 
 
@@ -897,6 +1003,10 @@ struct llg_ctx {
   llg_lpc* lpc = nullptr;   // learned correction applied by PrecondSolve
   int bx = kBX, by = kBY;   // 2D stencil workgroup shape (LLG_SetStencilBlock)
   double* d_mean = nullptr; // partial sums of LLG_MeanMagnetization
+  // f() handed to the ensemble layer instead of launched (LLG_SetRhsHook)
+  llg_rhs_hook rhs_hook = nullptr;
+  void* rhs_hook_data = nullptr;
+  long terms_ver = 0;       // bumped when a term set changes (batch tables re-read it)
 };
 
 namespace {
@@ -1144,9 +1254,14 @@ int LLG_GetCallCounts(llg_ctx* c, long* nf, long* njv, long* nps, long* npv) {
 // f(): [normalize] -> demag (overwrite hd) -> RHS (compact zeroes holes in-launch);
 // with rin the RHS pass writes the Newton residual built from f instead of f
 static int rhs_launch(llg_ctx* c, N_Vector y, N_Vector out, const ResIn* rin) {
-  c->nf++;
   double* yd = N_VGetDeviceArrayPointer(y);
   double* fd = N_VGetDeviceArrayPointer(out);
+  if (c->rhs_hook) {   // the ensemble batches this call with other instances'
+    llg_res_in r{};
+    if (rin) r = llg_res_in{rin->zn1, rin->ycor, rin->c0, rin->c1, rin->c2};
+    return c->rhs_hook(c->rhs_hook_data, c, yd, fd, rin ? &r : nullptr);
+  }
+  c->nf++;
   hipStream_t s = stream_of(out, c);
   if (c->cfg.normalize != LLG_NORM_NONE) {
     hipLaunchKernelGGL(k_normalize, dim3((c->ncell + k1D - 1) / k1D), dim3(k1D), 0, s, yd,
@@ -1606,6 +1721,7 @@ int LLG_SetAppliedField(llg_ctx* ctx, int which, const double* happ) {
     if (which & LLG_TERMS_JV) ctx->cfg.jv.happ[a] = happ[a];
     if (which & LLG_TERMS_PC) ctx->cfg.pc.happ[a] = happ[a];
   }
+  ctx->terms_ver++;
   return 0;
 }
 
@@ -1637,6 +1753,215 @@ int LLG_MeanMagnetization(llg_ctx* ctx, N_Vector y, double* m) {
     return -1;
   const double n = ctx->n_active > 0 ? (double)ctx->n_active : 1.0;
   for (int a = 0; a < 3; a++) m[a] = sum[a] / n;
+  return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- batches
+struct llg_batch {
+  std::vector<llg_ctx*> ctx;
+  std::vector<long> ver;          // terms_ver of each context at the last upload
+  std::vector<BatchStatic> h_st;
+  BatchStatic* d_st = nullptr;
+};
+
+namespace {
+
+BatchStatic static_of(const llg_ctx* c) {
+  BatchStatic S{};
+  S.g = geo_of(c);
+  S.rhs = to_dev(c->cfg.rhs);
+  S.jv = to_dev(c->cfg.jv);
+  S.hd = c->demag ? c->d_hd : nullptr;
+  S.ntx = (c->cfg.ng + kBatchBX - 1) / kBatchBX;
+  S.nty = (c->cfg.ny + kBatchBY - 1) / kBatchBY;
+  S.norm = c->cfg.normalize;
+  return S;
+}
+
+// re-read the term sets of contexts changed since the last upload (a field
+// ramp); the table is rewritten after the stream's earlier batches finished
+int batch_refresh(llg_batch* b, hipStream_t s) {
+  bool stale = false;
+  for (size_t i = 0; i < b->ctx.size(); i++) stale |= b->ver[i] != b->ctx[i]->terms_ver;
+  if (!stale) return 0;
+  for (size_t i = 0; i < b->ctx.size(); i++) {
+    b->h_st[i] = static_of(b->ctx[i]);
+    b->ver[i] = b->ctx[i]->terms_ver;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess ||
+      hipMemcpy(b->d_st, b->h_st.data(), sizeof(BatchStatic) * b->h_st.size(),
+                hipMemcpyHostToDevice) != hipSuccess) {
+    set_err("LLG batch: table upload failed");
+    return -1;
+  }
+  return 0;
+}
+
+// the stencil launch of instances `ids` (at most kBatchMax)
+template <class Fill, class Launch>
+void stencil_chunks(const llg_batch* b, const std::vector<int>& ids, Fill&& fill, Launch&& launch) {
+  for (size_t c0 = 0; c0 < ids.size(); c0 += kBatchMax) {
+    const int n = (int)std::min<size_t>(kBatchMax, ids.size() - c0);
+    StencilBatch sb{};
+    sb.n = n;
+    int t = 0;
+    for (int k = 0; k < n; k++) {
+      const int i = ids[c0 + k];
+      sb.t0[k] = t;
+      sb.idx[k] = i;
+      t += b->h_st[i].ntx * b->h_st[i].nty;
+      fill(sb, k, i);
+    }
+    sb.t0[n] = t;
+    launch(sb, t, c0, n);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+llg_batch* LLG_BatchCreate(int n, llg_ctx* const* ctx) {
+  if (n < 1 || !ctx) {
+    set_err("LLG_BatchCreate: no instances");
+    return nullptr;
+  }
+  for (int i = 0; i < n; i++) {
+    if (!ctx[i] || ctx[i]->cfg.exec != LLG_EXEC_YMSK) {
+      set_err("LLG_BatchCreate: NULL context or compact execution (dense-mask instances only)");
+      return nullptr;
+    }
+  }
+  llg_batch* b = new llg_batch();
+  b->ctx.assign(ctx, ctx + n);
+  b->ver.assign(n, -1);
+  b->h_st.resize(n);
+  if (hipMalloc(&b->d_st, sizeof(BatchStatic) * n) != hipSuccess) {
+    set_err("LLG_BatchCreate: hipMalloc failed");
+    delete b;
+    return nullptr;
+  }
+  if (batch_refresh(b, ctx[0]->stream) != 0) {
+    LLG_BatchDestroy(b);
+    return nullptr;
+  }
+  return b;
+}
+
+int LLG_BatchSize(const llg_batch* b) { return b ? (int)b->ctx.size() : -1; }
+
+int LLG_RhsBatch(llg_batch* b, double* const* y, double* const* ydot,
+                 const llg_res_in* const* res, void* stream) {
+  if (!b || !y || !ydot) return -1;
+  hipStream_t s = (hipStream_t)stream;
+  if (batch_refresh(b, s) != 0) return -1;
+  const int n = (int)b->ctx.size();
+  std::vector<int> all, plain, resid, norm;
+  for (int i = 0; i < n; i++) {
+    if (!y[i]) continue;
+    if (!ydot[i] || ydot[i] == y[i]) {
+      set_err("LLG_RhsBatch: missing or aliased output");
+      return -1;
+    }
+    all.push_back(i);
+    (res && res[i] ? resid : plain).push_back(i);
+    if (b->h_st[i].norm != LLG_NORM_NONE) norm.push_back(i);
+    b->ctx[i]->nf++;
+  }
+  // [normalize] -> demag -> RHS, the order of f()
+  for (size_t c0 = 0; c0 < norm.size(); c0 += kBatchMax) {
+    const int m = (int)std::min<size_t>(kBatchMax, norm.size() - c0);
+    StencilBatch sb{};
+    sb.n = m;
+    int maxn = 0;
+    for (int k = 0; k < m; k++) {
+      const int i = norm[c0 + k];
+      sb.idx[k] = i;
+      sb.y[k] = y[i];
+      maxn = std::max(maxn, b->ctx[i]->ncell);
+    }
+    hipLaunchKernelGGL(k_normalize_batch, dim3((maxn + k1D - 1) / k1D, m), dim3(k1D), 0, s,
+                       (const BatchStatic*)b->d_st, sb);
+  }
+  {
+    std::vector<DemagData*> dd;
+    std::vector<const double*> dy, dw;
+    std::vector<double*> dh;
+    for (int i : all) {
+      llg_ctx* c = b->ctx[i];
+      if (!c->demag) continue;
+      dd.push_back(c->demag);
+      dy.push_back(y[i]);
+      dw.push_back(c->cfg.rhs.window && c->d_w ? c->d_w : nullptr);
+      dh.push_back(c->d_hd);
+    }
+    if (!dd.empty() && Demag_ApplyBatch((int)dd.size(), dd.data(), dy.data(), dw.data(),
+                                        dh.data(), s) != 0) {
+      set_err(std::string("LLG_RhsBatch: ") + Demag_LastError());
+      return -1;
+    }
+  }
+  const dim3 block(kBatchBX, kBatchBY);
+  stencil_chunks(b, plain,
+                 [&](StencilBatch& sb, int k, int i) { sb.y[k] = y[i]; sb.out[k] = ydot[i]; },
+                 [&](const StencilBatch& sb, int tiles, size_t, int) {
+                   hipLaunchKernelGGL(k_rhs_batch<false>, dim3(tiles), block, 0, s,
+                                      (const BatchStatic*)b->d_st, sb, ResBatch{});
+                 });
+  stencil_chunks(b, resid,
+                 [&](StencilBatch& sb, int k, int i) { sb.y[k] = y[i]; sb.out[k] = ydot[i]; },
+                 [&](const StencilBatch& sb, int tiles, size_t c0, int m) {
+                   ResBatch rb{};
+                   for (int k = 0; k < m; k++) {
+                     const llg_res_in& r = *res[resid[c0 + k]];
+                     rb.r[k] = ResIn{r.zn1, r.ycor, r.c0, r.c1, r.c2};
+                   }
+                   hipLaunchKernelGGL(k_rhs_batch<true>, dim3(tiles), block, 0, s,
+                                      (const BatchStatic*)b->d_st, sb, rb);
+                 });
+  return launch_status("LLG_RhsBatch");
+}
+
+int LLG_JtvBatch(llg_batch* b, const double* const* v, double* const* Jv,
+                 const double* const* y, void* stream) {
+  if (!b || !v || !Jv || !y) return -1;
+  hipStream_t s = (hipStream_t)stream;
+  if (batch_refresh(b, s) != 0) return -1;
+  std::vector<int> ids;
+  for (int i = 0; i < (int)b->ctx.size(); i++) {
+    if (!v[i]) continue;
+    if (!Jv[i] || !y[i] || Jv[i] == v[i] || Jv[i] == y[i]) {
+      set_err("LLG_JtvBatch: missing or aliased vector");
+      return -1;
+    }
+    ids.push_back(i);
+    b->ctx[i]->njv++;
+  }
+  stencil_chunks(b, ids,
+                 [&](StencilBatch& sb, int k, int i) {
+                   sb.y[k] = y[i];
+                   sb.v[k] = v[i];
+                   sb.out[k] = Jv[i];
+                 },
+                 [&](const StencilBatch& sb, int tiles, size_t, int) {
+                   hipLaunchKernelGGL(k_jtv_batch, dim3(tiles), dim3(kBatchBX, kBatchBY), 0, s,
+                                      (const BatchStatic*)b->d_st, sb);
+                 });
+  return launch_status("LLG_JtvBatch");
+}
+
+void LLG_BatchDestroy(llg_batch* b) {
+  if (!b) return;
+  hipFree(b->d_st);
+  delete b;
+}
+
+int LLG_SetRhsHook(llg_ctx* ctx, llg_rhs_hook fn, void* data) {
+  if (!ctx) return -1;
+  ctx->rhs_hook = fn;
+  ctx->rhs_hook_data = fn ? data : nullptr;
   return 0;
 }
 

@@ -16,7 +16,7 @@ import numpy as np
 
 from . import native
 from .native import (llg_config, llg_terms, llg_run_opts, llg_run_result, llg_grain,
-                     llg_instance, llg_ensemble_result,
+                     llg_instance, llg_ensemble_result, llg_res_in,
                      EXEC_YMSK, EXEC_COMPACT, ANISO_LINEAR, ANISO_CUBIC, NORM_NONE,
                      NORM_EXACT, NORM_REGULARIZED, BC_PERIODIC, BC_NEUMANN, DEMAG_OFF,
                      DEMAG_PERIODIC, DEMAG_PADDED, GATE_NONE, GATE_MASK, GATE_WINDOW,
@@ -558,6 +558,19 @@ class Ensemble:
             raise RuntimeError("LLG_EnsembleGetState failed")
         return y
 
+    def set_batched(self, mode=1):
+        """mode 1: same-shape instances evaluate f together (one
+        LLG_RhsBatch per meeting); returns the number of shape groups."""
+        g = lib().LLG_EnsembleSetBatched(self.h, mode)
+        if g < 0:
+            raise RuntimeError("LLG_EnsembleSetBatched failed: " + lib().LLG_LastError().decode())
+        return g
+
+    def batch_stats(self):
+        launches, calls = C.c_long(0), C.c_long(0)
+        lib().LLG_EnsembleBatchStats(self.h, C.byref(launches), C.byref(calls))
+        return {"launches": launches.value, "calls": calls.value}
+
     def result(self, i):
         r = llg_run_result()
         lib().LLG_EnsembleGetResult(self.h, i, C.byref(r))
@@ -566,6 +579,64 @@ class Ensemble:
     def close(self):
         if getattr(self, "h", None):
             lib().LLG_EnsembleDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- batches
+class Batch:
+    """One launch per pass over several contexts (include/llg/llg_batch.h).
+    sims: Simulation objects (dense-mask execution)."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.ctx for s in self.sims])
+        self.h = lib().LLG_BatchCreate(len(self.sims), arr)
+        if not self.h:
+            raise RuntimeError("LLG_BatchCreate failed: " + lib().LLG_LastError().decode())
+
+    @staticmethod
+    def _dev(v):
+        return C.cast(lib().N_VGetDeviceArrayPointer_Hip(v.v), C.c_void_p).value
+
+    @classmethod
+    def _ptrs(cls, vs):
+        return (C.c_void_p * len(vs))(*[cls._dev(v) if v is not None else None for v in vs])
+
+    def rhs(self, ys, outs, res=None, stream=None):
+        """ys, outs: HipVector per instance (None skips it); res: per
+        instance None or (zn1, ycor, c0, c1, c2) with HipVector zn1, ycor.
+        Launches on `stream` (default: the first instance's) and waits for
+        the device, so the results are readable from any stream."""
+        keep, rp = [], None
+        if res is not None:
+            rp = (C.c_void_p * len(res))()
+            for i, r in enumerate(res):
+                if r is None:
+                    continue
+                zn1, ycor, c0, c1, c2 = r
+                st = llg_res_in(self._dev(zn1), self._dev(ycor), c0, c1, c2)
+                keep.append(st)
+                rp[i] = C.addressof(st)
+        rc = lib().LLG_RhsBatch(self.h, self._ptrs(ys), self._ptrs(outs), rp,
+                                stream or self.sims[0].stream)
+        if rc != 0 or lib().LLG_DeviceSynchronize() != 0:
+            raise RuntimeError("LLG_RhsBatch failed: " + lib().LLG_LastError().decode())
+
+    def jtv(self, vs, jvs, ys, stream=None):
+        rc = lib().LLG_JtvBatch(self.h, self._ptrs(vs), self._ptrs(jvs), self._ptrs(ys),
+                                stream or self.sims[0].stream)
+        if rc != 0 or lib().LLG_DeviceSynchronize() != 0:
+            raise RuntimeError("LLG_JtvBatch failed: " + lib().LLG_LastError().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().LLG_BatchDestroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -24,6 +24,7 @@ SOURCES = [
     "demag_fft.hip",
     "demag_reg_f64.hip",
     "demag_reg_f32.hip",
+    "demag_reg_batch_f64.hip",
     "cvode_bdf.cpp",
     "geometry.cpp",
     "llg_driver.cpp",

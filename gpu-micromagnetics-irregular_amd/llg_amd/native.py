@@ -89,6 +89,11 @@ class llg_instance(C.Structure):
     ]
 
 
+class llg_res_in(C.Structure):
+    _fields_ = [("zn1", C.c_void_p), ("ycor", C.c_void_p),
+                ("c0", C.c_double), ("c1", C.c_double), ("c2", C.c_double)]
+
+
 class llg_ensemble_result(C.Structure):
     _fields_ = [
         ("wall_s", C.c_double), ("inst_time", C.c_double), ("nst", C.c_long),
@@ -198,6 +203,8 @@ SIGNATURES = {
     "Demag_Apply": (None, [VP, DP, DP]),
     "Demag_ApplyWindowed": (None, [VP, DP, DP, DP]),
     "Demag_SetStream": (None, [VP, VP]),
+    "Demag_ApplyBatch": (C.c_int, [C.c_int, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP),
+                                   C.POINTER(VP), VP]),
     "Demag_GetSelfCoupling": (None, [VP, DP, DP, DP]),
     "Demag_Destroy": (None, [VP]),
     "Demag_SetLdsPadding": (C.c_int, [VP, C.c_int, C.c_int]),
@@ -308,6 +315,14 @@ SIGNATURES = {
     "LLG_EnsembleGetState": (C.c_int, [VP, C.c_int, DP]),
     "LLG_EnsembleGetResult": (C.c_int, [VP, C.c_int, C.POINTER(llg_run_result)]),
     "LLG_EnsembleDestroy": (None, [VP]),
+    "LLG_EnsembleSetBatched": (C.c_int, [VP, C.c_int]),
+    "LLG_EnsembleBatchStats": (C.c_int, [VP, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    # llg_batch.h
+    "LLG_BatchCreate": (VP, [C.c_int, C.POINTER(VP)]),
+    "LLG_BatchSize": (C.c_int, [VP]),
+    "LLG_RhsBatch": (C.c_int, [VP, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), VP]),
+    "LLG_JtvBatch": (C.c_int, [VP, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), VP]),
+    "LLG_BatchDestroy": (None, [VP]),
     "LLGCVodeInit": (C.c_int, [VP, VP, C.c_double, VP]),
     "LLGCVodeReInit": (C.c_int, [VP, C.c_double, VP]),
     "LLGCVodeSetUserData": (C.c_int, [VP, VP]),

@@ -37,6 +37,14 @@ void Demag_ApplyWindowed(DemagData* d, const double* y_dev, const double* w_dev,
                          double* h_out_dev);
 /* stream is a hipStream_t; NULL selects the library default stream */
 void Demag_SetStream(DemagData* d, void* stream);
+/* n fields in one set of launches on `stream`: h[i] = demag of y[i] (times
+ * w[i] when w and w[i] are non-NULL), each exactly what Demag_Apply(d[i],
+ * ...) writes.  Instances with the same FFT plan (grid, padding, FP64,
+ * register kernel variants) share one launch per pass, up to 32 at a time;
+ * the others run their own passes on `stream`.  No reference counterpart
+ * (one grid per process there).  Returns 0, or -1 on a launch error. */
+int  Demag_ApplyBatch(int n, DemagData* const* d, const double* const* y,
+                      const double* const* w, double* const* h, void* stream);
 /* N_aa(0) * strength; zeros when d == NULL (2d_i4/demag_fft.cu:365-377) */
 void Demag_GetSelfCoupling(DemagData* d, double* nxx0, double* nyy0, double* nzz0);
 void Demag_Destroy(DemagData* d);

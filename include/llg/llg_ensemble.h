@@ -48,6 +48,18 @@ int   LLG_EnsembleSize(void* ens);
 int   LLG_EnsembleGetState(void* ens, int i, double* y_host);
 /* statistics of instance i's last advance */
 int   LLG_EnsembleGetResult(void* ens, int i, llg_run_result* r);
+/* mode 1: instances of the same grid shape evaluate f together -- each
+ * instance's f() call waits until every live instance of its shape has
+ * called f, and one LLG_RhsBatch (llg_batch.h) then evaluates all of them:
+ * one normalisation launch, one launch per demag pass and one RHS launch
+ * for the group instead of one set per instance.  Each instance keeps its
+ * own step control, and its results stay bitwise those of its own run.
+ * Every instance then runs on a host thread of its own.  Dense-mask
+ * instances only.  mode 0 restores independent evaluation.  Returns the
+ * number of shape groups, or -1. */
+int   LLG_EnsembleSetBatched(void* ens, int mode);
+/* batched launches and the f() calls they served, summed over groups */
+int   LLG_EnsembleBatchStats(void* ens, long* launches, long* calls);
 void  LLG_EnsembleDestroy(void* ens);
 
 #ifdef __cplusplus

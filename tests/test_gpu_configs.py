@@ -98,8 +98,20 @@ def test_ensemble_64_instances_bitwise_and_against_oracle():
         assert np.array_equal(ens.state(i), run.state()), i
         run.close()
         sim.close()
-    ens.close()
     print("[gpu] all 64 members bit-equal to their single runs", flush=True)
+    # batched evaluation: same-shape members share one launch per pass of f
+    bat = la.Ensemble(members, nthreads=0, **kw)
+    assert bat.set_batched(1) == 4
+    outb = bat.advance(t_end)
+    st = bat.batch_stats()
+    print(f"[gpu] batched: {st['calls']} f calls in {st['launches']} group evaluations", flush=True)
+    assert outb["failed"] == 0 and st["launches"] < st["calls"]
+    for i in range(len(members)):
+        a, b = bat.result(i), ens.result(i)
+        assert (a["nst"], a["nli"], a["nfe"]) == (b["nst"], b["nli"], b["nfe"]), i
+        assert np.array_equal(bat.state(i), ens.state(i)), i
+    bat.close()
+    ens.close()
     small = [k for k, (c, _, _) in enumerate(members) if (c.ng, c.ny) == (256, 256)][:2]
     assert len(small) == 2
     for k in small:

@@ -169,6 +169,22 @@ def ensemble_bench(la, args):
     value = out["inst_time"] / out["wall_s"]
     print(f"[bench] ensemble advanced {out['inst_time']:.1f} instance-time units in "
           f"{out['wall_s']:.3f} s: nst={out['nst']} nli={out['nli']}", file=sys.stderr, flush=True)
+    # the same run with f evaluated per shape group (one launch per pass for
+    # all same-shape instances whose f calls meet); same trajectories
+    batched = {}
+    for mode, name in ((1, "all_shapes"), (2, "small_shapes")):
+        bat = la.Ensemble(members, nthreads=0, **kw)
+        groups = bat.set_batched(mode)
+        outb = bat.advance(args.steps * step)
+        stb = bat.batch_stats()
+        bat.close()
+        batched[name] = {
+            "value": round(outb["inst_time"] / outb["wall_s"], 4),
+            "ratio_to_independent": round(outb["inst_time"] / outb["wall_s"] / value, 3),
+            "shape_groups": groups, "f_calls_batched": stb["calls"],
+            "group_evaluations": stb["launches"], "nst": outb["nst"],
+            "same_step_counts": outb["nst"] == out["nst"]}
+        print(f"[bench] batched f ({name}): {batched[name]}", file=sys.stderr, flush=True)
     # the same instances one at a time (a sample of 4, a short interval each)
     seq_t, seq_wall = 0.0, 0.0
     for cfg, geom, y0 in members[:4]:
@@ -194,7 +210,8 @@ def ensemble_bench(la, args):
                                "rtol=atol=1e-4, own step control each",
                    "cells": ncell, "nst": out["nst"], "nli": out["nli"], "nfe": out["nfe"],
                    "failed": out["failed"],
-                   "sequential_sample_inst_time_per_wall_s": round(seq, 4) if seq else None},
+                   "sequential_sample_inst_time_per_wall_s": round(seq, 4) if seq else None,
+                   "batched_f": batched},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(rec), flush=True)

@@ -23,6 +23,9 @@
 
 namespace {
 
+// mode 2 batches the groups of at most this many cells per instance
+constexpr long kBatchSmallCells = 1L << 17;
+
 struct Ensemble;
 
 // Instances of one grid shape whose f() calls meet: each member's call
@@ -194,7 +197,7 @@ int LLG_EnsembleAdvance(void* ens, double t_end, llg_ensemble_result* res) {
       const int i = next.fetch_add(1);
       if (i >= n) return;
       e->rc[i] = LLG_RunAdvance(e->runs[i], t_end, &e->last[i]);
-      if (batched) group_leave(&e->members[i]);
+      if (batched && e->members[i].g) group_leave(&e->members[i]);
     }
   };
   // batched: every instance needs a thread of its own (a group waits for
@@ -225,7 +228,7 @@ int LLG_EnsembleAdvance(void* ens, double t_end, llg_ensemble_result* res) {
 
 int LLG_EnsembleSetBatched(void* ens, int mode) {
   Ensemble* e = static_cast<Ensemble*>(ens);
-  if (!e || mode < 0 || mode > 1) return -1;
+  if (!e || mode < 0 || mode > 2) return -1;
   hipDeviceSynchronize();
   clear_groups(e);
   if (mode == 0) return 0;
@@ -238,6 +241,9 @@ int LLG_EnsembleSetBatched(void* ens, int mode) {
   }
   e->members.assign(n, Member{});
   for (auto& kv : by_shape) {
+    // mode 2: only the small grids, whose instances are bound by kernel
+    // dispatch rather than by memory traffic
+    if (mode == 2 && (long)kv.first.first * kv.first.second > kBatchSmallCells) continue;
     auto g = std::make_unique<Group>();
     g->members = kv.second;
     const int m = (int)g->members.size();
@@ -265,6 +271,7 @@ int LLG_EnsembleSetBatched(void* ens, int mode) {
     e->groups.push_back(std::move(g));
   }
   for (int i = 0; i < n; i++) {
+    if (!e->members[i].g) continue;   // evaluates f on its own
     if (hipEventCreateWithFlags(&e->members[i].ready, hipEventDisableTiming) != hipSuccess) {
       clear_groups(e);
       return -1;

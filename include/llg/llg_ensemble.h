@@ -55,8 +55,10 @@ int   LLG_EnsembleGetResult(void* ens, int i, llg_run_result* r);
  * for the group instead of one set per instance.  Each instance keeps its
  * own step control, and its results stay bitwise those of its own run.
  * Every instance then runs on a host thread of its own.  Dense-mask
- * instances only.  mode 0 restores independent evaluation.  Returns the
- * number of shape groups, or -1. */
+ * instances only.  mode 2 batches only the shapes of at most 2^17 cells
+ * (whose instances are bound by kernel dispatch, not memory traffic); the
+ * others evaluate f on their own.  mode 0 restores independent
+ * evaluation.  Returns the number of batched shape groups, or -1. */
 int   LLG_EnsembleSetBatched(void* ens, int mode);
 /* batched launches and the f() calls they served, summed over groups */
 int   LLG_EnsembleBatchStats(void* ens, long* launches, long* calls);

@@ -183,3 +183,22 @@ def test_ensemble_batched_equals_independent_ensemble():
         assert np.array_equal(ens.state(i), ref.state(i)), i
     ens.close()
     ref.close()
+
+
+def test_ensemble_batched_small_shapes_only():
+    """mode 2: the small grid's instances evaluate f together, the large
+    grid's on their own; trajectories unchanged."""
+    members = la.ensemble_members(n=5, seed=5, grids=((256, 128), (512, 512)), frac=(0.3, 0.7))
+    kw = dict(rtol=1e-4, atol=1e-4)
+    ref = la.Ensemble(members, **kw)
+    ref.advance(0.3)
+    ens = la.Ensemble(members, **kw)
+    assert ens.set_batched(2) == 1
+    ens.advance(0.3)
+    st = ens.batch_stats()
+    assert 0 < st["launches"] <= st["calls"]
+    for i in range(len(members)):
+        assert ens.result(i)["nst"] == ref.result(i)["nst"], i
+        assert np.array_equal(ens.state(i), ref.state(i)), i
+    ens.close()
+    ref.close()

@@ -359,6 +359,79 @@ def gpu_same_window(la, sim, y0, t_end, ropts, dt=0.01):
             "nst": o["nst"], "unit": "sim-time/wall-s"}
 
 
+def hysteresis_bench(la, args):
+    """The hysteresis sweep on the 4096x4096 polycrystal (400 grains, 10 %
+    dead grains as holes, compact execution, zero-padded 8192x8192 FFT demag
+    of strength 1, the i3 constants): h_ext along +x ramped from -0.2 to
+    +0.2 in `--stages` quasi-static stages of `--t-stage` each (LLG_RunStage:
+    the field set in the RHS, Jv and preconditioner term sets, the
+    integrator re-initialised at the current state).  The plain FP64 run
+    and the FP32-spectral-demag form run side by side, stage by stage, from
+    the same random initial state; value = simulated time / summed stage
+    wall time of the FP64 run, and each stage reports the FP32 form's max
+    |dm| against the FP64 state at the stage's end.  Host copies of the
+    states for that comparison sit outside the timed stages."""
+    ng = ny = 4096
+    t0 = time.perf_counter()
+    geom = la.Geometry.polycrystal(ng, ny, num_grains=400, dead_frac=0.10, seed=5)
+    y0 = geom.ic_random_sphere(seed=3)
+    fields = np.zeros((args.stages, 3))
+    fields[:, 0] = np.linspace(-0.2, 0.2, args.stages)
+    which = la.TERMS_RHS | la.TERMS_JV | la.TERMS_PC
+    dead = int((geom.active == 0).sum())
+    print(f"[bench] hysteresis: 4096x4096 polycrystal, {dead} hole cells; geometry "
+          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    forms = {}
+    for name, fp32 in (("fp64", 0), ("fp32_demag", 1)):
+        cfg = la.preset("polycrystal", ng=ng, ny=ny, demag=1.0, compact=True, padded=True,
+                        happ=tuple(fields[0]))
+        cfg.demag_fp32 = fp32
+        sim = la.Simulation(cfg, geom)
+        run = sim.integrator(y0, rtol=1e-4, atol=1e-4, maxl=5, max_order=5,
+                             gstype=args.gstype, t_out=args.t_stage, mxstep=1000000)
+        forms[name] = {"sim": sim, "run": run, "wall": 0.0, "nst": 0, "nli": 0, "m": []}
+    per_stage = []
+    for k in range(args.stages):
+        ys = {}
+        for name, f in forms.items():
+            m, r = f["run"].stage(fields[k], args.t_stage, which=which)
+            f["wall"] += r["wall_s"]
+            f["nst"] += r["nst"]
+            f["nli"] += r["nli"]
+            f["m"].append(m.tolist())
+            ys[name] = f["run"].state()
+        dm = float(np.max(np.abs(ys["fp32_demag"] - ys["fp64"])))
+        per_stage.append({"hx": round(float(fields[k, 0]), 4), "mx": round(forms["fp64"]["m"][-1][0], 5),
+                          "max_dm_fp32": dm})
+        print(f"[bench] stage {k + 1}/{args.stages} hx={fields[k, 0]:+.3f}: m_x "
+              f"{forms['fp64']['m'][-1][0]:+.4f}, fp32 max|dm| {dm:.2e}", file=sys.stderr,
+              flush=True)
+    span = args.stages * args.t_stage
+    out = {}
+    for name, f in forms.items():
+        out[name] = {"value": round(span / f["wall"], 4), "wall_s": round(f["wall"], 3),
+                     "nst": f["nst"], "nli": f["nli"]}
+        f["run"].close()
+        f["sim"].close()
+    value = out["fp64"]["value"]
+    rec = {
+        "metric": "sim-time/wall-s at rtol=1e-4 (4096x4096 polycrystal hysteresis sweep)",
+        "value": value, "unit": "sim-time/wall-s", "n_gpus": 1, "steps": args.stages,
+        "warmup": 0, "ms_per_step": round(1e3 * out["fp64"]["wall_s"] / args.stages, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded polycrystal, random initial m)",
+        "config": {"workload": "4096x4096 polycrystal, 400 grains, 10% dead (holes), compact, "
+                               "zero-padded 8192x8192 FFT demag 1.0, h_ext along +x from -0.2 to "
+                               f"+0.2 in {args.stages} stages of {args.t_stage} time units, "
+                               "rtol=atol=1e-4, Krylov 5, block-Jacobi",
+                   "stages": args.stages, "t_stage": args.t_stage,
+                   "forms": out, "per_stage": per_stage,
+                   "fp32_over_fp64": round(out["fp32_demag"]["value"] / value, 3)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -378,7 +451,7 @@ def main():
                     help="reduction wait: 1 poll the pinned result word (default), 0 stream sync")
     ap.add_argument("--fp32-demag", action="store_true", help="FP32 spectral demag form")
     ap.add_argument("--workload", default="square_hole",
-                    choices=["square_hole", "ref512", "polycrystal", "ensemble"],
+                    choices=["square_hole", "ref512", "polycrystal", "ensemble", "hysteresis"],
                     help="square_hole: 1536x512 cells, padded demag (default); ref512: the "
                          "reference's published 512x512 periodic i1 run (use --steps 1000); "
                          "polycrystal: 1024x1024 Voronoi polycrystal (72 grains, axes uniform "
@@ -388,6 +461,9 @@ def main():
                     help="skip the FP32-demag / learned-preconditioner forms measured beside "
                          "the plain run (square_hole and polycrystal workloads)")
     ap.add_argument("--instances", type=int, default=64, help="ensemble size")
+    ap.add_argument("--stages", type=int, default=40, help="hysteresis: field stages")
+    ap.add_argument("--t-stage", type=float, default=0.05,
+                    help="hysteresis: simulated time per stage")
     ap.add_argument("--learned-pc", action="store_true",
                     help="train the learned block-Jacobi correction (on states of a "
                          "trajectory from another seed, untimed) and use it in the timed run")
@@ -437,6 +513,8 @@ def main():
     la.lib().NVHip_SetSyncMode(args.sync_mode)
     if args.workload == "ensemble":
         return ensemble_bench(la, args)
+    if args.workload == "hysteresis":
+        return hysteresis_bench(la, args)
 
     ref = args.workload == "ref512"
     poly = args.workload == "polycrystal"

@@ -459,6 +459,20 @@ class Run:
         lib().LLG_RunGetState(self.h, _dp(y))
         return y
 
+    def stage(self, happ, t_stage, which=TERMS_RHS | TERMS_PC):
+        """LLG_RunStage: set the applied field, re-initialise the integrator
+        at the current state and advance t_stage; returns the statistics
+        and the stage's mean magnetisation"""
+        h = np.ascontiguousarray(happ, dtype=np.float64).reshape(3)
+        m = np.zeros(3)
+        res = llg_run_result()
+        rc = lib().LLG_RunStage(self.h, which, _dp(h), t_stage, C.byref(res), _dp(m))
+        out = {"retval": res.retval, "wall_s": res.wall_s, "t": res.t_reached,
+               "sim_per_wall": res.sim_per_wall, **res.st.as_dict()}
+        if rc != 0:
+            raise RuntimeError(f"stage failed (rc {rc}) at t={res.t_reached}")
+        return m, out
+
     def ramp(self, fields, t_stage, which=TERMS_RHS | TERMS_PC):
         """LLG_RunRamp: one quasi-static stage per row of `fields` (n x 3),
         t_stage each; returns the per-stage mean magnetisation (n x 3) and

@@ -441,6 +441,9 @@ def main():
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--periodic", action="store_true", help="reference periodic demag instead of padded")
     ap.add_argument("--profile-kernel", default="demag_col", choices=sorted(KERNELS))
+    ap.add_argument("--profile-every", type=int, default=8,
+                    help="time one launch in this many of the roofline kernel (HIP events "
+                         "on its stream, inside the timed region)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -593,6 +596,9 @@ def main():
     run = sim.integrator(y0, **ropts)
     L = la.lib()
     L.LLG_ProfileStart(KERNELS[args.profile_kernel], 200000)
+    # every 8th launch of the kernel carries the timing events: an event pair
+    # delays the next dispatch (about 12 us per demag on this workload)
+    L.LLG_ProfileSetEvery(args.profile_every)
     out = run.advance(args.steps * step)
     tot_ms, nl, dropped = C.c_double(), C.c_long(), C.c_long()
     L.LLG_ProfileStop(C.byref(tot_ms), C.byref(nl), C.byref(dropped))
@@ -619,7 +625,7 @@ def main():
                                                           and ny == 512 and not args.fp32_demag)
                                         else "other"),
             "bytes_per_launch": nbytes, "avg_launch_us": round(avg_s * 1e6, 2),
-            "launches": nl.value}
+            "launches": nl.value, "timed_every": args.profile_every}
 
     ab = None
     if not args.no_ab and not ref and not args.learned_pc and not args.fp32_demag:

@@ -1082,6 +1082,9 @@ int LLG_ProfileStart(int kernel_id, int capacity) {
   st.ev.clear();
   st.used = 0;
   st.dropped = 0;
+  st.every = 1;
+  st.seen = 0;
+  st.open = false;
   st.target = kprof::NONE;
   if (kernel_id == kprof::NONE) return 0;
   if (capacity < 1) capacity = 1;
@@ -1089,6 +1092,14 @@ int LLG_ProfileStart(int kernel_id, int capacity) {
   for (auto& e : st.ev)
     if (hipEventCreate(&e) != hipSuccess) return -1;
   st.target = kernel_id;
+  return 0;
+}
+
+int LLG_ProfileSetEvery(int every) {
+  if (every < 1) return -1;
+  kprof::State& st = kprof::state();
+  st.every = every;
+  st.seen = 0;
   return 0;
 }
 

@@ -232,6 +232,7 @@ SIGNATURES = {
     "LLG_GetCallCounts": (C.c_int, [VP, C.POINTER(C.c_long), C.POINTER(C.c_long),
                                     C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     "LLG_ProfileStart": (C.c_int, [C.c_int, C.c_int]),
+    "LLG_ProfileSetEvery": (C.c_int, [C.c_int]),
     "LLG_ProfileStop": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     "LLG_EventCreate": (VP, []),
     "LLG_EventRecord": (C.c_int, [VP, VP]),

@@ -132,6 +132,10 @@ int      LLG_GetCallCounts(llg_ctx* ctx, long* nf, long* njv, long* npsetup, lon
  * 4 block-Jacobi solve, 5 demag row forward, 6 demag column, 7 demag row
  * inverse, 8 whole demag.  capacity = launches to record. */
 int LLG_ProfileStart(int kernel_id, int capacity);
+/* time one launch in `every` of the armed kernel (an event pair between two
+ * kernels delays the next dispatch by a few microseconds); after
+ * LLG_ProfileStart, which resets it to 1 */
+int LLG_ProfileSetEvery(int every);
 int LLG_ProfileStop(double* total_ms, long* launches, long* dropped);
 /* HIP event helpers for timing from a host binding (diagnostics) */
 void*  LLG_EventCreate(void);

@@ -129,6 +129,12 @@ struct Integrator {
   N_Vector ycur = nullptr, fcur = nullptr;   // borrowed
   N_Vector lsx = nullptr;                     // linear-solver solution
   bool newton_init = false;   // the predictor also set acor = 0, y = zn[0]
+  // the Newton correction of the current solve (fused.lag_update reads it)
+  N_Vector nls_ycor = nullptr;
+  // the solve's solution and first Newton update were queued behind it on
+  // the device (fused.lag_update); the update's norm is read with the ticket
+  bool upd_queued = false;
+  unsigned upd_ticket = 0;
   bool ewt_pending = false;   // the step's error weights left to the predictor
   // The Newton residual not yet formed: dst = sum_k c_k X_k is left to the
   // first pass of the lagged SPGMR solve, which forms it where it reads it
@@ -677,6 +683,14 @@ int spgmr_lagged_dev(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vect
   };
   for (int l = 0; l <= kq; l++)
     if (pass(l, l == kq) != 0) return kLaggedFallback;
+  // on a Newton iteration's first solve the solution and the Newton update
+  // follow each part on the device (they do nothing unless the solve ends
+  // there with a solution the iteration uses), so the iteration waits once
+  const bool upd = f.lag_update && f.lag_update_wait && m->nls_curiter == 0 && x == b &&
+                   m->nls_ycor && m->fused.newtonupd && !f.newtonupd_comb;
+  unsigned ticket = 0;
+  bool queued = upd && f.lag_update(b, l_max, V, s, x, m->nls_ycor, m->zn[0], m->ewt, m->y,
+                                    &ticket) == 0;
   llgcv_lag_result res;
   if (f.lag_wait(b, &res) != 0) return kLaggedFallback;
   if (res.status == LLGCV_LAG_RUNNING) {
@@ -684,6 +698,8 @@ int spgmr_lagged_dev(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vect
     for (int l = kq + 1; l < l_max; l++)
       if (pass(l, 0) != 0) return kLaggedFallback;
     if (f.lag_projnorm(W[(l_max - 1) & 1], l_max, V) != 0) return kLaggedFallback;
+    queued = upd && f.lag_update(b, l_max, V, s, x, m->nls_ycor, m->zn[0], m->ewt, m->y,
+                                 &ticket) == 0;
     if (f.lag_wait(b, &res) != 0) return kLaggedFallback;
   }
   m->lag_pred = std::min(res.passes + 1, l_max - 1);
@@ -712,6 +728,14 @@ int spgmr_lagged_dev(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vect
   for (int i = 1; i <= krydim; i++) m->yg[i] = 0.0;
   if (qr_sol(krydim, m->Hes, m->givens, m->yg) != 0) return LS_QRSOL_FAIL;
   if (converged || res.rho < res.r_norm) {
+    m->pend_b.on = false;   // x overwrites b
+    if (queued) {
+      // the device formed x and the Newton update with the same operations
+      // (its final step repeated this qr_sol and test)
+      m->upd_queued = true;
+      m->upd_ticket = ticket;
+      return converged ? LS_SUCCESS : LS_RES_REDUCED;
+    }
     auto& p = m->pend_x;
     p.k = krydim;
     for (int k = 0; k < krydim; k++) {
@@ -721,7 +745,6 @@ int spgmr_lagged_dev(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vect
     p.s = s;
     p.dst = x;
     p.on = true;
-    m->pend_b.on = false;   // x overwrites b
     if (!f.newtonupd_comb) form_x(m);
     return converged ? LS_SUCCESS : LS_RES_REDUCED;
   }
@@ -1268,6 +1291,8 @@ int newton_solve(Integrator* m, N_Vector ycor, double tol, bool call_lsetup, lon
   N_Vector del = m->delta;
   m->y_current = m->newton_init;   // y = zn[0] + ycor already formed by the predictor
   m->newton_init = false;
+  m->nls_ycor = ycor;
+  m->upd_queued = false;
   for (;;) {
     rv = nls_residual(m, ycor, del);
     if (rv != 0) break;
@@ -1281,6 +1306,8 @@ int newton_solve(Integrator* m, N_Vector ycor, double tol, bool call_lsetup, lon
     for (;;) {
       (*niters)++;
       rv = nls_lsolve(m, del);   // del holds -res: solve (I - gamma J) del = -res
+      const bool upd = m->upd_queued;
+      m->upd_queued = false;
       if (rv != 0) {
         m->pend_b.on = m->pend_x.on = false;   // del is not read again
         break;
@@ -1288,10 +1315,18 @@ int newton_solve(Integrator* m, N_Vector ycor, double tol, bool call_lsetup, lon
       double dn = 0.0, yn = -1.0;
       const bool vec2 = m->nls_curiter > 0 && del->ops->nvwrmsnormvectorarray;
       auto& px = m->pend_x;
-      const bool comb = px.on && px.dst == del && m->nls_curiter == 0 &&
+      const bool comb = !upd && px.on && px.dst == del && m->nls_curiter == 0 &&
                         m->fused.newtonupd_comb(ycor, px.k, px.c, px.X, px.s, m->zn[0], m->ewt,
                                                 m->y, &dn) == 0;
-      if (comb) {
+      if (upd) {
+        // ycor += del, y = zn[0] + ycor and the norm ran behind the solve
+        if (m->fused.lag_update_wait(ycor, m->upd_ticket, &dn) != 0) {
+          rv = LLGCV_LSOLVE_FAIL;
+          break;
+        }
+        m->y_current = true;
+        rv = nls_convtest_norms(m, ycor, dn, -1.0, tol);
+      } else if (comb) {
         // the solution formed where the update reads it (del not written)
         px.on = false;
         m->y_current = true;

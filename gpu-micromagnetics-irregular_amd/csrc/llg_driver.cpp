@@ -67,7 +67,7 @@ void LLG_DefaultRunOpts(llg_run_opts* o) {
   o->gstype = 0;        // auto: classical iff neq < 500000 (2d_i4/2d_fft.cu:645-650)
   o->max_order = 5;
   o->mxstep = 500;
-  o->fused = 1815;
+  o->fused = 3863;
   o->use_jtv = 1;
   o->use_prec = 1;
   o->pretype = SUN_PREC_LEFT;
@@ -149,6 +149,12 @@ void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_hos
         fo.lag_projnorm = NVHip_LagProjNorm;
         fo.lag_wait = lag_wait;
         fo.lag_next = NVHip_LagNext;
+        // bit 11: each solve's solution and first Newton update queued
+        // behind it (one wait per Newton iteration instead of two)
+        if (opts->fused & 2048) {
+          fo.lag_update = NVHip_LagUpdate;
+          fo.lag_update_wait = NVHip_LagUpdateWait;
+        }
       }
     }
     LLGCVodeSetFusedOps(r->cv, &fo);

@@ -22,6 +22,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -1990,7 +1991,12 @@ struct LagDev {
   double nu[kMaxK];
   double coef[kMaxK + 1];   // the next pass's a, b_0 .. b_l
   double r_norm, rotprod, rho, vk;
+  double xc[kMaxK];         // the solution's coefficients c_k = y_k / |u_k|
   int done, pskip, status, krydim, iters, passes;
+  int xk;                   // basis vectors in the solution
+  int xready;               // the solution is formed (status and qr_sol as the host's exit)
+  int xskip;                // the queued solution update found none (NVHip_LagUpdate)
+  int xpad;
 };
 struct LagPub {
   NVHipLagResult r;
@@ -2094,6 +2100,39 @@ __device__ bool lag_column(LagDev* st, int l, const double* r, int maxl, double 
   return false;
 }
 
+// the host loop's qr_sol: the rotations applied to the right-hand side,
+// then back substitution; false on a zero diagonal
+__device__ bool lag_qr_sol(int n, const double (*H)[kMaxK], const double* q, double* b) {
+  for (int k = 0; k < n; k++) {
+    const double c = q[2 * k], s = q[2 * k + 1], t1 = b[k], t2 = b[k + 1];
+    b[k] = c * t1 - s * t2;
+    b[k + 1] = s * t1 + c * t2;
+  }
+  for (int k = n - 1; k >= 0; k--) {
+    if (H[k][k] == 0.0) return false;
+    b[k] /= H[k][k];
+    for (int i = 0; i < k; i++) b[i] -= b[k] * H[i][k];
+  }
+  return true;
+}
+
+// a solve that has ended: the coefficients of its solution x =
+// sum_k c_k u_k / s, formed exactly as the host's exit forms them (the
+// same statuses, qr_sol and residual test; c_k = y_k / |u_k|)
+__device__ void lag_solution(LagDev* st) {
+  st->xready = 0;
+  if (st->status != NVHIP_LAG_CONVERGED && st->status != NVHIP_LAG_MAXL) return;
+  const int n = st->krydim;
+  double yg[kLagMax + 1];
+  yg[0] = st->r_norm;
+  for (int i = 1; i <= n; i++) yg[i] = 0.0;
+  if (!lag_qr_sol(n, st->H, st->q, yg)) return;
+  if (!(st->status == NVHIP_LAG_CONVERGED || st->rho < st->r_norm)) return;
+  for (int k = 0; k < n; k++) st->xc[k] = yg[k] / st->nu[k];
+  st->xk = n;
+  st->xready = 1;
+}
+
 __device__ void lag_publish(const LagDev* st, LagPub* pub, unsigned seq, int maxl) {
   NVHipLagResult& o = pub->r;
   o.status = st->status;
@@ -2122,6 +2161,7 @@ __device__ bool lag_run(LagDev* st, const double* r, int p, int mode, int maxl, 
   if (p == 0) {
     st->done = 0;
     st->pskip = 1;
+    st->xready = 0;
     st->status = LAG_RUNNING;
     st->iters = 0;
     st->passes = 0;
@@ -2173,11 +2213,33 @@ __global__ __launch_bounds__(kFinishBlock) void k_lag_step(int nblk, const doubl
   sum_partials<K, false, kFinishBlock / 64>(RED_SUM, nblk, partials, sm, r);   // barriers inside
   LagDev* st = reinterpret_cast<LagDev*>(lst);
   bool ended = false;
-  if (threadIdx.x == 0) ended = lag_run(st, r, p, mode, maxl, delta, bfloor, nlen);
+  if (threadIdx.x == 0) {
+    ended = lag_run(st, r, p, mode, maxl, delta, bfloor, nlen);
+    if (ended) lag_solution(st);
+  }
   __syncthreads();
   for (int i = threadIdx.x; i < NW; i += blockDim.x) reinterpret_cast<double*>(gst)[i] = lst[i];
   // a solve still running after the last step queued so far publishes too
   if (threadIdx.x == 0 && (ended || last)) lag_publish(st, pub, seq, maxl);
+}
+
+// x = sum_k c_k X_k / s with the coefficients of the ended solve (as
+// k_lincombdiv); when the solve left no solution to use, nothing -- and the
+// Newton update queued behind (skip flag xskip) does nothing either
+// (xinfo = {xk, xready} and xc of the state, xskip its flag: separate
+// restrict pointers, so the uniform reads are scalar loads)
+__global__ void k_lincombdiv_lag(const int* __restrict__ xinfo, const double* __restrict__ xc,
+                                 int* __restrict__ xskip, PtrPack X,
+                                 const double* __restrict__ sv, double* x, sunindextype n) {
+  const int go = xinfo[1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *xskip = go ? 0 : 1;
+  if (!go) return;
+  const int nv = xinfo[0];
+  GRID_STRIDE(i, n) {
+    double acc = 1.0 * 0.0;
+    for (int j = 0; j < nv; j++) acc = fma(xc[j], X.p[j][i], acc);
+    x[i] = acc / sv[i];
+  }
 }
 
 // u = a w - sum_j b_j U_j with (a, b) read from device memory, |u|^2
@@ -2279,6 +2341,54 @@ int NVHip_LagProjNorm(N_Vector wv, int nb, N_Vector* U) {
                      (const int*)&w->lag->pskip);
   lag_step_dispatch(1, s, g, w, 0, 1, 0);
   check_launch("lag projection norm");
+  return 0;
+}
+
+int NVHip_LagUpdate(N_Vector b, int nvec, N_Vector* V, N_Vector sv, N_Vector x, N_Vector ycor,
+                    N_Vector zn0, N_Vector w, N_Vector y, unsigned* ticket) {
+  if (!b || !V || !sv || !x || !ycor || !zn0 || !w || !y || !ticket || !is_hip(b) ||
+      nvec < 1 || nvec > kMaxK || g_red_mode == 0)
+    return -1;
+  const sunindextype n = LEN(b);
+  hipStream_t s = S(b);
+  if (LEN(sv) != n || LEN(x) != n || LEN(ycor) != n || LEN(zn0) != n || LEN(w) != n ||
+      LEN(y) != n || S(ycor) != s || S(x) != s)
+    return -1;
+  // the Newton update's aliasing rules (NVHip_NewtonUpdate), and x not a
+  // basis vector or the scale (every element is read before x is written,
+  // but x == X_j would change the values other elements read)
+  if (D(y) == D(ycor) || D(y) == D(x) || D(y) == D(zn0) || D(ycor) == D(x) || D(x) == D(sv))
+    return -1;
+  RedWs* wk = workspace(s);
+  if (!wk->lag) return -1;
+  PtrPack X{};
+  for (int j = 0; j < nvec; j++) {
+    if (!V[j] || LEN(V[j]) != n || D(V[j]) == D(x) || D(V[j]) == D(y)) return -1;
+    X.p[j] = D(V[j]);
+  }
+  static_assert(offsetof(LagDev, xready) == offsetof(LagDev, xk) + sizeof(int),
+                "xk, xready adjacent");
+  hipLaunchKernelGGL(k_lincombdiv_lag, dim3(ew_grid(n)), dim3(kBlock), 0, s,
+                     (const int*)&wk->lag->xk, (const double*)wk->lag->xc, &wk->lag->xskip, X,
+                     D(sv), D(x), n);
+  const int g = red_grid(n);
+  const unsigned seq = ++wk->seq;
+  const int* skip = &wk->lag->xskip;
+  hipLaunchKernelGGL(k_reduce1<FNewtonUpd0>, dim3(g), dim3(kBlock), 0, s, RED_SUM,
+                     FNewtonUpd0{D(x), D(w), D(zn0), D(ycor), D(y)}, n, wk->partials,
+                     (unsigned*)nullptr, wk->host_dev, (double*)nullptr, seq, skip);
+  launch_finish<1>(s, RED_SUM, g, wk, seq, 0, skip);
+  check_launch("lag update");
+  *ticket = seq;
+  return 0;
+}
+
+int NVHip_LagUpdateWait(N_Vector ycor, unsigned ticket, sunrealtype* dnorm) {
+  if (!ycor || !dnorm || !is_hip(ycor)) return -1;
+  hipStream_t s = S(ycor);
+  RedWs* w = workspace(s);
+  if (!wait_result_checked(s, w, ticket)) return -1;
+  *dnorm = std::sqrt(w->host[0] / (double)LEN(ycor));
   return 0;
 }
 

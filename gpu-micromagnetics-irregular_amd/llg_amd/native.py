@@ -190,6 +190,9 @@ SIGNATURES = {
                                  C.POINTER(VP), C.POINTER(VP), C.POINTER(VP)]),
     "NVHip_LagStep": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, C.c_int]),
     "NVHip_LagNext": (C.c_int, [VP]),
+    "NVHip_LagUpdate": (C.c_int, [VP, C.c_int, C.POINTER(VP), VP, VP, VP, VP, VP, VP,
+                                  C.POINTER(C.c_uint)]),
+    "NVHip_LagUpdateWait": (C.c_int, [VP, C.c_uint, DP]),
     "NVHip_LagProjNorm": (C.c_int, [VP, C.c_int, C.POINTER(VP)]),
     "NVHip_LagWait": (C.c_int, [VP, VP]),
     "NVHip_ProjNorm": (C.c_int, [C.c_int, C.c_double, DP, VP, C.POINTER(VP), VP, DP]),

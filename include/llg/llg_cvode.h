@@ -208,6 +208,15 @@ typedef struct llgcv_fused_ops {
   int (*lag_projnorm)(N_Vector w, int nb, N_Vector* U);
   int (*lag_wait)(N_Vector v, llgcv_lag_result* res);
   int (*lag_next)(N_Vector v);
+  /* with the device-driven solve, on a Newton iteration's first solve:
+   * the solution and the first Newton update queued behind each part of
+   * the solve before its wait (as NVHip_LagUpdate; they do nothing unless
+   * the solve ends there with a solution the iteration uses), and
+   * lag_update_wait for the update's norm (as NVHip_LagUpdateWait) -- one
+   * host wait less per Newton iteration, the same bits */
+  int (*lag_update)(N_Vector b, int nvec, N_Vector* V, N_Vector s, N_Vector x, N_Vector ycor,
+                    N_Vector zn0, N_Vector w, N_Vector y, unsigned* ticket);
+  int (*lag_update_wait)(N_Vector ycor, unsigned ticket, sunrealtype* dnorm);
 } llgcv_fused_ops;
 
 void* LLGCVodeCreate(int lmm, SUNContext sunctx);

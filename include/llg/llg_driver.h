@@ -57,9 +57,13 @@ typedef struct llg_run_opts {
                          first pass not formed in place): each lagged
                          SPGMR solve queued whole, its scalar steps on
                          the device (NVHip_Lag*), one host wait per
-                         solve; default 1815 (bits 0, 1, 2, 4, 8, 9, 10).
-                         Bits 1 to 3, 8, 9 and 10 change no value (bit 10
-                         none of bit 4's). */
+                         solve; bit 11 (with bit 10): the solution and
+                         the first Newton update of each iteration queued
+                         behind the solve (NVHip_LagUpdate), no host
+                         round trip between them; default 3863 (bits 0,
+                         1, 2, 4, 8, 9, 10, 11).  Bits 1 to 3 and 8 to 11
+                         change no value (bits 10 and 11 none of bit
+                         4's). */
   long   mxstep;      /* max steps per CVode call (CVODE default 500) */
   int    use_jtv;     /* analytic Jv (else difference quotient) */
   int    use_prec;    /* block-Jacobi preconditioner */

@@ -281,6 +281,22 @@ int NVHip_LagWait(N_Vector v, NVHipLagResult* res);
  * that wait NVHip_LagNext opens the next part (further passes, the
  * projection norm) with its own wait. */
 int NVHip_LagNext(N_Vector v);
+/* Queued behind the last part of a solve on b's stream: its solution
+ * x = sum_k c_k V[k] / s (the coefficients c_k = y_k / |u_k| from the
+ * solve's final step, which repeats the host exit's qr_sol and residual
+ * test; rounded as NVHip_LinCombDiv) and then the first Newton update with
+ * x (NVHip_NewtonUpdate with iter 0: ycor += x, y = zn0 + ycor, the sum of
+ * (x w)^2).  Both do nothing when the solve leaves no solution to use
+ * (status other than converged or Krylov cap, a zero diagonal, or a cap
+ * without residual reduction), and a part that ends with the solve still
+ * running leaves none.  nvec: the basis vectors passed (>= the solve's
+ * columns).  NVHip_LagUpdateWait waits for the update queued as *ticket:
+ * dnorm = sqrt(sum / n), or -1 when the update did nothing.  No other
+ * reduction of the stream may be waited for in between.  No reference
+ * counterpart. */
+int NVHip_LagUpdate(N_Vector b, int nvec, N_Vector* V, N_Vector s, N_Vector x, N_Vector ycor,
+                    N_Vector zn0, N_Vector w, N_Vector y, unsigned* ticket);
+int NVHip_LagUpdateWait(N_Vector ycor, unsigned ticket, sunrealtype* dnorm);
 
 /* Reductions for kernels outside this file (no counterpart in the
  * reference): a kernel queued on v's stream leaves K partial sums per

@@ -448,7 +448,7 @@ void k_col_cb(C2<T>* __restrict__ spec, int Hx, const C2<T>* __restrict__ tw,
 // exchanges only wait for the wave's own LDS accesses and the waves of a CU
 // drift apart.  The exchanges move real and imaginary parts separately
 // (RegFft::forward_split), so a wave's LDS buffer is kSlots values of T and
-// 16 waves of 1024-point transforms fit one CU.
+// the 12 waves a CU holds (register-bound, below) need 104 KB of LDS.
 //
 // A workgroup of W = 2 CB waves holds either CB adjacent columns' x and y
 // components (wave 2c: x of column c, wave 2c + 1: its y) or W adjacent
@@ -481,8 +481,14 @@ struct ColWave {
                              kLds <= 160 * 1024 && ROWS % (kThreads / CB) == 0;
 };
 
+// Three waves per SIMD (168 VGPRs): the tensor values are fetched before the
+// work that precedes their use (the z component's before its forward
+// transform, the x/y pair's first half before the exchange of spectra, the
+// second half before that half's exchange); at four waves per SIMD that
+// spilled, and 12 waves per CU with the loads in flight beat 16 waiting on
+// them.
 template <class T, class F, bool HALF, bool XYI, int CB, bool BATCH = false>
-__global__ __launch_bounds__(128 * CB) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__(128 * CB) __attribute__((amdgpu_waves_per_eu(3)))
 void k_col_wave(C2<T>* __restrict__ spec, int Hx, int nxy, const C2<T>* __restrict__ tw,
                 const T* __restrict__ K, BatchArg<T, BATCH> bt) {
   if constexpr (BATCH) {
@@ -541,15 +547,21 @@ void k_col_wave(C2<T>* __restrict__ spec, int Hx, int nxy, const C2<T>* __restri
       if constexpr (B < NB) r[B] = reinterpret_cast<const C*>(mybuf)[lane + 64 * B];
       else r[B] = mk<T>(T(0), T(0));                 // not read by the half-input stage
     });
-    F::template forward_split<HALF>(r, mybuf, lane, tw, WaveBar{});
-    // ---- tensor product
+    T kz[Z ? V : 1];
     if constexpr (Z) {
       const T* kzz = K + 2 * P + (long)kx * L;
       sfor<0, V>([&](auto I) {
         constexpr int i = I;
         const unsigned k = (unsigned)Last::k_of(rfft::opaque(lane), i / Q, i % Q);
-        const T cz = kzz[mirror(k)];
-        r[i] = mk<T>(cz * r[i].x, cz * r[i].y);
+        kz[i] = kzz[mirror(k)];
+      });
+    }
+    F::template forward_split<HALF>(r, mybuf, lane, tw, WaveBar{});
+    // ---- tensor product
+    if constexpr (Z) {
+      sfor<0, V>([&](auto I) {
+        constexpr int i = I;
+        r[i] = mk<T>(kz[i] * r[i].x, kz[i] * r[i].y);
       });
     } else {
       const int comp = wave & 1;
@@ -559,8 +571,21 @@ void k_col_wave(C2<T>* __restrict__ spec, int Hx, int nxy, const C2<T>* __restri
       const C* pbuf = reinterpret_cast<const C*>(lds_all + (wave ^ 1) * kBuf);
       // two rounds over halves of the registers: publish own values,
       // read the partner's at the same positions (same lane -> k map)
+      T pd[V / 2], px[V / 2];
+      auto fetch_half = [&](auto H) {
+        constexpr int HV = V / 2;
+        sfor<0, HV>([&](auto I) {
+          constexpr int i = H * HV + I;
+          const unsigned k = (unsigned)Last::k_of(rfft::opaque(lane), i / Q, i % Q);
+          const unsigned km = mirror(k);
+          pd[I] = kd[km];
+          px[I] = odd_sign(k) * kxyr[km];
+        });
+      };
+      fetch_half(std::integral_constant<int, 0>{});
       sfor<0, 2>([&](auto H) {
         constexpr int HV = V / 2;
+        if constexpr (H == 1) fetch_half(std::integral_constant<int, 1>{});
         __syncthreads();                             // the partner read the previous round
         sfor<0, HV>([&](auto I) { reinterpret_cast<C*>(mybuf)[lane + 64 * I] = r[H * HV + I]; });
         __syncthreads();
@@ -568,7 +593,7 @@ void k_col_wave(C2<T>* __restrict__ spec, int Hx, int nxy, const C2<T>* __restri
           constexpr int i = H * HV + I;
           const unsigned k = (unsigned)Last::k_of(rfft::opaque(lane), i / Q, i % Q);
           const unsigned km = mirror(k);
-          const T d = kd[km], xr = odd_sign(k) * kxyr[km];
+          const T d = pd[I], xr = px[I];
           const C o = pbuf[lane + 64 * I];          // the other component at this k
           C c;
           if constexpr (XYI) {

@@ -1049,9 +1049,11 @@ int Demag_ApplyBatch(int n, DemagData* const* d, const double* const* y,
     const bool reg = !a->fp32 && a->reg_row >= 0 && a->reg_col >= 0 &&
                      demag_reg::has_row_plan<double>(a->px / 2, a->reg_row) &&
                      demag_reg::has_col_plan<double>(a->py, a->reg_col);
-    if (!reg) {   // its own passes, on the batch's stream
+    if (!reg) {   // its own passes, on the batch's stream (its own stream kept)
+      hipStream_t own = a->stream;
       a->stream = s;
       apply(a, y[i], w ? w[i] : nullptr, h[i]);
+      a->stream = own;
       if (!g_derr.empty()) return -1;
       continue;
     }

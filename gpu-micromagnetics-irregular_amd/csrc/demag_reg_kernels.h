@@ -481,14 +481,14 @@ struct ColWave {
                              kLds <= 160 * 1024 && ROWS % (kThreads / CB) == 0;
 };
 
-// Three waves per SIMD (168 VGPRs): the tensor values are fetched before the
-// work that precedes their use (the z component's before its forward
-// transform, the x/y pair's first half before the exchange of spectra, the
-// second half before that half's exchange); at four waves per SIMD that
-// spilled, and 12 waves per CU with the loads in flight beat 16 waiting on
-// them.
+// FP64: three waves per SIMD (168 VGPRs): the tensor values are fetched
+// before the work that precedes their use (the z component's before its
+// forward transform, the x/y pair's first half before the exchange of
+// spectra, the second half before that half's exchange); at four waves per
+// SIMD that spilled, and 12 waves per CU with the loads in flight beat 16
+// waiting on them.  FP32 values take half the registers: four waves.
 template <class T, class F, bool HALF, bool XYI, int CB, bool BATCH = false>
-__global__ __launch_bounds__(128 * CB) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ __launch_bounds__(128 * CB) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4)))
 void k_col_wave(C2<T>* __restrict__ spec, int Hx, int nxy, const C2<T>* __restrict__ tw,
                 const T* __restrict__ K, BatchArg<T, BATCH> bt) {
   if constexpr (BATCH) {

@@ -89,6 +89,13 @@ def cpu_model():
     return "unknown"
 
 
+def m_norm_drift(y, active):
+    """max over the active cells of ||m| - 1| (the integration does not
+    renormalise m on this workload: the drift is the integration error's)"""
+    m = y.reshape(3, -1)[:, active != 0]
+    return float(f"{np.max(np.abs(np.sqrt((m * m).sum(0)) - 1.0)):.3e}") if m.size else 0.0
+
+
 def cpu_baseline(la, cfg, geom, y0, budget_s, threads):
     """The CPU oracle (OpenMP C restatement of the callbacks + serial
     N_Vector + the same BDF integrator) on the same workload from t = 0,
@@ -338,7 +345,8 @@ def north_star_ab(la, cfg, geom, y0, y_plain, out_plain, value_plain, t_end, rop
         forms[name] = {"value": round(v, 4), "ratio_to_plain_fp64": round(v / value_plain, 3),
                        "nst": o["nst"], "nli": o["nli"], "nni": o["nni"], "ncfn": o["ncfn"],
                        "max_dm_vs_rtol1e-6_reference": float(f"{np.max(np.abs(y - y_ref)[act]):.3e}"),
-                       "max_dm_vs_plain": float(f"{np.max(np.abs(y - y_plain)[act]):.3e}")}
+                       "max_dm_vs_plain": float(f"{np.max(np.abs(y - y_plain)[act]):.3e}"),
+                       "m_norm_drift": m_norm_drift(y, geom.active)}
         print(f"[bench] form {name}: {forms[name]}", file=sys.stderr, flush=True)
     apply_us = lpc_apply_us(la, sim64, y0)
     sim64.close()
@@ -346,7 +354,9 @@ def north_star_ab(la, cfg, geom, y0, y_plain, out_plain, value_plain, t_end, rop
             "plain_fp64": {"value": round(value_plain, 4), "nst": out_plain["nst"],
                            "nli": out_plain["nli"], "nni": out_plain["nni"],
                            "ncfn": out_plain["ncfn"],
-                           "max_dm_vs_rtol1e-6_reference": float(f"{dm_plain:.3e}")},
+                           "max_dm_vs_rtol1e-6_reference": float(f"{dm_plain:.3e}"),
+                           "m_norm_drift": m_norm_drift(y_plain, geom.active),
+                           "m_norm_drift_rtol1e-6_reference": m_norm_drift(y_ref, geom.active)},
             "forms": forms, "learned_pc": info, "learned_pc_apply_us": apply_us}
 
 
@@ -675,6 +685,7 @@ def main():
         "data": data,
         "config": {"workload": workload,
                    "t_end": args.steps * step,
+                   "m_norm_drift": m_norm_drift(y_final, geom.active),
                    "spgmr_iters_per_step": round(out["nli"] / max(out["nst"], 1), 3),
                    "nli_per_nni": round(out["nli"] / max(out["nni"], 1), 3),
                    "nst": out["nst"], "nfe": out["nfe"], "nni": out["nni"], "nli": out["nli"],

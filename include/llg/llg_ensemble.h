@@ -11,6 +11,9 @@
  * one's host round trips behind the others' work.  An instance computes
  * exactly what it computes alone (same kernels, same launch shapes, same
  * reduction order): results are bitwise those of single-instance runs.
+ * Optionally (LLG_EnsembleSetBatched) the f evaluations of same-shape
+ * instances are gathered into one LLG_RhsBatch each (llg_batch.h), with
+ * the same bits.
  */
 #ifndef LLG_ENSEMBLE_H
 #define LLG_ENSEMBLE_H

@@ -2018,147 +2018,179 @@ __device__ inline void lag_end(LagDev* st, int status) {
   st->done = 1;
 }
 
-// column l complete with h_{l+1,l} = nu_next: re-orthogonalisation test,
+// The step functions take the column index as a template argument: every
+// index into the state is then a constant (LDS accesses at immediate
+// offsets, the column and the sums in registers), which is what keeps the
+// serial step of one lane short.
+
+// column L complete with h_{L+1,L} = nu_next: re-orthogonalisation test,
 // its Givens rotation (qr_fact of the new column), residual estimate and
 // convergence; true when the solve has ended
-__device__ bool lag_finalize(LagDev* st, int l, double nu_next, int maxl, double delta) {
+template <int L>
+__device__ bool lag_finalize(LagDev* st, double nu_next, int maxl, double delta) {
   if ((kLagGsFactor * nu_next) < st->vk) {
     lag_end(st, NVHIP_LAG_FALLBACK);
     return true;
   }
-  st->nu[l + 1] = nu_next;
-  st->H[l + 1][l] = nu_next;
-  for (int k = 0; k < l; k++) {
-    const double t1 = st->H[k][l], t2 = st->H[k + 1][l], c = st->q[2 * k], s = st->q[2 * k + 1];
-    st->H[k][l] = c * t1 - s * t2;
-    st->H[k + 1][l] = s * t1 + c * t2;
+  st->nu[L + 1] = nu_next;
+  double h[L + 2];
+#pragma unroll
+  for (int k = 0; k <= L; k++) h[k] = st->H[k][L];
+  h[L + 1] = nu_next;
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    const double t1 = h[k], t2 = h[k + 1], c = st->q[2 * k], s = st->q[2 * k + 1];
+    h[k] = c * t1 - s * t2;
+    h[k + 1] = s * t1 + c * t2;
   }
   double c, s;
-  const double t1 = st->H[l][l], t2 = st->H[l + 1][l];
+  const double t1 = h[L], t2 = h[L + 1];
   sp_rotation(t1, t2, c, s);
-  st->q[2 * l] = c;
-  st->q[2 * l + 1] = s;
-  if ((st->H[l][l] = c * t1 - s * t2) == 0.0) {
+  st->q[2 * L] = c;
+  st->q[2 * L + 1] = s;
+  h[L] = c * t1 - s * t2;
+#pragma unroll
+  for (int k = 0; k <= L + 1; k++) st->H[k][L] = h[k];
+  if (h[L] == 0.0) {
     lag_end(st, NVHIP_LAG_QRFACT);
     return true;
   }
-  st->rotprod *= st->q[2 * l + 1];
+  st->rotprod *= s;
   st->rho = fabs(st->rotprod * st->r_norm);
   if (st->rho <= delta) {
     lag_end(st, NVHIP_LAG_CONVERGED);
     return true;
   }
-  if (l + 1 == maxl) {
+  if (L + 1 == maxl) {
     lag_end(st, NVHIP_LAG_MAXL);
     return true;
   }
   return false;
 }
 
-// column l from pass l's sums r: Hessenberg entries, the next pass's
-// coefficients, |A V_l| and the Pythagorean estimate of h_{l+1,l}; a column
+// column L from pass L's sums r: Hessenberg entries, the next pass's
+// coefficients, |A V_L| and the Pythagorean estimate of h_{L+1,L}; a column
 // the estimate shows converged (or the last one, when the estimate is well
 // conditioned) is completed here; true when the solve has ended
-__device__ bool lag_column(LagDev* st, int l, const double* r, int maxl, double delta) {
+template <int L>
+__device__ bool lag_column(LagDev* st, const double* r, int maxl, double delta) {
   st->iters++;
-  st->krydim = l + 1;
-  const double inv = 1.0 / st->nu[l];
+  st->krydim = L + 1;
+  const double inv = 1.0 / st->nu[L];
+  double h[L + 1];
   double hh = 0.0;
-  for (int j = 0; j <= l; j++) {
-    st->H[j][l] = (r[1 + j] * inv) / st->nu[j];
-    hh += st->H[j][l] * st->H[j][l];
-    st->coef[1 + j] = st->H[j][l] / st->nu[j];
+#pragma unroll
+  for (int j = 0; j <= L; j++) {
+    h[j] = (r[1 + j] * inv) / st->nu[j];
+    hh += h[j] * h[j];
+    st->H[j][L] = h[j];
+    st->coef[1 + j] = h[j] / st->nu[j];
   }
   st->coef[0] = inv;
-  const double vk_norm = sqrt(r[2 + l]) * inv;
+  const double vk_norm = sqrt(r[2 + L]) * inv;
   st->vk = vk_norm;
   const double est = vk_norm * vk_norm - hh;
   bool use_est = false;
   if (est > kLagEstFloor * vk_norm * vk_norm) {
-    if (l + 1 == maxl) {
+    if (L + 1 == maxl) {
       use_est = true;
     } else {
-      double t[kLagMax + 1];
-      for (int j = 0; j <= l; j++) t[j] = st->H[j][l];
-      for (int k = 0; k < l; k++) {
+      // the trial rotation on a copy of the column
+#pragma unroll
+      for (int k = 0; k < L; k++) {
         const double c = st->q[2 * k], sn = st->q[2 * k + 1];
-        const double t1 = t[k], t2 = t[k + 1];
-        t[k] = c * t1 - sn * t2;
-        t[k + 1] = sn * t1 + c * t2;
+        const double t1 = h[k], t2 = h[k + 1];
+        h[k] = c * t1 - sn * t2;
+        h[k + 1] = sn * t1 + c * t2;
       }
-      const double sn = lag_givens_sine(t[l], sqrt(est));
+      const double sn = lag_givens_sine(h[L], sqrt(est));
       use_est = fabs(st->rotprod * sn * st->r_norm) <= delta;
     }
   }
   if (use_est) {
     // the trial rotation is the one the completion forms, so an estimate
     // that shows convergence converges; anything else goes to the host
-    if (!lag_finalize(st, l, sqrt(est), maxl, delta)) lag_end(st, NVHIP_LAG_FALLBACK);
+    if (!lag_finalize<L>(st, sqrt(est), maxl, delta)) lag_end(st, NVHIP_LAG_FALLBACK);
     return true;
   }
-  if (l + 1 == maxl) st->pskip = 0;   // the last column's projection norm
+  if (L + 1 == maxl) st->pskip = 0;   // the last column's projection norm
   return false;
 }
 
-// the host loop's qr_sol: the rotations applied to the right-hand side,
-// then back substitution; false on a zero diagonal
-__device__ bool lag_qr_sol(int n, const double (*H)[kMaxK], const double* q, double* b) {
-  for (int k = 0; k < n; k++) {
-    const double c = q[2 * k], s = q[2 * k + 1], t1 = b[k], t2 = b[k + 1];
-    b[k] = c * t1 - s * t2;
-    b[k + 1] = s * t1 + c * t2;
-  }
-  for (int k = n - 1; k >= 0; k--) {
-    if (H[k][k] == 0.0) return false;
-    b[k] /= H[k][k];
-    for (int i = 0; i < k; i++) b[i] -= b[k] * H[i][k];
-  }
-  return true;
-}
-
-// a solve that has ended: the coefficients of its solution x =
-// sum_k c_k u_k / s, formed exactly as the host's exit forms them (the
-// same statuses, qr_sol and residual test; c_k = y_k / |u_k|)
+// a solve that has ended with n = krydim <= NMAX columns: the coefficients
+// of its solution x = sum_k c_k u_k / s, formed exactly as the host's exit
+// forms them (the same statuses, qr_sol -- the rotations applied to the
+// right-hand side, then back substitution -- and residual test; c_k =
+// y_k / |u_k|)
+template <int NMAX>
 __device__ void lag_solution(LagDev* st) {
   st->xready = 0;
   if (st->status != NVHIP_LAG_CONVERGED && st->status != NVHIP_LAG_MAXL) return;
   const int n = st->krydim;
-  double yg[kLagMax + 1];
+  if (n > NMAX) {   // not reached: pass P ends a solve of at most P + 1 columns
+    st->status = NVHIP_LAG_FALLBACK;
+    return;
+  }
+  double yg[NMAX + 1];
   yg[0] = st->r_norm;
-  for (int i = 1; i <= n; i++) yg[i] = 0.0;
-  if (!lag_qr_sol(n, st->H, st->q, yg)) return;
+#pragma unroll
+  for (int i = 1; i <= NMAX; i++) yg[i] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NMAX; k++) {
+    if (k < n) {
+      const double c = st->q[2 * k], s = st->q[2 * k + 1], t1 = yg[k], t2 = yg[k + 1];
+      yg[k] = c * t1 - s * t2;
+      yg[k + 1] = s * t1 + c * t2;
+    }
+  }
+#pragma unroll
+  for (int k = NMAX - 1; k >= 0; k--) {
+    if (k < n) {
+      if (st->H[k][k] == 0.0) return;
+      yg[k] /= st->H[k][k];
+#pragma unroll
+      for (int i = 0; i < k; i++) yg[i] -= yg[k] * st->H[i][k];
+    }
+  }
   if (!(st->status == NVHIP_LAG_CONVERGED || st->rho < st->r_norm)) return;
-  for (int k = 0; k < n; k++) st->xc[k] = yg[k] / st->nu[k];
+#pragma unroll
+  for (int k = 0; k < NMAX; k++)
+    if (k < n) st->xc[k] = yg[k] / st->nu[k];
   st->xk = n;
   st->xready = 1;
 }
 
+// the result for the host, written by every thread of the workgroup (one
+// store per value), each thread's stores fenced at system scope, then the
+// sequence word after the barrier
 __device__ void lag_publish(const LagDev* st, LagPub* pub, unsigned seq, int maxl) {
   NVHipLagResult& o = pub->r;
-  o.status = st->status;
-  o.krydim = st->krydim;
-  o.iters = st->iters;
-  o.passes = st->passes;
-  o.r_norm = st->r_norm;
-  o.rho = st->rho;
-  for (int i = 0; i <= maxl; i++) o.nu[i] = st->nu[i];
-  for (int i = 0; i <= maxl; i++)
-    for (int j = 0; j < maxl; j++) o.H[i][j] = st->H[i][j];
-  for (int k = 0; k < 2 * maxl; k++) o.q[k] = st->q[k];
+  const int t = threadIdx.x;
+  if (t == 0) {
+    o.status = st->status;
+    o.krydim = st->krydim;
+    o.iters = st->iters;
+    o.passes = st->passes;
+    o.r_norm = st->r_norm;
+    o.rho = st->rho;
+  }
+  if (t <= maxl) o.nu[t] = st->nu[t];
+  if (t < 2 * maxl) o.q[t] = st->q[t];
+  for (int e = t; e < (maxl + 1) * maxl; e += blockDim.x) {
+    const int i = e / maxl, j = e - i * maxl;
+    o.H[i][j] = st->H[i][j];
+  }
   __threadfence_system();
-  __hip_atomic_store(&pub->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  __syncthreads();
+  if (t == 0) __hip_atomic_store(&pub->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// one step of the solve on the state st with the sums r; true when the
-// solve has ended (the state is then to be published)
-__device__ bool lag_run(LagDev* st, const double* r, int p, int mode, int maxl, double delta,
-                        double bfloor, double nlen) {
-  if (mode == 1) {
-    st->pskip = 1;
-    lag_finalize(st, maxl - 1, sqrt(r[0]), maxl, delta);
-    return true;
-  }
-  if (p == 0) {
+// one step of the solve from pass P's sums r; true when the solve has ended
+// (the state is then to be published)
+template <int P>
+__device__ bool lag_run(LagDev* st, const double* r, int maxl, double delta, double bfloor,
+                        double nlen) {
+  if constexpr (P == 0) {
     st->done = 0;
     st->pskip = 1;
     st->xready = 0;
@@ -2180,47 +2212,61 @@ __device__ bool lag_run(LagDev* st, const double* r, int p, int mode, int maxl, 
       lag_end(st, NVHIP_LAG_ZERO);
       return true;
     }
-    for (int i = 0; i <= maxl; i++)
-      for (int j = 0; j < maxl; j++) st->H[i][j] = 0.0;
+    // H was zeroed when the state was staged
     st->nu[0] = r_norm;
-    return lag_column(st, 0, r, maxl, delta);
+    return lag_column<0>(st, r, maxl, delta);
+  } else {
+    st->passes++;
+    return lag_finalize<P - 1>(st, sqrt(r[0]), maxl, delta) || lag_column<P>(st, r, maxl, delta);
   }
-  st->passes++;
-  return lag_finalize(st, p - 1, sqrt(r[0]), maxl, delta) || lag_column(st, p, r, maxl, delta);
 }
 
 
-// mode 0: the sums of pass p (p = 0 starts the solve); mode 1: the last
-// column's projection norm |u|^2 (one sum).  last: no further step of the
-// solve is queued yet (the state is published also when the solve goes
-// on, with status LAG_RUNNING).  The state is copied to LDS by
-// all threads while the partials are summed, stepped there by thread 0
-// (its dependent accesses then cost LDS latency, not memory round trips)
-// and copied back.
-template <int K>
-__global__ __launch_bounds__(kFinishBlock) void k_lag_step(int nblk, const double* partials, int p,
-                                                           int mode, int last, LagDev* gst,
-                                                           LagPub* pub, unsigned seq, int maxl,
-                                                           double delta, double bfloor,
-                                                           double nlen) {
-  if (mode == 1 ? gst->pskip != 0 : (p > 0 && gst->done != 0)) return;   // uniform
+// PROJ false: the sums of pass P (P = 0 starts the solve); PROJ true: the
+// last column's projection norm |u|^2 (one sum; P = maxl - 1).  last: no
+// further step of the solve is queued yet (the state is published also
+// when the solve goes on, with status LAG_RUNNING).  The state is staged in
+// LDS by all threads while the partials are summed (its Hessenberg matrix
+// zeroed for a new solve), stepped there by thread 0 (its dependent
+// accesses then cost LDS latency, not memory round trips) and copied back.
+// A solve that has ended (a pass queued past convergence) or a projection
+// norm that is not needed leaves everything as it is.
+template <int P, bool PROJ>
+__global__ __launch_bounds__(kFinishBlock) void k_lag_step(int nblk, const double* partials,
+                                                           int last, LagDev* gst, LagPub* pub,
+                                                           unsigned seq, int maxl, double delta,
+                                                           double bfloor, double nlen) {
+  constexpr int K = PROJ ? 1 : (P == 0 ? 4 : P + 3);
   static_assert(sizeof(LagDev) % sizeof(double) == 0, "LagDev copied as doubles");
+  static_assert(offsetof(LagDev, H) == 0, "the Hessenberg matrix leads the state");
   constexpr int NW = (int)(sizeof(LagDev) / sizeof(double));
+  constexpr int NH = kMaxK * kMaxK;
+  constexpr bool fresh = !PROJ && P == 0;
   __shared__ double lst[NW];
   __shared__ double sm[K][kFinishBlock / 64];
-  for (int i = threadIdx.x; i < NW; i += blockDim.x) lst[i] = reinterpret_cast<const double*>(gst)[i];
+  __shared__ int publish_s;
+  for (int i = threadIdx.x; i < NW; i += blockDim.x)
+    lst[i] = (fresh && i < NH) ? 0.0 : reinterpret_cast<const double*>(gst)[i];
   double r[K];
   sum_partials<K, false, kFinishBlock / 64>(RED_SUM, nblk, partials, sm, r);   // barriers inside
   LagDev* st = reinterpret_cast<LagDev*>(lst);
-  bool ended = false;
+  if (PROJ ? st->pskip != 0 : (P > 0 && st->done != 0)) return;   // uniform
   if (threadIdx.x == 0) {
-    ended = lag_run(st, r, p, mode, maxl, delta, bfloor, nlen);
-    if (ended) lag_solution(st);
+    bool ended;
+    if constexpr (PROJ) {
+      st->pskip = 1;
+      lag_finalize<P>(st, sqrt(r[0]), maxl, delta);
+      ended = true;
+    } else {
+      ended = lag_run<P>(st, r, maxl, delta, bfloor, nlen);
+    }
+    if (ended) lag_solution<P + 1>(st);
+    // a solve still running after the last step queued so far publishes too
+    publish_s = ended || last;
   }
   __syncthreads();
   for (int i = threadIdx.x; i < NW; i += blockDim.x) reinterpret_cast<double*>(gst)[i] = lst[i];
-  // a solve still running after the last step queued so far publishes too
-  if (threadIdx.x == 0 && (ended || last)) lag_publish(st, pub, seq, maxl);
+  if (publish_s) lag_publish(st, pub, seq, maxl);   // uniform
 }
 
 // x = sum_k c_k X_k / s with the coefficients of the ended solve (as
@@ -2253,19 +2299,30 @@ struct FProjSqDev {
   }
 };
 
-void lag_step_dispatch(int K, hipStream_t s, int nblk, const RedWs* w, int p, int mode,
-                       int last) {
-  switch (K) {
-#define NVHIP_LAG_CASE(k)                                                                     \
-    case k:                                                                                   \
-      hipLaunchKernelGGL(k_lag_step<k>, dim3(1), dim3(kFinishBlock), 0, s, nblk, w->partials, \
-                         p, mode, last, w->lag, w->lagpub_dev, w->lagseq, w->lagmaxl,         \
-                         w->lagdelta, w->lagbfloor, w->lagnlen);                              \
-      break;
-    NVHIP_LAG_CASE(1) NVHIP_LAG_CASE(3) NVHIP_LAG_CASE(4) NVHIP_LAG_CASE(5)
-    NVHIP_LAG_CASE(6) NVHIP_LAG_CASE(7) NVHIP_LAG_CASE(8) NVHIP_LAG_CASE(9)
-    default: NVHIP_LAG_CASE(10)
-#undef NVHIP_LAG_CASE
+template <int P>
+void launch_lag_step(hipStream_t s, int nblk, const RedWs* w, bool proj, int last) {
+  if (proj)
+    hipLaunchKernelGGL((k_lag_step<P, true>), dim3(1), dim3(kFinishBlock), 0, s, nblk,
+                       w->partials, last, w->lag, w->lagpub_dev, w->lagseq, w->lagmaxl,
+                       w->lagdelta, w->lagbfloor, w->lagnlen);
+  else
+    hipLaunchKernelGGL((k_lag_step<P, false>), dim3(1), dim3(kFinishBlock), 0, s, nblk,
+                       w->partials, last, w->lag, w->lagpub_dev, w->lagseq, w->lagmaxl,
+                       w->lagdelta, w->lagbfloor, w->lagnlen);
+}
+
+// pass p's step (proj: the projection norm of column p = maxl - 1); 0 <= p < kLagMax
+void lag_step_dispatch(hipStream_t s, int nblk, const RedWs* w, int p, bool proj, int last) {
+  static_assert(kLagMax == 8, "one instantiation per column");
+  switch (p) {
+    case 0: launch_lag_step<0>(s, nblk, w, proj, last); break;
+    case 1: launch_lag_step<1>(s, nblk, w, proj, last); break;
+    case 2: launch_lag_step<2>(s, nblk, w, proj, last); break;
+    case 3: launch_lag_step<3>(s, nblk, w, proj, last); break;
+    case 4: launch_lag_step<4>(s, nblk, w, proj, last); break;
+    case 5: launch_lag_step<5>(s, nblk, w, proj, last); break;
+    case 6: launch_lag_step<6>(s, nblk, w, proj, last); break;
+    default: launch_lag_step<7>(s, nblk, w, proj, last); break;
   }
 }
 
@@ -2316,7 +2373,7 @@ int NVHip_LagStep(N_Vector v, int K, int nblk, int p, int last) {
     return -1;
   RedWs* w = workspace(S(v));
   if (!w->lag || p >= w->lagmaxl || K != (p == 0 ? 4 : p + 3)) return -1;
-  lag_step_dispatch(K, S(v), nblk, w, p, 0, last != 0);
+  lag_step_dispatch(S(v), nblk, w, p, false, last != 0);
   check_launch("lag step");
   return 0;
 }
@@ -2339,7 +2396,7 @@ int NVHip_LagProjNorm(N_Vector wv, int nb, N_Vector* U) {
   hipLaunchKernelGGL(k_reduce1<FProjSqDev>, dim3(g), dim3(kBlock), 0, s, RED_SUM, f, n,
                      w->partials, (unsigned*)nullptr, w->host_dev, (double*)nullptr, 0u,
                      (const int*)&w->lag->pskip);
-  lag_step_dispatch(1, s, g, w, 0, 1, 0);
+  lag_step_dispatch(s, g, w, nb - 1, true, 0);
   check_launch("lag projection norm");
   return 0;
 }

@@ -531,14 +531,10 @@ __device__ void finish(int kind, int nblk, const double* partials, double* out, 
   }
 }
 
+// the workgroup's share of a one-value reduction of f over [0, n), valid
+// in thread 0 (the accumulation every k_reduce1 form uses)
 template <class F>
-__global__ __launch_bounds__(kBlock) void k_reduce1(int kind, F f, sunindextype n,
-                                                    double* partials, unsigned* ticket,
-                                                    double* out, double* dout, unsigned seq,
-                                                    const int* skip) {
-  if (skip && *skip) return;   // a device-resident solve that has finished
-  __shared__ double sm[1][kWaves];
-  __shared__ bool last;
+__device__ double reduce1_block(int kind, const F& f, sunindextype n, double (*sm)[kWaves]) {
   // four independent accumulators so four loads per thread are in flight
   double a0 = identity(kind), a1 = a0, a2 = a0, a3 = a0;
   const sunindextype stride = (sunindextype)gridDim.x * blockDim.x;
@@ -553,6 +549,18 @@ __global__ __launch_bounds__(kBlock) void k_reduce1(int kind, F f, sunindextype 
   for (; i < n; i += stride) a0 = combine(kind, a0, f(i));
   double v[1] = {combine(kind, combine(kind, a0, a1), combine(kind, a2, a3))};
   block_reduce<1>(kind, v, sm);
+  return v[0];
+}
+
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_reduce1(int kind, F f, sunindextype n,
+                                                    double* partials, unsigned* ticket,
+                                                    double* out, double* dout, unsigned seq,
+                                                    const int* skip) {
+  if (skip && *skip) return;   // a device-resident solve that has finished
+  __shared__ double sm[1][kWaves];
+  __shared__ bool last;
+  double v[1] = {reduce1_block(kind, f, n, sm)};
   if (!ticket) {
     if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
     return;
@@ -2269,22 +2277,72 @@ __global__ __launch_bounds__(kFinishBlock) void k_lag_step(int nblk, const doubl
   if (publish_s) lag_publish(st, pub, seq, maxl);   // uniform
 }
 
-// x = sum_k c_k X_k / s with the coefficients of the ended solve (as
-// k_lincombdiv); when the solve left no solution to use, nothing -- and the
-// Newton update queued behind (skip flag xskip) does nothing either
-// (xinfo = {xk, xready} and xc of the state, xskip its flag: separate
-// restrict pointers, so the uniform reads are scalar loads)
-__global__ void k_lincombdiv_lag(const int* __restrict__ xinfo, const double* __restrict__ xc,
-                                 int* __restrict__ xskip, PtrPack X,
-                                 const double* __restrict__ sv, double* x, sunindextype n) {
-  const int go = xinfo[1];
+// The solution of an ended solve and the first Newton update in one pass:
+// x = (0 + sum_k c_k X_k) / s with the coefficients c_k of the solve's final
+// step (as k_lincombdiv), then FNewtonUpd0 with x (ycor += x, y = zn0 +
+// ycor, (x w)^2 summed).  NV, the number of basis vectors, is a template
+// argument so that the coefficients sit in registers.
+template <int NV>
+struct FNewtonUpdLag0 {
+  double c[NV]; PtrPack X; const double *s, *w, *zn0; double *x, *ycor, *y;
+  __device__ double operator()(sunindextype i) const {
+    double acc = 1.0 * 0.0;
+#pragma unroll
+    for (int j = 0; j < NV; j++) acc = fma(c[j], X.p[j][i], acc);
+    const double d = acc / s[i];
+    x[i] = d;
+    const double cy = fma(1.0, ycor[i], 1.0 * d);
+    ycor[i] = cy;
+    y[i] = fma(1.0, zn0[i], 1.0 * cy);
+    const double p = d * w[i];
+    return p * p;
+  }
+};
+
+template <int NV>
+__device__ void newton_upd_lag(const double* __restrict__ xc, const PtrPack& X, const double* s,
+                               const double* w, const double* zn0, double* x, double* ycor,
+                               double* y, sunindextype n, double* partials, double (*sm)[kWaves]) {
+  FNewtonUpdLag0<NV> f{};
+#pragma unroll
+  for (int j = 0; j < NV; j++) f.c[j] = xc[j];
+  f.X = X;
+  f.s = s;
+  f.w = w;
+  f.zn0 = zn0;
+  f.x = x;
+  f.ycor = ycor;
+  f.y = y;
+  const double v = reduce1_block(RED_SUM, f, n, sm);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+
+// xinfo = {xk, xready} and xc of the solve's state (separate restrict
+// pointers, so the uniform reads are scalar loads).  When the solve left no
+// solution to use, nothing, and the flag xskip stops the finish queued
+// behind.  Partials as k_reduce1 writes them (the same grid and
+// accumulation), so the update and its sum are those of k_lincombdiv and
+// k_reduce1<FNewtonUpd0> run one after the other.  At most 128 VGPRs, so
+// that four workgroups share a CU (the widest basis is register-heavy).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_newton_upd_lag(
+    const int* __restrict__ xinfo, const double* __restrict__ xc, int* __restrict__ xskip,
+    PtrPack X, const double* s, const double* w, const double* zn0, double* x, double* ycor,
+    double* y, sunindextype n, double* partials) {
+  __shared__ double sm[1][kWaves];
+  const int nv = xinfo[0];
+  const int go = xinfo[1] && nv >= 1 && nv <= kLagMax;
   if (blockIdx.x == 0 && threadIdx.x == 0) *xskip = go ? 0 : 1;
   if (!go) return;
-  const int nv = xinfo[0];
-  GRID_STRIDE(i, n) {
-    double acc = 1.0 * 0.0;
-    for (int j = 0; j < nv; j++) acc = fma(xc[j], X.p[j][i], acc);
-    x[i] = acc / sv[i];
+  static_assert(kLagMax == 8, "one instantiation per basis size");
+  switch (nv) {
+    case 1: newton_upd_lag<1>(xc, X, s, w, zn0, x, ycor, y, n, partials, sm); break;
+    case 2: newton_upd_lag<2>(xc, X, s, w, zn0, x, ycor, y, n, partials, sm); break;
+    case 3: newton_upd_lag<3>(xc, X, s, w, zn0, x, ycor, y, n, partials, sm); break;
+    case 4: newton_upd_lag<4>(xc, X, s, w, zn0, x, ycor, y, n, partials, sm); break;
+    case 5: newton_upd_lag<5>(xc, X, s, w, zn0, x, ycor, y, n, partials, sm); break;
+    case 6: newton_upd_lag<6>(xc, X, s, w, zn0, x, ycor, y, n, partials, sm); break;
+    case 7: newton_upd_lag<7>(xc, X, s, w, zn0, x, ycor, y, n, partials, sm); break;
+    default: newton_upd_lag<8>(xc, X, s, w, zn0, x, ycor, y, n, partials, sm); break;
   }
 }
 
@@ -2425,16 +2483,12 @@ int NVHip_LagUpdate(N_Vector b, int nvec, N_Vector* V, N_Vector sv, N_Vector x, 
   }
   static_assert(offsetof(LagDev, xready) == offsetof(LagDev, xk) + sizeof(int),
                 "xk, xready adjacent");
-  hipLaunchKernelGGL(k_lincombdiv_lag, dim3(ew_grid(n)), dim3(kBlock), 0, s,
-                     (const int*)&wk->lag->xk, (const double*)wk->lag->xc, &wk->lag->xskip, X,
-                     D(sv), D(x), n);
   const int g = red_grid(n);
   const unsigned seq = ++wk->seq;
-  const int* skip = &wk->lag->xskip;
-  hipLaunchKernelGGL(k_reduce1<FNewtonUpd0>, dim3(g), dim3(kBlock), 0, s, RED_SUM,
-                     FNewtonUpd0{D(x), D(w), D(zn0), D(ycor), D(y)}, n, wk->partials,
-                     (unsigned*)nullptr, wk->host_dev, (double*)nullptr, seq, skip);
-  launch_finish<1>(s, RED_SUM, g, wk, seq, 0, skip);
+  hipLaunchKernelGGL(k_newton_upd_lag, dim3(g), dim3(kBlock), 0, s, (const int*)&wk->lag->xk,
+                     (const double*)wk->lag->xc, &wk->lag->xskip, X, D(sv), D(w), D(zn0), D(x),
+                     D(ycor), D(y), n, wk->partials);
+  launch_finish<1>(s, RED_SUM, g, wk, seq, 0, &wk->lag->xskip);
   check_launch("lag update");
   *ticket = seq;
   return 0;

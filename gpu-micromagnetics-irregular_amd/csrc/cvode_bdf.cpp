@@ -135,6 +135,17 @@ struct Integrator {
   // the device (fused.lag_update); the update's norm is read with the ticket
   bool upd_queued = false;
   unsigned upd_ticket = 0;
+  // the order-change test's two norms queued behind the first Newton
+  // update (fused.pair_queue) for the end of the step: wanted on this
+  // attempt (with the cquot and l[q] the step's end will use), queued with
+  // the update that is in use (ticket), and usable once the Newton
+  // iteration converged with that update
+  struct {
+    bool want = false, on = false, valid = false;
+    int q = 0;
+    double cquot = 0.0, lq = 0.0;
+    unsigned ticket = 0;
+  } spec;
   bool ewt_pending = false;   // the step's error weights left to the predictor
   // The Newton residual not yet formed: dst = sum_k c_k X_k is left to the
   // first pass of the lagged SPGMR solve, which forms it where it reads it
@@ -689,8 +700,17 @@ int spgmr_lagged_dev(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vect
   const bool upd = f.lag_update && f.lag_update_wait && m->nls_curiter == 0 && x == b &&
                    m->nls_ycor && m->fused.newtonupd && !f.newtonupd_comb;
   unsigned ticket = 0;
+  // the order-change norms behind the update that may end the solve (each
+  // part's replaces the previous one's)
+  auto queue_pair = [&](bool q_ok) {
+    auto& sp = m->spec;
+    sp.on = q_ok && sp.want && m->nls_ycor == m->acor &&
+            f.pair_queue(m->zn[sp.q], sp.lq, m->acor, -sp.cquot, m->zn[m->qmax], 1.0, m->acor,
+                         m->ewt, &sp.ticket) == 0;
+  };
   bool queued = upd && f.lag_update(b, l_max, V, s, x, m->nls_ycor, m->zn[0], m->ewt, m->y,
                                     &ticket) == 0;
+  queue_pair(queued);
   llgcv_lag_result res;
   if (f.lag_wait(b, &res) != 0) return kLaggedFallback;
   if (res.status == LLGCV_LAG_RUNNING) {
@@ -700,6 +720,7 @@ int spgmr_lagged_dev(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vect
     if (f.lag_projnorm(W[(l_max - 1) & 1], l_max, V) != 0) return kLaggedFallback;
     queued = upd && f.lag_update(b, l_max, V, s, x, m->nls_ycor, m->zn[0], m->ewt, m->y,
                                  &ticket) == 0;
+    queue_pair(queued);
     if (f.lag_wait(b, &res) != 0) return kLaggedFallback;
   }
   m->lag_pred = std::min(res.passes + 1, l_max - 1);
@@ -1346,6 +1367,8 @@ int newton_solve(Integrator* m, N_Vector ycor, double tol, bool call_lsetup, lon
       }
       if (rv == 0) {
         m->nls_jcur = false;
+        // the queued order-change norms read this update's correction
+        m->spec.valid = m->spec.on && upd;
         return 0;
       }
       if (rv != NLS_CONTINUE) break;
@@ -1565,9 +1588,17 @@ void prepare_next_step(Integrator* m, double dsm) {
   // still left to the predictor
   auto& pc = m->pend_c;
   bool paired = false;
-  if (both && pc.on && !pc.scaled && pc.q == m->q && m->fused.wrms_pair_comp)
-    paired = m->fused.wrms_pair_comp(m->zn[m->q], pc.l[m->q], m->acor, -cquot, m->zn[m->qmax],
-                                     1.0, m->acor, m->ewt, nr) == 0;
+  auto& sp = m->spec;
+  if (both && pc.on && !pc.scaled && pc.q == m->q && m->fused.wrms_pair_comp) {
+    // the same norms queued behind the step's Newton update when the step
+    // began (the same vectors and coefficients)
+    paired = sp.valid && sp.q == m->q && sp.cquot == cquot && sp.lq == pc.l[m->q] &&
+             m->fused.pair_wait(m->acor, sp.ticket, nr) == 0;
+    if (!paired)
+      paired = m->fused.wrms_pair_comp(m->zn[m->q], pc.l[m->q], m->acor, -cquot,
+                                       m->zn[m->qmax], 1.0, m->acor, m->ewt, nr) == 0;
+  }
+  sp.valid = false;
   if (both && !paired && m->fused.wrms_pair) {
     flush_completion(m);
     paired = m->fused.wrms_pair(m->zn[m->q], -cquot, m->zn[m->qmax], 1.0, m->acor, m->ewt,
@@ -1586,6 +1617,25 @@ void prepare_next_step(Integrator* m, double dsm) {
   set_eta(m);
 }
 
+// Whether this attempt, when it succeeds without a failure before it,
+// ends with prepare_next_step's paired order-change norms (qwait reaches 0
+// in complete_step, both orders testable, the completion left pending),
+// and the cquot and l[q] it will use: complete_step moves tau[1] to tau[2]
+// and changes nothing else they read.
+void want_pair(Integrator* m, bool first_attempt) {
+  auto& sp = m->spec;
+  sp.want = sp.on = sp.valid = false;
+  const llgcv_fused_ops& f = m->fused;
+  if (!first_attempt || !f.pair_queue || !f.pair_wait || !f.wrms_pair_comp || !f.lag_update ||
+      !f.predict_complete || !f.predict_init || m->etamax == 1.0 || m->qwait != 1 ||
+      !(m->q > 1 && m->q != m->qmax && m->saved_tq5 != 0.0))
+    return;
+  sp.want = true;
+  sp.q = m->q;
+  sp.lq = m->l[m->q];
+  sp.cquot = (m->tq[5] / m->saved_tq5) * std::pow(m->h / m->tau[1], (double)m->L);
+}
+
 int cv_step(Integrator* m) {
   const double saved_t = m->tn;
   int ncf = 0, nef = 0, nflag = FIRST_CALL;
@@ -1594,6 +1644,7 @@ int cv_step(Integrator* m) {
   for (;;) {
     predict(m);
     set_coeffs(m);
+    want_pair(m, ncf == 0 && nef == 0);
     nflag = cv_nls(m, nflag);
     const int kflag = handle_nflag(m, &nflag, saved_t, &ncf);
     if (kflag == PREDICT_AGAIN) continue;

@@ -67,7 +67,7 @@ void LLG_DefaultRunOpts(llg_run_opts* o) {
   o->gstype = 0;        // auto: classical iff neq < 500000 (2d_i4/2d_fft.cu:645-650)
   o->max_order = 5;
   o->mxstep = 500;
-  o->fused = 3863;
+  o->fused = 7959;
   o->use_jtv = 1;
   o->use_prec = 1;
   o->pretype = SUN_PREC_LEFT;
@@ -154,6 +154,12 @@ void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_hos
         if (opts->fused & 2048) {
           fo.lag_update = NVHip_LagUpdate;
           fo.lag_update_wait = NVHip_LagUpdateWait;
+          // bit 12: a step's order-change norms queued behind its first
+          // Newton update (one wait less on such a step)
+          if (opts->fused & 4096) {
+            fo.pair_queue = NVHip_WrmsNormPairCompQueue;
+            fo.pair_wait = NVHip_WrmsNormPairWait;
+          }
         }
       }
     }

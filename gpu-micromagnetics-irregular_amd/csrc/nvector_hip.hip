@@ -833,16 +833,20 @@ void finish_dispatch(int K, hipStream_t s, int kind, int nblk, const RedWs* w, u
   }
 }
 
-// Wait for the reduction tagged `seq` on stream s.  Spin mode polls the
-// pinned sequence word (bounded: after 2 s it falls back to a stream sync,
-// which also surfaces any device error); otherwise hipStreamSynchronize.
-void wait_result(hipStream_t s, RedWs* w, unsigned seq) {
+// Wait for the reduction tagged `seq` on stream s (result slot `slot`).
+// Spin mode polls the pinned sequence word (bounded: after 2 s it falls
+// back to a stream sync, which also surfaces any device error; a later
+// sequence number in the slot ends the wait too); otherwise
+// hipStreamSynchronize.
+void wait_result(hipStream_t s, RedWs* w, unsigned seq, int slot = 0) {
   g_syncs++;
   if (g_sync_mode == 1 || g_sync_mode == 2) {
-    const unsigned* word = reinterpret_cast<const unsigned*>(w->host + kMaxK);   // slot 0
+    const unsigned* word = reinterpret_cast<const unsigned*>(w->host + slot * kSlot + kMaxK);
     auto t0 = std::chrono::steady_clock::now();
     for (unsigned it = 0;; it++) {
-      if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) return;
+      const unsigned got = __atomic_load_n(word, __ATOMIC_ACQUIRE);
+      if (got == seq) return;
+      if ((int)(got - seq) > 0) break;
       if (g_sync_mode == 2 && (it & 15u) == 15u) std::this_thread::yield();
       if ((it & 4095u) == 4095u &&
           std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2))
@@ -1506,9 +1510,10 @@ int NVHip_PredictComplete(int q, N_Vector* zn, N_Vector acor, const sunrealtype*
 
 // x_comp: nrm[0] is the norm of fma(cx, x_comp, x) (x + cx x_comp as
 // N_VScaleAddMulti forms it) when given
-static int wrms_pair(N_Vector x, double cx, N_Vector xc, double a, N_Vector y, double b,
-                     N_Vector z, N_Vector w, double* nrm) {
-  if (!x || !y || !z || !w || !nrm) return -1;
+// the two sums of a WRMS pair queued on x's stream, results in `slot`
+static int wrms_pair_queue(N_Vector x, double cx, N_Vector xc, double a, N_Vector y, double b,
+                           N_Vector z, N_Vector w, int slot, unsigned* seq_out) {
+  if (!x || !y || !z || !w) return -1;
   N_Vector vs[5] = {x, y, z, w, xc ? xc : x};
   for (N_Vector v : vs)
     if (v->ops->nvgetvectorid != N_VGetVectorID_Hip || LEN(v) != LEN(x)) return -1;
@@ -1524,9 +1529,19 @@ static int wrms_pair(N_Vector x, double cx, N_Vector xc, double a, N_Vector y, d
   else
     hipLaunchKernelGGL((k_reduce2<FWsq, FLinWsq>), dim3(g), dim3(kBlock), 0, s, RED_SUM,
                        FWsq{D(x), D(w)}, FLinWsq{a, b, D(y), D(z), D(w)}, n, ws->partials);
-  finish_dispatch(2, s, RED_SUM, g, ws, seq);
+  launch_finish<2>(s, RED_SUM, g, ws, seq, slot);
   check_launch("wrms pair");
-  wait_result(s, ws, seq);
+  *seq_out = seq;
+  return 0;
+}
+
+static int wrms_pair(N_Vector x, double cx, N_Vector xc, double a, N_Vector y, double b,
+                     N_Vector z, N_Vector w, double* nrm) {
+  unsigned seq = 0;
+  if (!nrm || wrms_pair_queue(x, cx, xc, a, y, b, z, w, 0, &seq) != 0) return -1;
+  const sunindextype n = LEN(x);
+  RedWs* ws = workspace(S(x));
+  wait_result(S(x), ws, seq);
   nrm[0] = std::sqrt(ws->host[0] / (double)n);
   nrm[1] = std::sqrt(ws->host[1] / (double)n);
   return 0;
@@ -1541,6 +1556,26 @@ int NVHip_WrmsNormPairComp(N_Vector x, sunrealtype cx, N_Vector xc, sunrealtype 
                            sunrealtype b, N_Vector z, N_Vector w, sunrealtype* nrm) {
   if (!xc) return -1;
   return wrms_pair(x, cx, xc, a, y, b, z, w, nrm);
+}
+
+int NVHip_WrmsNormPairCompQueue(N_Vector x, sunrealtype cx, N_Vector xc, sunrealtype a,
+                                N_Vector y, sunrealtype b, N_Vector z, N_Vector w,
+                                unsigned* ticket) {
+  if (!xc || !ticket || g_red_mode == 0) return -1;
+  return wrms_pair_queue(x, cx, xc, a, y, b, z, w, 1, ticket);
+}
+
+int NVHip_WrmsNormPairWait(N_Vector x, unsigned ticket, sunrealtype* nrm) {
+  if (!x || !nrm || x->ops->nvgetvectorid != N_VGetVectorID_Hip) return -1;
+  hipStream_t s = S(x);
+  RedWs* ws = workspace(s);
+  wait_result(s, ws, ticket, 1);
+  const unsigned* word = reinterpret_cast<const unsigned*>(ws->host + kSlot + kMaxK);
+  if (__atomic_load_n(word, __ATOMIC_ACQUIRE) != ticket) return -1;
+  const double n = (double)LEN(x);
+  nrm[0] = std::sqrt(ws->host[kSlot] / n);
+  nrm[1] = std::sqrt(ws->host[kSlot + 1] / n);
+  return 0;
 }
 
 int NVHip_EwtSet(N_Vector y, sunrealtype rtol, sunrealtype atol, N_Vector abstol, N_Vector w) {

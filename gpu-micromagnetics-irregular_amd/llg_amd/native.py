@@ -172,6 +172,9 @@ SIGNATURES = {
                                         C.c_double, VP, VP]),
     "NVHip_WrmsNormPairComp": (C.c_int, [VP, C.c_double, VP, C.c_double, VP, C.c_double, VP,
                                          VP, DP]),
+    "NVHip_WrmsNormPairCompQueue": (C.c_int, [VP, C.c_double, VP, C.c_double, VP, C.c_double,
+                                              VP, VP, C.POINTER(C.c_uint)]),
+    "NVHip_WrmsNormPairWait": (C.c_int, [VP, C.c_uint, DP]),
     "NVHip_ScaleDiv": (C.c_int, [C.c_double, VP, VP, VP, VP]),
     "NVHip_ProdDotMulti": (C.c_int, [VP, VP, VP, C.c_int, C.POINTER(VP), DP]),
     "NVHip_LinCombDot": (C.c_int, [C.c_int, DP, C.POINTER(VP), VP, DP]),

@@ -217,6 +217,15 @@ typedef struct llgcv_fused_ops {
   int (*lag_update)(N_Vector b, int nvec, N_Vector* V, N_Vector s, N_Vector x, N_Vector ycor,
                     N_Vector zn0, N_Vector w, N_Vector y, unsigned* ticket);
   int (*lag_update_wait)(N_Vector ycor, unsigned ticket, sunrealtype* dnorm);
+  /* with lag_update, on a step that will test an order change when it
+   * succeeds: the test's two norms (wrms_pair_comp's, from the pending
+   * completion and the step's coefficients) queued behind the update
+   * (as NVHip_WrmsNormPairCompQueue) and read after the step (as
+   * NVHip_WrmsNormPairWait) when the first Newton iteration converged --
+   * one host wait less on such a step, the same bits */
+  int (*pair_queue)(N_Vector x, sunrealtype cx, N_Vector xc, sunrealtype a, N_Vector y,
+                    sunrealtype b, N_Vector z, N_Vector w, unsigned* ticket);
+  int (*pair_wait)(N_Vector x, unsigned ticket, sunrealtype* nrm);
 } llgcv_fused_ops;
 
 void* LLGCVodeCreate(int lmm, SUNContext sunctx);

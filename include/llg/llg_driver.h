@@ -60,10 +60,14 @@ typedef struct llg_run_opts {
                          solve; bit 11 (with bit 10): the solution and
                          the first Newton update of each iteration queued
                          behind the solve (NVHip_LagUpdate), no host
-                         round trip between them; default 3863 (bits 0,
-                         1, 2, 4, 8, 9, 10, 11).  Bits 1 to 3 and 8 to 11
-                         change no value (bits 10 and 11 none of bit
-                         4's). */
+                         round trip between them; bit 12 (with bit 11):
+                         on a step that will test an order change, the
+                         test's two norms queued behind the first Newton
+                         update (NVHip_WrmsNormPairCompQueue), one host
+                         round trip less on such a step; default 7959
+                         (bits 0, 1, 2, 4, 8, 9, 10, 11, 12).  Bits 1 to 3
+                         and 8 to 12 change no value (bits 10 to 12 none
+                         of bit 4's). */
   long   mxstep;      /* max steps per CVode call (CVODE default 500) */
   int    use_jtv;     /* analytic Jv (else difference quotient) */
   int    use_prec;    /* block-Jacobi preconditioner */

@@ -152,6 +152,16 @@ int NVHip_WrmsNormPair(N_Vector x, sunrealtype a, N_Vector y, sunrealtype b, N_V
  * forms it (a pending completion of x), neither written. */
 int NVHip_WrmsNormPairComp(N_Vector x, sunrealtype cx, N_Vector xc, sunrealtype a, N_Vector y,
                            sunrealtype b, N_Vector z, N_Vector w, sunrealtype* nrm);
+/* NVHip_WrmsNormPairComp queued on x's stream without a wait: the result
+ * lands in the second result slot (the one a multi-dot projection uses), so
+ * reductions waited for on the first slot in between leave it alone; a
+ * later queue or projection overwrites it.  NVHip_WrmsNormPairWait waits
+ * for the pair queued as ticket and returns -1 when the slot holds another
+ * result.  No reference counterpart. */
+int NVHip_WrmsNormPairCompQueue(N_Vector x, sunrealtype cx, N_Vector xc, sunrealtype a,
+                                N_Vector y, sunrealtype b, N_Vector z, N_Vector w,
+                                unsigned* ticket);
+int NVHip_WrmsNormPairWait(N_Vector x, unsigned ticket, sunrealtype* nrm);
 
 /* One-pass kernels for the SPGMR loop (llgcv_fused_ops), each equal
  * element for element, and reduction for reduction, to the N_Vector

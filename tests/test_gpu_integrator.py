@@ -79,30 +79,38 @@ def test_device_driven_lagged_spgmr_is_bitwise_neutral(name, make, maxl):
     bit for bit; the host waits drop.  fused 3863 (1815 plus bit 11): the
     solution and the first Newton update of each iteration queued behind
     the solve (coefficients from the device's qr_sol), so no host round
-    trip sits between the solve and the update; the same bits.  maxl 1 makes every column the last:
-    its Pythagorean estimate or projection norm, and the Krylov cap."""
+    trip sits between the solve and the update; the same bits.  fused 7959
+    (3863 plus bit 12): on a step that will test an order change, the
+    test's two norms queued behind the first Newton update and used when
+    that iteration converged; the same bits and waits.  maxl 1 makes every
+    column the last: its Pythagorean estimate or projection norm, and the
+    Krylov cap."""
     cfg, geom, y0 = make()
     sim = la.Simulation(cfg, geom)
     res = {}
-    for fused in (791, 1815, 3863):
+    for fused in (791, 1815, 3863, 7959):
         run = sim.integrator(y0, rtol=1e-5, atol=1e-5, gstype=2, fused=fused, maxl=maxl)
         s0 = la.lib().NVHip_NumSyncs()      # a process-wide count
         out = run.advance(1.0)
         res[fused] = (run.state(), out, la.lib().NVHip_NumSyncs() - s0)
         run.close()
     (ya, sa, wa), (yb, sb, wb), (yc, sc, wc) = res[791], res[1815], res[3863]
+    yd, sd, wd = res[7959]
     print(f"[gpu] {name} maxl={maxl}: host-stepped nst={sa['nst']} nli={sa['nli']} "
           f"waits={wa}; device-stepped nst={sb['nst']} nli={sb['nli']} waits={wb}; "
           f"with the update queued behind the solve waits={wc}", flush=True)
     for k in ("nst", "nfe", "nni", "nli", "netf", "ncfn", "nsetups"):
-        assert sa[k] == sb[k] == sc[k], k
+        assert sa[k] == sb[k] == sc[k] == sd[k], k
     assert np.array_equal(ya, yb)
     assert np.array_equal(ya, yc)
+    assert np.array_equal(ya, yd)
     # maxl 1: both forms wait for pass 0 and, when needed, the projection norm
     assert wb < wa if maxl > 1 else wb <= wa
     # bit 11 keeps the waits (the solve's state, then the update's norm) but
     # the update is already queued when the solve's state arrives
     assert wc == wb
+    # bit 12 waits for the queued norms instead of the norms it would queue
+    assert wd == wc
     sim.close()
 
 

@@ -479,6 +479,36 @@ def test_wrms_norm_pair_comp_equals_completion_then_pair():
     assert np.array_equal(x.get(), x0) and np.array_equal(xc.get(), xc0)
 
 
+def test_wrms_norm_pair_queued_equals_pair_and_keeps_its_slot():
+    """NVHip_WrmsNormPairCompQueue then NVHip_WrmsNormPairWait ==
+    NVHip_WrmsNormPairComp bit for bit; reductions waited for in between
+    (first result slot) leave the queued pair alone; a later queue replaces
+    it, and the wait for the replaced ticket reports -1."""
+    L = la.lib()
+    rng = np.random.default_rng(72)
+    x, xc, yv, zv = _vecs(rng, 4)
+    w, = _vecs(rng, 1, lo=0.5)
+    a, b, c = -0.734, 1.0, 0.4166
+    ref = (C.c_double * 2)()
+    assert L.NVHip_WrmsNormPairComp(x.v, c, xc.v, a, yv.v, b, zv.v, w.v, ref) == 0
+    t1 = C.c_uint(0)
+    assert L.NVHip_WrmsNormPairCompQueue(x.v, c, xc.v, a, yv.v, b, zv.v, w.v, C.byref(t1)) == 0
+    d = L.N_VDotProd_Hip(yv.v, zv.v)          # a reduction waited for in between
+    assert dot_close(d, yv.get(), zv.get())
+    nr = (C.c_double * 2)()
+    assert L.NVHip_WrmsNormPairWait(x.v, t1.value, nr) == 0
+    assert list(nr) == list(ref)
+    t2 = C.c_uint(0)
+    assert L.NVHip_WrmsNormPairCompQueue(x.v, -c, xc.v, b, yv.v, a, zv.v, w.v, C.byref(t2)) == 0
+    assert t2.value != t1.value
+    assert L.NVHip_WrmsNormPairWait(x.v, t2.value, nr) == 0
+    bad = (C.c_double * 2)()
+    assert L.NVHip_WrmsNormPairWait(x.v, t1.value, bad) == -1
+    ref2 = (C.c_double * 2)()
+    assert L.NVHip_WrmsNormPairComp(x.v, -c, xc.v, b, yv.v, a, zv.v, w.v, ref2) == 0
+    assert list(nr) == list(ref2)
+
+
 def test_wrms_norm_pair_equals_two_norms():
     """NVHip_WrmsNormPair == N_VWrmsNorm(x, w) and N_VLinearSum(a, y, b, z,
     t) + N_VWrmsNorm(t, w), bit for bit, with one host wait"""

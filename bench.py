@@ -581,7 +581,7 @@ def main():
     step = 1.0
     ropts = dict(rtol=1e-4, atol=1e-4, maxl=5, gstype=args.gstype, max_order=5, t_out=0.1,
                  mxstep=100000)
-    ropts["fused"] = {"pipelined": 823, "lagged": 7959, "lagged-host": 791, "host": 775,
+    ropts["fused"] = {"pipelined": 823, "lagged": 16151, "lagged-host": 791, "host": 775,
                       "device": 783}[args.spgmr] | \
         args.fused_extra
     if args.red_blocks:

@@ -170,9 +170,10 @@ struct Integrator {
   // The step's completion zn[j] += l[j] acor (j = 0..q) not yet written,
   // with the rescale zn[j] *= c[j] (j = 1..q) of a step-size change when
   // scaled: left to the next predictor's pass (flush_completion writes it
-  // where another reader of zn needs it).
+  // where another reader of zn needs it).  save: the copy zn[qmax] = acor
+  // of the step is pending too (fused.predict_complete_save).
   struct {
-    bool on = false, scaled = false;
+    bool on = false, scaled = false, save = false;
     int q = 0;
     double l[kLMax] = {};
     double c[kLMax] = {};
@@ -310,6 +311,8 @@ void flush_completion(Integrator* m) {
   auto& p = m->pend_c;
   if (!p.on) return;
   p.on = false;
+  if (p.save) N_VScale(1.0, m->acor, m->zn[m->qmax]);
+  p.save = false;
   N_VScaleAddMulti(p.q + 1, p.l, m->acor, m->zn, m->zn);
   if (p.scaled) N_VScaleVectorArray(p.q, p.c + 1, m->zn + 1, m->zn + 1);
   p.scaled = false;
@@ -326,13 +329,20 @@ void predict(Integrator* m) {
   if (pc.on) {
     // the previous step's completion (and rescale) in the same pass
     pc.on = false;
-    m->newton_init = pc.q == m->q && m->fused.predict_complete &&
-                     m->fused.predict_complete(m->q, m->zn, m->acor, pc.l,
-                                               pc.scaled ? pc.c : nullptr, m->y, m->reltol,
-                                               m->sabstol, m->itol == 2 ? m->vabstol : nullptr,
-                                               w) == 0;
+    const double* c = pc.scaled ? pc.c : nullptr;
+    N_Vector av = m->itol == 2 ? m->vabstol : nullptr;
+    if (pc.save)
+      m->newton_init = pc.q == m->q && m->fused.predict_complete_save &&
+                       m->fused.predict_complete_save(m->q, m->zn, m->acor, pc.l, c, m->y,
+                                                      m->reltol, m->sabstol, av, w,
+                                                      m->zn[m->qmax]) == 0;
+    else
+      m->newton_init = pc.q == m->q && m->fused.predict_complete &&
+                       m->fused.predict_complete(m->q, m->zn, m->acor, pc.l, c, m->y,
+                                                 m->reltol, m->sabstol, av, w) == 0;
     if (m->newton_init) {
       pc.scaled = false;
+      pc.save = false;
       m->ewt_pending = false;
       return;
     }
@@ -1490,6 +1500,7 @@ void complete_step(Integrator* m) {
     auto& pc = m->pend_c;
     pc.on = true;
     pc.scaled = false;
+    pc.save = false;
     pc.q = m->q;
     for (int j = 0; j <= m->q; j++) pc.l[j] = m->l[j];
   } else {
@@ -1497,7 +1508,12 @@ void complete_step(Integrator* m) {
   }
   m->qwait--;
   if (m->qwait == 1 && m->q != m->qmax) {
-    N_VScale(1.0, m->acor, m->zn[m->qmax]);
+    // zn[qmax] = acor, for the order-change test two steps on: left to the
+    // next predictor's pass with the completion when it can be
+    if (m->pend_c.on && m->fused.predict_complete_save)
+      m->pend_c.save = true;
+    else
+      N_VScale(1.0, m->acor, m->zn[m->qmax]);
     m->saved_tq5 = m->tq[5];
     m->indx_acor = m->qmax;
   }

@@ -67,7 +67,7 @@ void LLG_DefaultRunOpts(llg_run_opts* o) {
   o->gstype = 0;        // auto: classical iff neq < 500000 (2d_i4/2d_fft.cu:645-650)
   o->max_order = 5;
   o->mxstep = 500;
-  o->fused = 7959;
+  o->fused = 16151;
   o->use_jtv = 1;
   o->use_prec = 1;
   o->pretype = SUN_PREC_LEFT;
@@ -125,6 +125,9 @@ void* LLG_RunCreate(llg_ctx* ctx, const llg_run_opts* opts, const double* y0_hos
     if (opts->fused & 512) {
       fo.predict_complete = NVHip_PredictComplete;
       fo.wrms_pair_comp = NVHip_WrmsNormPairComp;
+      // bit 13: a completed step's copy of acor into zn[qmax] left to the
+      // next predictor's pass as well
+      if (opts->fused & 8192) fo.predict_complete_save = NVHip_PredictCompleteSave;
     }
     if ((opts->fused & 4) && opts->use_jtv && opts->use_prec && opts->pretype == SUN_PREC_LEFT) {
       fo.krylov = LLG_Krylov;

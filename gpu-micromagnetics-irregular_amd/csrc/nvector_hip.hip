@@ -271,13 +271,14 @@ template <int Q, bool COMP>
 __global__ void k_pascal_init(OutPack Z, double* __restrict__ acor, double* __restrict__ y,
                               sunindextype n, double rtol, double atol,
                               const double* __restrict__ av, double* __restrict__ w,
-                              CoefPack lc, CoefPack sc) {
+                              CoefPack lc, CoefPack sc, double* __restrict__ save) {
   GRID_STRIDE(i, n) {
     double z[Q + 1];
 #pragma unroll
     for (int j = 0; j <= Q; j++) z[j] = Z.p[j][i];
     if constexpr (COMP) {
       const double a = acor[i];
+      if (save) save[i] = a;   // the completed step's correction kept (N_VScale(1, acor, save))
 #pragma unroll
       for (int j = 0; j <= Q; j++) z[j] = fma(lc.c[j], a, z[j]);
 #pragma unroll
@@ -1456,8 +1457,15 @@ int NVHip_Pascal(int q, N_Vector* zn, sunrealtype sign) {
 
 static int predict_launch(int q, N_Vector* zn, N_Vector acor, const double* lc,
                           const double* sc, N_Vector y, double rtol, double atol,
-                          N_Vector abstol, N_Vector ewt) {
+                          N_Vector abstol, N_Vector ewt, N_Vector save = nullptr) {
   if (q < 1 || q > 5 || !acor || !y) return -1;
+  if (save && (!lc || save->ops->nvgetvectorid != N_VGetVectorID_Hip ||
+               LEN(save) != LEN(zn[0]) || D(save) == D(acor) || D(save) == D(y) ||
+               (ewt && D(save) == D(ewt))))
+    return -1;
+  if (save)
+    for (int j = 0; j <= q; j++)
+      if (zn[j] && D(zn[j]) == D(save)) return -1;
   if (ewt && (ewt->ops->nvgetvectorid != N_VGetVectorID_Hip || LEN(ewt) != LEN(zn[0])))
     return -1;
   if (abstol && LEN(abstol) != LEN(zn[0])) return -1;
@@ -1482,12 +1490,13 @@ static int predict_launch(int q, N_Vector* zn, N_Vector acor, const double* lc,
     lp.c[j] = lc ? lc[j] : 0.0;
     sp.c[j] = (sc && j > 0) ? sc[j] : 1.0;
   }
+  double* const dsave = save ? D(save) : nullptr;
   switch (q * 2 + (lc ? 1 : 0)) {
 #define NVHIP_PI_CASE(k)                                                                      \
     case 2 * k: hipLaunchKernelGGL((k_pascal_init<k, false>), g, b, 0, s, zp, D(acor), D(y), n, \
-                                   rtol, atol, av, dw, lp, sp); break;                        \
+                                   rtol, atol, av, dw, lp, sp, (double*)nullptr); break;      \
     case 2 * k + 1: hipLaunchKernelGGL((k_pascal_init<k, true>), g, b, 0, s, zp, D(acor), D(y), \
-                                       n, rtol, atol, av, dw, lp, sp); break;
+                                       n, rtol, atol, av, dw, lp, sp, dsave); break;
     NVHIP_PI_CASE(1) NVHIP_PI_CASE(2) NVHIP_PI_CASE(3) NVHIP_PI_CASE(4) NVHIP_PI_CASE(5)
 #undef NVHIP_PI_CASE
     default: return -1;
@@ -1506,6 +1515,13 @@ int NVHip_PredictComplete(int q, N_Vector* zn, N_Vector acor, const sunrealtype*
                           N_Vector abstol, N_Vector ewt) {
   if (!l) return -1;
   return predict_launch(q, zn, acor, l, c, y, rtol, atol, abstol, ewt);
+}
+
+int NVHip_PredictCompleteSave(int q, N_Vector* zn, N_Vector acor, const sunrealtype* l,
+                              const sunrealtype* c, N_Vector y, sunrealtype rtol,
+                              sunrealtype atol, N_Vector abstol, N_Vector ewt, N_Vector save) {
+  if (!l || !save) return -1;
+  return predict_launch(q, zn, acor, l, c, y, rtol, atol, abstol, ewt, save);
 }
 
 // x_comp: nrm[0] is the norm of fma(cx, x_comp, x) (x + cx x_comp as

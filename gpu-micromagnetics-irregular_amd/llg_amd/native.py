@@ -170,6 +170,8 @@ SIGNATURES = {
     "NVHip_WrmsNormPair": (C.c_int, [VP, C.c_double, VP, C.c_double, VP, VP, DP]),
     "NVHip_PredictComplete": (C.c_int, [C.c_int, C.POINTER(VP), VP, DP, DP, VP, C.c_double,
                                         C.c_double, VP, VP]),
+    "NVHip_PredictCompleteSave": (C.c_int, [C.c_int, C.POINTER(VP), VP, DP, DP, VP, C.c_double,
+                                        C.c_double, VP, VP, VP]),
     "NVHip_WrmsNormPairComp": (C.c_int, [VP, C.c_double, VP, C.c_double, VP, C.c_double, VP,
                                          VP, DP]),
     "NVHip_WrmsNormPairCompQueue": (C.c_int, [VP, C.c_double, VP, C.c_double, VP, C.c_double,

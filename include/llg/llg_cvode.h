@@ -226,6 +226,12 @@ typedef struct llgcv_fused_ops {
   int (*pair_queue)(N_Vector x, sunrealtype cx, N_Vector xc, sunrealtype a, N_Vector y,
                     sunrealtype b, N_Vector z, N_Vector w, unsigned* ticket);
   int (*pair_wait)(N_Vector x, unsigned ticket, sunrealtype* nrm);
+  /* with predict_complete: the copy of acor into zn[qmax] that a completed
+   * step keeps for the next order-change test left to the next predictor's
+   * pass too (as NVHip_PredictCompleteSave) */
+  int (*predict_complete_save)(int q, N_Vector* zn, N_Vector acor, const sunrealtype* l,
+                               const sunrealtype* c, N_Vector y, sunrealtype rtol,
+                               sunrealtype atol, N_Vector abstol, N_Vector ewt, N_Vector save);
 } llgcv_fused_ops;
 
 void* LLGCVodeCreate(int lmm, SUNContext sunctx);

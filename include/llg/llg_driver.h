@@ -64,10 +64,12 @@ typedef struct llg_run_opts {
                          on a step that will test an order change, the
                          test's two norms queued behind the first Newton
                          update (NVHip_WrmsNormPairCompQueue), one host
-                         round trip less on such a step; default 7959
-                         (bits 0, 1, 2, 4, 8, 9, 10, 11, 12).  Bits 1 to 3
-                         and 8 to 12 change no value (bits 10 to 12 none
-                         of bit 4's). */
+                         round trip less on such a step; bit 13 (with bit
+                         9): a completed step's copy of acor into
+                         zn[qmax] left to the next predictor's pass too
+                         (NVHip_PredictCompleteSave); default 16151 (bits
+                         0, 1, 2, 4, 8 to 13).  Bits 1 to 3 and 8 to 13
+                         change no value (bits 10 to 12 none of bit 4's). */
   long   mxstep;      /* max steps per CVode call (CVODE default 500) */
   int    use_jtv;     /* analytic Jv (else difference quotient) */
   int    use_prec;    /* block-Jacobi preconditioner */

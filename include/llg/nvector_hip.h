@@ -143,6 +143,13 @@ int NVHip_PredictInit(int q, N_Vector* zn, N_Vector acor, N_Vector y, sunrealtyp
 int NVHip_PredictComplete(int q, N_Vector* zn, N_Vector acor, const sunrealtype* l,
                           const sunrealtype* c, N_Vector y, sunrealtype rtol, sunrealtype atol,
                           N_Vector abstol, N_Vector ewt);
+/* NVHip_PredictComplete that also keeps the completed step's correction:
+ * save = acor (N_VScale(1, acor, save)) before acor is zeroed, in the same
+ * pass; save may not alias zn[0..q], acor, y or ewt.  No reference
+ * counterpart. */
+int NVHip_PredictCompleteSave(int q, N_Vector* zn, N_Vector acor, const sunrealtype* l,
+                              const sunrealtype* c, N_Vector y, sunrealtype rtol,
+                              sunrealtype atol, N_Vector abstol, N_Vector ewt, N_Vector save);
 /* Two WRMS norms with one host wait: nrm[0] = |x|_WRMS(w) and nrm[1] =
  * |a y + b z|_WRMS(w), the combination not written; each equal to
  * N_VWrmsNorm (after N_VLinearSum for the second) bit for bit. */

@@ -40,18 +40,19 @@ def test_fused_integrator_kernels_are_bitwise_neutral(gstype, case):
     7: also the one-launch Krylov operator chain; 15: also the whole SPGMR
     solve queued on the device with one host wait (classical GS only);
     263: 7 with the Newton residual formed in the RHS pass; 775: 263 with
-    each step's completion left to the next predictor's pass.  All must
-    agree bit for bit, counters included."""
+    each step's completion left to the next predictor's pass; 8967: 775
+    with the completed step's copy of acor into zn[qmax] left to that pass
+    too.  All must agree bit for bit, counters included."""
     cfg, geom, y0 = _case() if case == "square_hole" else _compact_case()
     sim = la.Simulation(cfg, geom)
     res = {}
-    for fused in (1, 3, 7, 15, 263, 775):
+    for fused in (1, 3, 7, 15, 263, 775, 8967):
         run = sim.integrator(y0, rtol=1e-5, atol=1e-5, gstype=gstype, fused=fused)
         out = run.advance(1.0)
         res[fused] = (run.state(), out)
         run.close()
     ya, sa = res[1]
-    for fused in (3, 7, 15, 263, 775):
+    for fused in (3, 7, 15, 263, 775, 8967):
         yb, sb = res[fused]
         print(f"[gpu] {case} gs={gstype} fused 1: nst={sa['nst']} nli={sa['nli']} "
               f"syncs={sa['nsyncs']}; fused {fused}: nst={sb['nst']} nli={sb['nli']} "
@@ -82,13 +83,15 @@ def test_device_driven_lagged_spgmr_is_bitwise_neutral(name, make, maxl):
     trip sits between the solve and the update; the same bits.  fused 7959
     (3863 plus bit 12): on a step that will test an order change, the
     test's two norms queued behind the first Newton update and used when
-    that iteration converged; the same bits and waits.  maxl 1 makes every
+    that iteration converged; the same bits and waits.  fused 16151 (7959
+    plus bit 13): a completed step's copy of acor into zn[qmax] made by the
+    next predictor's pass; the same bits and waits.  maxl 1 makes every
     column the last: its Pythagorean estimate or projection norm, and the
     Krylov cap."""
     cfg, geom, y0 = make()
     sim = la.Simulation(cfg, geom)
     res = {}
-    for fused in (791, 1815, 3863, 7959):
+    for fused in (791, 1815, 3863, 7959, 16151):
         run = sim.integrator(y0, rtol=1e-5, atol=1e-5, gstype=2, fused=fused, maxl=maxl)
         s0 = la.lib().NVHip_NumSyncs()      # a process-wide count
         out = run.advance(1.0)
@@ -96,14 +99,16 @@ def test_device_driven_lagged_spgmr_is_bitwise_neutral(name, make, maxl):
         run.close()
     (ya, sa, wa), (yb, sb, wb), (yc, sc, wc) = res[791], res[1815], res[3863]
     yd, sd, wd = res[7959]
+    ye, se, we = res[16151]
     print(f"[gpu] {name} maxl={maxl}: host-stepped nst={sa['nst']} nli={sa['nli']} "
           f"waits={wa}; device-stepped nst={sb['nst']} nli={sb['nli']} waits={wb}; "
           f"with the update queued behind the solve waits={wc}", flush=True)
     for k in ("nst", "nfe", "nni", "nli", "netf", "ncfn", "nsetups"):
-        assert sa[k] == sb[k] == sc[k] == sd[k], k
+        assert sa[k] == sb[k] == sc[k] == sd[k] == se[k], k
     assert np.array_equal(ya, yb)
     assert np.array_equal(ya, yc)
     assert np.array_equal(ya, yd)
+    assert np.array_equal(ya, ye)
     # maxl 1: both forms wait for pass 0 and, when needed, the projection norm
     assert wb < wa if maxl > 1 else wb <= wa
     # bit 11 keeps the waits (the solve's state, then the update's norm) but
@@ -111,10 +116,11 @@ def test_device_driven_lagged_spgmr_is_bitwise_neutral(name, make, maxl):
     assert wc == wb
     # bit 12 waits for the queued norms instead of the norms it would queue
     assert wd == wc
+    assert we == wd
     sim.close()
 
 
-@pytest.mark.parametrize("fused", [7, 23, 55, 215, 279, 791, 1815])
+@pytest.mark.parametrize("fused", [7, 23, 55, 215, 279, 791, 1815, 16151])
 @pytest.mark.parametrize("name,kw", [("square_hole", {}), ("i6", {"demag": 4.0}),
                                      ("plain128", {})])
 def test_trajectory_matches_oracle(name, kw, fused):
@@ -126,7 +132,9 @@ def test_trajectory_matches_oracle(name, kw, fused):
     formed where they are read (never written); 279: 23 with the Newton
     residual formed in the RHS pass; 791: 279 with each step's completion
     left to the next predictor's pass; 1815: 791 with each SPGMR solve
-    queued whole and stepped on the device"""
+    queued whole and stepped on the device; 16151: the default (1815 with
+    the solution and Newton update, the order-change norms and the acor
+    copy queued behind the solve or left to the next predictor)"""
     ng, ny = (48, 32) if name != "plain128" else (32, 32)
     cfg = la.preset(name, ng=ng, ny=ny, **kw)
     if name == "plain128":

@@ -460,6 +460,40 @@ def test_predict_complete_equals_completion_rescale_then_predict_init(q, weights
                                    None) == -1   # no completion coefficients
 
 
+@pytest.mark.parametrize("q,scaled", [(2, False), (4, True)])
+def test_predict_complete_save_equals_copy_then_predict_complete(q, scaled):
+    """NVHip_PredictCompleteSave == N_VScale(1, acor, save) then
+    NVHip_PredictComplete, bit for bit; save may not alias zn, acor or y"""
+    L = la.lib()
+    n = 20_011
+    rng = np.random.default_rng(120 + q)
+    Z = [rng.standard_normal(n) for _ in range(q + 1)]
+    A = rng.standard_normal(n)
+    za, zb = [vec(a) for a in Z], [vec(a) for a in Z]
+    acor_a, acor_b = vec(A), vec(A)
+    y_a, y_b = vec(np.zeros(n)), vec(np.zeros(n))
+    w_a, w_b = vec(np.zeros(n)), vec(np.zeros(n))
+    sa, sb = vec(np.full(n, 3.0)), vec(np.full(n, 3.0))
+    lv = (C.c_double * 6)(*([1.0] + list(rng.uniform(-2.0, 2.0, 5))))
+    cv = (C.c_double * 6)(*[0.91 ** j for j in range(6)])
+    c = cv if scaled else None
+    L.N_VScale_Hip(1.0, acor_a.v, sa.v)
+    assert L.NVHip_PredictComplete(q, arr(za), acor_a.v, lv, c, y_a.v, 1e-4, 1e-5, None,
+                                   w_a.v) == 0
+    assert L.NVHip_PredictCompleteSave(q, arr(zb), acor_b.v, lv, c, y_b.v, 1e-4, 1e-5, None,
+                                       w_b.v, sb.v) == 0
+    assert np.array_equal(sb.get(), A)
+    assert np.array_equal(sa.get(), sb.get())
+    for j in range(q + 1):
+        assert np.array_equal(za[j].get(), zb[j].get()), j
+    assert np.array_equal(acor_a.get(), acor_b.get())
+    assert np.array_equal(y_a.get(), y_b.get())
+    assert np.array_equal(w_a.get(), w_b.get())
+    for bad in (zb[0], acor_b, y_b, w_b):
+        assert L.NVHip_PredictCompleteSave(q, arr(zb), acor_b.v, lv, c, y_b.v, 1e-4, 1e-5,
+                                           None, w_b.v, bad.v) == -1
+
+
 def test_wrms_norm_pair_comp_equals_completion_then_pair():
     """NVHip_WrmsNormPairComp(x, c, xc, ...) == N_VScaleAddMulti(1, c, xc, x)
     then NVHip_WrmsNormPair, bit for bit, x and xc left unchanged"""

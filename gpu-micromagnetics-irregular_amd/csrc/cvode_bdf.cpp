@@ -1794,7 +1794,8 @@ int LLGCVodeReInit(void* mem, sunrealtype t0, N_Vector y0) {
   if (!mem) return LLGCV_MEM_NULL;
   Integrator* m = static_cast<Integrator*>(mem);
   if (!m->zn[0]) return LLGCV_ILL_INPUT;
-  m->pend_c.on = m->pend_c.scaled = false;   // zn is replaced
+  m->pend_c.on = m->pend_c.scaled = m->pend_c.save = false;   // zn is replaced
+  m->spec.want = m->spec.on = m->spec.valid = false;
   m->tn = t0;
   m->tretlast = t0;
   N_VScale(1.0, y0, m->zn[0]);

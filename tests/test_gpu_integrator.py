@@ -69,7 +69,7 @@ LAG_CASES = [("square_hole_48x32", lambda: _case()),
                                             la.Geometry.full(64, 64).ic_stripes()))]
 
 
-@pytest.mark.parametrize("maxl", [5, 1])
+@pytest.mark.parametrize("maxl", [8, 5, 1])
 @pytest.mark.parametrize("name,make", LAG_CASES, ids=[c[0] for c in LAG_CASES])
 def test_device_driven_lagged_spgmr_is_bitwise_neutral(name, make, maxl):
     """fused 1815 (791 plus bit 10): every lagged SPGMR solve queued whole,
@@ -85,9 +85,10 @@ def test_device_driven_lagged_spgmr_is_bitwise_neutral(name, make, maxl):
     test's two norms queued behind the first Newton update and used when
     that iteration converged; the same bits and waits.  fused 16151 (7959
     plus bit 13): a completed step's copy of acor into zn[qmax] made by the
-    next predictor's pass; the same bits and waits.  maxl 1 makes every
-    column the last: its Pythagorean estimate or projection norm, and the
-    Krylov cap."""
+    next predictor's pass; the same bits and waits.  maxl 8, the widest
+    lagged solve, runs every pass and step kernel instantiation (and the
+    widest solution + update); maxl 1 makes every column the last: its
+    Pythagorean estimate or projection norm, and the Krylov cap."""
     cfg, geom, y0 = make()
     sim = la.Simulation(cfg, geom)
     res = {}
@@ -117,6 +118,37 @@ def test_device_driven_lagged_spgmr_is_bitwise_neutral(name, make, maxl):
     # bit 12 waits for the queued norms instead of the norms it would queue
     assert wd == wc
     assert we == wd
+    sim.close()
+
+
+def test_widest_lagged_solve_is_bitwise_neutral():
+    """Krylov dimension 8 on a workload whose solves go deeper than 5
+    iterations (the 128x64 square hole at rtol = atol = 1e-3 over three
+    units: the run differs from the same run at dimension 5, so some solve
+    used more than 5 columns): the pass and step kernels of columns 5..7,
+    the solve's second part and the widest solution + Newton update run,
+    and the device-stepped forms (1815, 16151) equal the host-stepped
+    lagged solve (791) bit for bit."""
+    cfg, geom, y0 = _case(ng=128, ny=64)
+    sim = la.Simulation(cfg, geom)
+    res = {}
+    for fused, maxl in ((791, 8), (1815, 8), (16151, 8), (16151, 5)):
+        run = sim.integrator(y0, rtol=1e-3, atol=1e-3, gstype=2, fused=fused, maxl=maxl,
+                             mxstep=100000)
+        out = run.advance(3.0)
+        res[(fused, maxl)] = (run.state(), out)
+        run.close()
+    ya, sa = res[(791, 8)]
+    print(f"[gpu] widest lagged solve: nst={sa['nst']} nni={sa['nni']} nli={sa['nli']} "
+          f"ncfn={sa['ncfn']}; at dimension 5 nli={res[(16151, 5)][1]['nli']}", flush=True)
+    for key in ((1815, 8), (16151, 8)):
+        yb, sb = res[key]
+        for k in ("nst", "nfe", "nni", "nli", "netf", "ncfn", "nsetups"):
+            assert sa[k] == sb[k], (key, k)
+        assert np.array_equal(ya, yb), key
+    y5, s5 = res[(16151, 5)]
+    assert s5["nli"] != sa["nli"] or not np.array_equal(y5, ya)
+    assert sa["nli"] > 4 * sa["nni"]
     sim.close()
 
 

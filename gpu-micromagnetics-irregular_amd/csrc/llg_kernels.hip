@@ -141,9 +141,18 @@ struct VTile {
   }
 };
 
+// the state y read by field / jtv_cell: stored, or staged in an LDS tile
+// with VTile's one-cell halo (the lagged Krylov pass stages it beside v)
+__device__ inline double yat(const double* __restrict__ y, int a, int N, int idx, int) {
+  return y[a * N + idx];
+}
+template <int SY, int SX>
+__device__ inline double yat(const VTile<SY, SX>& y, int a, int N, int idx, int nb) {
+  return y.at(a, N, idx, nb);
+}
 // h (without demag) from terms T; optional linearised partner dh for v
-template <bool LIN, class VA = VPlain>
-__device__ inline void field(const GeoD& g, const TermsD& T, const double* __restrict__ y,
+template <bool LIN, class VA = VPlain, class YP = const double*>
+__device__ inline void field(const GeoD& g, const TermsD& T, const YP& y,
                              const VA& v, int cell, int gx, int gy,
                              double h[3], double dh[3]) {
   const int N = g.ncell;
@@ -151,7 +160,7 @@ __device__ inline void field(const GeoD& g, const TermsD& T, const double* __res
   nbrs(g, gx, gy, lc, rc, uc, dc);
   double msk[3], nsk[3];
   axes(g, T, cell, msk, nsk);
-  const double m[3] = {y[cell], y[N + cell], y[2 * N + cell]};
+  const double m[3] = {yat(y, 0, N, cell, 0), yat(y, 1, N, cell, 0), yat(y, 2, N, cell, 0)};
   const bool win = T.window && g.w;
   const double wl = win ? g.w[lc] : 1.0, wr = win ? g.w[rc] : 1.0;
   const double wu = win ? g.w[uc] : 1.0, wd = win ? g.w[dc] : 1.0;
@@ -163,8 +172,8 @@ __device__ inline void field(const GeoD& g, const TermsD& T, const double* __res
   }
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    const double L = wl * y[a * N + lc], R = wr * y[a * N + rc];
-    const double U = wu * y[a * N + uc], D = wd * y[a * N + dc];
+    const double L = wl * yat(y, a, N, lc, 1), R = wr * yat(y, a, N, rc, 2);
+    const double U = wu * yat(y, a, N, uc, 3), D = wd * yat(y, a, N, dc, 4);
     double ex = L + R + U + D;
     if (g.exch_self) ex = ex - 4.0 * m[a];
     double an;
@@ -366,14 +375,14 @@ __global__ __launch_bounds__(BX* BY) void k_rhs_tiled(GeoD g, TermsD T,
 // ------------------------------------------------------------------- Jv
 // Jv = J(y) v; with AT the SPGMR operator z = v + (-gamma) Jv is written
 // instead, rounded exactly as N_VLinearSum(1, v, -gamma, Jv) rounds it
-template <bool AT, class VA>
-__device__ inline void jtv_cell(const GeoD& g, const TermsD& T, const double* __restrict__ y,
+template <bool AT, class VA, class YP = const double*>
+__device__ inline void jtv_cell(const GeoD& g, const TermsD& T, const YP& y,
                                 const VA& v, int cell, int gx, int gy, double mgamma,
                                 double o[3]) {
   const int N = g.ncell;
   double h[3], dh[3];
   field<true>(g, T, y, v, cell, gx, gy, h, dh);
-  const double m1 = y[cell], m2 = y[N + cell], m3 = y[2 * N + cell];
+  const double m1 = yat(y, 0, N, cell, 0), m2 = yat(y, 1, N, cell, 0), m3 = yat(y, 2, N, cell, 0);
   const double v1 = v.at(0, N, cell, 0), v2 = v.at(1, N, cell, 0), v3 = v.at(2, N, cell, 0);
   const double mh = m1 * h[0] + m2 * h[1] + m3 * h[2];
   const double dmh = (v1 * h[0] + v2 * h[1] + v3 * h[2]) + (m1 * dh[0] + m2 * dh[1] + m3 * dh[2]);
@@ -647,7 +656,8 @@ struct ArnIn {
 //     = U_0                            (L == 0 without PRE)
 //   w = s .* gate P (v + (-gamma) J v),  v = u / s
 // u is formed once per cell in an LDS tile with a one-cell halo (the halo
-// cell of an edge is the neighbour nbrs() names) and written for the
+// cell of an edge is the neighbour nbrs() names), staged there with the
+// state y that the operator reads at the same cells, and written for the
 // tile's own cells; w is written; each workgroup leaves its partial sums
 //   [|u|^2, <w,U_0>, .., <w,U_{L-1}>, <w,u>, |w|^2] (and with PRE also
 //   sum (win s)^2, the square of the WRMS norm of the right-hand side)
@@ -678,6 +688,7 @@ __global__ __launch_bounds__(BX* BY) void k_arnoldi(GeoD g, TermsD T, int pgate,
     for (int j = 0; j < L; j++) cb[j] = coef[1 + j];
   }
   __shared__ double tile[3][SY][SX];   // u / s with a one-cell halo
+  __shared__ double ytile[3][SY][SX];  // the state y, same halo
   __shared__ double own[3][BY][BX];    // u of the tile's own cells
   __shared__ double sm[K][NT / 64];
   const int N = g.ncell;
@@ -749,6 +760,7 @@ __global__ __launch_bounds__(BX* BY) void k_arnoldi(GeoD g, TermsD T, int pgate,
 #pragma unroll
       for (int a = 0; a < 3; a++) {
         tile[a][sy][sx] = u[a] / s[a * N + cidx];
+        ytile[a][sy][sx] = y[a * N + cidx];
         if (mine) {
           own[a][sy - 1][sx - 1] = u[a];
           if (L > 0 || PRE) in.ul[a * N + cidx] = u[a];
@@ -760,7 +772,8 @@ __global__ __launch_bounds__(BX* BY) void k_arnoldi(GeoD g, TermsD T, int pgate,
     if (gx < g.ng && gy < g.ny) {
       const int cell = gy * g.ng + gx;
       double o[3], z[3];
-      jtv_cell<true>(g, T, y, VTile<SY, SX>{tile, (int)threadIdx.x + 1, (int)threadIdx.y + 1},
+      jtv_cell<true>(g, T, VTile<SY, SX>{ytile, (int)threadIdx.x + 1, (int)threadIdx.y + 1},
+                     VTile<SY, SX>{tile, (int)threadIdx.x + 1, (int)threadIdx.y + 1},
                      cell, gx, gy, mgamma, o);
       apply_P_cell(g, pgate, P, cell, o, z);
 #pragma unroll

@@ -657,7 +657,8 @@ struct ArnIn {
 //   w = s .* gate P (v + (-gamma) J v),  v = u / s
 // u is formed once per cell in an LDS tile with a one-cell halo (the halo
 // cell of an edge is the neighbour nbrs() names), staged there with the
-// state y that the operator reads at the same cells, and written for the
+// state y that the operator reads at the same cells (and, for the own
+// cells, the scale s that w = s .* P(...) needs again), and written for the
 // tile's own cells; w is written; each workgroup leaves its partial sums
 //   [|u|^2, <w,U_0>, .., <w,U_{L-1}>, <w,u>, |w|^2] (and with PRE also
 //   sum (win s)^2, the square of the WRMS norm of the right-hand side)
@@ -690,6 +691,7 @@ __global__ __launch_bounds__(BX* BY) void k_arnoldi(GeoD g, TermsD T, int pgate,
   __shared__ double tile[3][SY][SX];   // u / s with a one-cell halo
   __shared__ double ytile[3][SY][SX];  // the state y, same halo
   __shared__ double own[3][BY][BX];    // u of the tile's own cells
+  __shared__ double sown[3][BY][BX];   // s of the tile's own cells
   __shared__ double sm[K][NT / 64];
   const int N = g.ncell;
   const int tid = threadIdx.y * BX + threadIdx.x;
@@ -759,10 +761,12 @@ __global__ __launch_bounds__(BX* BY) void k_arnoldi(GeoD g, TermsD T, int pgate,
       }
 #pragma unroll
       for (int a = 0; a < 3; a++) {
-        tile[a][sy][sx] = u[a] / s[a * N + cidx];
+        const double sv = s[a * N + cidx];
+        tile[a][sy][sx] = u[a] / sv;
         ytile[a][sy][sx] = y[a * N + cidx];
         if (mine) {
           own[a][sy - 1][sx - 1] = u[a];
+          sown[a][sy - 1][sx - 1] = sv;
           if (L > 0 || PRE) in.ul[a * N + cidx] = u[a];
         }
       }
@@ -778,7 +782,7 @@ __global__ __launch_bounds__(BX* BY) void k_arnoldi(GeoD g, TermsD T, int pgate,
       apply_P_cell(g, pgate, P, cell, o, z);
 #pragma unroll
       for (int a = 0; a < 3; a++) {
-        const double w = s[a * N + cell] * z[a];
+        const double w = sown[a][threadIdx.y][threadIdx.x] * z[a];
         wout[a * N + cell] = w;
         const double u = own[a][threadIdx.y][threadIdx.x];
         acc[0] = fma(u, u, acc[0]);

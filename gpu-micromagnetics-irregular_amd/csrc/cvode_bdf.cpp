@@ -718,8 +718,10 @@ int spgmr_lagged_dev(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vect
             f.pair_queue(m->zn[sp.q], sp.lq, m->acor, -sp.cquot, m->zn[m->qmax], 1.0, m->acor,
                          m->ewt, &sp.ticket) == 0;
   };
-  bool queued = upd && f.lag_update(b, l_max, V, s, x, m->nls_ycor, m->zn[0], m->ewt, m->y,
-                                    &ticket) == 0;
+  // x (the Newton correction del) is not read after the update, which forms
+  // ycor += x and its norm itself: the device does not write it
+  bool queued = upd && f.lag_update(b, l_max, V, s, nullptr, m->nls_ycor, m->zn[0], m->ewt,
+                                    m->y, &ticket) == 0;
   queue_pair(queued);
   llgcv_lag_result res;
   if (f.lag_wait(b, &res) != 0) return kLaggedFallback;
@@ -728,8 +730,8 @@ int spgmr_lagged_dev(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vect
     for (int l = kq + 1; l < l_max; l++)
       if (pass(l, 0) != 0) return kLaggedFallback;
     if (f.lag_projnorm(W[(l_max - 1) & 1], l_max, V) != 0) return kLaggedFallback;
-    queued = upd && f.lag_update(b, l_max, V, s, x, m->nls_ycor, m->zn[0], m->ewt, m->y,
-                                 &ticket) == 0;
+    queued = upd && f.lag_update(b, l_max, V, s, nullptr, m->nls_ycor, m->zn[0], m->ewt,
+                                 m->y, &ticket) == 0;
     queue_pair(queued);
     if (f.lag_wait(b, &res) != 0) return kLaggedFallback;
   }

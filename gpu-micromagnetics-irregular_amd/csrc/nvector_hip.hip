@@ -2341,7 +2341,7 @@ struct FNewtonUpdLag0 {
 #pragma unroll
     for (int j = 0; j < NV; j++) acc = fma(c[j], X.p[j][i], acc);
     const double d = acc / s[i];
-    x[i] = d;
+    if (x) x[i] = d;   // uniform: NULL when the caller does not read the solution
     const double cy = fma(1.0, ycor[i], 1.0 * d);
     ycor[i] = cy;
     y[i] = fma(1.0, zn0[i], 1.0 * cy);
@@ -2512,24 +2512,26 @@ int NVHip_LagProjNorm(N_Vector wv, int nb, N_Vector* U) {
 
 int NVHip_LagUpdate(N_Vector b, int nvec, N_Vector* V, N_Vector sv, N_Vector x, N_Vector ycor,
                     N_Vector zn0, N_Vector w, N_Vector y, unsigned* ticket) {
-  if (!b || !V || !sv || !x || !ycor || !zn0 || !w || !y || !ticket || !is_hip(b) ||
-      nvec < 1 || nvec > kMaxK || g_red_mode == 0)
+  if (!b || !V || !sv || !ycor || !zn0 || !w || !y || !ticket || !is_hip(b) || nvec < 1 ||
+      nvec > kMaxK || g_red_mode == 0)
     return -1;
   const sunindextype n = LEN(b);
   hipStream_t s = S(b);
-  if (LEN(sv) != n || LEN(x) != n || LEN(ycor) != n || LEN(zn0) != n || LEN(w) != n ||
-      LEN(y) != n || S(ycor) != s || S(x) != s)
+  if (LEN(sv) != n || LEN(ycor) != n || LEN(zn0) != n || LEN(w) != n || LEN(y) != n ||
+      S(ycor) != s || (x && (LEN(x) != n || S(x) != s)))
     return -1;
   // the Newton update's aliasing rules (NVHip_NewtonUpdate), and x not a
   // basis vector or the scale (every element is read before x is written,
   // but x == X_j would change the values other elements read)
-  if (D(y) == D(ycor) || D(y) == D(x) || D(y) == D(zn0) || D(ycor) == D(x) || D(x) == D(sv))
+  double* const dx = x ? D(x) : nullptr;
+  if (D(y) == D(ycor) || D(y) == D(zn0) ||
+      (dx && (D(y) == dx || D(ycor) == dx || dx == D(sv))))
     return -1;
   RedWs* wk = workspace(s);
   if (!wk->lag) return -1;
   PtrPack X{};
   for (int j = 0; j < nvec; j++) {
-    if (!V[j] || LEN(V[j]) != n || D(V[j]) == D(x) || D(V[j]) == D(y)) return -1;
+    if (!V[j] || LEN(V[j]) != n || (dx && D(V[j]) == dx) || D(V[j]) == D(y)) return -1;
     X.p[j] = D(V[j]);
   }
   static_assert(offsetof(LagDev, xready) == offsetof(LagDev, xk) + sizeof(int),
@@ -2537,7 +2539,7 @@ int NVHip_LagUpdate(N_Vector b, int nvec, N_Vector* V, N_Vector sv, N_Vector x, 
   const int g = red_grid(n);
   const unsigned seq = ++wk->seq;
   hipLaunchKernelGGL(k_newton_upd_lag, dim3(g), dim3(kBlock), 0, s, (const int*)&wk->lag->xk,
-                     (const double*)wk->lag->xc, &wk->lag->xskip, X, D(sv), D(w), D(zn0), D(x),
+                     (const double*)wk->lag->xc, &wk->lag->xskip, X, D(sv), D(w), D(zn0), dx,
                      D(ycor), D(y), n, wk->partials);
   launch_finish<1>(s, RED_SUM, g, wk, seq, 0, &wk->lag->xskip);
   check_launch("lag update");

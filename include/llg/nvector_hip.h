@@ -307,7 +307,8 @@ int NVHip_LagNext(N_Vector v);
  * (status other than converged or Krylov cap, a zero diagonal, or a cap
  * without residual reduction), and a part that ends with the solve still
  * running leaves none.  nvec: the basis vectors passed (>= the solve's
- * columns).  NVHip_LagUpdateWait waits for the update queued as *ticket:
+ * columns); x may be NULL when the caller does not read the solution (it
+ * is then not written).  NVHip_LagUpdateWait waits for the update queued as *ticket:
  * dnorm = sqrt(sum / n), or -1 when the update did nothing.  No other
  * reduction of the stream may be waited for in between.  No reference
  * counterpart. */

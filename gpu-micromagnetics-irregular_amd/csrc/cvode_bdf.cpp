@@ -706,7 +706,10 @@ int spgmr_lagged_dev(Integrator* m, N_Vector x, N_Vector b, double delta, N_Vect
     if (pass(l, l == kq) != 0) return kLaggedFallback;
   // on a Newton iteration's first solve the solution and the Newton update
   // follow each part on the device (they do nothing unless the solve ends
-  // there with a solution the iteration uses), so the iteration waits once
+  // there with a solution the iteration uses), so the iteration waits once;
+  // the correction ycor is still zero there (cv_nls or the predictor zeroed
+  // it, as does a restart after a convergence failure), which the queued
+  // update relies on
   const bool upd = f.lag_update && f.lag_update_wait && m->nls_curiter == 0 && x == b &&
                    m->nls_ycor && m->fused.newtonupd && !f.newtonupd_comb;
   unsigned ticket = 0;

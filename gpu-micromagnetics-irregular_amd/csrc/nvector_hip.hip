@@ -2331,7 +2331,8 @@ __global__ __launch_bounds__(kFinishBlock) void k_lag_step(int nblk, const doubl
 // The solution of an ended solve and the first Newton update in one pass:
 // x = (0 + sum_k c_k X_k) / s with the coefficients c_k of the solve's final
 // step (as k_lincombdiv), then FNewtonUpd0 with x (ycor += x, y = zn0 +
-// ycor, (x w)^2 summed).  NV, the number of basis vectors, is a template
+// ycor, (x w)^2 summed) on the zero correction of a Newton iteration's
+// start (ycor is written, not read).  NV, the number of basis vectors, is a template
 // argument so that the coefficients sit in registers.
 template <int NV>
 struct FNewtonUpdLag0 {
@@ -2342,7 +2343,9 @@ struct FNewtonUpdLag0 {
     for (int j = 0; j < NV; j++) acc = fma(c[j], X.p[j][i], acc);
     const double d = acc / s[i];
     if (x) x[i] = d;   // uniform: NULL when the caller does not read the solution
-    const double cy = fma(1.0, ycor[i], 1.0 * d);
+    // ycor is zero before a Newton iteration's first update: +0.0 in place of
+    // its load gives the bits of fma(1, ycor, d)
+    const double cy = fma(1.0, 0.0, 1.0 * d);
     ycor[i] = cy;
     y[i] = fma(1.0, zn0[i], 1.0 * cy);
     const double p = d * w[i];

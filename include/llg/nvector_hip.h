@@ -303,7 +303,8 @@ int NVHip_LagNext(N_Vector v);
  * solve's final step, which repeats the host exit's qr_sol and residual
  * test; rounded as NVHip_LinCombDiv) and then the first Newton update with
  * x (NVHip_NewtonUpdate with iter 0: ycor += x, y = zn0 + ycor, the sum of
- * (x w)^2).  Both do nothing when the solve leaves no solution to use
+ * (x w)^2; ycor must be zero on entry, as it is at a Newton iteration's
+ * start, and is written without being read).  Both do nothing when the solve leaves no solution to use
  * (status other than converged or Krylov cap, a zero diagonal, or a cap
  * without residual reduction), and a part that ends with the solve still
  * running leaves none.  nvec: the basis vectors passed (>= the solve's

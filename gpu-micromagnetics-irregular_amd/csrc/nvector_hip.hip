@@ -2289,7 +2289,8 @@ __device__ bool lag_run(LagDev* st, const double* r, int maxl, double delta, dou
 // zeroed for a new solve), stepped there by thread 0 (its dependent
 // accesses then cost LDS latency, not memory round trips) and copied back.
 // A solve that has ended (a pass queued past convergence) or a projection
-// norm that is not needed leaves everything as it is.
+// norm that is not needed leaves everything as it is: its flag, read with
+// the state, is tested before the partials are summed.
 template <int P, bool PROJ>
 __global__ __launch_bounds__(kFinishBlock) void k_lag_step(int nblk, const double* partials,
                                                            int last, LagDev* gst, LagPub* pub,
@@ -2304,12 +2305,14 @@ __global__ __launch_bounds__(kFinishBlock) void k_lag_step(int nblk, const doubl
   __shared__ double lst[NW];
   __shared__ double sm[K][kFinishBlock / 64];
   __shared__ int publish_s;
+  // the skip flag read with the state, tested before the partials' sum
+  const int skip_now = PROJ ? gst->pskip : (P > 0 ? gst->done : 0);
   for (int i = threadIdx.x; i < NW; i += blockDim.x)
     lst[i] = (fresh && i < NH) ? 0.0 : reinterpret_cast<const double*>(gst)[i];
+  if (skip_now) return;   // uniform
   double r[K];
   sum_partials<K, false, kFinishBlock / 64>(RED_SUM, nblk, partials, sm, r);   // barriers inside
   LagDev* st = reinterpret_cast<LagDev*>(lst);
-  if (PROJ ? st->pskip != 0 : (P > 0 && st->done != 0)) return;   // uniform
   if (threadIdx.x == 0) {
     bool ended;
     if constexpr (PROJ) {
